@@ -1,0 +1,87 @@
+/*
+ * libsais.h -- drop-in header of the MI355X suffix-array engine.
+ *
+ * Same prototypes, version macros and return-code conventions as libsais 2.10.4
+ * (include/libsais.h of the reference), for the entry points this engine provides:
+ * suffix array, generalized suffix array, BWT, PLCP (plain and GSA) and LCP.  The
+ * reference's context (_ctx), OpenMP (_omp), integer-alphabet (_int), sampled-BWT
+ * (_aux) and inverse-BWT (unbwt) functions are not provided, so a program that uses
+ * them fails at link time instead of silently running something else.
+ *
+ * All buffers are caller-owned host memory; calls are synchronous and may be made
+ * from several host threads at once.  Work runs on GPU 0.
+ */
+#ifndef LIBSAIS_H
+#define LIBSAIS_H 1
+
+#define LIBSAIS_VERSION_MAJOR   2
+#define LIBSAIS_VERSION_MINOR   10
+#define LIBSAIS_VERSION_PATCH   4
+#define LIBSAIS_VERSION_STRING  "2.10.4"
+
+#ifdef _WIN32
+    #ifdef LIBSAIS_SHARED
+        #ifdef LIBSAIS_EXPORTS
+            #define LIBSAIS_API __declspec(dllexport)
+        #else
+            #define LIBSAIS_API __declspec(dllimport)
+        #endif
+    #else
+        #define LIBSAIS_API
+    #endif
+#else
+    #define LIBSAIS_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#include <stdint.h>
+
+/*
+ * Suffix array of T[0..n-1] into SA[0..n-1] (SA has room for n+fs entries; only the
+ * first n are defined).  freq (256 entries, may be NULL) receives symbol counts.
+ * Returns 0, -1 for invalid arguments, -2 when memory cannot be obtained.
+ * Replaces reference include/libsais.h:84 (src/libsais.c:7018-7032).
+ */
+LIBSAIS_API int32_t libsais(const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq);
+
+/*
+ * Generalized suffix array of a 0-separated string set (T[n-1] must be 0; empty
+ * strings are rejected with -1).  Every separator sorts as a distinct symbol,
+ * smaller than every letter, ordered by position.
+ * Replaces reference include/libsais.h:95 (src/libsais.c:7034-7048, 6886-6889).
+ */
+LIBSAIS_API int32_t libsais_gsa(const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq);
+
+/*
+ * Burrows-Wheeler transform of T into U (U may alias T); A is temporary space of
+ * n+fs entries whose content is unspecified on return.  Returns the primary index,
+ * or -1 / -2.  Replaces reference include/libsais.h:182 (src/libsais.c:7097-7121).
+ */
+LIBSAIS_API int32_t libsais_bwt(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq);
+
+/*
+ * Permuted LCP array: PLCP[i] = lcp of suffix i and the suffix preceding it in SA.
+ * Returns 0 or -1.  Replaces reference include/libsais.h:368 (src/libsais.c:8363-8379).
+ */
+LIBSAIS_API int32_t libsais_plcp(const uint8_t * T, const int32_t * SA, int32_t * PLCP, int32_t n);
+
+/*
+ * PLCP of a generalized suffix array: matches stop at separators.
+ * Replaces reference include/libsais.h:378 (src/libsais.c:8381-8397).
+ */
+LIBSAIS_API int32_t libsais_plcp_gsa(const uint8_t * T, const int32_t * SA, int32_t * PLCP, int32_t n);
+
+/*
+ * LCP[i] = PLCP[SA[i]] (LCP may alias SA).  Returns 0 or -1.
+ * Replaces reference include/libsais.h:398 (src/libsais.c:8417-8432).
+ */
+LIBSAIS_API int32_t libsais_lcp(const int32_t * PLCP, const int32_t * SA, int32_t * LCP, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

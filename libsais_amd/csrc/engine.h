@@ -1,0 +1,80 @@
+// engine.h -- internal interfaces between the host orchestration (engine.hip) and
+// the kernel translation units.  Nothing here is part of the public C ABI.
+#pragma once
+
+#include "common.h"
+
+#include <stddef.h>
+
+// ------------------------------------------------------------------ radix sort
+// Onesweep LSD radix sort, 8-bit digits, stable.  Keys are u64 or u32, values u32.
+// The result ends in (k0,v0) or in (k1,v1); *result_in_alt says which.
+struct RadixScratch {
+    u64 *hist;        // [8][256] global digit histograms
+    u64 *lookback;    // [lb_tiles][256] look-back words (epoch-tagged)
+    u32 *tile_ctr;    // [8] tile tickets, one per pass
+    u64 lb_tiles;     // capacity of lookback in tiles
+    u32 epoch;        // advanced once per pass; lookback words of older passes are ignored
+    u64 *hist_host;   // pinned host copy of hist
+    // Optional hook called right before (phase 0) and after (phase 1) each pass
+    // launch; the engine uses it to bracket passes with timing events.
+    void (*hook)(void *user, int phase);
+    void *hook_user;
+};
+
+enum { RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS };
+
+int radix_sort_u64(RadixScratch &rs, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit,
+                   int end_bit, hipStream_t s, int *result_in_alt, int *passes_run);
+int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit,
+                   int end_bit, hipStream_t s, int *result_in_alt, int *passes_run);
+
+// ------------------------------------------------------- group segmentation
+// After a sort, mark group heads/tails, give every element its group head (SA
+// position of the group's first slot), write ISA for everyone, SA for retired
+// singletons, and compact the still-unsorted elements into the next active list.
+enum { SG_THREADS = 256, SG_ITEMS = 16, SG_TILE = SG_THREADS * SG_ITEMS };
+
+struct SegScratch {
+    u32 *tile_cnt;    // [tiles] active count per tile (scanned in place)
+    u32 *tile_head;   // [tiles] max head position per tile (scanned in place)
+    u32 *total;       // [1] number of still-active elements
+    u64 tiles_cap;
+};
+
+// Initial grouping: keys sorted, vals = suffix index.  sepmask != 0 marks GSA keys
+// that contain a separator (low symbol slot zero) as singletons.
+int seg_init(SegScratch &ss, const u64 *keys, const u32 *vals, u64 n, u64 sepmask, u32 *SA,
+             u32 *ISA, u32 *act_i, u32 *act_g, u32 *act_p, hipStream_t s);
+// Round grouping: comp sorted (old group << 32 | key2), vals = suffix index,
+// act_p = SA slot of each active position (unchanged by the sort).
+int seg_round(SegScratch &ss, const u64 *comp, const u32 *vals, const u32 *act_p, u64 a, u32 *SA,
+              u32 *ISA, u32 *nact_i, u32 *nact_g, u32 *nact_p, hipStream_t s);
+
+// ------------------------------------------------------------ SA kernels
+int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
+int launch_pack_keys(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u64 *keys,
+                     u32 *vals, hipStream_t s);
+int launch_round_keys(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h,
+                      u64 *comp, u32 *vals, hipStream_t s);
+
+// ------------------------------------------------------------ BWT / PLCP / LCP
+struct LcpScratch {
+    u32 *queue;       // [cap] positions whose lcp did not finish in the per-thread pass
+    u32 *queue_l;     // [cap] lcp matched so far
+    u32 *queue_n;     // [1]
+    u64 queue_cap;
+    u32 *tile_max;    // [tiles] per-tile max for the prefix-max scan
+    u64 tiles_cap;
+};
+
+int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
+int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
+// PLCP from either PHI (isa == nullptr) or ISA+SA (phi computed on the fly).
+int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
+                const u32 *SA, u32 *PLCP, hipStream_t s);
+int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
+
+// Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).
+int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
+int launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);

@@ -1,0 +1,730 @@
+// engine.hip -- device context, pooled HBM, and the prefix-doubling pipeline.
+//
+// Pipeline for one text (all on one stream):
+//   histogram (freq, alphabet)  ->  packed keys of the first k symbols
+//   ->  onesweep sort  ->  groups (ISA = group head, SA for singletons, active list)
+//   ->  rounds h = k, 2k, 4k, ...: key (group, rank of i+h) -> onesweep sort
+//       -> regroup, retire singletons, compact the active list   until none left
+//   ->  optional BWT gather, PLCP (irreducible + prefix max), LCP gather.
+#include "engine.h"
+#include "../../include/sagpu.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#define SAGPU_HIDDEN __attribute__((visibility("hidden")))
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+void sagpu_set_error(const char *msg, const char *file, int line)
+{
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s (%s:%d)", msg, file, line);
+    g_err = buf;
+}
+
+// ------------------------------------------------------------------ pool
+namespace {
+
+struct Pool {
+    std::multimap<size_t, void *> cache;
+    std::unordered_map<void *, size_t> live;
+    size_t limit = 0, live_bytes = 0;
+
+    static size_t round(size_t b)
+    {
+        if (b < (1u << 20)) return (b + 255) & ~(size_t)255;
+        return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    }
+
+    void *get(size_t bytes)
+    {
+        bytes = round(bytes ? bytes : 1);
+        if (limit && live_bytes + bytes > limit) {
+            sagpu_set_error("device memory limit reached", __FILE__, __LINE__);
+            return nullptr;
+        }
+        auto it = cache.lower_bound(bytes);
+        if (it != cache.end() && it->first <= bytes + bytes / 4 + (4u << 20)) {
+            void *p = it->second;
+            size_t sz = it->first;
+            cache.erase(it);
+            live[p] = sz;
+            live_bytes += sz;
+            return p;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                sagpu_set_error("hipMalloc failed", __FILE__, __LINE__);
+                return nullptr;
+            }
+        }
+        live[p] = bytes;
+        live_bytes += bytes;
+        return p;
+    }
+
+    void put(void *p)
+    {
+        if (!p) return;
+        auto it = live.find(p);
+        if (it == live.end()) return;
+        cache.emplace(it->second, p);
+        live_bytes -= it->second;
+        live.erase(it);
+    }
+
+    void trim()
+    {
+        for (auto &kv : cache) (void)hipFree(kv.second);
+        cache.clear();
+    }
+};
+
+struct Timer {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> free_ev;
+    struct Rec { int family; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+};
+
+enum Family { F_ONESWEEP = 0, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_COUNT };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    RadixScratch rs{};
+    u64 *pinned = nullptr;      // small pinned host staging (counts, histograms)
+    Timer timer;
+    sagpu_stats stats{};
+};
+
+std::mutex g_mu;
+Ctx *g_ctx = nullptr;
+bool g_timing = false;
+
+int ctx_get(Ctx **out)
+{
+    if (g_ctx) { *out = g_ctx; return SAGPU_OK; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        sagpu_set_error("no HIP device available", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    Ctx *c = new Ctx();
+    SAGPU_CHECK(hipSetDevice(c->device));
+    SAGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SAGPU_CHECK(hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault));
+    c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
+    g_ctx = c;
+    *out = c;
+    return SAGPU_OK;
+}
+
+// Per-call scope: every buffer taken from the pool returns to it on exit.
+struct Scope {
+    Ctx &c;
+    std::vector<void *> held;
+    explicit Scope(Ctx &ctx) : c(ctx) {}
+    ~Scope()
+    {
+        for (void *p : held) c.pool.put(p);
+    }
+    template <typename T> T *get(u64 count)
+    {
+        void *p = c.pool.get((size_t)count * sizeof(T));
+        if (p) held.push_back(p);
+        return (T *)p;
+    }
+};
+
+#define GET(var, T, count)                             \
+    T *var = sc.get<T>(count);                         \
+    if (!var) return SAGPU_ENOMEM
+
+// ------------------------------------------------------------------ timing
+struct Span {
+    Ctx &c;
+    int fam;
+    hipEvent_t a = nullptr, b = nullptr;
+    Span(Ctx &ctx, int family) : c(ctx), fam(family)
+    {
+        if (!c.timer.on) return;
+        if (!c.timer.free_ev.empty()) {
+            a = c.timer.free_ev.back().first;
+            b = c.timer.free_ev.back().second;
+            c.timer.free_ev.pop_back();
+        } else {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+        }
+        (void)hipEventRecord(a, c.stream);
+    }
+    ~Span()
+    {
+        if (!a) return;
+        (void)hipEventRecord(b, c.stream);
+        c.timer.recs.push_back({fam, a, b});
+    }
+};
+
+void timer_collect(Ctx &c)
+{
+    if (c.timer.recs.empty()) return;
+    (void)hipStreamSynchronize(c.stream);
+    for (auto &r : c.timer.recs) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        switch (r.family) {
+        case F_ONESWEEP: c.stats.ms_onesweep += ms; c.stats.onesweep_launches++; break;
+        case F_INITIAL: c.stats.ms_initial += ms; break;
+        case F_ROUNDS: c.stats.ms_rounds += ms; break;
+        case F_BWT: c.stats.ms_bwt += ms; break;
+        case F_PLCP: c.stats.ms_plcp += ms; break;
+        case F_LCP: c.stats.ms_lcp += ms; break;
+        case F_TOTAL: c.stats.ms_total += ms; break;
+        }
+        c.timer.free_ev.push_back({r.a, r.b});
+    }
+    c.timer.recs.clear();
+}
+
+// ------------------------------------------------------------------ pieces
+int bits_for(u64 v)
+{
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) b++;
+    return b;
+}
+
+int ensure_radix_scratch(Ctx &c, Scope &sc, u64 n)
+{
+    u64 tiles = (n + RS_TILE - 1) / RS_TILE + 1;
+    if (!c.rs.hist) {
+        c.rs.hist = (u64 *)c.pool.get(16 * 256 * sizeof(u64));
+        c.rs.tile_ctr = (u32 *)c.pool.get(64 * sizeof(u32));
+        if (!c.rs.hist || !c.rs.tile_ctr) return SAGPU_ENOMEM;
+    }
+    if (c.rs.lb_tiles < tiles) {
+        if (c.rs.lookback) c.pool.put(c.rs.lookback);
+        c.rs.lookback = (u64 *)c.pool.get(tiles * 256 * sizeof(u64));
+        if (!c.rs.lookback) { c.rs.lb_tiles = 0; return SAGPU_ENOMEM; }
+        c.rs.lb_tiles = tiles;
+        SAGPU_TRY(launch_fill_u64(c.rs.lookback, tiles * 256, 0, c.stream));
+        c.rs.epoch = 0;
+    }
+    (void)sc;
+    return SAGPU_OK;
+}
+
+// Brackets every onesweep pass launch with a pair of events (timing mode only).
+void onesweep_hook(void *user, int phase)
+{
+    Ctx &c = *(Ctx *)user;
+    static thread_local Span *open = nullptr;
+    if (phase == 0) open = new Span(c, F_ONESWEEP);
+    else { delete open; open = nullptr; }
+}
+
+int sort_u64(Ctx &c, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, int end_bit, int *alt)
+{
+    int passes = 0;
+    c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
+    c.rs.hook_user = &c;
+    int rc = radix_sort_u64(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
+    c.stats.sort_passes += passes;
+    c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
+    return rc;
+}
+
+int read_u32(Ctx &c, const u32 *d, u32 *out)
+{
+    SAGPU_CHECK(hipMemcpyAsync(c.pinned, d, sizeof(u32), hipMemcpyDeviceToHost, c.stream));
+    SAGPU_CHECK(hipStreamSynchronize(c.stream));
+    *out = *(u32 *)c.pinned;
+    return SAGPU_OK;
+}
+
+// Suffix array of the padded device text dT[0..n) (n >= 2, n < 2^32).  On return
+// dSA holds the suffix array and dISA its inverse.  freq_out (host, 256) optional.
+int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out)
+{
+    hipStream_t s = c.stream;
+    sagpu_stats &st = c.stats;
+
+    GET(d_hist, u64, 256 + 8);
+    u32 *d_bad = (u32 *)(d_hist + 256);
+    u64 freq[256];
+    int bits, k;
+    {
+        Span sp(c, F_INITIAL);
+        SAGPU_TRY(launch_fill_u64(d_hist, 256 + 8, 0, s));
+        SAGPU_TRY(launch_histogram(dT, n, gsa, d_hist, d_bad, s));
+    }
+    SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_hist, (256 + 1) * sizeof(u64), hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    memcpy(freq, c.pinned, sizeof freq);
+    u32 bad = *(u32 *)(c.pinned + 256);
+    if (freq_out) memcpy(freq_out, freq, sizeof freq);
+    if (gsa && bad) return SAGPU_EINVAL;
+
+    uint8_t *lut_h = (uint8_t *)(c.pinned + 4096);
+    int sigma = 0;
+    for (int ch = 0; ch < 256; ch++) {
+        lut_h[ch] = (uint8_t)sigma;
+        if (freq[ch]) sigma++;
+    }
+    bits = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
+    k = 64 / bits;
+    if (k > 64) k = 64;
+    st.initial_symbols = k;
+    st.bits_per_symbol = bits;
+
+    GET(d_lut, uint8_t, 256);
+    SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
+
+    GET(keysA, u64, n);
+    GET(keysB, u64, n);
+    GET(valsA, u32, n);
+    GET(valsB, u32, n);
+    GET(act_i, u32, n);
+    GET(act_g, u32, n);
+    GET(act_p0, u32, n);
+    GET(act_p1, u32, n);
+    u32 *act_p[2] = {act_p0, act_p1};
+
+    SegScratch ss{};
+    const u64 seg_tiles = (n + SG_TILE - 1) / SG_TILE + 1;
+    ss.tiles_cap = seg_tiles;
+    {
+        GET(t_cnt, u32, seg_tiles);
+        GET(t_head, u32, seg_tiles);
+        GET(t_total, u32, 64);
+        ss.tile_cnt = t_cnt;
+        ss.tile_head = t_head;
+        ss.total = t_total;
+    }
+    SAGPU_TRY(ensure_radix_scratch(c, sc, n));
+
+    u32 a = 0;
+    {
+        Span sp(c, F_INITIAL);
+        SAGPU_TRY(launch_pack_keys(dT, n, d_lut, bits, k, gsa, keysA, valsA, s));
+    }
+    int alt = 0;
+    SAGPU_TRY(sort_u64(c, keysA, valsA, keysB, valsB, n, 0, k * bits, &alt));
+    {
+        Span sp(c, F_INITIAL);
+        const u64 sepmask = gsa ? ((1ull << bits) - 1ull) : 0ull;
+        SAGPU_TRY(seg_init(ss, alt ? keysB : keysA, alt ? valsB : valsA, n, sepmask, dSA, dISA, act_i, act_g,
+                           act_p[0], s));
+    }
+    SAGPU_TRY(read_u32(c, ss.total, &a));
+
+    const int gbits = bits_for(n - 1);
+    u64 h = (u64)k;
+    int cur = 0;
+    st.rounds = 0;
+    while (a > 0) {
+        if (st.rounds < 64) st.active[st.rounds] = a;
+        st.rounds++;
+        {
+            Span sp(c, F_ROUNDS);
+            SAGPU_TRY(launch_round_keys(act_i, act_g, a, dISA, n, h, keysA, valsA, s));
+        }
+        SAGPU_TRY(sort_u64(c, keysA, valsA, keysB, valsB, a, 0, 32 + gbits, &alt));
+        {
+            Span sp(c, F_ROUNDS);
+            SAGPU_TRY(seg_round(ss, alt ? keysB : keysA, alt ? valsB : valsA, act_p[cur], a, dSA, dISA, act_i, act_g,
+                                act_p[cur ^ 1], s));
+        }
+        SAGPU_TRY(read_u32(c, ss.total, &a));
+        cur ^= 1;
+        h *= 2;
+        if (h > 4 * n + 64) {
+            sagpu_set_error("doubling did not converge", __FILE__, __LINE__);
+            return SAGPU_EDEVICE;
+        }
+    }
+    return SAGPU_OK;
+}
+
+int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
+             u32 *PLCP)
+{
+    LcpScratch ls{};
+    ls.queue_cap = n / 16 + 4096;
+    ls.tiles_cap = n / 4096 + 2;
+    GET(q, u32, ls.queue_cap);
+    GET(ql, u32, ls.queue_cap);
+    GET(qn, u32, 64);
+    GET(tm, u32, ls.tiles_cap);
+    ls.queue = q;
+    ls.queue_l = ql;
+    ls.queue_n = qn;
+    ls.tile_max = tm;
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, c.stream);
+}
+
+// Copies a text into a pool buffer with 64 zero bytes of padding.
+int padded_text(Ctx &c, Scope &sc, const uint8_t *src, u64 n, bool src_on_host, uint8_t **out)
+{
+    GET(dT, uint8_t, n + 64);
+    SAGPU_CHECK(hipMemsetAsync(dT + n, 0, 64, c.stream));
+    if (n) SAGPU_CHECK(hipMemcpyAsync(dT, src, n, src_on_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                                      c.stream));
+    *out = dT;
+    return SAGPU_OK;
+}
+
+void stats_reset(Ctx &c)
+{
+    c.stats = sagpu_stats{};
+    c.timer.on = g_timing;
+}
+
+// Fused device pipeline.  Returns 0 or the primary index when U is requested.
+int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *dSA, uint8_t *dU, u32 *dPLCP,
+                  u32 *dLCP, u64 *freq)
+{
+    if (!dSA) {
+        dSA = sc.get<u32>(n);
+        if (!dSA) return SAGPU_ENOMEM;
+    }
+    GET(dISA, u32, n);
+    int rc;
+    {
+        Span total(c, F_TOTAL);
+        rc = sa_build(c, sc, dTpad, n, gsa, dSA, dISA, freq);
+        if (rc != SAGPU_OK) return rc;
+        if (dU) {
+            Span sp(c, F_BWT);
+            SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
+        }
+        if (dLCP && !dPLCP) {
+            dPLCP = sc.get<u32>(n);
+            if (!dPLCP) return SAGPU_ENOMEM;
+        }
+        if (dPLCP) {
+            Span sp(c, F_PLCP);
+            SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP));
+        }
+        if (dLCP) {
+            Span sp(c, F_LCP);
+            SAGPU_TRY(launch_lcp(dPLCP, dSA, n, dLCP, c.stream));
+        }
+    }
+    int64_t ret = 0;
+    if (dU) {
+        u32 j0 = 0;
+        SAGPU_TRY(read_u32(c, dISA, &j0));
+        ret = (int64_t)j0 + 1;
+    }
+    SAGPU_CHECK(hipStreamSynchronize(c.stream));
+    timer_collect(c);
+    return ret;
+}
+
+__global__ void k_widen(const u32 *in, int64_t *out, u64 n)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (int64_t)in[i];
+}
+
+__global__ void k_narrow(const int64_t *in, u32 *out, u64 n)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (u32)in[i];
+}
+
+unsigned grid_n(u64 n)
+{
+    u64 b = (n + 2047) / 2048;
+    return (unsigned)(b < 1 ? 1 : b > 65535 ? 65535 : b);
+}
+
+// Download u32 device array into an int32 or int64 host array.
+int download_idx(Ctx &c, Scope &sc, const u32 *d, void *h, u64 n, int width)
+{
+    if (width == 4) {
+        SAGPU_CHECK(hipMemcpyAsync(h, d, n * 4, hipMemcpyDeviceToHost, c.stream));
+    } else {
+        GET(w, int64_t, n);
+        hipLaunchKernelGGL(k_widen, dim3(grid_n(n)), dim3(256), 0, c.stream, d, w, n);
+        SAGPU_CHECK(hipMemcpyAsync(h, w, n * 8, hipMemcpyDeviceToHost, c.stream));
+    }
+    return SAGPU_OK;
+}
+
+int upload_idx(Ctx &c, Scope &sc, const void *h, u64 n, int width, u32 **out)
+{
+    GET(d, u32, n);
+    if (width == 4) {
+        SAGPU_CHECK(hipMemcpyAsync(d, h, n * 4, hipMemcpyHostToDevice, c.stream));
+    } else {
+        GET(w, int64_t, n);
+        SAGPU_CHECK(hipMemcpyAsync(w, h, n * 8, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_narrow, dim3(grid_n(n)), dim3(256), 0, c.stream, w, d, n);
+    }
+    *out = d;
+    return SAGPU_OK;
+}
+
+void store_freq(const u64 *f, void *freq, int width)
+{
+    if (!freq) return;
+    for (int ch = 0; ch < 256; ch++) {
+        if (width == 4) ((int32_t *)freq)[ch] = (int32_t)f[ch];
+        else ((int64_t *)freq)[ch] = (int64_t)f[ch];
+    }
+}
+
+} // namespace
+
+// ======================================================== host-buffer layer
+// Used by the libsais-compatible wrappers (api.c).  Arguments are already
+// validated there; n >= 2.  width = 4 (int32 arrays) or 8 (int64 arrays).
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_t n, int32_t gsa, void *freq,
+                                              int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    int64_t rc;
+    {
+        Scope sc(*c);
+        uint8_t *dT;
+        SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+        u32 *dSA = sc.get<u32>((u64)n);
+        if (!dSA) return SAGPU_ENOMEM;
+        u64 f[256];
+        rc = build_dev(*c, sc, dT, (u64)n, gsa, dSA, nullptr, nullptr, nullptr, f);
+        if (rc < 0) return rc;
+        SAGPU_TRY(download_idx(*c, sc, dSA, SA, (u64)n, width));
+        SAGPU_CHECK(hipStreamSynchronize(c->stream));
+        store_freq(f, freq, width);
+    }
+    return 0;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    uint8_t *dU = sc.get<uint8_t>((u64)n);
+    if (!dU) return SAGPU_ENOMEM;
+    u64 f[256];
+    int64_t primary = build_dev(*c, sc, dT, (u64)n, 0, nullptr, dU, nullptr, nullptr, f);
+    if (primary < 0) return primary;
+    SAGPU_CHECK(hipMemcpyAsync(U, dU, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    store_freq(f, freq, width);
+    return primary;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,
+                                                int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    u32 *dSA;
+    SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
+    u32 *dPHI = sc.get<u32>((u64)n);
+    u32 *dP = sc.get<u32>((u64)n);
+    if (!dPHI || !dP) return SAGPU_ENOMEM;
+    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP));
+    SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA, void *LCP, int64_t n, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    Scope sc(*c);
+    u32 *dSA, *dP;
+    SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
+    SAGPU_TRY(upload_idx(*c, sc, PLCP, (u64)n, width, &dP));
+    u32 *dL = sc.get<u32>((u64)n);
+    if (!dL) return SAGPU_ENOMEM;
+    SAGPU_TRY(launch_lcp(dP, dSA, (u64)n, dL, c->stream));
+    SAGPU_TRY(download_idx(*c, sc, dL, LCP, (u64)n, width));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ======================================================== public device ABI
+
+extern "C" int32_t sagpu_init(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    return ctx_get(&c);
+}
+
+extern "C" void sagpu_shutdown(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx) return;
+    (void)hipStreamSynchronize(g_ctx->stream);
+    if (g_ctx->rs.hist) g_ctx->pool.put(g_ctx->rs.hist);
+    if (g_ctx->rs.tile_ctr) g_ctx->pool.put(g_ctx->rs.tile_ctr);
+    if (g_ctx->rs.lookback) g_ctx->pool.put(g_ctx->rs.lookback);
+    g_ctx->pool.trim();
+    for (auto &kv : g_ctx->pool.live) (void)hipFree(kv.first);
+    (void)hipHostFree(g_ctx->pinned);
+    (void)hipStreamDestroy(g_ctx->stream);
+    delete g_ctx;
+    g_ctx = nullptr;
+}
+
+extern "C" const char *sagpu_last_error(void) { return g_err.c_str(); }
+
+extern "C" void *sagpu_malloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void sagpu_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+extern "C" int32_t sagpu_memcpy_h2d(void *d, const void *h, size_t bytes)
+{
+    return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : SAGPU_EDEVICE;
+}
+
+extern "C" int32_t sagpu_memcpy_d2h(void *h, const void *d, size_t bytes)
+{
+    return hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : SAGPU_EDEVICE;
+}
+
+extern "C" int32_t sagpu_memset(void *d, int value, size_t bytes)
+{
+    return hipMemset(d, value, bytes) == hipSuccess ? 0 : SAGPU_EDEVICE;
+}
+
+extern "C" int32_t sagpu_synchronize(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? 0 : SAGPU_EDEVICE;
+}
+
+extern "C" void sagpu_set_memory_limit(uint64_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    if (ctx_get(&c) == SAGPU_OK) c->pool.limit = (size_t)bytes;
+}
+
+extern "C" void sagpu_trim(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_ctx) g_ctx->pool.trim();
+}
+
+extern "C" void sagpu_set_timing(int32_t enabled) { g_timing = enabled != 0; }
+
+extern "C" int32_t sagpu_get_stats(sagpu_stats *out)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx || !out) return SAGPU_EINVAL;
+    *out = g_ctx->stats;
+    return 0;
+}
+
+extern "C" int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA, uint8_t *d_U,
+                                 int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq)
+{
+    if (!d_T || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    u64 f[256];
+    int64_t rc = build_dev(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, (u32 *)d_SA, d_U, (u32 *)d_PLCP,
+                           (u32 *)d_LCP, f);
+    if (rc >= 0 && h_freq)
+        for (int ch = 0; ch < 256; ch++) h_freq[ch] = (int64_t)f[ch];
+    return rc;
+}
+
+extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags)
+{
+    if (!d_T || !d_SA || !d_PLCP || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    u32 *dPHI = sc.get<u32>((u64)n);
+    if (!dPHI) return SAGPU_ENOMEM;
+    SAGPU_TRY(launch_phi((const u32 *)d_SA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int32_t sagpu_lcp32(const int32_t *d_PLCP, const int32_t *d_SA, int32_t *d_LCP, int64_t n)
+{
+    if (!d_PLCP || !d_SA || !d_LCP || n < 1 || n > INT32_MAX) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    SAGPU_TRY(launch_lcp((const u32 *)d_PLCP, (const u32 *)d_SA, (u64)n, (u32 *)d_LCP, c->stream));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}

@@ -1,0 +1,316 @@
+// lcp_kernels.hip -- BWT gather, Phi, PLCP (irreducible positions + prefix max), LCP.
+//
+// PLCP[i] = lcp(T[i..], T[Phi[i]..]) where Phi[i] is the suffix that precedes i in
+// SA (Phi[SA[0]] = n, giving 0).  When T[i-1] == T[Phi[i]-1] the position is
+// "reducible": Phi[i-1] = Phi[i]-1 and PLCP[i] = PLCP[i-1]-1 (Kasai's bound holds
+// with equality).  So only irreducible positions are compared character by
+// character; with v[i] = PLCP[i] + i for irreducible i and 0 otherwise, v is
+// non-decreasing over irreducible positions and PLCP[i] = max_{j<=i} v[j] - i.
+// Position 0, positions whose predecessor is suffix 0 or the sentinel row, and (in
+// GSA mode) positions preceded by a separator are always irreducible.
+#include "engine.h"
+
+namespace {
+
+unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
+{
+    u64 b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// BWT with the libsais layout: U[0] = T[n-1]; row j (j != j0, SA[j0] == 0) goes to
+// U[j + (j < j0)].  j0 = ISA[0].
+__global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, const u32 *__restrict__ SA,
+                                             const u32 *__restrict__ ISA, u64 n, uint8_t *__restrict__ U)
+{
+    const u64 j0 = ISA[0];
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        u32 s = SA[j];
+        if (j == j0) U[0] = T[n - 1];
+        else U[j + (j < j0)] = (s - 1u < n) ? T[s - 1] : (uint8_t)0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_phi(const u32 *__restrict__ SA, u64 n, u32 *__restrict__ PHI)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    // Entries outside [0, n) (an invalid SA from the caller) are skipped rather than
+    // written out of bounds.
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        u32 p = SA[j];
+        if (p < n) PHI[p] = j == 0 ? (u32)n : SA[j - 1];
+    }
+}
+
+// 8 bytes starting at an arbitrary address; the text buffer carries >= 16 bytes of
+// zero padding so both aligned words are in bounds.
+__device__ __forceinline__ u64 load8u(const uint8_t *p)
+{
+    uintptr_t a = (uintptr_t)p;
+    const u64 *q = reinterpret_cast<const u64 *>(a & ~(uintptr_t)7);
+    u32 sh = (u32)(a & 7) * 8;
+    u64 lo = q[0];
+    if (sh == 0) return lo;
+    u64 hi = q[1];
+    return (lo >> sh) | (hi << (64 - sh));
+}
+
+// Compare up to `limit` more bytes from offset l.  Returns the new l; *done is set
+// when the comparison ended (mismatch, separator, or bound).
+__device__ __forceinline__ u64 match_bytes(const uint8_t *T, u64 i, u64 k, u64 l, u64 m, int gsa, u64 limit,
+                                           bool *done)
+{
+    const u64 stop_at = l + limit;
+    while (true) {
+        if (l >= m) { *done = true; return m; }
+        u64 x = load8u(T + i + l), y = load8u(T + k + l);
+        u64 stop = x ^ y;
+        if (gsa) stop |= (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+        if (stop) {
+            l += (u64)(__builtin_ctzll(stop) >> 3);
+            *done = true;
+            return l < m ? l : m;
+        }
+        l += 8;
+        if (l >= stop_at) { *done = (l >= m); return l < m ? l : m; }
+    }
+}
+
+constexpr u64 PL_THREAD_LIMIT = 64;
+
+// Per position: decide reducibility, compare irreducible ones up to 64 bytes, queue
+// the rest.  Writes v[i] (PLCP[i] + i, or 0 for reducible positions) into PLCP.
+__global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T, u64 n, int gsa,
+                                                  const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
+                                                  const u32 *__restrict__ SA, u32 *__restrict__ V,
+                                                  u32 *__restrict__ queue, u32 *__restrict__ queue_l,
+                                                  u32 *__restrict__ queue_n, u64 queue_cap)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 k;
+        if (PHI) k = PHI[i];
+        else {
+            u32 r = ISA[i];
+            k = r == 0 ? n : SA[r - 1];
+        }
+        if (k > n) k = n;
+        bool irr = (i == 0) || (k == 0) || (k >= n);
+        if (!irr) {
+            uint8_t a = T[i - 1], b = T[k - 1];
+            irr = (a != b) || (gsa && a == 0);
+        }
+        if (!irr) { V[i] = 0; continue; }
+        if (k >= n) { V[i] = (u32)i; continue; }
+        u64 m = n - (i > k ? i : k); // also bounds GSA matches (they stop earlier)
+        bool done = false;
+        u64 l = match_bytes(T, i, k, 0, m, gsa, PL_THREAD_LIMIT, &done);
+        if (done) {
+            V[i] = (u32)(l + i);
+        } else {
+            u32 slot = atomicAdd(queue_n, 1u);
+            if (slot < queue_cap) {
+                queue[slot] = (u32)i;
+                queue_l[slot] = (u32)l;
+                V[i] = (u32)(l + i); // provisional; overwritten by k_plcp_long
+            } else {
+                // queue full: finish here (correct, only slower)
+                l = match_bytes(T, i, k, l, m, gsa, ~0ull >> 1, &done);
+                V[i] = (u32)(l + i);
+            }
+        }
+    }
+}
+
+// One workgroup per queued position: 256 lanes x 8 bytes per step, first mismatch
+// found with a workgroup min-reduction.
+__global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T, u64 n, int gsa,
+                                                   const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
+                                                   const u32 *__restrict__ SA, u32 *__restrict__ V,
+                                                   const u32 *__restrict__ queue, const u32 *__restrict__ queue_l,
+                                                   const u32 *__restrict__ queue_n, u64 queue_cap)
+{
+    __shared__ u64 s_min[4];
+    __shared__ u64 s_res;
+    const u32 cnt = (u32)((u64)*queue_n < queue_cap ? (u64)*queue_n : queue_cap);
+    for (u32 q = blockIdx.x; q < cnt; q += gridDim.x) {
+        const u64 i = queue[q];
+        u64 k;
+        if (PHI) k = PHI[i];
+        else {
+            u32 r = ISA[i];
+            k = r == 0 ? n : SA[r - 1];
+        }
+        if (k > n) k = n;
+        const u64 m = n - (i > k ? i : k);
+        u64 l = queue_l[q];
+        for (;;) {
+            u64 off = l + (u64)threadIdx.x * 8;
+            u64 first = ~0ull;
+            if (off < m) {
+                u64 x = load8u(T + i + off), y = load8u(T + k + off);
+                u64 stop = x ^ y;
+                if (gsa) stop |= (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+                if (stop) {
+                    first = off + (u64)(__builtin_ctzll(stop) >> 3);
+                    if (first > m) first = m;
+                }
+            } else {
+                first = m;
+            }
+            // workgroup min
+            for (int d = 32; d >= 1; d >>= 1) {
+                u64 o = __shfl_xor(first, d, SAGPU_WAVE);
+                first = first < o ? first : o;
+            }
+            if (lane_id() == 0) s_min[threadIdx.x / SAGPU_WAVE] = first;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                u64 r = s_min[0];
+                for (int w = 1; w < 4; w++) r = r < s_min[w] ? r : s_min[w];
+                s_res = r;
+            }
+            __syncthreads();
+            u64 r = s_res;
+            __syncthreads();
+            if (r != ~0ull) {
+                if (threadIdx.x == 0) V[i] = (u32)(r + i);
+                break;
+            }
+            l += 256 * 8;
+        }
+    }
+}
+
+// Prefix max over V, then PLCP[i] = max - i (in place).
+constexpr int MS_THREADS = 256, MS_ITEMS = 16, MS_TILE = MS_THREADS * MS_ITEMS;
+
+__global__ __launch_bounds__(MS_THREADS) void k_max_reduce(const u32 *__restrict__ V, u64 n, u32 *__restrict__ tmax)
+{
+    __shared__ u32 s_w[4];
+    const u64 base = (u64)blockIdx.x * MS_TILE;
+    u32 m = 0;
+    for (int j = 0; j < MS_ITEMS; j++) {
+        u64 e = base + (u64)j * MS_THREADS + threadIdx.x;
+        if (e < n) { u32 v = V[e]; m = m > v ? m : v; }
+    }
+    for (int d = 32; d >= 1; d >>= 1) { u32 o = __shfl_xor(m, d, SAGPU_WAVE); m = m > o ? m : o; }
+    if (lane_id() == 0) s_w[threadIdx.x / SAGPU_WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 r = s_w[0];
+        for (int w = 1; w < 4; w++) r = r > s_w[w] ? r : s_w[w];
+        tmax[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_max_scan_tiles(u32 *tmax, u64 tiles)
+{
+    __shared__ u32 s_w[16];
+    __shared__ u32 s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (u64 base = 0; base < tiles; base += 1024) {
+        u64 i = base + threadIdx.x;
+        u32 v = i < tiles ? tmax[i] : 0u;
+        u32 inc = wave_incl_max(v);
+        if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
+        __syncthreads();
+        u32 c = s_carry;
+        for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = c > s_w[q] ? c : s_w[q];
+        u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
+        if (lane_id() > 0) c = c > prev ? c : prev;
+        if (i < tiles) tmax[i] = c;
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            u32 t = s_carry;
+            for (int q = 0; q < 16; q++) t = t > s_w[q] ? t : s_w[q];
+            s_carry = t;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(MS_THREADS) void k_max_apply(u32 *__restrict__ V, u64 n, const u32 *__restrict__ tmax)
+{
+    __shared__ u32 s_w[4];
+    // blocked layout: thread handles MS_ITEMS consecutive elements
+    const u64 base = (u64)blockIdx.x * MS_TILE + (u64)threadIdx.x * MS_ITEMS;
+    u32 v[MS_ITEMS];
+    u32 m = 0;
+#pragma unroll
+    for (int j = 0; j < MS_ITEMS; j++) {
+        u64 e = base + j;
+        v[j] = e < n ? V[e] : 0u;
+        m = m > v[j] ? m : v[j];
+    }
+    u32 inc = wave_incl_max(m);
+    if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
+    __syncthreads();
+    u32 c = tmax[blockIdx.x];
+    for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = c > s_w[q] ? c : s_w[q];
+    u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
+    if (lane_id() > 0) c = c > prev ? c : prev;
+#pragma unroll
+    for (int j = 0; j < MS_ITEMS; j++) {
+        u64 e = base + j;
+        c = c > v[j] ? c : v[j];
+        if (e < n) V[e] = (u32)(c - e);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lcp(const u32 *__restrict__ PLCP, const u32 *__restrict__ SA, u64 n,
+                                             u32 *__restrict__ LCP)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        u32 p = SA[j];
+        LCP[j] = p < n ? PLCP[p] : 0u;
+    }
+}
+
+} // namespace
+
+int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bwt, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_phi, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, SA, n, PHI);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
+                u32 *PLCP, hipStream_t s)
+{
+    const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
+    if (tiles > ls.tiles_cap) {
+        sagpu_set_error("plcp scratch too small", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
+    hipLaunchKernelGGL(k_plcp_irr, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP,
+                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap);
+    hipLaunchKernelGGL(k_plcp_long, dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP, ls.queue, ls.queue_l,
+                       ls.queue_n, ls.queue_cap);
+    hipLaunchKernelGGL(k_max_reduce, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, PLCP, n, ls.tile_max);
+    hipLaunchKernelGGL(k_max_scan_tiles, dim3(1), dim3(1024), 0, s, ls.tile_max, tiles);
+    hipLaunchKernelGGL(k_max_apply, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, PLCP, n, ls.tile_max);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lcp, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, PLCP, SA, n, LCP);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}

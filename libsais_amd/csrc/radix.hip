@@ -1,0 +1,275 @@
+// radix.hip -- onesweep LSD radix sort for gfx950 (8-bit digits, wave64).
+//
+// One upfront kernel builds the 256-bin histograms of every digit position at once
+// (LDS-privatised, one read of the keys).  Each pass is then a single kernel:
+//   * a workgroup takes a tile ticket (atomic counter), so every tile it waits on
+//     belongs to a workgroup that is already running;
+//   * keys are ranked inside the wave with 8 ballots per item (peer mask of equal
+//     digits, popcount below the lane) and per-wave LDS digit counters -- stable by
+//     construction, no atomics on the ranking path;
+//   * the per-digit tile count is published to a look-back word, and the tile's
+//     global digit offsets come from decoupled look-back over preceding tiles;
+//   * keys (then values) are reordered in LDS by digit and written out as runs, so
+//     consecutive lanes store to consecutive addresses.
+#include "engine.h"
+
+#include <string.h>
+
+namespace {
+
+constexpr int RS_WAVES = RS_THREADS / SAGPU_WAVE;
+constexpr int RS_WAVE_ITEMS = SAGPU_WAVE * RS_ITEMS;
+constexpr int LB_VAL_BITS = 40;
+constexpr u64 LB_VAL_MASK = (1ull << LB_VAL_BITS) - 1ull;
+constexpr u64 LB_AGG = 1ull, LB_PRE = 2ull;
+
+__device__ __forceinline__ u64 lb_pack(u32 epoch, u64 flag, u64 v)
+{
+    return ((u64)epoch << 42) | (flag << LB_VAL_BITS) | v;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, u64 n, int begin_bit,
+                                                    int npass, u64 *__restrict__ hist)
+{
+    __shared__ u32 s_h[8][256];
+    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        K k = keys[i];
+        for (int p = 0; p < npass; p++) {
+            u32 d = (u32)(k >> (begin_bit + 8 * p)) & 255u;
+            atomicAdd(&s_h[p][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += blockDim.x) {
+        u32 v = (&s_h[0][0])[i];
+        if (v) atomicAdd(&hist[i], (u64)v);
+    }
+}
+
+// Exclusive scan of each pass's 256-bin histogram into bucket offsets (in place).
+__global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
+{
+    __shared__ u64 s_w[RS_WAVES];
+    for (int p = 0; p < npass; p++) {
+        u64 v = hist[p * 256 + threadIdx.x];
+        u64 inc = wave_incl_sum64(v);
+        if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
+        __syncthreads();
+        u64 carry = 0;
+        for (u32 w = 0; w < threadIdx.x / SAGPU_WAVE; w++) carry += s_w[w];
+        hist[8 * 256 + p * 256 + threadIdx.x] = carry + inc - v;
+        __syncthreads();
+    }
+}
+
+template <typename K, bool HAS_VALS>
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ kin, const u32 *__restrict__ vin,
+                                                         K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
+                                                         int shift, const u64 *__restrict__ offsets,
+                                                         u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
+                                                         u32 epoch)
+{
+    __shared__ u32 s_tile;
+    __shared__ u32 s_wcnt[RS_WAVES][256];
+    __shared__ u32 s_dstart[256];
+    __shared__ u64 s_gbase[256];
+    __shared__ u32 s_wsum[RS_WAVES];
+    __shared__ K s_keys[RS_TILE];
+    __shared__ uint8_t s_dig[RS_TILE];
+
+    const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; q++) s_wcnt[q][tid] = 0;
+    __syncthreads();
+
+    const u64 tile = s_tile;
+    const u64 wbase = tile * RS_TILE + (u64)w * RS_WAVE_ITEMS;
+
+    K key[RS_ITEMS];
+    u32 val[RS_ITEMS];
+    u32 rank[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+        bool ok = pos < n;
+        key[j] = ok ? kin[pos] : (K)0;
+        if (HAS_VALS) val[j] = ok ? vin[pos] : 0u;
+    }
+
+    // Wave-level stable ranking: order inside the wave is (item j, lane).
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+        bool ok = pos < n;
+        u32 d = (u32)(key[j] >> shift) & 255u;
+        u64 m = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            u64 bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        u32 below = __popcll(m & lanemask_lt());
+        u32 old = 0;
+        if (ok) old = s_wcnt[w][d];
+        if (ok && below == 0) s_wcnt[w][d] = old + (u32)__popcll(m);
+        rank[j] = old + below;
+    }
+    __syncthreads();
+
+    // Per digit: wave offsets, tile count, publish, look back.
+    {
+        const u32 d = tid;
+        u32 run = 0;
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; q++) {
+            u32 c = s_wcnt[q][d];
+            s_wcnt[q][d] = run;
+            run += c;
+        }
+        u64 *my = lookback + tile * 256 + d;
+        if (tile == 0) st_release_u64(my, lb_pack(epoch, LB_PRE, run));
+        else st_release_u64(my, lb_pack(epoch, LB_AGG, run));
+
+        // tile-local exclusive digit starts
+        u32 inc = wave_incl_sum(run);
+        if (lane == SAGPU_WAVE - 1) s_wsum[w] = inc;
+
+        u64 excl = 0;
+        if (tile > 0) {
+            u64 t = tile - 1;
+            for (;;) {
+                const u64 *src = lookback + t * 256 + d;
+                u64 v = ld_acquire_u64(src);
+                while ((u32)(v >> 42) != epoch || ((v >> LB_VAL_BITS) & 3ull) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    v = ld_acquire_u64(src);
+                }
+                excl += v & LB_VAL_MASK;
+                if (((v >> LB_VAL_BITS) & 3ull) == LB_PRE) break;
+                --t;
+            }
+            st_release_u64(my, lb_pack(epoch, LB_PRE, excl + run));
+        }
+        s_gbase[d] = offsets[d] + excl;
+        __syncthreads();
+        u32 carry = 0;
+        for (u32 q = 0; q < w; q++) carry += s_wsum[q];
+        s_dstart[d] = carry + inc - run;
+    }
+    __syncthreads();
+
+    // Reorder keys in LDS by digit.
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+        if (pos < n) {
+            u32 d = (u32)(key[j] >> shift) & 255u;
+            u32 lp = s_dstart[d] + s_wcnt[w][d] + rank[j];
+            s_keys[lp] = key[j];
+            s_dig[lp] = (uint8_t)d;
+            rank[j] = lp;
+        }
+    }
+    __syncthreads();
+    const u64 tbase = tile * RS_TILE;
+    const u32 tile_n = (u32)((n - tbase) < (u64)RS_TILE ? (n - tbase) : (u64)RS_TILE);
+    for (u32 idx = tid; idx < tile_n; idx += RS_THREADS) {
+        u32 d = s_dig[idx];
+        kout[s_gbase[d] + (idx - s_dstart[d])] = s_keys[idx];
+    }
+    if (HAS_VALS) {
+        __syncthreads();
+        u32 *s_vals = reinterpret_cast<u32 *>(s_keys);
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++) {
+            u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+            if (pos < n) s_vals[rank[j]] = val[j];
+        }
+        __syncthreads();
+        for (u32 idx = tid; idx < tile_n; idx += RS_THREADS) {
+            u32 d = s_dig[idx];
+            vout[s_gbase[d] + (idx - s_dstart[d])] = s_vals[idx];
+        }
+    }
+}
+
+template <typename K>
+int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
+                    hipStream_t s, int *result_in_alt, int *passes_run)
+{
+    *result_in_alt = 0;
+    if (passes_run) *passes_run = 0;
+    if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
+    const int npass = (end_bit - begin_bit + 7) / 8;
+    const u64 tiles = (n + RS_TILE - 1) / RS_TILE;
+    if (tiles > rs.lb_tiles || npass > 8) {
+        sagpu_set_error("radix scratch too small", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+
+    SAGPU_TRY(launch_fill_u64(rs.hist, 16 * 256, 0, s));
+    {
+        int blocks = (int)((n + 256 * 64 - 1) / (256 * 64));
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3(blocks), dim3(256), 0, s, k0, n, begin_bit, npass, rs.hist);
+        hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass);
+        SAGPU_CHECK(hipGetLastError());
+    }
+    // Histograms decide which passes can be skipped (all keys share the digit).
+    SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, 8 * 256 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+
+    SAGPU_TRY(launch_fill_u32(rs.tile_ctr, 8, 0, s));
+    K *ki = k0, *ko = k1;
+    u32 *vi = v0, *vo = v1;
+    int alt = 0, ran = 0;
+    for (int p = 0; p < npass; p++) {
+        bool trivial = false;
+        for (int d = 0; d < 256; d++)
+            if (rs.hist_host[p * 256 + d] == n) trivial = true;
+        if (trivial) continue;
+        rs.epoch = (rs.epoch + 1) & ((1u << 22) - 1);
+        if (rs.epoch == 0) {
+            SAGPU_TRY(launch_fill_u64(rs.lookback, rs.lb_tiles * 256, 0, s));
+            rs.epoch = 1;
+        }
+        if (rs.hook) rs.hook(rs.hook_user, 0);
+        if (v0)
+            hipLaunchKernelGGL((k_onesweep<K, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n,
+                               begin_bit + 8 * p, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
+                               rs.epoch);
+        else
+            hipLaunchKernelGGL((k_onesweep<K, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo,
+                               n, begin_bit + 8 * p, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
+                               rs.epoch);
+        SAGPU_CHECK(hipGetLastError());
+        if (rs.hook) rs.hook(rs.hook_user, 1);
+        K *tk = ki; ki = ko; ko = tk;
+        u32 *tv = vi; vi = vo; vo = tv;
+        alt ^= 1;
+        ran++;
+    }
+    *result_in_alt = alt;
+    if (passes_run) *passes_run = ran;
+    return SAGPU_OK;
+}
+
+} // namespace
+
+int radix_sort_u64(RadixScratch &rs, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
+                   hipStream_t s, int *result_in_alt, int *passes_run)
+{
+    return radix_sort_impl<u64>(rs, k0, v0, k1, v1, n, begin_bit, end_bit, s, result_in_alt, passes_run);
+}
+
+int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
+                   hipStream_t s, int *result_in_alt, int *passes_run)
+{
+    return radix_sort_impl<u32>(rs, k0, v0, k1, v1, n, begin_bit, end_bit, s, result_in_alt, passes_run);
+}

@@ -1,0 +1,293 @@
+/*
+ * oracle.c -- CPU restatement of the libsais 2.10.4 hot path (SA, GSA, BWT, PLCP,
+ * LCP).  TEST INFRASTRUCTURE ONLY: the product library never links or calls it.
+ *
+ * The suffix array is built by a compact textbook SA-IS (classify S/L, place LMS,
+ * induce L then S, name LMS substrings, recurse, place sorted LMS, induce again),
+ * the algorithm family of libsais.c:6879-6923 (libsais_main_8u) without any of the
+ * reference's block/cache/OpenMP machinery.  GSA separators are mapped to distinct
+ * integer symbols ordered by position, which is the order libsais_gsa produces
+ * (libsais.c:4038-4055 marker flip and first-LMS reinsertion, 6206 terminal
+ * separator, 5484 no induction from separator-preceded positions).
+ *
+ * BWT, PLCP and LCP restate libsais.c:7110-7118, 8116-8238 and 8311-8338 line by
+ * line in meaning.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t I;
+
+void oracle_freq(const uint8_t *T, int64_t n, int64_t *freq)
+{
+    memset(freq, 0, 256 * sizeof(int64_t));
+    for (I i = 0; i < n; i++) freq[T[i]]++;
+}
+
+/* libsais.c:7036 (T[n-1] must be 0) and 6886-6889: after S/L typing, the zero
+ * symbol may only occur as an S-type 0 preceded by an L-type letter, plus exactly
+ * one L-type 0 (the terminal one).  For byte strings that is: T[0] != 0 and no two
+ * adjacent zeros. */
+int oracle_gsa_valid(const uint8_t *T, int64_t n)
+{
+    if (n > 0 && T[n - 1] != 0) return 0;
+    if (n <= 1) return 1;
+    if (T[0] == 0) return 0;
+    for (I i = 0; i + 1 < n; i++)
+        if (T[i] == 0 && T[i + 1] == 0) return 0;
+    return 1;
+}
+
+/* ---------------------------------------------------------------- SA-IS ---- */
+
+static void buckets(const I *s, I n, I K, I *bkt, int ends)
+{
+    memset(bkt, 0, (size_t)K * sizeof(I));
+    for (I i = 0; i < n; i++) bkt[s[i]]++;
+    I sum = 0;
+    for (I c = 0; c < K; c++) {
+        sum += bkt[c];
+        bkt[c] = ends ? sum : sum - bkt[c];
+    }
+}
+
+/* t[i] = 1 for S-type.  A virtual sentinel, smaller than every symbol, follows
+ * s[n-1]; it is never stored. */
+static int is_lms(const uint8_t *t, I i) { return i > 0 && t[i] && !t[i - 1]; }
+
+static void induce(const I *s, I *SA, I n, I K, const uint8_t *t, I *bkt)
+{
+    buckets(s, n, K, bkt, 0);
+    SA[bkt[s[n - 1]]++] = n - 1; /* induced by the virtual sentinel */
+    for (I i = 0; i < n; i++) {
+        I p = SA[i];
+        if (p > 0 && !t[p - 1]) SA[bkt[s[p - 1]]++] = p - 1;
+    }
+    buckets(s, n, K, bkt, 1);
+    for (I i = n - 1; i >= 0; i--) {
+        I p = SA[i];
+        if (p > 0 && t[p - 1]) SA[--bkt[s[p - 1]]] = p - 1;
+    }
+}
+
+static int sais(const I *s, I *SA, I n, I K)
+{
+    if (n == 1) { SA[0] = 0; return 0; }
+    uint8_t *t = (uint8_t *)malloc((size_t)n);
+    I *bkt = (I *)malloc((size_t)K * sizeof(I));
+    if (!t || !bkt) { free(t); free(bkt); return -2; }
+
+    t[n - 1] = 0;
+    for (I i = n - 2; i >= 0; i--) t[i] = (s[i] < s[i + 1]) || (s[i] == s[i + 1] && t[i + 1]);
+
+    /* 1. LMS suffixes at bucket ends, then induce: sorts LMS substrings. */
+    for (I i = 0; i < n; i++) SA[i] = -1;
+    buckets(s, n, K, bkt, 1);
+    for (I i = n - 1; i >= 1; i--)
+        if (is_lms(t, i)) SA[--bkt[s[i]]] = i;
+    induce(s, SA, n, K, t, bkt);
+
+    /* 2. Compact sorted LMS positions into SA[0..m) and name the substrings. */
+    I m = 0;
+    for (I i = 0; i < n; i++)
+        if (is_lms(t, SA[i])) SA[m++] = SA[i];
+    for (I i = m; i < n; i++) SA[i] = -1;
+    I names = 0, prev = -1;
+    for (I i = 0; i < m; i++) {
+        I pos = SA[i];
+        int diff = (prev < 0);
+        for (I d = 0; !diff; d++) {
+            if (pos + d == n || prev + d == n) { diff = 1; break; }
+            if (s[pos + d] != s[prev + d] || t[pos + d] != t[prev + d]) { diff = 1; break; }
+            if (d > 0 && (is_lms(t, pos + d) || is_lms(t, prev + d))) {
+                diff = !(is_lms(t, pos + d) && is_lms(t, prev + d));
+                break;
+            }
+        }
+        if (diff) { names++; prev = pos; }
+        SA[m + pos / 2] = names - 1;
+    }
+    I j = n - 1;
+    for (I i = n - 1; i >= m; i--)
+        if (SA[i] >= 0) SA[j--] = SA[i];
+
+    /* 3. Sort the reduced string (recursively when names repeat). */
+    I *s1 = SA + n - m, *SA1 = SA;
+    if (names < m) {
+        int rc = sais(s1, SA1, m, names);
+        if (rc) { free(t); free(bkt); return rc; }
+    } else {
+        for (I i = 0; i < m; i++) SA1[s1[i]] = i;
+    }
+
+    /* 4. Map back to text positions, place at bucket ends, final induce. */
+    I k = 0;
+    for (I i = 1; i < n; i++)
+        if (is_lms(t, i)) s1[k++] = i;
+    for (I i = 0; i < m; i++) SA1[i] = s1[SA1[i]];
+    for (I i = m; i < n; i++) SA[i] = -1;
+    buckets(s, n, K, bkt, 1);
+    for (I i = m - 1; i >= 0; i--) {
+        I p = SA[i];
+        SA[i] = -1;
+        SA[--bkt[s[p]]] = p;
+    }
+    induce(s, SA, n, K, t, bkt);
+
+    free(t);
+    free(bkt);
+    return 0;
+}
+
+int oracle_sa(const uint8_t *T, int64_t *SA, int64_t n, int gsa)
+{
+    if (n <= 0) return 0;
+    if (gsa && !oracle_gsa_valid(T, n)) return -1;
+    if (n == 1) { SA[0] = 0; return 0; }
+    I *s = (I *)malloc((size_t)n * sizeof(I));
+    if (!s) return -2;
+    I K = 256;
+    if (gsa) {
+        I z = 0;
+        for (I i = 0; i < n; i++)
+            if (T[i] == 0) s[i] = z++;
+        for (I i = 0; i < n; i++)
+            if (T[i] != 0) s[i] = z + T[i];
+        K = z + 256;
+    } else {
+        for (I i = 0; i < n; i++) s[i] = T[i];
+    }
+    int rc = sais(s, SA, n, K);
+    free(s);
+    return rc;
+}
+
+/* ------------------------------------------------------- SA verification ---- */
+
+int oracle_check_sa(const uint8_t *T, const int64_t *SA, int64_t n, int gsa)
+{
+    if (n <= 0) return 1;
+    I *isa = (I *)malloc((size_t)n * sizeof(I));
+    if (!isa) return 0;
+    for (I i = 0; i < n; i++) isa[i] = -1;
+    int ok = 1;
+    for (I j = 0; j < n && ok; j++) {
+        I p = SA[j];
+        if (p < 0 || p >= n || isa[p] >= 0) ok = 0;
+        else isa[p] = j;
+    }
+    for (I j = 1; j < n && ok; j++) {
+        I a = SA[j - 1], b = SA[j];
+        if (T[a] < T[b]) continue;
+        if (T[a] > T[b]) { ok = 0; break; }
+        if (gsa && T[a] == 0) { ok = a < b; continue; }
+        if (a + 1 == n) continue;           /* a is a proper prefix of b */
+        if (b + 1 == n) { ok = 0; break; }  /* b is a proper prefix of a */
+        ok = isa[a + 1] < isa[b + 1];
+    }
+    free(isa);
+    return ok;
+}
+
+/* ---------------------------------------------------------- BWT / LCP ---- */
+
+int64_t oracle_bwt(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { U[0] = T[0]; return 1; }
+    I j0 = -1;
+    for (I j = 0; j < n; j++)
+        if (SA[j] == 0) { j0 = j; break; }
+    /* libsais.c:7113-7117: U[0] = T[n-1]; rows before the primary shift right by
+     * one, the primary row itself is dropped. */
+    U[0] = T[n - 1];
+    for (I j = 0; j < n; j++) {
+        if (j == j0) continue;
+        U[j + (j < j0)] = T[SA[j] - 1];
+    }
+    return j0 + 1;
+}
+
+void oracle_plcp(const uint8_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int gsa)
+{
+    if (n <= 0) return;
+    if (n == 1) { PLCP[0] = 0; return; }
+    I k = n;
+    for (I i = 0; i < n; i++) { PLCP[SA[i]] = k; k = SA[i]; } /* phi */
+    I l = 0;
+    for (I i = 0; i < n; i++) {
+        k = PLCP[i];
+        if (!gsa) {
+            I m = n - (i > k ? i : k);
+            while (l < m && T[i + l] == T[k + l]) l++;
+        } else {
+            /* The separator stops the match; i + l and k + l stay below n for a
+             * valid GSA input because T[n-1] == 0. */
+            while (i + l < n && k + l < n && T[i + l] > 0 && T[i + l] == T[k + l]) l++;
+        }
+        PLCP[i] = l;
+        l -= (l != 0);
+    }
+}
+
+void oracle_lcp(const int64_t *PLCP, const int64_t *SA, int64_t *LCP, int64_t n)
+{
+    for (I i = 0; i < n; i++) LCP[i] = PLCP[SA[i]];
+}
+
+/* ------------------------------------------------------ 32-bit wrappers ---- */
+
+static I *widen(const int32_t *a, I n)
+{
+    I *w = (I *)calloc((size_t)(n > 0 ? n : 1), sizeof(I));
+    if (!w) return NULL;
+    for (I i = 0; i < n; i++) w[i] = a[i];
+    return w;
+}
+
+int oracle_sa32(const uint8_t *T, int32_t *SA, int64_t n, int gsa)
+{
+    I *w = (I *)malloc((size_t)(n > 0 ? n : 1) * sizeof(I));
+    if (!w) return -2;
+    int rc = oracle_sa(T, w, n, gsa);
+    if (rc == 0) for (I i = 0; i < n; i++) SA[i] = (int32_t)w[i];
+    free(w);
+    return rc;
+}
+
+int oracle_check_sa32(const uint8_t *T, const int32_t *SA, int64_t n, int gsa)
+{
+    I *w = widen(SA, n);
+    if (!w) return 0;
+    int ok = oracle_check_sa(T, w, n, gsa);
+    free(w);
+    return ok;
+}
+
+int64_t oracle_bwt32(const uint8_t *T, const int32_t *SA, uint8_t *U, int64_t n)
+{
+    I *w = widen(SA, n);
+    if (!w) return -2;
+    I r = oracle_bwt(T, w, U, n);
+    free(w);
+    return r;
+}
+
+void oracle_plcp32(const uint8_t *T, const int32_t *SA, int32_t *PLCP, int64_t n, int gsa)
+{
+    I *w = widen(SA, n);
+    I *p = (I *)calloc((size_t)(n > 0 ? n : 1), sizeof(I));
+    if (w && p) {
+        oracle_plcp(T, w, p, n, gsa);
+        for (I i = 0; i < n; i++) PLCP[i] = (int32_t)p[i];
+    }
+    free(w);
+    free(p);
+}
+
+void oracle_lcp32(const int32_t *PLCP, const int32_t *SA, int32_t *LCP, int64_t n)
+{
+    for (I i = 0; i < n; i++) LCP[i] = PLCP[SA[i]];
+}

@@ -1,0 +1,178 @@
+"""ctypes access to the checkers under oracle/ (test infrastructure only).
+
+liboracle.so is the C restatement (oracle/oracle.c); libsais_ref.so is the
+reference libsais 2.10.4 compiled from its own sources by oracle/Makefile.  The
+product library never loads either.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libsais_ref.so")
+
+_o = None
+_r = None
+_vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+
+
+def oracle():
+    global _o
+    if _o is None:
+        o = ctypes.CDLL(ORACLE_PATH, mode=ctypes.RTLD_LOCAL)
+        o.oracle_sa32.argtypes = [_vp, _vp, _i64, ctypes.c_int]
+        o.oracle_sa32.restype = ctypes.c_int
+        o.oracle_sa.argtypes = [_vp, _vp, _i64, ctypes.c_int]
+        o.oracle_sa.restype = ctypes.c_int
+        o.oracle_check_sa32.argtypes = [_vp, _vp, _i64, ctypes.c_int]
+        o.oracle_check_sa32.restype = ctypes.c_int
+        o.oracle_check_sa.argtypes = [_vp, _vp, _i64, ctypes.c_int]
+        o.oracle_check_sa.restype = ctypes.c_int
+        o.oracle_bwt32.argtypes = [_vp, _vp, _vp, _i64]
+        o.oracle_bwt32.restype = _i64
+        o.oracle_plcp32.argtypes = [_vp, _vp, _vp, _i64, ctypes.c_int]
+        o.oracle_plcp32.restype = None
+        o.oracle_lcp32.argtypes = [_vp, _vp, _vp, _i64]
+        o.oracle_lcp32.restype = None
+        o.oracle_freq.argtypes = [_vp, _i64, _vp]
+        o.oracle_freq.restype = None
+        o.oracle_gsa_valid.argtypes = [_vp, _i64]
+        o.oracle_gsa_valid.restype = ctypes.c_int
+        _o = o
+    return _o
+
+
+def have_ref():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _r
+    if _r is None:
+        r = ctypes.CDLL(REF_PATH, mode=ctypes.RTLD_LOCAL)
+        for name in ("libsais", "libsais_gsa"):
+            getattr(r, name).argtypes = [_vp, _vp, _i32, _i32, _vp]
+            getattr(r, name).restype = _i32
+        r.libsais_bwt.argtypes = [_vp, _vp, _vp, _i32, _i32, _vp]
+        r.libsais_bwt.restype = _i32
+        for name in ("libsais_plcp", "libsais_plcp_gsa", "libsais_lcp"):
+            getattr(r, name).argtypes = [_vp, _vp, _vp, _i32]
+            getattr(r, name).restype = _i32
+        for name in ("libsais_omp", "libsais_gsa_omp"):
+            getattr(r, name).argtypes = [_vp, _vp, _i32, _i32, _vp, _i32]
+            getattr(r, name).restype = _i32
+        r.libsais_bwt_omp.argtypes = [_vp, _vp, _vp, _i32, _i32, _vp, _i32]
+        r.libsais_bwt_omp.restype = _i32
+        for name in ("libsais_plcp_omp", "libsais_plcp_gsa_omp", "libsais_lcp_omp"):
+            getattr(r, name).argtypes = [_vp, _vp, _vp, _i32, _i32]
+            getattr(r, name).restype = _i32
+        for name in ("libsais64", "libsais64_gsa"):
+            getattr(r, name).argtypes = [_vp, _vp, _i64, _i64, _vp]
+            getattr(r, name).restype = _i64
+        r.libsais64_bwt.argtypes = [_vp, _vp, _vp, _i64, _i64, _vp]
+        r.libsais64_bwt.restype = _i64
+        for name in ("libsais64_plcp", "libsais64_plcp_gsa", "libsais64_lcp"):
+            getattr(r, name).argtypes = [_vp, _vp, _vp, _i64]
+            getattr(r, name).restype = _i64
+        _r = r
+    return _r
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------- oracle
+def sa(T, gsa=False):
+    n = len(T)
+    SA = np.zeros(max(n, 1), np.int32)
+    rc = oracle().oracle_sa32(_p(T), _p(SA), n, int(gsa))
+    return rc, SA[:n]
+
+
+def check_sa(T, SA, gsa=False):
+    SA = np.ascontiguousarray(SA, dtype=np.int32)
+    return bool(oracle().oracle_check_sa32(_p(T), _p(SA), len(T), int(gsa)))
+
+
+def bwt(T, SA):
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint8)
+    SA = np.ascontiguousarray(SA, dtype=np.int32)
+    idx = oracle().oracle_bwt32(_p(T), _p(SA), _p(U), n)
+    return U[:n], idx
+
+
+def plcp(T, SA, gsa=False):
+    n = len(T)
+    P = np.zeros(max(n, 1), np.int32)
+    SA = np.ascontiguousarray(SA, dtype=np.int32)
+    oracle().oracle_plcp32(_p(T), _p(SA), _p(P), n, int(gsa))
+    return P[:n]
+
+
+def lcp(P, SA):
+    n = len(SA)
+    L = np.zeros(max(n, 1), np.int32)
+    oracle().oracle_lcp32(_p(np.ascontiguousarray(P, np.int32)), _p(np.ascontiguousarray(SA, np.int32)), _p(L), n)
+    return L[:n]
+
+
+def freq(T):
+    f = np.zeros(256, np.int64)
+    oracle().oracle_freq(_p(T), len(T), _p(f))
+    return f
+
+
+def gsa_valid(T):
+    return bool(oracle().oracle_gsa_valid(_p(T), len(T)))
+
+
+# ---------------------------------------------------------------- reference
+def ref_sa(T, gsa=False, threads=0):
+    n = len(T)
+    SA = np.zeros(max(n, 1), np.int32)
+    f = np.zeros(256, np.int32)
+    r = ref()
+    if threads:
+        rc = (r.libsais_gsa_omp if gsa else r.libsais_omp)(_p(T), _p(SA), n, 0, _p(f), threads)
+    else:
+        rc = (r.libsais_gsa if gsa else r.libsais)(_p(T), _p(SA), n, 0, _p(f))
+    return rc, SA[:n], f
+
+
+def ref_bwt(T, threads=0):
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint8)
+    A = np.zeros(max(n, 1), np.int32)
+    r = ref()
+    if threads:
+        idx = r.libsais_bwt_omp(_p(T), _p(U), _p(A), n, 0, None, threads)
+    else:
+        idx = r.libsais_bwt(_p(T), _p(U), _p(A), n, 0, None)
+    return U[:n], idx
+
+
+def ref_plcp(T, SA, gsa=False, threads=0):
+    n = len(T)
+    P = np.zeros(max(n, 1), np.int32)
+    SA = np.ascontiguousarray(SA, dtype=np.int32)
+    r = ref()
+    if threads:
+        (r.libsais_plcp_gsa_omp if gsa else r.libsais_plcp_omp)(_p(T), _p(SA), _p(P), n, threads)
+    else:
+        (r.libsais_plcp_gsa if gsa else r.libsais_plcp)(_p(T), _p(SA), _p(P), n)
+    return P[:n]
+
+
+def ref_lcp(P, SA, threads=0):
+    n = len(SA)
+    L = np.zeros(max(n, 1), np.int32)
+    r = ref()
+    if threads:
+        r.libsais_lcp_omp(_p(P), _p(SA), _p(L), n, threads)
+    else:
+        r.libsais_lcp(_p(P), _p(SA), _p(L), n)
+    return L[:n]

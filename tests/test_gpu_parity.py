@@ -1,0 +1,153 @@
+"""GPU parity: every libsais entry point through the C ABI against the oracle.
+
+Inputs are seeded and small enough for the oracle to finish in seconds.  SA, BWT,
+primary index, PLCP, LCP and freq must be bit-identical.
+"""
+import numpy as np
+import pytest
+
+import libsais_amd as la
+from libsais_amd import workloads
+from tests import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    rng = np.random.default_rng(1234)
+    cases = []
+    for n in (2, 3, 5, 17, 64, 1000, 4095, 4096, 4097, 65536, 200003):
+        cases.append((f"acgt{n}", workloads.iid(n, seed=n, alphabet=b"ACGT")))
+    for k in (1, 2, 3, 5, 20, 255, 256):
+        n = 50000
+        cases.append((f"bytes_k{k}", rng.integers(0, k, n).astype(np.uint8)))
+    cases.append(("bytes_with_zeros", rng.integers(0, 3, 30000).astype(np.uint8)))
+    cases.append(("all_a_100", np.full(100, 65, np.uint8)))
+    cases.append(("all_a_70000", np.full(70000, 65, np.uint8)))
+    cases.append(("all_zero_5000", np.zeros(5000, np.uint8)))
+    cases.append(("periodic17", workloads.periodic(100000, 17, 3)))
+    cases.append(("fibonacci", workloads.fibonacci(100000)))
+    cases.append(("dna_like_1m", workloads.dna_like(1 << 20, seed=2)))
+    cases.append(("banana", np.frombuffer(b"banana", np.uint8).copy()))
+    cases.append(("mississippi", np.frombuffer(b"mississippi", np.uint8).copy()))
+    return cases
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name,T", CASES, ids=[c[0] for c in CASES])
+def test_sa_bwt_plcp_lcp(name, T):
+    print(f"case {name} n={len(T)}", flush=True)
+    n = len(T)
+    rc, SA_ref = ol.sa(T)
+    assert rc == 0
+    SA = np.zeros(n, np.int32)
+    freq = np.zeros(256, np.int32)
+    assert la.libsais(T, SA, n, 0, freq) == 0, la.last_error()
+    assert np.array_equal(SA, SA_ref)
+    assert np.array_equal(freq.astype(np.int64), ol.freq(T))
+
+    U_ref, idx_ref = ol.bwt(T, SA_ref)
+    U = np.zeros(n, np.uint8)
+    A = np.zeros(n, np.int32)
+    idx = la.libsais_bwt(T, U, A, n, 0, None)
+    assert idx == idx_ref
+    assert np.array_equal(U, U_ref)
+
+    P_ref = ol.plcp(T, SA_ref)
+    P = np.zeros(n, np.int32)
+    assert la.libsais_plcp(T, SA, P, n) == 0
+    assert np.array_equal(P, P_ref)
+
+    L = np.zeros(n, np.int32)
+    assert la.libsais_lcp(P, SA, L, n) == 0
+    assert np.array_equal(L, ol.lcp(P_ref, SA_ref))
+
+
+def _gsa_set(rng, count, maxlen, alphabet=b"ACGT"):
+    al = np.frombuffer(alphabet, np.uint8)
+    parts = []
+    for _ in range(count):
+        L = int(rng.integers(1, maxlen + 1))
+        parts.append(rng.choice(al, L))
+        parts.append(np.zeros(1, np.uint8))
+    return np.concatenate(parts)
+
+
+GSA_CASES = [
+    ("gsa_small", _gsa_set(np.random.default_rng(5), 10, 20)),
+    ("gsa_many", _gsa_set(np.random.default_rng(6), 3000, 40)),
+    ("gsa_binary", _gsa_set(np.random.default_rng(7), 500, 200, b"ab")),
+    ("gsa_repeats", np.tile(np.frombuffer(b"ACGTACGT\x00", np.uint8), 2000)),
+    ("gsa_single", np.frombuffer(b"banana\x00", np.uint8).copy()),
+]
+
+
+@pytest.mark.parametrize("name,T", GSA_CASES, ids=[c[0] for c in GSA_CASES])
+def test_gsa(name, T):
+    print(f"case {name} n={len(T)}", flush=True)
+    n = len(T)
+    rc, SA_ref = ol.sa(T, gsa=True)
+    assert rc == 0
+    SA = np.zeros(n, np.int32)
+    assert la.libsais_gsa(T, SA, n, 0, None) == 0, la.last_error()
+    assert np.array_equal(SA, SA_ref)
+    P = np.zeros(n, np.int32)
+    assert la.libsais_plcp_gsa(T, SA, P, n) == 0
+    assert np.array_equal(P, ol.plcp(T, SA_ref, gsa=True))
+
+
+def test_gsa_invalid_on_gpu():
+    for bad in (b"\x00ab\x00", b"ab\x00\x00cd\x00", b"a\x00\x00"):
+        T = np.frombuffer(bad, np.uint8).copy()
+        SA = np.zeros(len(T), np.int32)
+        assert la.libsais_gsa(T, SA, len(T), 0, None) == -1
+
+
+def test_64bit_delegation():
+    T = workloads.iid(30000, seed=9, alphabet=b"ACGT")
+    n = len(T)
+    rc, SA_ref = ol.sa(T)
+    SA = np.zeros(n, np.int64)
+    f = np.zeros(256, np.int64)
+    assert la.libsais64(T, SA, n, 0, f) == 0
+    assert np.array_equal(SA, SA_ref.astype(np.int64))
+    assert np.array_equal(f, ol.freq(T))
+    U = np.zeros(n, np.uint8)
+    A = np.zeros(n, np.int64)
+    U_ref, idx_ref = ol.bwt(T, SA_ref)
+    assert la.libsais64_bwt(T, U, A, n, 0, None) == idx_ref
+    assert np.array_equal(U, U_ref)
+    P = np.zeros(n, np.int64)
+    assert la.libsais64_plcp(T, SA, P, n) == 0
+    P_ref = ol.plcp(T, SA_ref)
+    assert np.array_equal(P, P_ref.astype(np.int64))
+    L = np.zeros(n, np.int64)
+    assert la.libsais64_lcp(P, SA, L, n) == 0
+    assert np.array_equal(L, ol.lcp(P_ref, SA_ref).astype(np.int64))
+
+
+def test_bwt_output_aliases_input():
+    T = workloads.iid(10000, seed=3, alphabet=b"ACGT")
+    U_ref, idx_ref = ol.bwt(T, ol.sa(T)[1])
+    buf = T.copy()
+    A = np.zeros(len(T), np.int32)
+    assert la.libsais_bwt(buf, buf, A, len(T), 0, None) == idx_ref
+    assert np.array_equal(buf, U_ref)
+
+
+def test_lcp_aliases_sa():
+    T = workloads.iid(10000, seed=4, alphabet=b"ACGT")
+    SA = ol.sa(T)[1].copy()
+    P = ol.plcp(T, SA)
+    L_ref = ol.lcp(P, SA)
+    assert la.libsais_lcp(P, SA, SA, len(T)) == 0
+    assert np.array_equal(SA, L_ref)
+
+
+def test_repeatable():
+    T = workloads.dna_like(1 << 18, seed=77)
+    a = la.suffix_array(T)
+    b = la.suffix_array(T)
+    assert np.array_equal(a, b)
