@@ -1,0 +1,248 @@
+"""Benchmark: suffix array + BWT + PLCP + LCP of a 256 MiB DNA-like string on one MI355X.
+
+One step = one fused sagpu_build32() call on a device-resident text (order-3 Markov
+ACGT with 2% tandem repeats and 5% copied segments, seed 2): SA, BWT + primary
+index, PLCP and LCP, all int32, outputs left in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+
+roofline: the dominant kernel is the onesweep radix-sort pass.  Its algorithmic
+traffic per launch is elements x (8 B key + 4 B value) x 2 (read + write); the
+launch durations come from HIP events recorded on the engine stream around every
+pass inside the timed steps.
+pipeline: B_model = 269 n + 244 sum_r a_r bytes per step (prefix-doubling yardstick
+with 8 initial symbols; a_r counts suffixes with max(LCP[j], LCP[j+1]) >= 8*2^(r-1),
+computed from the LCP this run produced), divided by the step time.
+cpu_baseline: the reference libsais 2.10.4 (built from its sources by oracle/Makefile)
+with OpenMP on this host's cores, on the same input: libsais_omp + libsais_bwt_omp
++ libsais_plcp_omp + libsais_lcp_omp, median of the repetitions.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SA+BWT input MB/s @256 MiB DNA; batched vs plain aggregate MB/s; p99 latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def a_r_from_lcp(L: np.ndarray, h0: int = 8):
+    """a_r = #{j : max(LCP[j], LCP[j+1]) >= h0 * 2^(r-1)}, LCP[n] := 0."""
+    n = len(L)
+    m = np.maximum(L, np.append(L[1:], 0).astype(L.dtype))
+    out = []
+    h = h0
+    while True:
+        a = int(np.count_nonzero(m >= h))
+        if a == 0:
+            break
+        out.append(a)
+        h *= 2
+        if h > 4 * n:
+            break
+    return out
+
+
+def cpu_baseline(T: np.ndarray, reps: int):
+    """Reference libsais with OpenMP, or the oracle restatement if the reference
+    build is absent."""
+    from tests import oracle_lib as ol
+
+    n = len(T)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, 64)
+    if ol.have_ref():
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            rc, SA, _ = ol.ref_sa(T, threads=cores)
+            t1 = time.perf_counter()
+            U, idx = ol.ref_bwt(T, threads=cores)
+            t2 = time.perf_counter()
+            P = ol.ref_plcp(T, SA, threads=cores)
+            L = ol.ref_lcp(P, SA, threads=cores)
+            t3 = time.perf_counter()
+            times.append(t3 - t0)
+            log(f"cpu reference: sa {t1 - t0:.2f}s bwt {t2 - t1:.2f}s plcp+lcp {t3 - t2:.2f}s")
+        t = float(np.median(times))
+        return {"value": round(n / t / 1e6, 2), "unit": "MB/s", "cores": cores, "kind": "reference",
+                "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp, "
+                          f"median of {reps}"}
+    # oracle port, single thread, on a bounded prefix
+    m = min(n, 1 << 24)
+    S = np.ascontiguousarray(T[:m])
+    t0 = time.perf_counter()
+    rc, SA = ol.sa(S)
+    ol.bwt(S, SA)
+    P = ol.plcp(S, SA)
+    ol.lcp(P, SA)
+    t = time.perf_counter() - t0
+    return {"value": round(m / t / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port",
+            "sample": f"first {m} bytes, SA+BWT+PLCP+LCP with the oracle restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 28)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on exactly one GPU; multi-GPU runs are not part of "
+                                   "this build (see DESIGN.md for how the path would shard)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import libsais_amd as la
+    from libsais_amd import workloads
+
+    lib = la.load()
+    n = args.n
+    t0 = time.perf_counter()
+    T = workloads.dna_like(n, seed=args.seed)
+    log(f"generated {n} bytes in {time.perf_counter() - t0:.1f}s")
+
+    if lib.sagpu_init() != 0:
+        raise RuntimeError(la.last_error())
+    dT = lib.sagpu_malloc(n)
+    dSA = lib.sagpu_malloc(4 * n)
+    dU = lib.sagpu_malloc(n)
+    dP = lib.sagpu_malloc(4 * n)
+    dL = lib.sagpu_malloc(4 * n)
+    if not all((dT, dSA, dU, dP, dL)):
+        raise MemoryError("device allocation failed")
+    assert lib.sagpu_memcpy_h2d(dT, T.ctypes.data, n) == 0
+    freq = np.zeros(256, np.int64)
+
+    def step():
+        r = lib.sagpu_build32(dT, n, 0, dSA, dU, dP, dL, freq.ctypes.data)
+        if r < 0:
+            raise RuntimeError(f"sagpu_build32 failed ({r}): {la.last_error()}")
+        return r
+
+    lib.sagpu_set_timing(0)
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done")
+
+    lib.sagpu_set_timing(1)
+    sweep_ms, sweep_launch, sweep_elems = 0.0, 0, 0
+    dev_ms = 0.0
+    fam = {"initial": 0.0, "rounds": 0.0, "bwt": 0.0, "plcp": 0.0, "lcp": 0.0}
+    lib.sagpu_synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        primary = step()
+        st = la.stats()
+        sweep_ms += st.ms_onesweep
+        sweep_launch += st.onesweep_launches
+        sweep_elems += st.sorted_elems
+        dev_ms += st.ms_total
+        fam["initial"] += st.ms_initial
+        fam["rounds"] += st.ms_rounds
+        fam["bwt"] += st.ms_bwt
+        fam["plcp"] += st.ms_plcp
+        fam["lcp"] += st.ms_lcp
+    lib.sagpu_synchronize()
+    wall = time.perf_counter() - t0
+    lib.sagpu_set_timing(0)
+    ms_per_step = wall * 1000.0 / args.steps
+    value = n / (ms_per_step / 1000.0) / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} MB/s")
+
+    st = la.stats()
+    rounds = st.rounds
+    active = [int(st.active[r]) for r in range(min(rounds, 64))]
+
+    # dominant kernel roofline (onesweep pass)
+    bytes_per_elem = 2 * (8 + 4)
+    sweep_bytes = sweep_elems * bytes_per_elem
+    achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
+
+    # pipeline yardstick from this run's LCP
+    L = np.empty(n, np.int32)
+    lib.sagpu_memcpy_d2h(L.ctypes.data, dL, 4 * n)
+    a_r = a_r_from_lcp(L)
+    b_model = 269 * n + 244 * sum(a_r)
+    del L
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": "dna_like_256MiB_sa_bwt_plcp_lcp" if n == 1 << 28 else f"dna_like_{n}B_sa_bwt_plcp_lcp",
+            "n": n,
+            "seed": args.seed,
+            "outputs": ["SA", "BWT+primary", "PLCP", "LCP"],
+            "index": "int32",
+            "inputs_resident": "HBM",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_onesweep (radix pass, u64 key + u32 value)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": sweep_launch,
+            "avg_launch_ms": round(sweep_ms / max(1, sweep_launch), 4),
+            "algorithmic_bytes_per_elem": bytes_per_elem,
+        },
+        "pipeline": {
+            "b_model_bytes": b_model,
+            "a_r": a_r,
+            "achieved_GBs": round(b_model / (ms_per_step / 1000.0) / 1e9, 1),
+            "frac": round(b_model / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+            "device_ms_per_step": round(dev_ms / args.steps, 3),
+            "family_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam.items()},
+            "engine_rounds": rounds,
+            "engine_active": active,
+            "initial_symbols": st.initial_symbols,
+            "primary": int(primary),
+        },
+    }
+    for p in (dT, dSA, dU, dP, dL):
+        lib.sagpu_free(p)
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(T, args.cpu_reps)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
