@@ -1,0 +1,241 @@
+"""Pins the oracle (CPU, no GPU).
+
+Each oracle function is checked against something independent of its own code:
+brute-force suffix sorting, hand-derived closed forms, the compiled reference
+(oracle/_ref, when it was built here), committed golden vectors made from that
+reference, and planted mistakes that the checker must reject.
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from libsais_amd import workloads
+from tests import oracle_lib as ol
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+needs_ref = pytest.mark.skipif(not ol.have_ref(), reason="reference build oracle/_ref absent")
+
+
+def brute_sa(T, gsa=False):
+    T = bytes(T)
+    n = len(T)
+    if not gsa:
+        return sorted(range(n), key=lambda i: T[i:])
+
+    def key(i):
+        out = []
+        for p in range(i, n):
+            if T[p] == 0:
+                out.append((0, p))
+                break
+            out.append((1, T[p]))
+        return out
+
+    return sorted(range(n), key=key)
+
+
+def brute_plcp(T, SA, gsa=False):
+    T = bytes(T)
+    n = len(T)
+    P = [0] * n
+    for j in range(n):
+        i = SA[j]
+        if j == 0:
+            P[i] = 0
+            continue
+        k = SA[j - 1]
+        l = 0
+        while i + l < n and k + l < n and T[i + l] == T[k + l] and not (gsa and T[i + l] == 0):
+            l += 1
+        P[i] = l
+    return P
+
+
+def test_exhaustive_binary_strings():
+    for n in range(1, 12):
+        for t in itertools.product(b"ab", repeat=n):
+            T = np.array(t, np.uint8)
+            rc, SA = ol.sa(T)
+            assert rc == 0
+            assert SA.tolist() == brute_sa(T), t
+
+
+def test_exhaustive_gsa_strings():
+    checked = 0
+    for n in range(1, 10):
+        for t in itertools.product((0, 1, 2), repeat=n):
+            T = np.array(t, np.uint8)
+            valid = ol.gsa_valid(T)
+            rc, SA = ol.sa(T, gsa=True)
+            if not valid:
+                assert rc == -1
+                continue
+            assert rc == 0
+            assert SA.tolist() == brute_sa(T, gsa=True), t
+            assert ol.plcp(T, SA, gsa=True).tolist() == brute_plcp(T, SA.tolist(), gsa=True)
+            checked += 1
+    assert checked > 1000
+
+
+def test_random_plcp_against_direct_comparison():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = int(rng.integers(2, 200))
+        T = rng.integers(0, int(rng.integers(1, 5)), n).astype(np.uint8)
+        rc, SA = ol.sa(T)
+        P = ol.plcp(T, SA)
+        assert P.tolist() == brute_plcp(T, SA.tolist())
+        L = ol.lcp(P, SA)
+        assert L.tolist() == [P[SA[j]] for j in range(n)]
+
+
+def test_closed_forms():
+    T = np.frombuffer(b"banana", np.uint8).copy()
+    rc, SA = ol.sa(T)
+    assert SA.tolist() == [5, 3, 1, 0, 4, 2]
+    U, idx = ol.bwt(T, SA)
+    assert bytes(U) == b"annbaa" and idx == 4
+    P = ol.plcp(T, SA)
+    assert P.tolist() == [0, 3, 2, 1, 0, 0]
+    assert ol.lcp(P, SA).tolist() == [0, 1, 3, 0, 0, 2]
+
+    T = np.frombuffer(b"mississippi", np.uint8).copy()
+    rc, SA = ol.sa(T)
+    assert SA.tolist() == [10, 7, 4, 1, 0, 9, 8, 6, 3, 5, 2]
+    U, idx = ol.bwt(T, SA)
+    assert bytes(U) == b"ipssmpissii" and idx == 5
+    assert ol.lcp(ol.plcp(T, SA), SA).tolist() == [0, 1, 1, 4, 0, 0, 1, 0, 2, 1, 3]
+
+    for n in (1, 2, 7, 1000):
+        T = np.full(n, 65, np.uint8)
+        rc, SA = ol.sa(T)
+        assert SA.tolist() == list(range(n - 1, -1, -1))
+        P = ol.plcp(T, SA)
+        assert P.tolist() == [n - 1 - i for i in range(n)] if n > 1 else [0]
+        assert ol.lcp(P, SA).tolist() == list(range(n))
+        U, idx = ol.bwt(T, SA)
+        assert (U == 65).all() and idx == n
+
+
+def test_checker_rejects_planted_mistakes():
+    T = workloads.iid(5000, 5, b"ACGT")
+    rc, SA = ol.sa(T)
+    assert ol.check_sa(T, SA)
+    bad = SA.copy()
+    bad[100], bad[101] = bad[101], bad[100]
+    assert not ol.check_sa(T, bad)
+    bad = SA.copy()
+    bad[7] = bad[8]
+    assert not ol.check_sa(T, bad)
+    bad = SA.copy()
+    bad[0] = len(T)
+    assert not ol.check_sa(T, bad)
+    # suffix that ends early must come first among equals
+    T2 = np.frombuffer(b"aaaa", np.uint8).copy()
+    assert ol.check_sa(T2, np.array([3, 2, 1, 0], np.int32))
+    assert not ol.check_sa(T2, np.array([2, 3, 1, 0], np.int32))
+    # GSA: separators ordered by position
+    G = np.frombuffer(b"ab\x00ab\x00", np.uint8).copy()
+    rc, GSA = ol.sa(G, gsa=True)
+    assert ol.check_sa(G, GSA, gsa=True)
+    swapped = GSA.copy()
+    swapped[0], swapped[1] = swapped[1], swapped[0]
+    assert not ol.check_sa(G, swapped, gsa=True)
+
+
+def test_checker_accepts_all_workload_shapes():
+    for T in (workloads.periodic(20000, 17, 3), workloads.fibonacci(20000), workloads.dna_like(1 << 16, 2),
+              np.full(3000, 65, np.uint8)):
+        rc, SA = ol.sa(T)
+        assert ol.check_sa(T, SA)
+
+
+def test_golden_small_vectors():
+    z = np.load(os.path.join(GOLDEN, "small.npz"))
+    names = sorted({k.split("__")[0] for k in z.files})
+    assert len(names) >= 10
+    for name in names:
+        T = z[f"{name}__T"]
+        gsa = name.startswith("gsa")
+        rc, SA = ol.sa(T, gsa=gsa)
+        assert rc == 0
+        assert np.array_equal(SA, z[f"{name}__SA"]), name
+        assert np.array_equal(ol.plcp(T, SA, gsa=gsa), z[f"{name}__PLCP"]), name
+        assert np.array_equal(ol.freq(T), z[f"{name}__freq"].astype(np.int64)), name
+        if not gsa:
+            U, idx = ol.bwt(T, SA)
+            assert np.array_equal(U, z[f"{name}__U"]) and idx == int(z[f"{name}__primary"][0]), name
+            assert np.array_equal(ol.lcp(ol.plcp(T, SA), SA), z[f"{name}__LCP"]), name
+
+
+def test_golden_1mib_digest():
+    import hashlib
+
+    g = json.load(open(os.path.join(GOLDEN, "golden.json")))["acgt_1MiB_seed1"]
+    T = workloads.iid_acgt(g["n"], seed=1)
+    assert hashlib.sha256(T.tobytes()).hexdigest() == g["T_sha256"]
+    rc, SA = ol.sa(T)
+    assert hashlib.sha256(SA.tobytes()).hexdigest() == g["SA_int32_sha256"]
+    U, idx = ol.bwt(T, SA)
+    assert hashlib.sha256(U.tobytes()).hexdigest() == g["U_sha256"] and idx == g["primary"]
+    P = ol.plcp(T, SA)
+    assert hashlib.sha256(P.tobytes()).hexdigest() == g["PLCP_int32_sha256"]
+    assert hashlib.sha256(ol.lcp(P, SA).tobytes()).hexdigest() == g["LCP_int32_sha256"]
+
+
+@needs_ref
+def test_gsa_validity_matches_reference():
+    for n in range(1, 9):
+        for t in itertools.product((0, 1, 2), repeat=n):
+            T = np.array(t, np.uint8)
+            if T[-1] != 0:
+                continue
+            rc, _, _ = ol.ref_sa(T, gsa=True)
+            assert (rc == 0) == ol.gsa_valid(T), t
+
+
+@needs_ref
+@pytest.mark.parametrize("alphabet", [1, 2, 4, 20, 256])
+def test_random_against_reference(alphabet):
+    rng = np.random.default_rng(alphabet)
+    for n in (2, 3, 10, 257, 5000, 70000):
+        T = rng.integers(0, alphabet, n).astype(np.uint8)
+        rc, SA_r, f_r = ol.ref_sa(T)
+        rc2, SA = ol.sa(T)
+        assert rc == rc2 == 0
+        assert np.array_equal(SA, SA_r)
+        U_r, idx_r = ol.ref_bwt(T)
+        U, idx = ol.bwt(T, SA)
+        assert idx == idx_r and np.array_equal(U, U_r)
+        P_r = ol.ref_plcp(T, SA_r)
+        assert np.array_equal(ol.plcp(T, SA), P_r)
+        assert np.array_equal(ol.lcp(P_r, SA), ol.ref_lcp(P_r, SA_r))
+        assert np.array_equal(ol.freq(T), f_r.astype(np.int64))
+
+
+@needs_ref
+def test_workload_shapes_against_reference():
+    for T in (workloads.dna_like(1 << 20, 2), workloads.periodic(1 << 18, 17, 3), workloads.fibonacci(1 << 18),
+              np.full(1 << 16, 65, np.uint8)):
+        rc, SA_r, _ = ol.ref_sa(T)
+        assert np.array_equal(ol.sa(T)[1], SA_r)
+        assert np.array_equal(ol.plcp(T, SA_r), ol.ref_plcp(T, SA_r))
+
+
+@needs_ref
+def test_gsa_sets_against_reference():
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        parts = []
+        for _ in range(int(rng.integers(1, 60))):
+            parts.append(rng.integers(1, int(rng.integers(2, 6)), int(rng.integers(1, 50))).astype(np.uint8))
+            parts.append(np.zeros(1, np.uint8))
+        T = np.concatenate(parts)
+        rc, SA_r, _ = ol.ref_sa(T, gsa=True)
+        rc2, SA = ol.sa(T, gsa=True)
+        assert rc == rc2 == 0
+        assert np.array_equal(SA, SA_r)
+        assert np.array_equal(ol.plcp(T, SA, gsa=True), ol.ref_plcp(T, SA_r, gsa=True))
