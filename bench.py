@@ -147,7 +147,7 @@ def main():
         log(f"warmup {i} done")
 
     lib.sagpu_set_timing(1)
-    sweep_ms, sweep_launch, sweep_elems = 0.0, 0, 0
+    sweep_ms, sweep_launch, sweep_elems, sweep_bytes = 0.0, 0, 0, 0
     dev_ms = 0.0
     fam = {"initial": 0.0, "rounds": 0.0, "bwt": 0.0, "plcp": 0.0, "lcp": 0.0}
     lib.sagpu_synchronize()
@@ -158,6 +158,7 @@ def main():
         sweep_ms += st.ms_onesweep
         sweep_launch += st.onesweep_launches
         sweep_elems += st.sorted_elems
+        sweep_bytes += st.sorted_bytes
         dev_ms += st.ms_total
         fam["initial"] += st.ms_initial
         fam["rounds"] += st.ms_rounds
@@ -175,9 +176,9 @@ def main():
     rounds = st.rounds
     active = [int(st.active[r]) for r in range(min(rounds, 64))]
 
-    # dominant kernel roofline (onesweep pass)
-    bytes_per_elem = 2 * (8 + 4)
-    sweep_bytes = sweep_elems * bytes_per_elem
+    # dominant kernel roofline (radix pass): (key + value) bytes read and written
+    # per element moved, 16 B for u32 keys, 24 B for u64 keys
+    bytes_per_elem = sweep_bytes / max(1, sweep_elems)
     achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
 
     # pipeline yardstick from this run's LCP
@@ -210,7 +211,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_onesweep (radix pass, u64 key + u32 value)",
+            "kernel": "k_radix_pass (onesweep LSD pass; u32 or u64 key + u32 value)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -218,7 +219,8 @@ def main():
             "traffic": None,
             "launches": sweep_launch,
             "avg_launch_ms": round(sweep_ms / max(1, sweep_launch), 4),
-            "algorithmic_bytes_per_elem": bytes_per_elem,
+            "algorithmic_bytes_per_elem": round(bytes_per_elem, 2),
+            "elements_moved": sweep_elems,
         },
         "pipeline": {
             "b_model_bytes": b_model,

@@ -71,7 +71,10 @@ typedef struct sagpu_stats {
     int32_t bits_per_symbol;
     int32_t sort_passes;          /* onesweep passes executed in total             */
     int64_t active[64];           /* active[r] = unsorted suffixes entering round r */
-    int64_t sorted_elems;         /* sum over onesweep passes of elements moved     */
+    int64_t sorted_elems;         /* sum over radix passes of elements moved        */
+    int64_t sorted_bytes;         /* algorithmic bytes of those passes (read+write) */
+    int32_t big_rounds;           /* rounds that sorted groups too big for LDS      */
+    int64_t big_elems;            /* elements those rounds sorted globally          */
     double  ms_total;             /* first to last kernel of the build             */
     double  ms_onesweep;          /* all onesweep passes                            */
     int32_t onesweep_launches;
@@ -81,6 +84,30 @@ typedef struct sagpu_stats {
     double  ms_plcp;
     double  ms_lcp;
 } sagpu_stats;
+
+/*
+ * Stable LSD radix sort of (key, value) pairs in device memory, in place, by key
+ * bits [begin_bit, end_bit).  Exposes the engine's sort primitive on its own.
+ */
+SAGPU_API int32_t sagpu_sort_pairs_u32(uint32_t *d_keys, uint32_t *d_vals, int64_t n, int32_t begin_bit,
+                                       int32_t end_bit);
+SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64_t n, int32_t begin_bit,
+                                       int32_t end_bit);
+
+/*
+ * Tuning knobs (process-wide):
+ *   "radix_variant"  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
+ * Returns 0, or -1 for an unknown name or value.
+ */
+SAGPU_API int32_t sagpu_set_option(const char *name, int64_t value);
+
+/*
+ * Sort micro-benchmark: sorts n pseudo-random (u32 or u64) keys with u32 values
+ * `reps` times on device-generated data and returns the mean ms per sort and per
+ * radix pass.  key_bytes = 4 or 8.
+ */
+SAGPU_API int32_t sagpu_sort_bench(int64_t n, int32_t key_bytes, int32_t key_bits, int32_t reps, double *ms_per_sort,
+                                   double *ms_per_pass);
 
 /* timing = 1 records events around kernel families (adds a little overhead). */
 SAGPU_API void    sagpu_set_timing(int32_t enabled);

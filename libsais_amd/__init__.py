@@ -61,6 +61,9 @@ class SagpuStats(ctypes.Structure):
         ("sort_passes", ctypes.c_int32),
         ("active", ctypes.c_int64 * 64),
         ("sorted_elems", ctypes.c_int64),
+        ("sorted_bytes", ctypes.c_int64),
+        ("big_rounds", ctypes.c_int32),
+        ("big_elems", ctypes.c_int64),
         ("ms_total", ctypes.c_double),
         ("ms_onesweep", ctypes.c_double),
         ("onesweep_launches", ctypes.c_int32),
@@ -107,6 +110,11 @@ ABI_SAGPU = [
     ("sagpu_build32", _i64, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("sagpu_plcp32", _i32, [_vp, _vp, _vp, _i64, _i32]),
     ("sagpu_lcp32", _i32, [_vp, _vp, _vp, _i64]),
+    ("sagpu_sort_pairs_u32", _i32, [_vp, _vp, _i64, _i32, _i32]),
+    ("sagpu_sort_pairs_u64", _i32, [_vp, _vp, _i64, _i32, _i32]),
+    ("sagpu_set_option", _i32, [ctypes.c_char_p, _i64]),
+    ("sagpu_sort_bench", _i32, [_i64, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
     ("sagpu_set_timing", None, [_i32]),
     ("sagpu_get_stats", _i32, [ctypes.POINTER(SagpuStats)]),
 ]

@@ -6,6 +6,14 @@
 
 #include <stddef.h>
 
+// ------------------------------------------------------------------ scan
+// Exclusive prefix sum of a u32 array in place (reduce-then-scan, 3 kernels).
+struct ScanScratch {
+    u32 *partials;    // [cap] per-tile sums
+    u64 cap;          // tiles of 4096 elements
+};
+int scan_exclusive_u32(ScanScratch &sc, u32 *data, u64 count, hipStream_t s);
+
 // ------------------------------------------------------------------ radix sort
 // Onesweep LSD radix sort, 8-bit digits, stable.  Keys are u64 or u32, values u32.
 // The result ends in (k0,v0) or in (k1,v1); *result_in_alt says which.
@@ -20,7 +28,13 @@ struct RadixScratch {
     // launch; the engine uses it to bracket passes with timing events.
     void (*hook)(void *user, int phase);
     void *hook_user;
+    int variant;      // RADIX_ONESWEEP or RADIX_REDUCE_THEN_SCAN
+    u32 *counts;      // [counts_cap] per-tile digit counts (reduce-then-scan)
+    u64 counts_cap;
+    ScanScratch scan;
 };
+
+enum { RADIX_ONESWEEP = 0, RADIX_REDUCE_THEN_SCAN = 1 };
 
 enum { RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS };
 
@@ -51,10 +65,38 @@ int seg_init(SegScratch &ss, const u64 *keys, const u32 *vals, u64 n, u64 sepmas
 int seg_round(SegScratch &ss, const u64 *comp, const u32 *vals, const u32 *act_p, u64 a, u32 *SA,
               u32 *ISA, u32 *nact_i, u32 *nact_g, u32 *nact_p, hipStream_t s);
 
+// ------------------------------------------------------- rounds (rounds.hip)
+// LCP marks written during construction: LCP_UNKNOWN | lower bound for pairs that
+// were separated by a doubling round (their lcp is resolved afterwards).
+#define LCP_UNKNOWN 0x80000000u
+#define LCP_BOUND_CAP 0x40000000u
+enum { RT_SMALL = 256 };   // groups up to this size are sorted inside one workgroup
+
+struct WsScratch {
+    void *tile_agg;   // [tiles_cap] 12-byte tile aggregates
+    u64 tiles_cap;    // tiles of 4096 elements
+    u32 *total;       // [1]
+    u32 *base;        // [1] reserved output base
+};
+
+int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
+               u32 *act_i, u32 *act_g, u32 *LCP, hipStream_t s);
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,
+                 hipStream_t s);
+int round_tile(const u32 *act_i, const u32 *act_g, const u32 *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
+               u32 *out_g, u32 *out_counter, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter, u64 big_cap,
+               u32 *LCP, hipStream_t s);
+int big_extract(const u32 *act_i, const u32 *key2, const u32 *bhead, const u32 *boff, u32 nb, u64 total, u64 *comp,
+                u32 *vals, hipStream_t s);
+int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, u32 *SA, u32 *ISA,
+             u32 *out_i, u32 *out_g, u32 *out_counter, u32 *LCP, hipStream_t s);
+
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
 int launch_pack_keys(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u64 *keys,
                      u32 *vals, hipStream_t s);
+int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
+                       hipStream_t s);
 int launch_round_keys(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h,
                       u64 *comp, u32 *vals, hipStream_t s);
 
@@ -71,8 +113,16 @@ struct LcpScratch {
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
 int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // PLCP from either PHI (isa == nullptr) or ISA+SA (phi computed on the fly).
+// marks = 1: PLCP already holds exact values or LCP_UNKNOWN | lower bound for
+// every position (scattered from the construction's LCP marks); only unknown
+// positions are compared.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
-                const u32 *SA, u32 *PLCP, hipStream_t s);
+                const u32 *SA, u32 *PLCP, int marks, hipStream_t s);
+// PLCP[SA[j]] = LCP[j] for all j (marks included); exact values are first clamped
+// to n - max(SA[j-1], SA[j]) (written back to LCP).
+int launch_scatter_marks(u32 *LCP, const u32 *SA, u64 n, u32 *PLCP, hipStream_t s);
+// LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
+int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 
 // Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).

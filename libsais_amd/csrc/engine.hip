@@ -114,6 +114,7 @@ struct Ctx {
 std::mutex g_mu;
 Ctx *g_ctx = nullptr;
 bool g_timing = false;
+int g_radix_variant = RADIX_ONESWEEP;
 
 int ctx_get(Ctx **out)
 {
@@ -129,6 +130,7 @@ int ctx_get(Ctx **out)
     SAGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     SAGPU_CHECK(hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault));
     c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
+    c->rs.variant = g_radix_variant;
     g_ctx = c;
     *out = c;
     return SAGPU_OK;
@@ -218,6 +220,15 @@ int ensure_radix_scratch(Ctx &c, Scope &sc, u64 n)
         c.rs.tile_ctr = (u32 *)c.pool.get(64 * sizeof(u32));
         if (!c.rs.hist || !c.rs.tile_ctr) return SAGPU_ENOMEM;
     }
+    if (c.rs.counts_cap < tiles * 256) {
+        if (c.rs.counts) c.pool.put(c.rs.counts);
+        if (c.rs.scan.partials) c.pool.put(c.rs.scan.partials);
+        c.rs.counts = (u32 *)c.pool.get(tiles * 256 * sizeof(u32));
+        c.rs.scan.cap = tiles * 256 / 4096 + 2;
+        c.rs.scan.partials = (u32 *)c.pool.get(c.rs.scan.cap * sizeof(u32));
+        if (!c.rs.counts || !c.rs.scan.partials) { c.rs.counts_cap = 0; return SAGPU_ENOMEM; }
+        c.rs.counts_cap = tiles * 256;
+    }
     if (c.rs.lb_tiles < tiles) {
         if (c.rs.lookback) c.pool.put(c.rs.lookback);
         c.rs.lookback = (u64 *)c.pool.get(tiles * 256 * sizeof(u64));
@@ -247,6 +258,19 @@ int sort_u64(Ctx &c, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, i
     int rc = radix_sort_u64(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
     c.stats.sort_passes += passes;
     c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
+    c.stats.sorted_bytes += (int64_t)passes * (int64_t)n * 24;
+    return rc;
+}
+
+int sort_u32(Ctx &c, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, int end_bit, int *alt)
+{
+    int passes = 0;
+    c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
+    c.rs.hook_user = &c;
+    int rc = radix_sort_u32(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
+    c.stats.sort_passes += passes;
+    c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
+    c.stats.sorted_bytes += (int64_t)passes * (int64_t)n * 16;
     return rc;
 }
 
@@ -258,9 +282,11 @@ int read_u32(Ctx &c, const u32 *d, u32 *out)
     return SAGPU_OK;
 }
 
-// Suffix array of the padded device text dT[0..n) (n >= 2, n < 2^32).  On return
-// dSA holds the suffix array and dISA its inverse.  freq_out (host, 256) optional.
-int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out)
+// Suffix array of the padded device text dT[0..n) (n >= 2, n <= 2^31 - 1).  On
+// return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
+// optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
+// previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
+int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
@@ -268,7 +294,6 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     GET(d_hist, u64, 256 + 8);
     u32 *d_bad = (u32 *)(d_hist + 256);
     u64 freq[256];
-    int bits, k;
     {
         Span sp(c, F_INITIAL);
         SAGPU_TRY(launch_fill_u64(d_hist, 256 + 8, 0, s));
@@ -287,72 +312,131 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         lut_h[ch] = (uint8_t)sigma;
         if (freq[ch]) sigma++;
     }
-    bits = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
-    k = 64 / bits;
-    if (k > 64) k = 64;
-    st.initial_symbols = k;
+    const int bits = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
+    const int ksym = 32 / bits;
+    st.initial_symbols = ksym;
     st.bits_per_symbol = bits;
 
     GET(d_lut, uint8_t, 256);
     SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
 
-    GET(keysA, u64, n);
-    GET(keysB, u64, n);
-    GET(valsA, u32, n);
-    GET(valsB, u32, n);
-    GET(act_i, u32, n);
-    GET(act_g, u32, n);
-    GET(act_p0, u32, n);
-    GET(act_p1, u32, n);
-    u32 *act_p[2] = {act_p0, act_p1};
+    // Sort buffers: u32 keys for the initial sort, u64 (group, key2) for big groups.
+    GET(kb0, u64, n);
+    GET(kb1, u64, n);
+    GET(vb0, u32, n);
+    GET(vb1, u32, n);
+    GET(ai0, u32, n);
+    GET(ag0, u32, n);
+    GET(ai1, u32, n);
+    GET(ag1, u32, n);
+    GET(key2, u32, n);
+    const u64 big_cap = n / (RT_SMALL + 1) + 16;
+    GET(big_head, u32, big_cap);
+    GET(big_g, u32, big_cap);
+    GET(big_size, u32, big_cap);
+    GET(d_cnt, u32, 64);
+    GET(d_boff, u32, big_cap + 1);
+    GET(d_bhead, u32, big_cap);
+    GET(d_bslot, u32, big_cap);
+    u32 *act_i[2] = {ai0, ai1}, *act_g[2] = {ag0, ag1};
 
-    SegScratch ss{};
-    const u64 seg_tiles = (n + SG_TILE - 1) / SG_TILE + 1;
-    ss.tiles_cap = seg_tiles;
+    WsScratch ws{};
+    ws.tiles_cap = n / 4096 + 2;
     {
-        GET(t_cnt, u32, seg_tiles);
-        GET(t_head, u32, seg_tiles);
-        GET(t_total, u32, 64);
-        ss.tile_cnt = t_cnt;
-        ss.tile_head = t_head;
-        ss.total = t_total;
+        GET(tagg, u32, ws.tiles_cap * 3);
+        ws.tile_agg = tagg;
+        ws.total = d_cnt + 8;
+        ws.base = d_cnt + 9;
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
 
-    u32 a = 0;
     {
         Span sp(c, F_INITIAL);
-        SAGPU_TRY(launch_pack_keys(dT, n, d_lut, bits, k, gsa, keysA, valsA, s));
+        SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, (u32 *)kb0, vb0, s));
     }
     int alt = 0;
-    SAGPU_TRY(sort_u64(c, keysA, valsA, keysB, valsB, n, 0, k * bits, &alt));
+    SAGPU_TRY(sort_u32(c, (u32 *)kb0, vb0, (u32 *)kb1, vb1, n, 0, ksym * bits, &alt));
     {
         Span sp(c, F_INITIAL);
-        const u64 sepmask = gsa ? ((1ull << bits) - 1ull) : 0ull;
-        SAGPU_TRY(seg_init(ss, alt ? keysB : keysA, alt ? valsB : valsA, n, sepmask, dSA, dISA, act_i, act_g,
-                           act_p[0], s));
+        SAGPU_TRY(group_init(ws, alt ? (u32 *)kb1 : (u32 *)kb0, alt ? vb1 : vb0, n, bits, ksym, gsa, dSA, dISA,
+                             act_i[0], act_g[0], dLCP, s));
     }
-    SAGPU_TRY(read_u32(c, ss.total, &a));
+    u32 a = 0;
+    SAGPU_TRY(read_u32(c, ws.total, &a));
 
-    const int gbits = bits_for(n - 1);
-    u64 h = (u64)k;
+    u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
+    std::vector<u32> bh, bg, bs, order, hoff, hhead, hslot;
     while (a > 0) {
         if (st.rounds < 64) st.active[st.rounds] = a;
         st.rounds++;
+        const int nxt = cur ^ 1;
         {
             Span sp(c, F_ROUNDS);
-            SAGPU_TRY(launch_round_keys(act_i, act_g, a, dISA, n, h, keysA, valsA, s));
+            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, dSA, s));
+            SAGPU_TRY(launch_fill_u32(d_cnt, 3, 0, s));
+            SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, a, h, dSA, dISA, act_i[nxt], act_g[nxt], d_cnt,
+                                 big_head, big_g, big_size, d_cnt + 1, big_cap, dLCP, s));
         }
-        SAGPU_TRY(sort_u64(c, keysA, valsA, keysB, valsB, a, 0, 32 + gbits, &alt));
-        {
-            Span sp(c, F_ROUNDS);
-            SAGPU_TRY(seg_round(ss, alt ? keysB : keysA, alt ? valsB : valsA, act_p[cur], a, dSA, dISA, act_i, act_g,
-                                act_p[cur ^ 1], s));
+        SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 3 * sizeof(u32), hipMemcpyDeviceToHost, s));
+        SAGPU_CHECK(hipStreamSynchronize(s));
+        u32 out_n = ((u32 *)c.pinned)[0];
+        const u32 nb = ((u32 *)c.pinned)[1];
+        if (nb > big_cap || ((u32 *)c.pinned)[2] != 0 || out_n > a) {
+            sagpu_set_error("round bookkeeping inconsistent", __FILE__, __LINE__);
+            return SAGPU_EDEVICE;
         }
-        SAGPU_TRY(read_u32(c, ss.total, &a));
-        cur ^= 1;
+        if (nb > 0) {
+            // Big groups, in list order so the extraction is deterministic.
+            bh.resize(nb);
+            bg.resize(nb);
+            bs.resize(nb);
+            SAGPU_CHECK(hipMemcpyAsync(bh.data(), big_head, nb * 4, hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(bg.data(), big_g, nb * 4, hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(bs.data(), big_size, nb * 4, hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipStreamSynchronize(s));
+            order.resize(nb);
+            for (u32 q = 0; q < nb; q++) order[q] = q;
+            std::sort(order.begin(), order.end(), [&](u32 x, u32 y) { return bh[x] < bh[y]; });
+            hoff.resize(nb + 1);
+            hhead.resize(nb);
+            hslot.resize(nb);
+            u64 total = 0;
+            for (u32 q = 0; q < nb; q++) {
+                hoff[q] = (u32)total;
+                hhead[q] = bh[order[q]];
+                hslot[q] = bg[order[q]];
+                total += bs[order[q]];
+            }
+            hoff[nb] = (u32)total;
+            for (u32 q = 0; q < nb; q++) {
+                u64 hq = hhead[q], sq = hoff[q + 1] - hoff[q];
+                if (sq <= RT_SMALL || hq + sq > a || (q + 1 < nb && hq + sq > hhead[q + 1])) total = ~0ull;
+            }
+            if (total > a) {
+                sagpu_set_error("big-group list inconsistent", __FILE__, __LINE__);
+                return SAGPU_EDEVICE;
+            }
+            SAGPU_CHECK(hipMemcpyAsync(d_boff, hoff.data(), (nb + 1) * 4, hipMemcpyHostToDevice, s));
+            SAGPU_CHECK(hipMemcpyAsync(d_bhead, hhead.data(), nb * 4, hipMemcpyHostToDevice, s));
+            SAGPU_CHECK(hipMemcpyAsync(d_bslot, hslot.data(), nb * 4, hipMemcpyHostToDevice, s));
+            {
+                Span sp(c, F_ROUNDS);
+                SAGPU_TRY(big_extract(act_i[cur], key2, d_bhead, d_boff, nb, total, kb0, vb0, s));
+            }
+            SAGPU_TRY(sort_u64(c, kb0, vb0, kb1, vb1, total, 0, 32 + bits_for(nb - 1), &alt));
+            {
+                Span sp(c, F_ROUNDS);
+                SAGPU_TRY(big_emit(ws, alt ? kb1 : kb0, alt ? vb1 : vb0, d_bslot, total, h, dSA, dISA, act_i[nxt],
+                                   act_g[nxt], d_cnt, dLCP, s));
+            }
+            SAGPU_TRY(read_u32(c, d_cnt, &out_n));
+            st.big_rounds++;
+            st.big_elems += (int64_t)total;
+        }
+        a = out_n;
+        cur = nxt;
         h *= 2;
         if (h > 4 * n + 64) {
             sagpu_set_error("doubling did not converge", __FILE__, __LINE__);
@@ -363,7 +447,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 }
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-             u32 *PLCP)
+             u32 *PLCP, int marks)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -376,7 +460,7 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     ls.queue_l = ql;
     ls.queue_n = qn;
     ls.tile_max = tm;
-    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, c.stream);
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, marks, c.stream);
 }
 
 // Copies a text into a pool buffer with 64 zero bytes of padding.
@@ -405,26 +489,35 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (!dSA) return SAGPU_ENOMEM;
     }
     GET(dISA, u32, n);
+    const bool want_lcp = dPLCP || dLCP;
+    u32 *marks = dLCP;
+    if (want_lcp && !marks) {
+        marks = sc.get<u32>(n);
+        if (!marks) return SAGPU_ENOMEM;
+    }
+    if (want_lcp && !dPLCP) {
+        dPLCP = sc.get<u32>(n);
+        if (!dPLCP) return SAGPU_ENOMEM;
+    }
     int rc;
     {
         Span total(c, F_TOTAL);
-        rc = sa_build(c, sc, dTpad, n, gsa, dSA, dISA, freq);
+        rc = sa_build(c, sc, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
             SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
         }
-        if (dLCP && !dPLCP) {
-            dPLCP = sc.get<u32>(n);
-            if (!dPLCP) return SAGPU_ENOMEM;
-        }
-        if (dPLCP) {
-            Span sp(c, F_PLCP);
-            SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP));
-        }
-        if (dLCP) {
-            Span sp(c, F_LCP);
-            SAGPU_TRY(launch_lcp(dPLCP, dSA, n, dLCP, c.stream));
+        if (want_lcp) {
+            {
+                Span sp(c, F_PLCP);
+                SAGPU_TRY(launch_scatter_marks(marks, dSA, n, dPLCP, c.stream));
+                SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, 1));
+            }
+            if (dLCP) {
+                Span sp(c, F_LCP);
+                SAGPU_TRY(launch_lcp_fix(dPLCP, dSA, n, dLCP, c.stream));
+            }
         }
     }
     int64_t ret = 0;
@@ -570,7 +663,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
     u32 *dP = sc.get<u32>((u64)n);
     if (!dPHI || !dP) return SAGPU_ENOMEM;
     SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
-    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, 0));
     SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -674,6 +767,133 @@ extern "C" void sagpu_trim(void)
 
 extern "C" void sagpu_set_timing(int32_t enabled) { g_timing = enabled != 0; }
 
+extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
+{
+    if (!name) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!strcmp(name, "radix_variant")) {
+        if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
+        g_radix_variant = (int)value;
+        if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    return SAGPU_EINVAL;
+}
+
+namespace {
+
+template <typename K>
+int sort_pairs_dev(K *d_keys, u32 *d_vals, int64_t n, int begin_bit, int end_bit)
+{
+    if (!d_keys || !d_vals || n < 0 || begin_bit < 0 || end_bit > (int)(8 * sizeof(K)) || begin_bit > end_bit)
+        return SAGPU_EINVAL;
+    if (n <= 1) return 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    Scope sc(*c);
+    K *k1 = sc.get<K>((u64)n);
+    u32 *v1 = sc.get<u32>((u64)n);
+    if (!k1 || !v1) return SAGPU_ENOMEM;
+    SAGPU_TRY(ensure_radix_scratch(*c, sc, (u64)n));
+    int alt = 0, passes = 0;
+    c->rs.hook = nullptr;
+    int rc = sizeof(K) == 8
+                 ? radix_sort_u64(c->rs, (u64 *)d_keys, d_vals, (u64 *)k1, v1, (u64)n, begin_bit, end_bit, c->stream,
+                                  &alt, &passes)
+                 : radix_sort_u32(c->rs, (u32 *)d_keys, d_vals, (u32 *)k1, v1, (u64)n, begin_bit, end_bit, c->stream,
+                                  &alt, &passes);
+    if (rc) return rc;
+    if (alt) {
+        SAGPU_CHECK(hipMemcpyAsync(d_keys, k1, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, c->stream));
+        SAGPU_CHECK(hipMemcpyAsync(d_vals, v1, (size_t)n * sizeof(u32), hipMemcpyDeviceToDevice, c->stream));
+    }
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+__global__ void k_bench_keys(u64 *k64, u32 *k32, u32 *v, u64 n, u64 seed, u64 mask)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 z = (i + seed) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        if (k64) k64[i] = z & mask;
+        else k32[i] = (u32)(z & mask);
+        v[i] = (u32)i;
+    }
+}
+
+} // namespace
+
+extern "C" int32_t sagpu_sort_pairs_u32(uint32_t *d_keys, uint32_t *d_vals, int64_t n, int32_t b, int32_t e)
+{
+    return sort_pairs_dev<u32>(d_keys, d_vals, n, b, e);
+}
+
+extern "C" int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64_t n, int32_t b, int32_t e)
+{
+    return sort_pairs_dev<u64>((u64 *)d_keys, d_vals, n, b, e);
+}
+
+extern "C" int32_t sagpu_sort_bench(int64_t n, int32_t key_bytes, int32_t key_bits, int32_t reps, double *ms_per_sort,
+                                    double *ms_per_pass)
+{
+    if (n < 2 || (key_bytes != 4 && key_bytes != 8) || key_bits < 1 || key_bits > 8 * key_bytes || reps < 1)
+        return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    Scope sc(*c);
+    u64 *k64a = nullptr, *k64b = nullptr;
+    u32 *k32a = nullptr, *k32b = nullptr;
+    if (key_bytes == 8) {
+        k64a = sc.get<u64>((u64)n);
+        k64b = sc.get<u64>((u64)n);
+        if (!k64a || !k64b) return SAGPU_ENOMEM;
+    } else {
+        k32a = sc.get<u32>((u64)n);
+        k32b = sc.get<u32>((u64)n);
+        if (!k32a || !k32b) return SAGPU_ENOMEM;
+    }
+    u32 *va = sc.get<u32>((u64)n), *vb = sc.get<u32>((u64)n);
+    if (!va || !vb) return SAGPU_ENOMEM;
+    SAGPU_TRY(ensure_radix_scratch(*c, sc, (u64)n));
+    const u64 mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1ull);
+    hipEvent_t e0, e1;
+    SAGPU_CHECK(hipEventCreate(&e0));
+    SAGPU_CHECK(hipEventCreate(&e1));
+    double total = 0;
+    int passes_total = 0;
+    c->rs.hook = nullptr;
+    for (int r = 0; r < reps + 1; r++) {
+        hipLaunchKernelGGL(k_bench_keys, dim3(4096), dim3(256), 0, c->stream, k64a, k32a, va, (u64)n,
+                           (u64)r * 7919u, mask);
+        SAGPU_CHECK(hipEventRecord(e0, c->stream));
+        int alt = 0, passes = 0;
+        int rc = key_bytes == 8 ? radix_sort_u64(c->rs, k64a, va, k64b, vb, (u64)n, 0, key_bits, c->stream, &alt,
+                                                 &passes)
+                                : radix_sort_u32(c->rs, k32a, va, k32b, vb, (u64)n, 0, key_bits, c->stream, &alt,
+                                                 &passes);
+        if (rc) return rc;
+        SAGPU_CHECK(hipEventRecord(e1, c->stream));
+        SAGPU_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        SAGPU_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) {
+            total += ms;
+            passes_total += passes;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_sort) *ms_per_sort = total / reps;
+    if (ms_per_pass) *ms_per_pass = passes_total ? total / passes_total : 0.0;
+    return 0;
+}
+
 extern "C" int32_t sagpu_get_stats(sagpu_stats *out)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -713,7 +933,7 @@ extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t
     u32 *dPHI = sc.get<u32>((u64)n);
     if (!dPHI) return SAGPU_ENOMEM;
     SAGPU_TRY(launch_phi((const u32 *)d_SA, (u64)n, dPHI, c->stream));
-    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP, 0));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -87,10 +87,19 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                                                   const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                   const u32 *__restrict__ SA, u32 *__restrict__ V,
                                                   u32 *__restrict__ queue, u32 *__restrict__ queue_l,
-                                                  u32 *__restrict__ queue_n, u64 queue_cap)
+                                                  u32 *__restrict__ queue_n, u64 queue_cap, int marks)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 lb = 0;
+        if (marks) {
+            u32 p = V[i];
+            if (!(p & LCP_UNKNOWN)) {
+                V[i] = (u32)(p + i);
+                continue;
+            }
+            lb = p & ~LCP_UNKNOWN;
+        }
         u64 k;
         if (PHI) k = PHI[i];
         else {
@@ -107,7 +116,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
         if (k >= n) { V[i] = (u32)i; continue; }
         u64 m = n - (i > k ? i : k); // also bounds GSA matches (they stop earlier)
         bool done = false;
-        u64 l = match_bytes(T, i, k, 0, m, gsa, PL_THREAD_LIMIT, &done);
+        u64 l = match_bytes(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
         if (done) {
             V[i] = (u32)(l + i);
         } else {
@@ -288,8 +297,58 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s)
     return SAGPU_OK;
 }
 
+namespace {
+
+__global__ __launch_bounds__(256) void k_scatter_marks(u32 *__restrict__ LCP, const u32 *__restrict__ SA, u64 n,
+                                                       u32 *__restrict__ PLCP)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        u32 p = SA[j];
+        u32 v = LCP[j];
+        if (!(v & LCP_UNKNOWN) && j > 0) {
+            // exact lcp of two suffixes whose padded keys first differ at v
+            u32 q = SA[j - 1];
+            u64 m = n - (u64)(p > q ? p : q);
+            if (v > m) {
+                v = (u32)m;
+                LCP[j] = v;
+            }
+        }
+        if (p < n) PLCP[p] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lcp_fix(const u32 *__restrict__ PLCP, const u32 *__restrict__ SA, u64 n,
+                                                 u32 *__restrict__ LCP)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        if (LCP[j] & LCP_UNKNOWN) {
+            u32 p = SA[j];
+            LCP[j] = p < n ? PLCP[p] : 0u;
+        }
+    }
+}
+
+} // namespace
+
+int launch_scatter_marks(u32 *LCP, const u32 *SA, u64 n, u32 *PLCP, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scatter_marks, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, LCP, SA, n, PLCP);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lcp_fix, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, PLCP, SA, n, LCP);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, hipStream_t s)
+                u32 *PLCP, int marks, hipStream_t s)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {
@@ -298,7 +357,7 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
     }
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
     hipLaunchKernelGGL(k_plcp_irr, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP,
-                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap);
+                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
     hipLaunchKernelGGL(k_plcp_long, dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP, ls.queue, ls.queue_l,
                        ls.queue_n, ls.queue_cap);
     hipLaunchKernelGGL(k_max_reduce, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, PLCP, n, ls.tile_max);

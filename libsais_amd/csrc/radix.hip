@@ -22,6 +22,7 @@ constexpr int RS_WAVE_ITEMS = SAGPU_WAVE * RS_ITEMS;
 constexpr int LB_VAL_BITS = 40;
 constexpr u64 LB_VAL_MASK = (1ull << LB_VAL_BITS) - 1ull;
 constexpr u64 LB_AGG = 1ull, LB_PRE = 2ull;
+constexpr int LB_WIN = 8;
 
 __device__ __forceinline__ u64 lb_pack(u32 epoch, u64 flag, u64 v)
 {
@@ -30,7 +31,7 @@ __device__ __forceinline__ u64 lb_pack(u32 epoch, u64 flag, u64 v)
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, u64 n, int begin_bit,
-                                                    int npass, u64 *__restrict__ hist)
+                                                    int end_bit, int npass, u64 *__restrict__ hist)
 {
     __shared__ u32 s_h[8][256];
     for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&s_h[0][0])[i] = 0;
@@ -39,7 +40,9 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         K k = keys[i];
         for (int p = 0; p < npass; p++) {
-            u32 d = (u32)(k >> (begin_bit + 8 * p)) & 255u;
+            const int sh = begin_bit + 8 * p;
+            const u32 dm = end_bit - sh >= 8 ? 255u : ((1u << (end_bit - sh)) - 1u);
+            u32 d = (u32)(k >> sh) & dm;
             atomicAdd(&s_h[p][d], 1u);
         }
     }
@@ -66,12 +69,15 @@ __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
     }
 }
 
-template <typename K, bool HAS_VALS>
-__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ kin, const u32 *__restrict__ vin,
-                                                         K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
-                                                         int shift, const u64 *__restrict__ offsets,
-                                                         u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
-                                                         u32 epoch)
+// One radix pass over a tile of RS_TILE keys.  LOOKBACK = true: onesweep (tile
+// ticket, aggregate publish, decoupled look-back).  LOOKBACK = false: the tile's
+// global digit offsets were produced by an upsweep + scan (reduce-then-scan).
+template <typename K, bool HAS_VALS, bool LOOKBACK>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
+                                                           K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
+                                                           int shift, u32 dmask, const u64 *__restrict__ offsets,
+                                                           u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
+                                                           u32 epoch, const u32 *__restrict__ scanned, u64 tiles)
 {
     __shared__ u32 s_tile;
     __shared__ u32 s_wcnt[RS_WAVES][256];
@@ -82,12 +88,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
     __shared__ uint8_t s_dig[RS_TILE];
 
     const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
-    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    if (LOOKBACK) {
+        if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    }
 #pragma unroll
     for (int q = 0; q < RS_WAVES; q++) s_wcnt[q][tid] = 0;
     __syncthreads();
 
-    const u64 tile = s_tile;
+    const u64 tile = LOOKBACK ? (u64)s_tile : (u64)blockIdx.x;
     const u64 wbase = tile * RS_TILE + (u64)w * RS_WAVE_ITEMS;
 
     K key[RS_ITEMS];
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
     for (int j = 0; j < RS_ITEMS; j++) {
         u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
         bool ok = pos < n;
-        u32 d = (u32)(key[j] >> shift) & 255u;
+        u32 d = (u32)(key[j] >> shift) & dmask;
         u64 m = __ballot(ok);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -122,7 +130,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
     }
     __syncthreads();
 
-    // Per digit: wave offsets, tile count, publish, look back.
+    // Per digit: wave offsets, tile count, global offset.
     {
         const u32 d = tid;
         u32 run = 0;
@@ -132,31 +140,46 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
             s_wcnt[q][d] = run;
             run += c;
         }
-        u64 *my = lookback + tile * 256 + d;
-        if (tile == 0) st_release_u64(my, lb_pack(epoch, LB_PRE, run));
-        else st_release_u64(my, lb_pack(epoch, LB_AGG, run));
-
-        // tile-local exclusive digit starts
         u32 inc = wave_incl_sum(run);
         if (lane == SAGPU_WAVE - 1) s_wsum[w] = inc;
 
-        u64 excl = 0;
-        if (tile > 0) {
-            u64 t = tile - 1;
-            for (;;) {
-                const u64 *src = lookback + t * 256 + d;
-                u64 v = ld_acquire_u64(src);
-                while ((u32)(v >> 42) != epoch || ((v >> LB_VAL_BITS) & 3ull) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    v = ld_acquire_u64(src);
+        u64 gbase;
+        if (LOOKBACK) {
+            u64 *my = lookback + tile * 256 + d;
+            if (tile == 0) st_release_u64(my, lb_pack(epoch, LB_PRE, run));
+            else st_release_u64(my, lb_pack(epoch, LB_AGG, run));
+            u64 excl = 0;
+            if (tile > 0) {
+                // Windowed look-back: the loads for up to LB_WIN predecessors are
+                // issued together so their latencies overlap; entries are consumed
+                // from the nearest one until an inclusive prefix is found.
+                long long t = (long long)tile - 1;
+                bool found = false;
+                while (!found) {
+                    const int cnt = t + 1 < LB_WIN ? (int)(t + 1) : LB_WIN;
+                    u64 v[LB_WIN];
+#pragma unroll
+                    for (int q = 0; q < LB_WIN; q++)
+                        if (q < cnt) v[q] = ld_acquire_u64(lookback + (u64)(t - q) * 256 + d);
+#pragma unroll
+                    for (int q = 0; q < LB_WIN; q++) {
+                        if (q >= cnt || found) continue;
+                        while ((u32)(v[q] >> 42) != epoch || ((v[q] >> LB_VAL_BITS) & 3ull) == 0) {
+                            __builtin_amdgcn_s_sleep(1);
+                            v[q] = ld_acquire_u64(lookback + (u64)(t - q) * 256 + d);
+                        }
+                        excl += v[q] & LB_VAL_MASK;
+                        if (((v[q] >> LB_VAL_BITS) & 3ull) == LB_PRE) found = true;
+                    }
+                    t -= cnt;
                 }
-                excl += v & LB_VAL_MASK;
-                if (((v >> LB_VAL_BITS) & 3ull) == LB_PRE) break;
-                --t;
+                st_release_u64(my, lb_pack(epoch, LB_PRE, excl + run));
             }
-            st_release_u64(my, lb_pack(epoch, LB_PRE, excl + run));
+            gbase = offsets[d] + excl;
+        } else {
+            gbase = scanned[(u64)d * tiles + tile];
         }
-        s_gbase[d] = offsets[d] + excl;
+        s_gbase[d] = gbase;
         __syncthreads();
         u32 carry = 0;
         for (u32 q = 0; q < w; q++) carry += s_wsum[q];
@@ -169,7 +192,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
     for (int j = 0; j < RS_ITEMS; j++) {
         u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
         if (pos < n) {
-            u32 d = (u32)(key[j] >> shift) & 255u;
+            u32 d = (u32)(key[j] >> shift) & dmask;
             u32 lp = s_dstart[d] + s_wcnt[w][d] + rank[j];
             s_keys[lp] = key[j];
             s_dig[lp] = (uint8_t)d;
@@ -199,6 +222,29 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const K *__restrict__ k
     }
 }
 
+// Reduce-then-scan upsweep: per-tile digit counts, digit-major (counts[d*tiles+t]).
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rts_upsweep(const K *__restrict__ kin, u64 n, int shift,
+                                                            u32 dmask, u32 *__restrict__ counts, u64 tiles)
+{
+    __shared__ u32 s_h[RS_WAVES][256];
+    const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; q++) s_h[q][tid] = 0;
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        u64 pos = base + (u64)j * RS_THREADS + tid;
+        if (pos < n) atomicAdd(&s_h[w][(u32)(kin[pos] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    u32 c = 0;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; q++) c += s_h[q][tid];
+    counts[(u64)tid * tiles + blockIdx.x] = c;
+}
+
 template <typename K>
 int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
                     hipStream_t s, int *result_in_alt, int *passes_run)
@@ -208,7 +254,7 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
     const int npass = (end_bit - begin_bit + 7) / 8;
     const u64 tiles = (n + RS_TILE - 1) / RS_TILE;
-    if (tiles > rs.lb_tiles || npass > 8) {
+    if (tiles > rs.lb_tiles || npass > 8 || (rs.variant == RADIX_REDUCE_THEN_SCAN && tiles * 256 > rs.counts_cap)) {
         sagpu_set_error("radix scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
@@ -217,7 +263,7 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     {
         int blocks = (int)((n + 256 * 64 - 1) / (256 * 64));
         if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(blocks), dim3(256), 0, s, k0, n, begin_bit, npass, rs.hist);
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3(blocks), dim3(256), 0, s, k0, n, begin_bit, end_bit, npass, rs.hist);
         hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass);
         SAGPU_CHECK(hipGetLastError());
     }
@@ -235,19 +281,35 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
             if (rs.hist_host[p * 256 + d] == n) trivial = true;
         if (trivial) continue;
         rs.epoch = (rs.epoch + 1) & ((1u << 22) - 1);
-        if (rs.epoch == 0) {
+        if (rs.epoch == 0 && rs.variant != RADIX_REDUCE_THEN_SCAN) {
             SAGPU_TRY(launch_fill_u64(rs.lookback, rs.lb_tiles * 256, 0, s));
             rs.epoch = 1;
         }
+        const int shift = begin_bit + 8 * p;
+        const u32 dmask = end_bit - shift >= 8 ? 255u : ((1u << (end_bit - shift)) - 1u);
+        if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
+            hipLaunchKernelGGL(k_rts_upsweep<K>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n, shift,
+                               dmask, rs.counts, tiles);
+            SAGPU_TRY(scan_exclusive_u32(rs.scan, rs.counts, tiles * 256, s));
+        }
         if (rs.hook) rs.hook(rs.hook_user, 0);
-        if (v0)
-            hipLaunchKernelGGL((k_onesweep<K, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n,
-                               begin_bit + 8 * p, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
-                               rs.epoch);
-        else
-            hipLaunchKernelGGL((k_onesweep<K, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo,
-                               n, begin_bit + 8 * p, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
-                               rs.epoch);
+        if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
+            if (v0)
+                hipLaunchKernelGGL((k_radix_pass<K, true, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
+                                   vi, ko, vo, n, shift, dmask, nullptr, nullptr, nullptr, 0u, rs.counts, tiles);
+            else
+                hipLaunchKernelGGL((k_radix_pass<K, false, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
+                                   vi, ko, vo, n, shift, dmask, nullptr, nullptr, nullptr, 0u, rs.counts, tiles);
+        } else {
+            if (v0)
+                hipLaunchKernelGGL((k_radix_pass<K, true, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
+                                   vi, ko, vo, n, shift, dmask, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
+                                   rs.epoch, nullptr, tiles);
+            else
+                hipLaunchKernelGGL((k_radix_pass<K, false, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
+                                   vi, ko, vo, n, shift, dmask, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
+                                   rs.epoch, nullptr, tiles);
+        }
         SAGPU_CHECK(hipGetLastError());
         if (rs.hook) rs.hook(rs.hook_user, 1);
         K *tk = ki; ki = ko; ko = tk;

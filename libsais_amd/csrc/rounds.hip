@@ -1,0 +1,524 @@
+// rounds.hip -- grouping after the initial sort and the prefix-doubling rounds.
+//
+// State between rounds is the active list: act_i[k] (suffix index) and act_g[k]
+// (SA slot of the first element of its group), with every group contiguous.  The
+// list order across groups is free, so kernels reserve output ranges with one
+// atomic per workgroup.  ISA[i] is the rank of suffix i (its group's first slot,
+// the final position once it is alone).
+//
+// Round with offset h (all group members share their first h symbols):
+//   k_round_gather   key2[k] = rank of i+h (or a length code past the end), and
+//                    SA[g] = end of the group in the list (SA slots of unsorted
+//                    groups are free until their suffixes are placed)
+//   k_round_tile     groups of <= RT_SMALL members whose first element lies in a
+//                    2048-wide stretch of the list: ranked by key2 inside LDS,
+//                    regrouped, ISA/SA written, survivors emitted
+//   big path         larger groups: extracted, sorted by (group, key2) with the
+//                    radix sort, regrouped with a segmented scan (BigF)
+// A pair of neighbours separated in a round has lcp in [h, 2h); the slot of the
+// second one gets LCP_UNKNOWN | h so the LCP pass can start comparing at h.
+#include "engine.h"
+
+namespace {
+
+// ---------------------------------------------------- warp-striped segmented scan
+// A tile of 4096 elements is split into four 1024-element wave spans, each walked
+// as 16 rows of 64 consecutive elements, so every load and store of a row is
+// coalesced.  The scan carries one sum (output index of active elements) and two
+// max-scans (positions of the latest group starts).
+
+struct Agg {
+    u32 s, m1, m2;
+};
+
+__device__ __forceinline__ u32 umax(u32 a, u32 b) { return a > b ? a : b; }
+
+constexpr int WS_THREADS = 256, WS_ROWS = 16, WS_TILE = 4096, WS_SPAN = 1024, WS_WAVES = 4;
+
+template <class F>
+__global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__ tagg)
+{
+    __shared__ Agg s_w[WS_WAVES];
+    const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    u32 s = 0, m1 = 0, m2 = 0;
+    for (int r = 0; r < WS_ROWS; r++) {
+        u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        if (e < f.n) {
+            bool act;
+            u32 v1, v2;
+            f.scan_inputs(e, act, v1, v2);
+            s += act;
+            m1 = umax(m1, v1);
+            m2 = umax(m2, v2);
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d, SAGPU_WAVE);
+        m1 = umax(m1, __shfl_xor(m1, d, SAGPU_WAVE));
+        m2 = umax(m2, __shfl_xor(m2, d, SAGPU_WAVE));
+    }
+    if (lane == 0) s_w[w] = Agg{s, m1, m2};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Agg t{0, 0, 0};
+        for (int q = 0; q < WS_WAVES; q++) {
+            t.s += s_w[q].s;
+            t.m1 = umax(t.m1, s_w[q].m1);
+            t.m2 = umax(t.m2, s_w[q].m2);
+        }
+        tagg[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of tile aggregates (one workgroup).  Writes the grand total of
+// active elements and, when out_counter is given, reserves that many slots in the
+// output list: *base = atomicAdd(out_counter, total).
+__global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u32 *total, u32 *out_counter, u32 *base)
+{
+    __shared__ Agg s_w[16];
+    __shared__ Agg s_carry;
+    if (threadIdx.x == 0) s_carry = Agg{0, 0, 0};
+    __syncthreads();
+    for (u64 b0 = 0; b0 < tiles; b0 += 1024) {
+        u64 i = b0 + threadIdx.x;
+        Agg v = i < tiles ? tagg[i] : Agg{0, 0, 0};
+        u32 is = wave_incl_sum(v.s), i1 = wave_incl_max(v.m1), i2 = wave_incl_max(v.m2);
+        const u32 w = threadIdx.x / SAGPU_WAVE;
+        if (lane_id() == SAGPU_WAVE - 1) s_w[w] = Agg{is, i1, i2};
+        __syncthreads();
+        Agg c = s_carry;
+        for (u32 q = 0; q < w; q++) {
+            c.s += s_w[q].s;
+            c.m1 = umax(c.m1, s_w[q].m1);
+            c.m2 = umax(c.m2, s_w[q].m2);
+        }
+        u32 p1 = __shfl_up(i1, 1, SAGPU_WAVE), p2 = __shfl_up(i2, 1, SAGPU_WAVE);
+        if (lane_id() > 0) {
+            c.m1 = umax(c.m1, p1);
+            c.m2 = umax(c.m2, p2);
+        }
+        c.s += is - v.s;
+        if (i < tiles) tagg[i] = c;
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            Agg t = s_carry;
+            for (int q = 0; q < 16; q++) {
+                t.s += s_w[q].s;
+                t.m1 = umax(t.m1, s_w[q].m1);
+                t.m2 = umax(t.m2, s_w[q].m2);
+            }
+            s_carry = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *total = s_carry.s;
+        if (out_counter) *base = atomicAdd(out_counter, s_carry.s);
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restrict__ texcl, const u32 *__restrict__ base)
+{
+    __shared__ Agg s_w[WS_WAVES];
+    const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    bool act[WS_ROWS];
+    u32 v1[WS_ROWS], v2[WS_ROWS];
+    u32 s = 0, m1 = 0, m2 = 0;
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        act[r] = false;
+        v1[r] = v2[r] = 0;
+        if (e < f.n) f.scan_inputs(e, act[r], v1[r], v2[r]);
+        s += act[r];
+        m1 = umax(m1, v1[r]);
+        m2 = umax(m2, v2[r]);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d, SAGPU_WAVE);
+        m1 = umax(m1, __shfl_xor(m1, d, SAGPU_WAVE));
+        m2 = umax(m2, __shfl_xor(m2, d, SAGPU_WAVE));
+    }
+    if (lane == 0) s_w[w] = Agg{s, m1, m2};
+    __syncthreads();
+    Agg c = texcl[blockIdx.x];
+    for (u32 q = 0; q < w; q++) {
+        c.s += s_w[q].s;
+        c.m1 = umax(c.m1, s_w[q].m1);
+        c.m2 = umax(c.m2, s_w[q].m2);
+    }
+    const u32 ob = base ? *base : 0u;
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        u32 is = wave_incl_sum((u32)act[r]);
+        u32 i1 = umax(c.m1, wave_incl_max(v1[r]));
+        u32 i2 = umax(c.m2, wave_incl_max(v2[r]));
+        if (e < f.n) f.emit(e, act[r], ob + c.s + is - (u32)act[r], i1, i2);
+        c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
+        c.m1 = __shfl(i1, SAGPU_WAVE - 1, SAGPU_WAVE);
+        c.m2 = __shfl(i2, SAGPU_WAVE - 1, SAGPU_WAVE);
+    }
+}
+
+// ------------------------------------------------------- initial grouping
+// Keys are the packed first `ksym` symbols (u32, low ksym*bits bits), sorted.
+struct InitF {
+    const u32 *keys;
+    const u32 *vals;
+    u64 n;
+    u32 sepmask;     // GSA: low symbol slot; keys with it zero contain a separator
+    int bits, ksym, gsa;
+    u32 *SA, *ISA, *act_i, *act_g, *LCP;
+
+    __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
+    __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
+    {
+        u32 k = keys[e];
+        bool s = sep(k);
+        bool head = (e == 0) || s || keys[e - 1] != k;
+        bool tail = (e + 1 == n) || s || keys[e + 1] != k;
+        act = !(head && tail);
+        v1 = head ? (u32)e : 0u;
+        v2 = 0;
+    }
+    // Leading symbols two packed keys share, stopped at a separator in GSA mode.
+    __device__ u32 key_lcp(u32 k, u32 kp) const
+    {
+        u32 l = (u32)ksym;
+        u32 x = k ^ kp;
+        if (x) l = (u32)(__clz(x) - (32 - ksym * bits)) / (u32)bits;
+        if (gsa) {
+            for (int t = 0; t < ksym; t++) {
+                if (((k >> (bits * (ksym - 1 - t))) & sepmask) == 0) {
+                    l = (u32)t < l ? (u32)t : l;
+                    break;
+                }
+            }
+        }
+        return l;
+    }
+    __device__ void emit(u64 e, bool act, u32 o, u32 g, u32) const
+    {
+        u32 v = vals[e];
+        ISA[v] = g;
+        if (act) {
+            act_i[o] = v;
+            act_g[o] = g;
+        } else {
+            SA[e] = v;
+        }
+        if (LCP) {
+            u32 out = 0;
+            if (e > 0) {
+                // Raw key lcp: it is clamped to n - max(SA[j-1], SA[j]) of the final
+                // neighbours later, because a padded key can tie with a real one.
+                u32 k = keys[e], kp = keys[e - 1];
+                u32 l = key_lcp(k, kp);
+                out = (l >= (u32)ksym && !sep(k)) ? (LCP_UNKNOWN | (u32)ksym) : l;
+            }
+            LCP[e] = out;
+        }
+    }
+};
+
+// ------------------------------------------------------- big-group emission
+struct BigF {
+    const u64 *comp;     // (dense big-group index << 32) | key2, sorted
+    const u32 *vals;     // suffix index
+    const u32 *gslot;    // first SA slot of each big group
+    u64 n;
+    u32 h;
+    u32 *SA, *ISA, *out_i, *out_g, *LCP;
+
+    __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
+    {
+        u64 c = comp[e];
+        u64 cp = e > 0 ? comp[e - 1] : ~0ull;
+        u64 cn = e + 1 < n ? comp[e + 1] : ~0ull;
+        bool head = (e == 0) || c != cp;
+        bool tail = (e + 1 == n) || c != cn;
+        bool gch = (e == 0) || (c >> 32) != (cp >> 32);
+        act = !(head && tail);
+        v1 = gch ? (u32)e : 0u;
+        v2 = head ? (u32)e : 0u;
+    }
+    __device__ void emit(u64 e, bool act, u32 o, u32 gstart, u32 hstart) const
+    {
+        u32 v = vals[e];
+        u64 c = comp[e];
+        u32 g = gslot[(u32)(c >> 32)];
+        u32 r = g + (hstart - gstart);
+        ISA[v] = r;
+        if (act) {
+            out_i[o] = v;
+            out_g[o] = r;
+        } else {
+            SA[r] = v;
+        }
+        if (LCP && hstart == (u32)e && hstart != gstart) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+    }
+};
+
+template <class F>
+int ws_run(WsScratch &ws, const F &f, u64 n, u32 *out_counter, hipStream_t s)
+{
+    const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
+    if (tiles > ws.tiles_cap) {
+        sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
+    hipLaunchKernelGGL(k_ws_reduce<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, out_counter, ws.base);
+    hipLaunchKernelGGL(k_ws_emit<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg,
+                       out_counter ? ws.base : (const u32 *)nullptr);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+// ------------------------------------------------------------- round kernels
+__global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
+                                                      u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
+                                                      u32 *__restrict__ key2, u32 *__restrict__ SA)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < a; k += stride) {
+        u32 i = act_i[k];
+        u32 g = act_g[k];
+        u64 ih = (u64)i + h;
+        key2[k] = ih < n ? (u32)(h + ISA[ih]) : (u32)(n - 1 - i);
+        if (k + 1 == a || act_g[k + 1] != g) SA[g] = (u32)(k + 1);
+    }
+}
+
+constexpr int RT_THREADS = 256, RT_RANGE = 2048, RT_CAP = RT_RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
+static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
+
+__global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
+                                                           const u32 *__restrict__ key2, u64 a, u32 h,
+                                                           u32 *__restrict__ SA, u32 *__restrict__ ISA,
+                                                           u32 *__restrict__ out_i, u32 *__restrict__ out_g,
+                                                           u32 *__restrict__ out_counter, u32 *__restrict__ big_head,
+                                                           u32 *__restrict__ big_g, u32 *__restrict__ big_size,
+                                                           u32 *__restrict__ big_counter, u64 big_cap,
+                                                           u32 *__restrict__ LCP)
+{
+    __shared__ u32 s_i[RT_CAP], s_k[RT_CAP], s_g[RT_CAP], s_x0[RT_CAP];
+    __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
+    __shared__ uint8_t s_os[RT_CAP];
+    __shared__ u32 s_first, s_end, s_wsum[4], s_base;
+
+    const u32 tid = threadIdx.x;
+    const u64 lo = (u64)blockIdx.x * RT_RANGE;
+    const u64 hi = lo + RT_RANGE < a ? lo + RT_RANGE : a;
+    if (tid == 0) {
+        s_first = 0xFFFFFFFFu;
+        s_end = 0;
+    }
+    __syncthreads();
+
+    // 1. group heads in [lo, hi): small ones define the tile's element range, big
+    //    ones are handed to the big path.
+    for (u64 k = lo + tid; k < hi; k += RT_THREADS) {
+        u32 g = act_g[k];
+        if (k == 0 || act_g[k - 1] != g) {
+            u32 end = SA[g];
+            u32 size = end - (u32)k;
+            if (size <= RT_SMALL) {
+                atomicMin(&s_first, (u32)(k - lo));
+                atomicMax(&s_end, (u32)(end - lo));
+            } else {
+                u32 slot = atomicAdd(big_counter, 1u);
+                if (slot < big_cap) {
+                    big_head[slot] = (u32)k;
+                    big_g[slot] = g;
+                    big_size[slot] = size;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (s_first == 0xFFFFFFFFu) return;
+    const u32 first = s_first, len = s_end - s_first;
+    if (s_end < s_first || len > RT_CAP) {
+        // inconsistent group bookkeeping: report instead of overrunning LDS
+        if (tid == 0) atomicOr(out_counter + 2, 1u);
+        return;
+    }
+    const u64 gfirst = lo + first;
+
+    // 2. load the range; local group start by a max-scan over head positions
+    for (u32 x = tid; x < len; x += RT_THREADS) {
+        u64 k = gfirst + x;
+        s_i[x] = act_i[k];
+        s_k[x] = key2[k];
+        s_g[x] = act_g[k];
+    }
+    __syncthreads();
+    {
+        u32 loc[RT_PER];
+        u32 m = 0;
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            u32 x = tid * RT_PER + j;
+            bool head = x < len && (x == 0 || s_g[x - 1] != s_g[x]);
+            if (head) m = x;
+            loc[j] = m;
+        }
+        u32 inc = wave_incl_max(m);
+        if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
+        __syncthreads();
+        u32 c = 0;
+        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) c = umax(c, s_wsum[q]);
+        u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
+        if (lane_id() > 0) c = umax(c, prev);
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            u32 x = tid * RT_PER + j;
+            if (x < len) s_x0[x] = umax(c, loc[j]);
+        }
+    }
+    __syncthreads();
+
+    // 3. rank inside the group by key2 (count smaller, equal, equal-before)
+    for (u32 x = tid; x < len; x += RT_THREADS) s_os[x] = 0;
+    __syncthreads();
+    for (u32 x = tid; x < len; x += RT_THREADS) {
+        const u32 x0 = s_x0[x];
+        const u32 g = s_g[x];
+        const u32 gsz = SA[g] - (u32)(gfirst + x0); // group end was stored at its first slot
+        if (gsz > RT_SMALL) continue;              // big group: not ours
+        const u32 key = s_k[x];
+        u32 lt = 0, eq = 0, eqb = 0;
+        for (u32 y = x0; y < x0 + gsz; y++) {
+            u32 ky = s_k[y];
+            lt += ky < key;
+            eq += ky == key;
+            eqb += (ky == key) & (y < x);
+        }
+        const u32 pos = x0 + lt + eqb;
+        const u32 r = g + lt;
+        s_oi[pos] = s_i[x];
+        s_og[pos] = r;
+        s_os[pos] = eq > 1 ? 2 : 1;
+        if (LCP && lt > 0 && eqb == 0) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+    }
+    __syncthreads();
+    // ISA and SA writes happen after every thread has read SA[g] (group end marker).
+    for (u32 x = tid; x < len; x += RT_THREADS) {
+        u32 st = s_os[x];
+        if (st == 0) continue;
+        ISA[s_oi[x]] = s_og[x];
+        if (st == 1) SA[s_og[x]] = s_oi[x];
+    }
+
+    // 4. survivors, in sorted order, to the next active list
+    {
+        u32 cnt[RT_PER];
+        u32 tot = 0;
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            u32 x = tid * RT_PER + j;
+            cnt[j] = (x < len && s_os[x] == 2) ? 1u : 0u;
+            tot += cnt[j];
+        }
+        u32 inc = wave_incl_sum(tot);
+        __syncthreads();
+        if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
+        __syncthreads();
+        if (tid == 0) {
+            u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+            s_base = all ? atomicAdd(out_counter, all) : 0u;
+        }
+        __syncthreads();
+        u32 o = s_base + inc - tot;
+        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            u32 x = tid * RT_PER + j;
+            if (cnt[j]) {
+                out_i[o] = s_oi[x];
+                out_g[o] = s_og[x];
+                o++;
+            }
+        }
+    }
+}
+
+// Copies the members of the big groups (sorted by list position) into the sort
+// buffers: comp = (dense group index << 32) | key2, val = suffix index.
+__global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act_i, const u32 *__restrict__ key2,
+                                                     const u32 *__restrict__ bhead, const u32 *__restrict__ boff,
+                                                     u32 nb, u64 total, u64 *__restrict__ comp, u32 *__restrict__ vals)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        u32 lo = 0, hi = nb - 1;
+        while (lo < hi) {
+            u32 mid = (lo + hi + 1) >> 1;
+            if ((u64)boff[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        u64 k = (u64)bhead[lo] + (e - boff[lo]);
+        comp[e] = ((u64)lo << 32) | key2[k];
+        vals[e] = act_i[k];
+    }
+}
+
+unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
+{
+    u64 b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+} // namespace
+
+int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
+               u32 *act_i, u32 *act_g, u32 *LCP, hipStream_t s)
+{
+    InitF f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP};
+    return ws_run(ws, f, n, nullptr, s);
+}
+
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA, n, h,
+                       key2, SA);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int round_tile(const u32 *act_i, const u32 *act_g, const u32 *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
+               u32 *out_g, u32 *out_counter, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter, u64 big_cap,
+               u32 *LCP, hipStream_t s)
+{
+    u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
+    hipLaunchKernelGGL(k_round_tile, dim3(ceil_div_u(a, RT_RANGE)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, hh,
+                       SA, ISA, out_i, out_g, out_counter, big_head, big_g, big_size, big_counter, big_cap, LCP);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_extract(const u32 *act_i, const u32 *key2, const u32 *bhead, const u32 *boff, u32 nb, u64 total, u64 *comp,
+                u32 *vals, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_big_extract, dim3(grid_for(total, 256 * 4, 65535)), dim3(256), 0, s, act_i, key2, bhead, boff,
+                       nb, total, comp, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, u32 *SA, u32 *ISA,
+             u32 *out_i, u32 *out_g, u32 *out_counter, u32 *LCP, hipStream_t s)
+{
+    u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
+    BigF f{comp, vals, gslot, total, hh, SA, ISA, out_i, out_g, LCP};
+    return ws_run(ws, f, total, out_counter, s);
+}

@@ -88,6 +88,41 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_keys(const uint8_t *__restr
     }
 }
 
+// 32-bit variant of the packed keys (k*bits <= 32): same layout, u32 keys.
+__global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__restrict__ T, u64 n,
+                                                            const uint8_t *__restrict__ lut, int bits, int k,
+                                                            int gsa, u32 *__restrict__ keys, u32 *__restrict__ vals)
+{
+    __shared__ uint8_t s_lut[256];
+    __shared__ uint8_t s_c[PK_TILE + PK_HALO];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * PK_TILE;
+    for (int i = threadIdx.x; i < PK_TILE + PK_HALO; i += PK_THREADS) {
+        u64 p = base + i;
+        s_c[i] = p < n ? s_lut[T[p]] : 0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int j = 0; j < PK_ITEMS; j++) {
+        int li = j * PK_THREADS + threadIdx.x;
+        u64 p = base + li;
+        if (p >= n) break;
+        u32 key = 0;
+        bool sep = false;
+        for (int t = 0; t < k; t++) {
+            u32 c = s_c[li + t];
+            if (gsa) {
+                if (sep) c = 0;
+                sep = sep || c == 0;
+            }
+            key = (key << bits) | c;
+        }
+        keys[p] = key;
+        vals[p] = (u32)p;
+    }
+}
+
 // Doubling round: composite key (group head << 32) | key2, where
 //   key2 = h + ISA[i+h]   if i+h < n,
 //   key2 = n - 1 - i      otherwise (the suffix ends inside the shared prefix, so
@@ -145,6 +180,19 @@ int launch_pack_keys(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int 
         return SAGPU_EDEVICE;
     }
     hipLaunchKernelGGL(k_pack_keys, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
+                       keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
+                       hipStream_t s)
+{
+    if (k > PK_HALO || k * bits > 32) {
+        sagpu_set_error("pack32: bad symbol count", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
                        keys, vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;

@@ -151,7 +151,7 @@ def test_edge_cases_match_reference():
                 outs = []
                 for mod in (la, ref):
                     T = np.frombuffer(tbytes, np.uint8).copy()
-                    SA = np.full(2, 77, np.int32)
+                    SA = np.array([1, 0], np.int32)  # stays a valid index for the n == 1 lcp call
                     U = np.full(2, 77, np.uint8)
                     A = np.zeros(2, np.int32)
                     P = np.array([3, 4], np.int32)

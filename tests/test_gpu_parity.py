@@ -13,6 +13,31 @@ from tests import oracle_lib as ol
 pytestmark = pytest.mark.gpu
 
 
+def fused(T, gsa=False):
+    """sagpu_build32 on device buffers: (SA, U, primary, PLCP, LCP, freq)."""
+    lib = la.load()
+    n = len(T)
+    dT = lib.sagpu_malloc(n)
+    bufs = [lib.sagpu_malloc(4 * n), None if gsa else lib.sagpu_malloc(n), lib.sagpu_malloc(4 * n),
+            lib.sagpu_malloc(4 * n)]
+    assert lib.sagpu_memcpy_h2d(dT, T.ctypes.data, n) == 0
+    f64 = np.zeros(256, np.int64)
+    r = lib.sagpu_build32(dT, n, la.SAGPU_FLAG_GSA if gsa else 0, bufs[0], bufs[1], bufs[2], bufs[3],
+                          f64.ctypes.data)
+    assert r >= 0, la.last_error()
+    outs = []
+    for b, dt in zip(bufs, (np.int32, np.uint8, np.int32, np.int32)):
+        if b is None:
+            outs.append(None)
+            continue
+        o = np.empty(n, dt)
+        lib.sagpu_memcpy_d2h(o.ctypes.data, b, o.nbytes)
+        lib.sagpu_free(b)
+        outs.append(o)
+    lib.sagpu_free(dT)
+    return outs[0], outs[1], r, outs[2], outs[3], f64
+
+
 def _cases():
     rng = np.random.default_rng(1234)
     cases = []
@@ -28,6 +53,13 @@ def _cases():
     cases.append(("periodic17", workloads.periodic(100000, 17, 3)))
     cases.append(("fibonacci", workloads.fibonacci(100000)))
     cases.append(("dna_like_1m", workloads.dna_like(1 << 20, seed=2)))
+    for run in (255, 256, 257, 300, 2100):
+        T = workloads.iid(20000, seed=run, alphabet=b"ACGT")
+        T[5000:5000 + run] = 65
+        T[12000:12000 + run] = 65
+        cases.append((f"runs{run}", T))
+    rep = workloads.iid(3000, seed=99, alphabet=b"ACGT")
+    cases.append(("copies300", np.tile(rep, 300)))
     cases.append(("banana", np.frombuffer(b"banana", np.uint8).copy()))
     cases.append(("mississippi", np.frombuffer(b"mississippi", np.uint8).copy()))
     return cases
@@ -62,7 +94,16 @@ def test_sa_bwt_plcp_lcp(name, T):
 
     L = np.zeros(n, np.int32)
     assert la.libsais_lcp(P, SA, L, n) == 0
-    assert np.array_equal(L, ol.lcp(P_ref, SA_ref))
+    L_ref = ol.lcp(P_ref, SA_ref)
+    assert np.array_equal(L, L_ref)
+
+    if n >= 2:
+        fSA, fU, fidx, fP, fL, ff = fused(T)
+        assert np.array_equal(fSA, SA_ref)
+        assert fidx == idx_ref and np.array_equal(fU, U_ref)
+        assert np.array_equal(fP, P_ref)
+        assert np.array_equal(fL, L_ref)
+        assert np.array_equal(ff, ol.freq(T))
 
 
 def _gsa_set(rng, count, maxlen, alphabet=b"ACGT"):
@@ -95,7 +136,12 @@ def test_gsa(name, T):
     assert np.array_equal(SA, SA_ref)
     P = np.zeros(n, np.int32)
     assert la.libsais_plcp_gsa(T, SA, P, n) == 0
-    assert np.array_equal(P, ol.plcp(T, SA_ref, gsa=True))
+    P_ref = ol.plcp(T, SA_ref, gsa=True)
+    assert np.array_equal(P, P_ref)
+    fSA, _, _, fP, fL, _ = fused(T, gsa=True)
+    assert np.array_equal(fSA, SA_ref)
+    assert np.array_equal(fP, P_ref)
+    assert np.array_equal(fL, ol.lcp(P_ref, SA_ref))
 
 
 def test_gsa_invalid_on_gpu():
