@@ -43,28 +43,6 @@ int radix_sort_u64(RadixScratch &rs, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, 
 int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit,
                    int end_bit, hipStream_t s, int *result_in_alt, int *passes_run);
 
-// ------------------------------------------------------- group segmentation
-// After a sort, mark group heads/tails, give every element its group head (SA
-// position of the group's first slot), write ISA for everyone, SA for retired
-// singletons, and compact the still-unsorted elements into the next active list.
-enum { SG_THREADS = 256, SG_ITEMS = 16, SG_TILE = SG_THREADS * SG_ITEMS };
-
-struct SegScratch {
-    u32 *tile_cnt;    // [tiles] active count per tile (scanned in place)
-    u32 *tile_head;   // [tiles] max head position per tile (scanned in place)
-    u32 *total;       // [1] number of still-active elements
-    u64 tiles_cap;
-};
-
-// Initial grouping: keys sorted, vals = suffix index.  sepmask != 0 marks GSA keys
-// that contain a separator (low symbol slot zero) as singletons.
-int seg_init(SegScratch &ss, const u64 *keys, const u32 *vals, u64 n, u64 sepmask, u32 *SA,
-             u32 *ISA, u32 *act_i, u32 *act_g, u32 *act_p, hipStream_t s);
-// Round grouping: comp sorted (old group << 32 | key2), vals = suffix index,
-// act_p = SA slot of each active position (unchanged by the sort).
-int seg_round(SegScratch &ss, const u64 *comp, const u32 *vals, const u32 *act_p, u64 a, u32 *SA,
-              u32 *ISA, u32 *nact_i, u32 *nact_g, u32 *nact_p, hipStream_t s);
-
 // ------------------------------------------------------- rounds (rounds.hip)
 // LCP marks written during construction: LCP_UNKNOWN | lower bound for pairs that
 // were separated by a doubling round (their lcp is resolved afterwards).
@@ -93,13 +71,8 @@ int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, 
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
-int launch_pack_keys(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u64 *keys,
-                     u32 *vals, hipStream_t s);
 int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
                        hipStream_t s);
-int launch_round_keys(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h,
-                      u64 *comp, u32 *vals, hipStream_t s);
-
 // ------------------------------------------------------------ BWT / PLCP / LCP
 struct LcpScratch {
     u32 *queue;       // [cap] positions whose lcp did not finish in the per-thread pass
@@ -112,15 +85,11 @@ struct LcpScratch {
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
 int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
-// PLCP from either PHI (isa == nullptr) or ISA+SA (phi computed on the fly).
-// marks = 1: PLCP already holds exact values or LCP_UNKNOWN | lower bound for
-// every position (scattered from the construction's LCP marks); only unknown
-// positions are compared.
+// PLCP from either PHI (ISA == nullptr) or ISA+SA (phi computed on the fly).
+// marks (SA order, with ISA+SA only): exact lcps or LCP_UNKNOWN | lower bound from
+// the construction; position i reads marks[ISA[i]] and only unknown ones compare.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
-                const u32 *SA, u32 *PLCP, int marks, hipStream_t s);
-// PLCP[SA[j]] = LCP[j] for all j (marks included); exact values are first clamped
-// to n - max(SA[j-1], SA[j]) (written back to LCP).
-int launch_scatter_marks(u32 *LCP, const u32 *SA, u64 n, u32 *PLCP, hipStream_t s);
+                const u32 *SA, u32 *PLCP, const u32 *marks, hipStream_t s);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);

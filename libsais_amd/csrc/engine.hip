@@ -447,7 +447,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 }
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-             u32 *PLCP, int marks)
+             u32 *PLCP, const u32 *marks)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -511,8 +511,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (want_lcp) {
             {
                 Span sp(c, F_PLCP);
-                SAGPU_TRY(launch_scatter_marks(marks, dSA, n, dPLCP, c.stream));
-                SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, 1));
+                SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks));
             }
             if (dLCP) {
                 Span sp(c, F_LCP);
@@ -663,7 +662,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
     u32 *dP = sc.get<u32>((u64)n);
     if (!dPHI || !dP) return SAGPU_ENOMEM;
     SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
-    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, 0));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, nullptr));
     SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -933,7 +932,7 @@ extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t
     u32 *dPHI = sc.get<u32>((u64)n);
     if (!dPHI) return SAGPU_ENOMEM;
     SAGPU_TRY(launch_phi((const u32 *)d_SA, (u64)n, dPHI, c->stream));
-    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP, 0));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP, nullptr));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }

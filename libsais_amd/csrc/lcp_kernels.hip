@@ -87,23 +87,26 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                                                   const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                   const u32 *__restrict__ SA, u32 *__restrict__ V,
                                                   u32 *__restrict__ queue, u32 *__restrict__ queue_l,
-                                                  u32 *__restrict__ queue_n, u64 queue_cap, int marks)
+                                                  u32 *__restrict__ queue_n, u64 queue_cap,
+                                                  const u32 *__restrict__ marks)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 lb = 0;
-        if (marks) {
-            u32 p = V[i];
-            if (!(p & LCP_UNKNOWN)) {
-                V[i] = (u32)(p + i);
-                continue;
-            }
-            lb = p & ~LCP_UNKNOWN;
-        }
         u64 k;
-        if (PHI) k = PHI[i];
-        else {
+        if (PHI) {
+            k = PHI[i];
+        } else {
             u32 r = ISA[i];
+            if (marks) {
+                // construction already knows most lcps: read it where i sits in SA
+                u32 p = r < n ? marks[r] : 0u;
+                if (!(p & LCP_UNKNOWN)) {
+                    V[i] = (u32)(p + i);
+                    continue;
+                }
+                lb = p & ~LCP_UNKNOWN;
+            }
             k = r == 0 ? n : SA[r - 1];
         }
         if (k > n) k = n;
@@ -299,26 +302,6 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s)
 
 namespace {
 
-__global__ __launch_bounds__(256) void k_scatter_marks(u32 *__restrict__ LCP, const u32 *__restrict__ SA, u64 n,
-                                                       u32 *__restrict__ PLCP)
-{
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        u32 p = SA[j];
-        u32 v = LCP[j];
-        if (!(v & LCP_UNKNOWN) && j > 0) {
-            // exact lcp of two suffixes whose padded keys first differ at v
-            u32 q = SA[j - 1];
-            u64 m = n - (u64)(p > q ? p : q);
-            if (v > m) {
-                v = (u32)m;
-                LCP[j] = v;
-            }
-        }
-        if (p < n) PLCP[p] = v;
-    }
-}
-
 __global__ __launch_bounds__(256) void k_lcp_fix(const u32 *__restrict__ PLCP, const u32 *__restrict__ SA, u64 n,
                                                  u32 *__restrict__ LCP)
 {
@@ -333,13 +316,6 @@ __global__ __launch_bounds__(256) void k_lcp_fix(const u32 *__restrict__ PLCP, c
 
 } // namespace
 
-int launch_scatter_marks(u32 *LCP, const u32 *SA, u64 n, u32 *PLCP, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_scatter_marks, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, LCP, SA, n, PLCP);
-    SAGPU_CHECK(hipGetLastError());
-    return SAGPU_OK;
-}
-
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)
 {
     hipLaunchKernelGGL(k_lcp_fix, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, PLCP, SA, n, LCP);
@@ -348,7 +324,7 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
 }
 
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, int marks, hipStream_t s)
+                u32 *PLCP, const u32 *marks, hipStream_t s)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {

@@ -185,6 +185,13 @@ struct InitF {
         v1 = head ? (u32)e : 0u;
         v2 = 0;
     }
+    // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
+    __device__ u32 pad_tail(u32 k) const
+    {
+        u32 t = 0;
+        while (t < (u32)ksym && ((k >> (bits * t)) & sepmask) == 0) t++;
+        return t;
+    }
     // Leading symbols two packed keys share, stopped at a separator in GSA mode.
     __device__ u32 key_lcp(u32 k, u32 kp) const
     {
@@ -214,11 +221,18 @@ struct InitF {
         if (LCP) {
             u32 out = 0;
             if (e > 0) {
-                // Raw key lcp: it is clamped to n - max(SA[j-1], SA[j]) of the final
-                // neighbours later, because a padded key can tie with a real one.
                 u32 k = keys[e], kp = keys[e - 1];
                 u32 l = key_lcp(k, kp);
-                out = (l >= (u32)ksym && !sep(k)) ? (LCP_UNKNOWN | (u32)ksym) : l;
+                if (l >= (u32)ksym && !sep(k)) {
+                    out = LCP_UNKNOWN | (u32)ksym;
+                } else if (!gsa && (pad_tail(k) + l >= (u32)ksym || pad_tail(kp) + l >= (u32)ksym)) {
+                    // One of the two groups may hold a suffix that ends within the
+                    // first l symbols (its key is padded with code 0 there); its true
+                    // lcp is then shorter, so leave it to the LCP pass with bound l.
+                    out = LCP_UNKNOWN | l;
+                } else {
+                    out = l;
+                }
             }
             LCP[e] = out;
         }

@@ -46,49 +46,14 @@ __global__ __launch_bounds__(256) void k_histogram(const uint8_t *__restrict__ T
     if (gsa && nbad) atomicAdd(bad, nbad);
 }
 
-// Packed keys of the first k symbols of every suffix: symbol t of suffix i sits in
-// bits [bits*(k-1-t), bits*(k-t)).  Positions past the end read code 0, the
-// smallest code; ties that this creates are broken by the doubling rounds, where a
-// suffix shorter than the offset ranks by its length.  In GSA mode (separator = code
-// 0, letters >= 1) every symbol after the first separator is forced to 0, so keys
-// that contain a separator end in a zero slot and are singletons.
 constexpr int PK_THREADS = 256, PK_ITEMS = 16, PK_TILE = PK_THREADS * PK_ITEMS, PK_HALO = 64;
 
-__global__ __launch_bounds__(PK_THREADS) void k_pack_keys(const uint8_t *__restrict__ T, u64 n,
-                                                          const uint8_t *__restrict__ lut, int bits, int k,
-                                                          int gsa, u64 *__restrict__ keys, u32 *__restrict__ vals)
-{
-    __shared__ uint8_t s_lut[256];
-    __shared__ uint8_t s_c[PK_TILE + PK_HALO];
-    s_lut[threadIdx.x] = lut[threadIdx.x];
-    __syncthreads();
-    const u64 base = (u64)blockIdx.x * PK_TILE;
-    for (int i = threadIdx.x; i < PK_TILE + PK_HALO; i += PK_THREADS) {
-        u64 p = base + i;
-        s_c[i] = p < n ? s_lut[T[p]] : 0;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int j = 0; j < PK_ITEMS; j++) {
-        int li = j * PK_THREADS + threadIdx.x;
-        u64 p = base + li;
-        if (p >= n) break;
-        u64 key = 0;
-        bool sep = false;
-        for (int t = 0; t < k; t++) {
-            u32 c = s_c[li + t];
-            if (gsa) {
-                if (sep) c = 0;
-                sep = sep || c == 0;
-            }
-            key = (key << bits) | c;
-        }
-        keys[p] = key;
-        vals[p] = (u32)p;
-    }
-}
-
-// 32-bit variant of the packed keys (k*bits <= 32): same layout, u32 keys.
+// Packed keys of the first k symbols of every suffix (k*bits <= 32): symbol t of
+// suffix i sits in bits [bits*(k-1-t), bits*(k-t)).  Positions past the end read
+// code 0, the smallest code; ties that this creates are broken by the doubling
+// rounds, where a suffix shorter than the offset ranks by its length.  In GSA mode
+// (separator = code 0, letters >= 1) every symbol after the first separator is
+// forced to 0, so keys that contain a separator end in a zero slot.
 __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__restrict__ T, u64 n,
                                                             const uint8_t *__restrict__ lut, int bits, int k,
                                                             int gsa, u32 *__restrict__ keys, u32 *__restrict__ vals)
@@ -123,25 +88,6 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__res
     }
 }
 
-// Doubling round: composite key (group head << 32) | key2, where
-//   key2 = h + ISA[i+h]   if i+h < n,
-//   key2 = n - 1 - i      otherwise (the suffix ends inside the shared prefix, so
-//                         it precedes every longer group member; shorter first).
-__global__ __launch_bounds__(256) void k_round_keys(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
-                                                    u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
-                                                    u64 *__restrict__ comp, u32 *__restrict__ vals)
-{
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < a; t += stride) {
-        u32 i = act_i[t];
-        u32 g = act_g[t];
-        u64 ih = (u64)i + h;
-        u32 key2 = ih < n ? (u32)(h + ISA[ih]) : (u32)(n - 1 - i);
-        comp[t] = ((u64)g << 32) | key2;
-        vals[t] = i;
-    }
-}
-
 __global__ void k_fill_u32(u32 *p, u64 count, u32 v)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -172,19 +118,6 @@ int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_ba
     return SAGPU_OK;
 }
 
-int launch_pack_keys(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u64 *keys, u32 *vals,
-                     hipStream_t s)
-{
-    if (k > PK_HALO) {
-        sagpu_set_error("pack: k exceeds halo", __FILE__, __LINE__);
-        return SAGPU_EDEVICE;
-    }
-    hipLaunchKernelGGL(k_pack_keys, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
-                       keys, vals);
-    SAGPU_CHECK(hipGetLastError());
-    return SAGPU_OK;
-}
-
 int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
                        hipStream_t s)
 {
@@ -194,15 +127,6 @@ int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, in
     }
     hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
                        keys, vals);
-    SAGPU_CHECK(hipGetLastError());
-    return SAGPU_OK;
-}
-
-int launch_round_keys(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u64 *comp, u32 *vals,
-                      hipStream_t s)
-{
-    hipLaunchKernelGGL(k_round_keys, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA, n, h,
-                       comp, vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
