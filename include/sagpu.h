@@ -55,6 +55,19 @@ SAGPU_API void    sagpu_trim(void);
 SAGPU_API int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA, uint8_t *d_U,
                       int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq);
 
+/*
+ * Batched generalized suffix arrays: d_T holds `count` strings, each terminated by a
+ * 0 byte; string k occupies [h_offsets[k], h_offsets[k+1]) (host array of count+1
+ * entries, h_offsets[0] = 0, h_offsets[count] = n, no empty strings).  For every
+ * string the result equals libsais_gsa on that string alone (SA of length
+ * h_offsets[k+1]-h_offsets[k], local indices) and, when d_LCP is given,
+ * libsais_plcp_gsa + libsais_lcp on it; both are written into
+ * d_SA[h_offsets[k] ..] and d_LCP[h_offsets[k] ..].  One pipeline run serves all
+ * strings: keys carry the string id, so groups never span strings.
+ */
+SAGPU_API int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_t *h_offsets, int32_t count,
+                                    int32_t *d_SA, int32_t *d_LCP);
+
 /* PLCP from a device-resident text and suffix array (libsais_plcp semantics). */
 SAGPU_API int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags);
 /* LCP[i] = PLCP[SA[i]] on device buffers (libsais_lcp semantics). */

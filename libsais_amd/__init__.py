@@ -108,6 +108,7 @@ ABI_SAGPU = [
     ("sagpu_set_memory_limit", None, [ctypes.c_uint64]),
     ("sagpu_trim", None, []),
     ("sagpu_build32", _i64, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    ("sagpu_gsa_batch32", _i32, [_vp, _i64, _vp, _i32, _vp, _vp]),
     ("sagpu_plcp32", _i32, [_vp, _vp, _vp, _i64, _i32]),
     ("sagpu_lcp32", _i32, [_vp, _vp, _vp, _i64]),
     ("sagpu_sort_pairs_u32", _i32, [_vp, _vp, _i64, _i32, _i32]),

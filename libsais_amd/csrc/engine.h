@@ -57,8 +57,12 @@ struct WsScratch {
     u32 *base;        // [1] reserved output base
 };
 
+// G: scratch of n entries (group head per sorted position, SA order).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, hipStream_t s);
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
+// Batched GSA: keys = (string id << 32) | packed symbols.
+int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,
                  hipStream_t s);
 int round_tile(const u32 *act_i, const u32 *act_g, const u32 *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
@@ -73,6 +77,16 @@ int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, 
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
 int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
                        hipStream_t s);
+// Batched GSA: sid[p] = number of 0 bytes before p (string id); keys get sid in
+// their high half.  launch_string_ids needs a ScanScratch for n entries.
+int launch_string_ids(const uint8_t *T, u64 n, u32 *sid, ScanScratch &sc, hipStream_t s);
+int launch_pack_keys64_sid(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, const u32 *sid, u64 *keys,
+                           u32 *vals, hipStream_t s);
+// Checks that string k starts at offsets[k] (sid[offsets[k]] == k and the byte
+// before it is a separator); sets *bad otherwise.
+int launch_check_offsets(const uint8_t *T, const u32 *sid, const int64_t *offsets, u32 count, u32 *bad, hipStream_t s);
+// SA_local[j] = SA[j] - offsets[sid[j]] (slot j lies in the same string block).
+int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n, u32 *out, hipStream_t s);
 // ------------------------------------------------------------ BWT / PLCP / LCP
 struct LcpScratch {
     u32 *queue;       // [cap] positions whose lcp did not finish in the per-thread pass

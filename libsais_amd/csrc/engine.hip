@@ -286,7 +286,8 @@ int read_u32(Ctx &c, const u32 *d, u32 *out)
 // return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
 // optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
 // previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
-int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP)
+int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
+             const u32 *sid = nullptr, u32 nstrings = 0)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
@@ -350,16 +351,27 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
 
-    {
-        Span sp(c, F_INITIAL);
-        SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, (u32 *)kb0, vb0, s));
-    }
     int alt = 0;
-    SAGPU_TRY(sort_u32(c, (u32 *)kb0, vb0, (u32 *)kb1, vb1, n, 0, ksym * bits, &alt));
-    {
+    if (!sid) {
+        {
+            Span sp(c, F_INITIAL);
+            SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, (u32 *)kb0, vb0, s));
+        }
+        SAGPU_TRY(sort_u32(c, (u32 *)kb0, vb0, (u32 *)kb1, vb1, n, 0, ksym * bits, &alt));
         Span sp(c, F_INITIAL);
+        // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? (u32 *)kb1 : (u32 *)kb0, alt ? vb1 : vb0, n, bits, ksym, gsa, dSA, dISA,
-                             act_i[0], act_g[0], dLCP, s));
+                             act_i[0], act_g[0], dLCP, alt ? (u32 *)kb0 : (u32 *)kb1, s));
+    } else {
+        // batched GSA: (string id, symbols) keys, so every group stays inside one string
+        {
+            Span sp(c, F_INITIAL);
+            SAGPU_TRY(launch_pack_keys64_sid(dT, n, d_lut, bits, ksym, sid, kb0, vb0, s));
+        }
+        SAGPU_TRY(sort_u64(c, kb0, vb0, kb1, vb1, n, 0, 32 + bits_for(nstrings ? nstrings - 1 : 0), &alt));
+        Span sp(c, F_INITIAL);
+        SAGPU_TRY(group_init64(ws, alt ? kb1 : kb0, alt ? vb1 : vb0, n, bits, ksym, dSA, dISA, act_i[0], act_g[0],
+                               dLCP, (u32 *)(alt ? kb0 : kb1), s));
     }
     u32 a = 0;
     SAGPU_TRY(read_u32(c, ws.total, &a));
@@ -945,5 +957,61 @@ extern "C" int32_t sagpu_lcp32(const int32_t *d_PLCP, const int32_t *d_SA, int32
     SAGPU_TRY(ctx_get(&c));
     SAGPU_TRY(launch_lcp((const u32 *)d_PLCP, (const u32 *)d_SA, (u64)n, (u32 *)d_LCP, c->stream));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_t *h_offsets, int32_t count,
+                                     int32_t *d_SA, int32_t *d_LCP)
+{
+    if (!d_T || !h_offsets || !d_SA || count < 1 || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
+    if (h_offsets[0] != 0 || h_offsets[count] != n) return SAGPU_EINVAL;
+    for (int32_t q = 0; q < count; q++)
+        if (h_offsets[q + 1] <= h_offsets[q] + 1) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    stats_reset(*c);
+    Scope sc(*c);
+    hipStream_t s = c->stream;
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    u32 *sid = sc.get<u32>((u64)n);
+    int64_t *d_off = sc.get<int64_t>((u64)count + 1);
+    u32 *d_bad = sc.get<u32>(64);
+    u32 *part = sc.get<u32>((u64)n / 4096 + 2);
+    if (!sid || !d_off || !d_bad || !part) return SAGPU_ENOMEM;
+    ScanScratch ssc{part, (u64)n / 4096 + 2};
+    SAGPU_TRY(launch_string_ids(dT, (u64)n, sid, ssc, s));
+    SAGPU_CHECK(hipMemcpyAsync(d_off, h_offsets, ((size_t)count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    SAGPU_TRY(launch_fill_u32(d_bad, 1, 0, s));
+    SAGPU_TRY(launch_check_offsets(dT, sid, d_off, (u32)count, d_bad, s));
+    u32 bad = 0;
+    SAGPU_TRY(read_u32(*c, d_bad, &bad));
+    if (bad) return SAGPU_EINVAL;
+
+    u32 *gSA = sc.get<u32>((u64)n), *ISA = sc.get<u32>((u64)n);
+    if (!gSA || !ISA) return SAGPU_ENOMEM;
+    u32 *marks = nullptr, *P = nullptr;
+    if (d_LCP) {
+        marks = (u32 *)d_LCP;
+        P = sc.get<u32>((u64)n);
+        if (!P) return SAGPU_ENOMEM;
+    }
+    {
+        Span total(*c, F_TOTAL);
+        int rc = sa_build(*c, sc, dT, (u64)n, 1, gSA, ISA, nullptr, marks, sid, (u32)count);
+        if (rc != SAGPU_OK) return rc;
+        if (d_LCP) {
+            {
+                Span sp(*c, F_PLCP);
+                SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, 1, nullptr, ISA, gSA, P, marks));
+            }
+            Span sp(*c, F_LCP);
+            SAGPU_TRY(launch_lcp_fix(P, gSA, (u64)n, (u32 *)d_LCP, s));
+        }
+        SAGPU_TRY(launch_localize(gSA, sid, d_off, (u64)n, (u32 *)d_SA, s));
+    }
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    timer_collect(*c);
     return 0;
 }

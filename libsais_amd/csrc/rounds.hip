@@ -165,20 +165,23 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
 }
 
 // ------------------------------------------------------- initial grouping
-// Keys are the packed first `ksym` symbols (u32, low ksym*bits bits), sorted.
+// Keys are the packed first `ksym` symbols in the low 32 bits (low ksym*bits
+// bits), sorted.  With K = u64 the high half holds the string id of batched GSA,
+// so groups never span two strings.
+template <typename K>
 struct InitF {
-    const u32 *keys;
+    const K *keys;
     const u32 *vals;
     u64 n;
     u32 sepmask;     // GSA: low symbol slot; keys with it zero contain a separator
     int bits, ksym, gsa;
-    u32 *SA, *ISA, *act_i, *act_g, *LCP;
+    u32 *SA, *G, *act_i, *act_g, *LCP;   // G[e] = group head (ISA is scattered later)
 
     __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
-        u32 k = keys[e];
-        bool s = sep(k);
+        K k = keys[e];
+        bool s = sep((u32)k);
         bool head = (e == 0) || s || keys[e - 1] != k;
         bool tail = (e + 1 == n) || s || keys[e + 1] != k;
         act = !(head && tail);
@@ -211,7 +214,7 @@ struct InitF {
     __device__ void emit(u64 e, bool act, u32 o, u32 g, u32) const
     {
         u32 v = vals[e];
-        ISA[v] = g;
+        G[e] = g;
         if (act) {
             act_i[o] = v;
             act_g[o] = g;
@@ -221,7 +224,9 @@ struct InitF {
         if (LCP) {
             u32 out = 0;
             if (e > 0) {
-                u32 k = keys[e], kp = keys[e - 1];
+                // a pair from two different strings (batched GSA) always has a
+                // separator suffix second, whose lcp is 0 by the separator rule
+                u32 k = (u32)keys[e], kp = (u32)keys[e - 1];
                 u32 l = key_lcp(k, kp);
                 if (l >= (u32)ksym && !sep(k)) {
                     out = LCP_UNKNOWN | (u32)ksym;
@@ -292,6 +297,19 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u32 *out_counter, hipStream_t s)
                        out_counter ? ws.base : (const u32 *)nullptr);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
+}
+
+// ISA[vals[e]] = G[e] for the destinations inside [lo, hi).  Called once per
+// 64 Mi-entry window: random 4-byte writes confined to 256 MiB are merged by the
+// memory-side cache instead of costing a partial-line write each.
+__global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals, const u32 *__restrict__ G, u64 n,
+                                                    u32 *__restrict__ ISA, u32 lo, u32 hi)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        u32 v = vals[e];
+        if (v >= lo && v < hi) ISA[v] = G[e];
+    }
 }
 
 // ------------------------------------------------------------- round kernels
@@ -493,11 +511,32 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 
 } // namespace
 
-int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, hipStream_t s)
+template <typename K>
+int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
 {
-    InitF f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP};
-    return ws_run(ws, f, n, nullptr, s);
+    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP};
+    SAGPU_TRY(ws_run(ws, f, n, nullptr, s));
+    const u64 win = 1ull << 26;
+    for (u64 lo = 0; lo < n; lo += win) {
+        u64 hi = lo + win < n ? lo + win : n;
+        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, (u32)lo,
+                           (u32)hi);
+    }
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
+{
+    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, G, s);
+}
+
+int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
+{
+    return group_init_t<u64>(ws, keys, vals, n, bits, ksym, 1, SA, ISA, act_i, act_g, LCP, G, s);
 }
 
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,

@@ -88,6 +88,63 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__res
     }
 }
 
+__global__ __launch_bounds__(256) void k_zero_flags(const uint8_t *__restrict__ T, u64 n, u32 *__restrict__ f)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) f[p] = T[p] == 0;
+}
+
+__global__ __launch_bounds__(PK_THREADS) void k_pack_keys64_sid(const uint8_t *__restrict__ T, u64 n,
+                                                                const uint8_t *__restrict__ lut, int bits, int k,
+                                                                const u32 *__restrict__ sid, u64 *__restrict__ keys,
+                                                                u32 *__restrict__ vals)
+{
+    __shared__ uint8_t s_lut[256];
+    __shared__ uint8_t s_c[PK_TILE + PK_HALO];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * PK_TILE;
+    for (int i = threadIdx.x; i < PK_TILE + PK_HALO; i += PK_THREADS) {
+        u64 p = base + i;
+        s_c[i] = p < n ? s_lut[T[p]] : 0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int j = 0; j < PK_ITEMS; j++) {
+        int li = j * PK_THREADS + threadIdx.x;
+        u64 p = base + li;
+        if (p >= n) break;
+        u32 key = 0;
+        bool sep = false;
+        for (int t = 0; t < k; t++) {
+            u32 c = s_c[li + t];
+            if (sep) c = 0;
+            sep = sep || c == 0;
+            key = (key << bits) | c;
+        }
+        keys[p] = ((u64)sid[p] << 32) | key;
+        vals[p] = (u32)p;
+    }
+}
+
+__global__ void k_check_offsets(const uint8_t *__restrict__ T, const u32 *__restrict__ sid,
+                                const int64_t *__restrict__ off, u32 count, u32 *bad)
+{
+    for (u32 q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        int64_t o = off[q];
+        bool ok = sid[o] == q && off[q + 1] > o + 1 && T[off[q + 1] - 1] == 0 && (q == 0 || T[o - 1] == 0);
+        if (!ok) atomicOr(bad, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_localize(const u32 *__restrict__ SA, const u32 *__restrict__ sid,
+                                                  const int64_t *__restrict__ off, u64 n, u32 *__restrict__ out)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+        out[j] = SA[j] - (u32)off[sid[j]];
+}
+
 __global__ void k_fill_u32(u32 *p, u64 count, u32 v)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -127,6 +184,41 @@ int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, in
     }
     hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
                        keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_string_ids(const uint8_t *T, u64 n, u32 *sid, ScanScratch &sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_flags, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, sid);
+    SAGPU_CHECK(hipGetLastError());
+    return scan_exclusive_u32(sc, sid, n, s);
+}
+
+int launch_pack_keys64_sid(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, const u32 *sid, u64 *keys,
+                           u32 *vals, hipStream_t s)
+{
+    if (k > PK_HALO || k * bits > 32) {
+        sagpu_set_error("pack64: bad symbol count", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    hipLaunchKernelGGL(k_pack_keys64_sid, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k,
+                       sid, keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_check_offsets(const uint8_t *T, const u32 *sid, const int64_t *offsets, u32 count, u32 *bad, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_check_offsets, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, T, sid, offsets, count,
+                       bad);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n, u32 *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_localize, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, SA, sid, offsets, n, out);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
