@@ -68,6 +68,55 @@ SAGPU_API int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, in
 SAGPU_API int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_t *h_offsets, int32_t count,
                                     int32_t *d_SA, int32_t *d_LCP);
 
+/*
+ * Serving.  A job is one text and the outputs it asks for (host buffers the caller
+ * keeps alive until the job completes):
+ *   SAGPU_JOB_SA   SA[0..n)                      (libsais)
+ *   SAGPU_JOB_BWT  SA[0..n), U[0..n), primary    (libsais + libsais_bwt)
+ *   SAGPU_JOB_LCP  SA[0..n), LCP[0..n)           (libsais + libsais_plcp + libsais_lcp)
+ * status: 1 while pending, then 0 or a libsais error code.  Times are seconds on
+ * sagpu_clock()'s monotonic clock.
+ */
+#define SAGPU_JOB_SA  0
+#define SAGPU_JOB_BWT 1
+#define SAGPU_JOB_LCP 2
+
+typedef struct sagpu_job {
+    const uint8_t *T;
+    int64_t n;
+    int32_t kind;
+    int32_t *SA;
+    uint8_t *U;
+    int32_t *LCP;
+    int64_t primary;
+    volatile int32_t status;
+    int32_t batched;        /* 1 when the job ran inside a multi-job batch */
+    double t_submit, t_start, t_done;
+    int32_t batchable;      /* set by the server: text has no 0 byte */
+    int32_t reserved_;
+} sagpu_job;
+
+/* Runs one job synchronously on the calling thread's default stream (the plain,
+ * one-job-at-a-time path). */
+SAGPU_API int32_t sagpu_run_job(sagpu_job *job);
+
+/*
+ * Continuous-batching server: `slots` worker streams, each with its own pooled HBM.
+ * A free slot takes the oldest waiting job; if that job is small (< 1 MiB, no 0
+ * bytes) it also takes the following small jobs, up to batch_bytes of text, and
+ * runs them as one batched build (string-id tagged keys).  Work is admitted only
+ * while the estimated device memory of everything running stays within hbm_budget
+ * (0 = 90% of the device's free memory); a job that can never fit fails with -2.
+ * Jobs that arrive while others run are admitted as soon as a slot and memory are
+ * free, and slots run concurrently on the device.
+ */
+SAGPU_API void   *sagpu_server_create(int32_t slots, uint64_t hbm_budget, int64_t batch_bytes);
+SAGPU_API int32_t sagpu_server_submit(void *server, sagpu_job *job);
+SAGPU_API int32_t sagpu_server_wait(void *server, sagpu_job *job);
+SAGPU_API int32_t sagpu_server_drain(void *server);
+SAGPU_API void    sagpu_server_destroy(void *server);
+SAGPU_API double  sagpu_clock(void);
+
 /* PLCP from a device-resident text and suffix array (libsais_plcp semantics). */
 SAGPU_API int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags);
 /* LCP[i] = PLCP[SA[i]] on device buffers (libsais_lcp semantics). */

@@ -109,6 +109,13 @@ ABI_SAGPU = [
     ("sagpu_trim", None, []),
     ("sagpu_build32", _i64, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("sagpu_gsa_batch32", _i32, [_vp, _i64, _vp, _i32, _vp, _vp]),
+    ("sagpu_run_job", _i32, [_vp]),
+    ("sagpu_server_create", _vp, [_i32, ctypes.c_uint64, _i64]),
+    ("sagpu_server_submit", _i32, [_vp, _vp]),
+    ("sagpu_server_wait", _i32, [_vp, _vp]),
+    ("sagpu_server_drain", _i32, [_vp]),
+    ("sagpu_server_destroy", None, [_vp]),
+    ("sagpu_clock", ctypes.c_double, []),
     ("sagpu_plcp32", _i32, [_vp, _vp, _vp, _i64, _i32]),
     ("sagpu_lcp32", _i32, [_vp, _vp, _vp, _i64]),
     ("sagpu_sort_pairs_u32", _i32, [_vp, _vp, _i64, _i32, _i32]),

@@ -85,6 +85,14 @@ int launch_pack_keys64_sid(const uint8_t *T, u64 n, const uint8_t *lut, int bits
 // Checks that string k starts at offsets[k] (sid[offsets[k]] == k and the byte
 // before it is a separator); sets *bad otherwise.
 int launch_check_offsets(const uint8_t *T, const u32 *sid, const int64_t *offsets, u32 count, u32 *bad, hipStream_t s);
+// Batched plain-text jobs: job k's text is followed by one separator and occupies
+// [off[k], off[k+1]); slot off[k] of the batched suffix array is that separator's
+// suffix, slots off[k]+1 .. off[k+1]-1 are job k's suffix array.  Packed outputs
+// of job k start at off[k] - k.  outLCP / outU may be null.
+int launch_batch_extract(const u32 *gSA, const u32 *gLCP, const u32 *sid, const int64_t *off, const u32 *ISA,
+                         const uint8_t *T, u64 N, u32 *outSA, u32 *outLCP, uint8_t *outU, hipStream_t s);
+// prim[k] = ISA[off[k]] - off[k]  (1-based primary index of job k's BWT)
+int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u32 *prim, hipStream_t s);
 // SA_local[j] = SA[j] - offsets[sid[j]] (slot j lies in the same string block).
 int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n, u32 *out, hipStream_t s);
 // ------------------------------------------------------------ BWT / PLCP / LCP

@@ -107,6 +107,8 @@ struct Ctx {
     Pool pool;
     RadixScratch rs{};
     u64 *pinned = nullptr;      // small pinned host staging (counts, histograms)
+    uint8_t *stage = nullptr;   // large pinned staging for batched jobs
+    size_t stage_cap = 0;
     Timer timer;
     sagpu_stats stats{};
 };
@@ -116,9 +118,10 @@ Ctx *g_ctx = nullptr;
 bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
 
-int ctx_get(Ctx **out)
+// A context is one stream with its own pooled buffers and scratch; calls on one
+// context are serialised, different contexts run concurrently on the device.
+int ctx_create(Ctx **out)
 {
-    if (g_ctx) { *out = g_ctx; return SAGPU_OK; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         (void)hipGetLastError();
@@ -131,6 +134,40 @@ int ctx_get(Ctx **out)
     SAGPU_CHECK(hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault));
     c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
     c->rs.variant = g_radix_variant;
+    *out = c;
+    return SAGPU_OK;
+}
+
+void ctx_destroy(Ctx *c)
+{
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->rs.hist) c->pool.put(c->rs.hist);
+    if (c->rs.tile_ctr) c->pool.put(c->rs.tile_ctr);
+    if (c->rs.lookback) c->pool.put(c->rs.lookback);
+    if (c->rs.counts) c->pool.put(c->rs.counts);
+    if (c->rs.scan.partials) c->pool.put(c->rs.scan.partials);
+    c->pool.trim();
+    for (auto &kv : c->pool.live) (void)hipFree(kv.first);
+    for (auto &r : c->timer.recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto &e : c->timer.free_ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (c->stage) (void)hipHostFree(c->stage);
+    (void)hipHostFree(c->pinned);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ctx_get(Ctx **out)
+{
+    if (g_ctx) { *out = g_ctx; return SAGPU_OK; }
+    Ctx *c;
+    SAGPU_TRY(ctx_create(&c));
     g_ctx = c;
     *out = c;
     return SAGPU_OK;
@@ -714,15 +751,7 @@ extern "C" void sagpu_shutdown(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_ctx) return;
-    (void)hipStreamSynchronize(g_ctx->stream);
-    if (g_ctx->rs.hist) g_ctx->pool.put(g_ctx->rs.hist);
-    if (g_ctx->rs.tile_ctr) g_ctx->pool.put(g_ctx->rs.tile_ctr);
-    if (g_ctx->rs.lookback) g_ctx->pool.put(g_ctx->rs.lookback);
-    g_ctx->pool.trim();
-    for (auto &kv : g_ctx->pool.live) (void)hipFree(kv.first);
-    (void)hipHostFree(g_ctx->pinned);
-    (void)hipStreamDestroy(g_ctx->stream);
-    delete g_ctx;
+    ctx_destroy(g_ctx);
     g_ctx = nullptr;
 }
 
@@ -1014,4 +1043,365 @@ extern "C" int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_
     SAGPU_CHECK(hipStreamSynchronize(s));
     timer_collect(*c);
     return 0;
+}
+
+// ======================================================== jobs and serving
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+namespace {
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device bytes one build of an n-byte text may hold at once (buffers of sa_build,
+// outputs and staging), used for admission.
+u64 job_footprint(u64 n) { return 72ull * n + (64ull << 20); }
+
+int run_tiny(sagpu_job *j)
+{
+    if (j->n == 1) {
+        j->SA[0] = 0;
+        if (j->kind == SAGPU_JOB_BWT) { j->U[0] = j->T[0]; j->primary = 1; }
+        if (j->kind == SAGPU_JOB_LCP) j->LCP[0] = 0;
+    } else if (j->kind == SAGPU_JOB_BWT) {
+        j->primary = 0;
+    }
+    return 0;
+}
+
+// One job on one context: SA (+ BWT or LCP) of a host text into host buffers.
+int run_single(Ctx &c, sagpu_job *j)
+{
+    if (j->n <= 1) return run_tiny(j);
+    const u64 n = (u64)j->n;
+    if (n > (u64)INT32_MAX) return SAGPU_ENOMEM;
+    stats_reset(c);
+    Scope sc(c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(c, sc, j->T, n, true, &dT));
+    GET(dSA, u32, n);
+    uint8_t *dU = nullptr;
+    u32 *dL = nullptr;
+    if (j->kind == SAGPU_JOB_BWT) {
+        dU = sc.get<uint8_t>(n);
+        if (!dU) return SAGPU_ENOMEM;
+    }
+    if (j->kind == SAGPU_JOB_LCP) {
+        dL = sc.get<u32>(n);
+        if (!dL) return SAGPU_ENOMEM;
+    }
+    int64_t r = build_dev(c, sc, dT, n, 0, dSA, dU, nullptr, dL, nullptr);
+    if (r < 0) return (int)r;
+    SAGPU_CHECK(hipMemcpyAsync(j->SA, dSA, n * 4, hipMemcpyDeviceToHost, c.stream));
+    if (dU) SAGPU_CHECK(hipMemcpyAsync(j->U, dU, n, hipMemcpyDeviceToHost, c.stream));
+    if (dL) SAGPU_CHECK(hipMemcpyAsync(j->LCP, dL, n * 4, hipMemcpyDeviceToHost, c.stream));
+    SAGPU_CHECK(hipStreamSynchronize(c.stream));
+    if (dU) j->primary = r;
+    return 0;
+}
+
+int ensure_stage(Ctx &c, size_t bytes)
+{
+    if (c.stage_cap >= bytes) return SAGPU_OK;
+    if (c.stage) (void)hipHostFree(c.stage);
+    c.stage = nullptr;
+    c.stage_cap = 0;
+    size_t cap = bytes + (bytes >> 2);
+    SAGPU_CHECK(hipHostMalloc((void **)&c.stage, cap, hipHostMallocDefault));
+    c.stage_cap = cap;
+    return SAGPU_OK;
+}
+
+// Several small jobs (texts without 0 bytes) as one batched build: every text is
+// followed by a separator and the keys carry the job id, so the batched suffix
+// array of job k, minus its separator suffix, is exactly job k's suffix array;
+// lcps stop at the separator, which is where job k's text ends.  Returns -1 when
+// a text turns out to contain a 0 byte (caller then runs the jobs one by one).
+int run_batch(Ctx &c, sagpu_job **jobs, int count)
+{
+    std::vector<int64_t> off((size_t)count + 1);
+    u64 N = 0;
+    bool any_u = false, any_l = false;
+    for (int q = 0; q < count; q++) {
+        off[q] = (int64_t)N;
+        N += (u64)jobs[q]->n + 1;
+        any_u |= jobs[q]->kind == SAGPU_JOB_BWT;
+        any_l |= jobs[q]->kind == SAGPU_JOB_LCP;
+    }
+    off[count] = (int64_t)N;
+    if (N > (u64)INT32_MAX) return SAGPU_ENOMEM;
+    const u64 M = N - (u64)count;  // packed output entries
+    SAGPU_TRY(ensure_stage(c, (size_t)(N + 8 * M + 64)));
+    uint8_t *hin = c.stage;
+    for (int q = 0; q < count; q++) {
+        memcpy(hin + off[q], jobs[q]->T, (size_t)jobs[q]->n);
+        hin[off[q + 1] - 1] = 0;
+    }
+    stats_reset(c);
+    Scope sc(c);
+    hipStream_t s = c.stream;
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(c, sc, hin, N, true, &dT));
+    GET(sid, u32, N);
+    GET(d_off, int64_t, (u64)count + 1);
+    GET(d_bad, u32, 64);
+    GET(part, u32, N / 4096 + 2);
+    ScanScratch ssc{part, N / 4096 + 2};
+    SAGPU_TRY(launch_string_ids(dT, N, sid, ssc, s));
+    SAGPU_CHECK(hipMemcpyAsync(d_off, off.data(), ((size_t)count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    SAGPU_TRY(launch_fill_u32(d_bad, 1, 0, s));
+    SAGPU_TRY(launch_check_offsets(dT, sid, d_off, (u32)count, d_bad, s));
+    u32 bad = 0;
+    SAGPU_TRY(read_u32(c, d_bad, &bad));
+    if (bad) return SAGPU_EINVAL;
+
+    GET(gSA, u32, N);
+    GET(ISA, u32, N);
+    u32 *marks = nullptr, *P = nullptr, *gL = nullptr;
+    if (any_l) {
+        marks = sc.get<u32>(N);
+        P = sc.get<u32>(N);
+        gL = marks;
+        if (!marks || !P) return SAGPU_ENOMEM;
+    }
+    SAGPU_TRY(sa_build(c, sc, dT, N, 1, gSA, ISA, nullptr, marks, sid, (u32)count));
+    if (any_l) {
+        SAGPU_TRY(plcp_run(c, sc, dT, N, 1, nullptr, ISA, gSA, P, marks));
+        SAGPU_TRY(launch_lcp_fix(P, gSA, N, gL, s));
+    }
+    GET(oSA, u32, M);
+    u32 *oL = nullptr;
+    uint8_t *oU = nullptr;
+    if (any_l) {
+        oL = sc.get<u32>(M);
+        if (!oL) return SAGPU_ENOMEM;
+    }
+    if (any_u) {
+        oU = sc.get<uint8_t>(M);
+        if (!oU) return SAGPU_ENOMEM;
+    }
+    GET(prim, u32, (u64)count);
+    SAGPU_TRY(launch_batch_extract(gSA, gL, sid, d_off, ISA, dT, N, oSA, oL, oU, s));
+    SAGPU_TRY(launch_batch_primary(ISA, d_off, (u32)count, prim, s));
+    uint8_t *hSA = c.stage + ((N + 63) & ~(u64)63);
+    uint8_t *hL = hSA + 4 * M;
+    uint8_t *hU = c.stage;  // input text no longer needed on the host side
+    std::vector<u32> hprim((size_t)count);
+    SAGPU_CHECK(hipMemcpyAsync(hSA, oSA, 4 * M, hipMemcpyDeviceToHost, s));
+    if (oL) SAGPU_CHECK(hipMemcpyAsync(hL, oL, 4 * M, hipMemcpyDeviceToHost, s));
+    if (oU) SAGPU_CHECK(hipMemcpyAsync(hU, oU, M, hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipMemcpyAsync(hprim.data(), prim, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < count; q++) {
+        sagpu_job *j = jobs[q];
+        const u64 po = (u64)off[q] - (u64)q, nk = (u64)j->n;
+        memcpy(j->SA, hSA + 4 * po, 4 * nk);
+        if (j->kind == SAGPU_JOB_LCP) memcpy(j->LCP, hL + 4 * po, 4 * nk);
+        if (j->kind == SAGPU_JOB_BWT) {
+            memcpy(j->U, hU + po, nk);
+            j->primary = hprim[q];
+        }
+        j->batched = 1;
+    }
+    return 0;
+}
+
+struct Server {
+    std::vector<Ctx *> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<sagpu_job *> queue;
+    u64 budget = 0, reserved = 0;
+    u64 batch_bytes = 0;
+    u64 small_max = 1ull << 20;
+    int batch_max_jobs = 8192;
+    bool stop = false;
+};
+
+void finish(Server &sv, std::vector<sagpu_job *> &take, int rc)
+{
+    double t = now_s();
+    for (sagpu_job *j : take) {
+        j->t_done = t;
+        __atomic_store_n(&j->status, rc, __ATOMIC_RELEASE);
+    }
+    sv.cv.notify_all();
+}
+
+void worker(Server *svp, int slot)
+{
+    Server &sv = *svp;
+    Ctx &c = *sv.ctxs[slot];
+    (void)hipSetDevice(c.device);
+    std::vector<sagpu_job *> take;
+    for (;;) {
+        u64 need = 0;
+        {
+            std::unique_lock<std::mutex> lk(sv.mu);
+            for (;;) {
+                if (sv.stop && sv.queue.empty()) return;
+                if (sv.queue.empty()) {
+                    sv.cv.wait(lk);
+                    continue;
+                }
+                take.clear();
+                sagpu_job *f = sv.queue.front();
+                u64 bytes = 0;
+                if (sv.batch_bytes && f->batchable && (u64)f->n < sv.small_max) {
+                    for (sagpu_job *j : sv.queue) {
+                        if (!j->batchable || (u64)j->n >= sv.small_max) break;
+                        if (!take.empty() && (bytes + (u64)j->n + 1 > sv.batch_bytes ||
+                                              (int)take.size() >= sv.batch_max_jobs))
+                            break;
+                        take.push_back(j);
+                        bytes += (u64)j->n + 1;
+                    }
+                } else {
+                    take.push_back(f);
+                    bytes = (u64)f->n + 1;
+                }
+                need = job_footprint(bytes);
+                if (need > sv.budget) {
+                    // can never be admitted: fail the oldest job alone
+                    sv.queue.pop_front();
+                    take.assign(1, f);
+                    lk.unlock();
+                    finish(sv, take, SAGPU_ENOMEM);
+                    lk.lock();
+                    continue;
+                }
+                if (sv.reserved + need > sv.budget) {
+                    sv.cv.wait(lk);
+                    continue;
+                }
+                for (size_t q = 0; q < take.size(); q++) sv.queue.pop_front();
+                sv.reserved += need;
+                break;
+            }
+        }
+        double t0 = now_s();
+        for (sagpu_job *j : take) j->t_start = t0;
+        int rc;
+        if (take.size() == 1) {
+            rc = run_single(c, take[0]);
+        } else {
+            rc = run_batch(c, take.data(), (int)take.size());
+            if (rc == SAGPU_EINVAL) {
+                rc = 0;
+                for (sagpu_job *j : take) {
+                    int r = run_single(c, j);
+                    if (r) rc = r;
+                }
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(sv.mu);
+            sv.reserved -= need;
+        }
+        finish(sv, take, rc < -2 ? SAGPU_ENOMEM : rc);
+    }
+}
+
+} // namespace
+
+extern "C" double sagpu_clock(void) { return now_s(); }
+
+extern "C" int32_t sagpu_run_job(sagpu_job *job)
+{
+    if (!job || !job->T || !job->SA || job->n < 0 || job->kind < 0 || job->kind > 2 ||
+        (job->kind == SAGPU_JOB_BWT && !job->U) || (job->kind == SAGPU_JOB_LCP && !job->LCP))
+        return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    job->t_start = now_s();
+    int rc = run_single(*c, job);
+    job->t_done = now_s();
+    job->status = rc < -2 ? SAGPU_ENOMEM : rc;
+    return job->status;
+}
+
+extern "C" void *sagpu_server_create(int32_t slots, uint64_t hbm_budget, int64_t batch_bytes)
+{
+    if (slots < 1 || slots > 16 || batch_bytes < 0) return nullptr;
+    Server *sv = new Server();
+    for (int q = 0; q < slots; q++) {
+        Ctx *c = nullptr;
+        if (ctx_create(&c) != SAGPU_OK) {
+            for (Ctx *x : sv->ctxs) ctx_destroy(x);
+            delete sv;
+            return nullptr;
+        }
+        sv->ctxs.push_back(c);
+    }
+    if (!hbm_budget) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 16ull << 30;
+        hbm_budget = (uint64_t)(0.9 * (double)fr);
+    }
+    sv->budget = hbm_budget;
+    sv->batch_bytes = (u64)batch_bytes;
+    for (int q = 0; q < slots; q++) sv->workers.emplace_back(worker, sv, q);
+    return sv;
+}
+
+extern "C" int32_t sagpu_server_submit(void *server, sagpu_job *job)
+{
+    if (!server || !job || !job->T || !job->SA || job->n < 0 || job->kind < 0 || job->kind > 2 ||
+        (job->kind == SAGPU_JOB_BWT && !job->U) || (job->kind == SAGPU_JOB_LCP && !job->LCP))
+        return SAGPU_EINVAL;
+    Server &sv = *(Server *)server;
+    job->batched = 0;
+    job->primary = 0;
+    job->batchable = job->n >= 2 && memchr(job->T, 0, (size_t)job->n) == nullptr;
+    job->t_submit = now_s();
+    job->t_start = job->t_done = 0;
+    __atomic_store_n(&job->status, 1, __ATOMIC_RELEASE);
+    {
+        std::lock_guard<std::mutex> lk(sv.mu);
+        if (sv.stop) return SAGPU_EINVAL;
+        sv.queue.push_back(job);
+    }
+    sv.cv.notify_all();
+    return 0;
+}
+
+extern "C" int32_t sagpu_server_wait(void *server, sagpu_job *job)
+{
+    if (!server || !job) return SAGPU_EINVAL;
+    Server &sv = *(Server *)server;
+    std::unique_lock<std::mutex> lk(sv.mu);
+    sv.cv.wait(lk, [&] { return __atomic_load_n(&job->status, __ATOMIC_ACQUIRE) != 1; });
+    return job->status;
+}
+
+extern "C" int32_t sagpu_server_drain(void *server)
+{
+    if (!server) return SAGPU_EINVAL;
+    Server &sv = *(Server *)server;
+    std::unique_lock<std::mutex> lk(sv.mu);
+    sv.cv.wait(lk, [&] { return sv.queue.empty() && sv.reserved == 0; });
+    return 0;
+}
+
+extern "C" void sagpu_server_destroy(void *server)
+{
+    if (!server) return;
+    Server *sv = (Server *)server;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv.notify_all();
+    for (auto &t : sv->workers) t.join();
+    for (Ctx *c : sv->ctxs) ctx_destroy(c);
+    delete sv;
 }

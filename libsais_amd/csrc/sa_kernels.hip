@@ -145,6 +145,38 @@ __global__ __launch_bounds__(256) void k_localize(const u32 *__restrict__ SA, co
         out[j] = SA[j] - (u32)off[sid[j]];
 }
 
+__global__ __launch_bounds__(256) void k_batch_extract(const u32 *__restrict__ gSA, const u32 *__restrict__ gLCP,
+                                                       const u32 *__restrict__ sid, const int64_t *__restrict__ off,
+                                                       const u32 *__restrict__ ISA, const uint8_t *__restrict__ T,
+                                                       u64 N, u32 *__restrict__ outSA, u32 *__restrict__ outLCP,
+                                                       uint8_t *__restrict__ outU)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+        const u32 k = sid[j];
+        const u64 o = (u64)off[k];
+        if (j == o) continue;                 // the separator's own suffix
+        const u64 jj = j - o - 1;             // slot inside job k's suffix array
+        const u64 po = o - k;                 // packed output offset of job k
+        const u32 v = gSA[j] - (u32)o;        // suffix index inside job k
+        outSA[po + jj] = v;
+        if (outLCP) outLCP[po + jj] = gLCP[j];
+        if (outU) {
+            const u64 nk = (u64)off[k + 1] - o - 1;
+            const u64 j0 = (u64)ISA[o] - o - 1;
+            if (v == 0) outU[po] = T[o + nk - 1];
+            else outU[po + jj + (jj < j0)] = T[o + v - 1];
+        }
+    }
+}
+
+__global__ void k_batch_primary(const u32 *__restrict__ ISA, const int64_t *__restrict__ off, u32 count,
+                                u32 *__restrict__ prim)
+{
+    for (u32 q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
+        prim[q] = ISA[off[q]] - (u32)off[q];
+}
+
 __global__ void k_fill_u32(u32 *p, u64 count, u32 v)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -219,6 +251,22 @@ int launch_check_offsets(const uint8_t *T, const u32 *sid, const int64_t *offset
 int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n, u32 *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_localize, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, SA, sid, offsets, n, out);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_batch_extract(const u32 *gSA, const u32 *gLCP, const u32 *sid, const int64_t *off, const u32 *ISA,
+                         const uint8_t *T, u64 N, u32 *outSA, u32 *outLCP, uint8_t *outU, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_extract, dim3(grid_for(N, 256 * 8, 65535)), dim3(256), 0, s, gSA, gLCP, sid, off, ISA,
+                       T, N, outSA, outLCP, outU);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u32 *prim, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_primary, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, ISA, off, count, prim);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
