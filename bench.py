@@ -97,6 +97,76 @@ def cpu_baseline(T: np.ndarray, reps: int):
             "sample": f"first {m} bytes, SA+BWT+PLCP+LCP with the oracle restatement"}
 
 
+def bench_batched(reps: int = 2, plain_sample: int = 1000):
+    """Batched GSA of 20,000 separator-terminated strings (log-normal lengths, median
+    4 KiB, clipped to 256 B - 4 MiB, ~512 MiB, 10% with a shared 1-20 KiB prefix):
+    one sagpu_gsa_batch32 call (SA + LCP per string) against the per-string path
+    (one build per string) on a random sample of the same strings."""
+    import libsais_amd as la
+    from libsais_amd import batch, workloads
+
+    T, offs = workloads.gsa_batch(count=20000, seed=5)
+    dev = batch.DeviceText(T, want_lcp=True)
+    try:
+        rc = batch.batched_on_device(dev, offs)  # warm-up
+        if rc != 0:
+            raise RuntimeError(f"batched build failed ({rc}): {la.last_error()}")
+        t_b = batch.timed(batch.batched_on_device, dev, offs, reps=reps)
+        rng = np.random.default_rng(0)
+        sample = sorted(rng.choice(len(offs) - 1, min(plain_sample, len(offs) - 1), replace=False).tolist())
+        sample_bytes = int(sum(offs[k + 1] - offs[k] for k in sample))
+        batch.plain_on_device(dev, offs, sample[:20])  # warm-up
+        t_p = batch.timed(batch.plain_on_device, dev, offs, sample)
+    finally:
+        dev.free()
+    b_mbps = len(T) / t_b / 1e6
+    p_mbps = sample_bytes / t_p / 1e6
+    return {
+        "workload": "gsa_batch_20000_strings_512MiB_sa_lcp",
+        "strings": int(len(offs) - 1),
+        "bytes": int(len(T)),
+        "batched_ms": round(t_b * 1e3, 2),
+        "batched_MBps": round(b_mbps, 1),
+        "plain_MBps": round(p_mbps, 1),
+        "plain_sample": f"{len(sample)} random strings, {sample_bytes} bytes, one build per string",
+        "ratio": round(b_mbps / p_mbps, 2),
+    }
+
+
+def bench_serving(jobs: int, max_bytes: int, slots: int = 4):
+    """Jobs with sizes log-uniform from 4 KiB, order-2 Markov ACGT, SA / SA+BWT /
+    SA+LCP = 0.4/0.4/0.2, replayed twice: Poisson arrivals at 200 req/s (latency
+    under the specified load) and all at once (throughput when the GPU is the
+    bottleneck).  Served = slots + HBM admission + continuous batching; plain =
+    one job at a time on one stream, same order."""
+    from libsais_amd import serving
+
+    spec = serving.TraceSpec(jobs=jobs, rate=200.0, min_bytes=4 << 10, max_bytes=max_bytes, seed=6)
+    reqs = serving.make_trace(spec)
+    out = {"workload": f"{jobs}_jobs_4KiB_{max_bytes >> 20}MiB_markov2_sa_bwt_lcp", "slots": slots}
+    with serving.Server(slots=slots) as sv:
+        warm = [serving.Request(r.T, r.kind) for r in reqs[:8]]
+        for r in warm:
+            sv.submit(r)
+        sv.drain()
+        t0 = serving.replay(reqs, sv)
+        out["poisson200_served"] = serving.summarize(reqs, t0)
+        arrivals = [r.arrival for r in reqs]
+        for r in reqs:
+            r.arrival = 0.0
+        t0 = serving.replay(reqs, sv)
+        out["saturated_served"] = serving.summarize(reqs, t0)
+    t1 = serving.replay(reqs, None)
+    out["saturated_plain"] = serving.summarize(reqs, t1)
+    for r, a in zip(reqs, arrivals):
+        r.arrival = a
+    t1 = serving.replay(reqs, None)
+    out["poisson200_plain"] = serving.summarize(reqs, t1)
+    out["saturated_ratio_MBps"] = round(out["saturated_served"]["aggregate_MBps"] /
+                                        out["saturated_plain"]["aggregate_MBps"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +176,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and serving sections")
+    ap.add_argument("--mode", choices=["main", "serving-full", "batched"], default="main")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,6 +190,12 @@ def main():
     from libsais_amd import workloads
 
     lib = la.load()
+    if args.mode == "serving-full":
+        print(json.dumps({"serving": bench_serving(2000, 64 << 20)}), flush=True)
+        return
+    if args.mode == "batched":
+        print(json.dumps({"batched": bench_batched()}), flush=True)
+        return
     n = args.n
     t0 = time.perf_counter()
     T = workloads.dna_like(n, seed=args.seed)
@@ -238,6 +316,13 @@ def main():
     for p in (dT, dSA, dU, dP, dL):
         lib.sagpu_free(p)
 
+    if not args.no_extras:
+        log("batched GSA ...")
+        result["batched"] = bench_batched()
+        log(f"batched: {result['batched']}")
+        log("serving trace ...")
+        result["serving"] = bench_serving(300, 16 << 20)
+        log(f"serving: {result['serving']}")
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(T, args.cpu_reps)

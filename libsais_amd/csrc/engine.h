@@ -29,6 +29,7 @@ struct RadixScratch {
     void (*hook)(void *user, int phase);
     void *hook_user;
     int variant;      // RADIX_ONESWEEP or RADIX_REDUCE_THEN_SCAN
+    int items;        // keys per thread per tile: 16 or 32 (tile = 256 * items)
     u32 *counts;      // [counts_cap] per-tile digit counts (reduce-then-scan)
     u64 counts_cap;
     ScanScratch scan;

@@ -117,6 +117,7 @@ std::mutex g_mu;
 Ctx *g_ctx = nullptr;
 bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
+int g_radix_items = 32;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -134,6 +135,7 @@ int ctx_create(Ctx **out)
     SAGPU_CHECK(hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault));
     c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
     c->rs.variant = g_radix_variant;
+    c->rs.items = g_radix_items;
     *out = c;
     return SAGPU_OK;
 }
@@ -811,6 +813,12 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
 {
     if (!name) return SAGPU_EINVAL;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (!strcmp(name, "radix_items")) {
+        if (value != 16 && value != 32) return SAGPU_EINVAL;
+        g_radix_items = (int)value;
+        if (g_ctx) g_ctx->rs.items = g_radix_items;
+        return 0;
+    }
     if (!strcmp(name, "radix_variant")) {
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;

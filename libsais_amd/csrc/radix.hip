@@ -18,7 +18,6 @@
 namespace {
 
 constexpr int RS_WAVES = RS_THREADS / SAGPU_WAVE;
-constexpr int RS_WAVE_ITEMS = SAGPU_WAVE * RS_ITEMS;
 constexpr int LB_VAL_BITS = 40;
 constexpr u64 LB_VAL_MASK = (1ull << LB_VAL_BITS) - 1ull;
 constexpr u64 LB_AGG = 1ull, LB_PRE = 2ull;
@@ -72,20 +71,22 @@ __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
 // One radix pass over a tile of RS_TILE keys.  LOOKBACK = true: onesweep (tile
 // ticket, aggregate publish, decoupled look-back).  LOOKBACK = false: the tile's
 // global digit offsets were produced by an upsweep + scan (reduce-then-scan).
-template <typename K, bool HAS_VALS, bool LOOKBACK>
+template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
                                                            K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
                                                            int shift, u32 dmask, const u64 *__restrict__ offsets,
                                                            u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
                                                            u32 epoch, const u32 *__restrict__ scanned, u64 tiles)
 {
+    constexpr int TILE_ = RS_THREADS * ITEMS;
+    constexpr int WAVE_ITEMS_ = SAGPU_WAVE * ITEMS;
     __shared__ u32 s_tile;
     __shared__ u32 s_wcnt[RS_WAVES][256];
     __shared__ u32 s_dstart[256];
     __shared__ u64 s_gbase[256];
     __shared__ u32 s_wsum[RS_WAVES];
-    __shared__ K s_keys[RS_TILE];
-    __shared__ uint8_t s_dig[RS_TILE];
+    __shared__ K s_keys[TILE_];
+    __shared__ uint8_t s_dig[TILE_];
 
     const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
     if (LOOKBACK) {
@@ -96,13 +97,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     __syncthreads();
 
     const u64 tile = LOOKBACK ? (u64)s_tile : (u64)blockIdx.x;
-    const u64 wbase = tile * RS_TILE + (u64)w * RS_WAVE_ITEMS;
+    const u64 wbase = tile * TILE_ + (u64)w * WAVE_ITEMS_;
 
-    K key[RS_ITEMS];
-    u32 val[RS_ITEMS];
-    u32 rank[RS_ITEMS];
+    K key[ITEMS];
+    u32 val[ITEMS];
+    u32 rank[ITEMS];
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
         bool ok = pos < n;
         key[j] = ok ? kin[pos] : (K)0;
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
 
     // Wave-level stable ranking: order inside the wave is (item j, lane).
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
         bool ok = pos < n;
         u32 d = (u32)(key[j] >> shift) & dmask;
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
 
     // Reorder keys in LDS by digit.
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
         if (pos < n) {
             u32 d = (u32)(key[j] >> shift) & dmask;
@@ -200,8 +201,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
         }
     }
     __syncthreads();
-    const u64 tbase = tile * RS_TILE;
-    const u32 tile_n = (u32)((n - tbase) < (u64)RS_TILE ? (n - tbase) : (u64)RS_TILE);
+    const u64 tbase = tile * TILE_;
+    const u32 tile_n = (u32)((n - tbase) < (u64)TILE_ ? (n - tbase) : (u64)TILE_);
     for (u32 idx = tid; idx < tile_n; idx += RS_THREADS) {
         u32 d = s_dig[idx];
         kout[s_gbase[d] + (idx - s_dstart[d])] = s_keys[idx];
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
         __syncthreads();
         u32 *s_vals = reinterpret_cast<u32 *>(s_keys);
 #pragma unroll
-        for (int j = 0; j < RS_ITEMS; j++) {
+        for (int j = 0; j < ITEMS; j++) {
             u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
             if (pos < n) s_vals[rank[j]] = val[j];
         }
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
 }
 
 // Reduce-then-scan upsweep: per-tile digit counts, digit-major (counts[d*tiles+t]).
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rts_upsweep(const K *__restrict__ kin, u64 n, int shift,
                                                             u32 dmask, u32 *__restrict__ counts, u64 tiles)
 {
@@ -232,9 +233,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rts_upsweep(const K *__restrict_
 #pragma unroll
     for (int q = 0; q < RS_WAVES; q++) s_h[q][tid] = 0;
     __syncthreads();
-    const u64 base = (u64)blockIdx.x * RS_TILE;
+    const u64 base = (u64)blockIdx.x * (RS_THREADS * ITEMS);
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         u64 pos = base + (u64)j * RS_THREADS + tid;
         if (pos < n) atomicAdd(&s_h[w][(u32)(kin[pos] >> shift) & dmask], 1u);
     }
@@ -253,7 +254,9 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     if (passes_run) *passes_run = 0;
     if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
     const int npass = (end_bit - begin_bit + 7) / 8;
-    const u64 tiles = (n + RS_TILE - 1) / RS_TILE;
+    const int items = rs.items == 32 ? 32 : 16;
+    const u64 tile_keys = (u64)RS_THREADS * items;
+    const u64 tiles = (n + tile_keys - 1) / tile_keys;
     if (tiles > rs.lb_tiles || npass > 8 || (rs.variant == RADIX_REDUCE_THEN_SCAN && tiles * 256 > rs.counts_cap)) {
         sagpu_set_error("radix scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
@@ -288,28 +291,31 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         const int shift = begin_bit + 8 * p;
         const u32 dmask = end_bit - shift >= 8 ? 255u : ((1u << (end_bit - shift)) - 1u);
         if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
-            hipLaunchKernelGGL(k_rts_upsweep<K>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n, shift,
-                               dmask, rs.counts, tiles);
+            if (items == 32)
+                hipLaunchKernelGGL((k_rts_upsweep<K, 32>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                   shift, dmask, rs.counts, tiles);
+            else
+                hipLaunchKernelGGL((k_rts_upsweep<K, 16>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                   shift, dmask, rs.counts, tiles);
             SAGPU_TRY(scan_exclusive_u32(rs.scan, rs.counts, tiles * 256, s));
         }
         if (rs.hook) rs.hook(rs.hook_user, 0);
-        if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
-            if (v0)
-                hipLaunchKernelGGL((k_radix_pass<K, true, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
-                                   vi, ko, vo, n, shift, dmask, nullptr, nullptr, nullptr, 0u, rs.counts, tiles);
-            else
-                hipLaunchKernelGGL((k_radix_pass<K, false, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
-                                   vi, ko, vo, n, shift, dmask, nullptr, nullptr, nullptr, 0u, rs.counts, tiles);
+        const bool lb = rs.variant != RADIX_REDUCE_THEN_SCAN;
+        const u64 *offs = lb ? rs.hist + 8 * 256 + p * 256 : nullptr;
+        u64 *lbk = lb ? rs.lookback : nullptr;
+        u32 *tctr = lb ? rs.tile_ctr + p : nullptr;
+        const u32 *cnts = lb ? nullptr : rs.counts;
+#define RS_LAUNCH(HV, LB, IT)                                                                                     \
+    hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
+                       n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles)
+        if (v0) {
+            if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else RS_LAUNCH(true, true, 16); }
+            else { if (items == 32) RS_LAUNCH(true, false, 32); else RS_LAUNCH(true, false, 16); }
         } else {
-            if (v0)
-                hipLaunchKernelGGL((k_radix_pass<K, true, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
-                                   vi, ko, vo, n, shift, dmask, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
-                                   rs.epoch, nullptr, tiles);
-            else
-                hipLaunchKernelGGL((k_radix_pass<K, false, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki,
-                                   vi, ko, vo, n, shift, dmask, rs.hist + 8 * 256 + p * 256, rs.lookback, rs.tile_ctr + p,
-                                   rs.epoch, nullptr, tiles);
+            if (lb) { if (items == 32) RS_LAUNCH(false, true, 32); else RS_LAUNCH(false, true, 16); }
+            else { if (items == 32) RS_LAUNCH(false, false, 32); else RS_LAUNCH(false, false, 16); }
         }
+#undef RS_LAUNCH
         SAGPU_CHECK(hipGetLastError());
         if (rs.hook) rs.hook(rs.hook_user, 1);
         K *tk = ki; ki = ko; ko = tk;
