@@ -160,6 +160,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  * Tuning knobs (process-wide):
  *   "radix_variant"  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
  *   "radix_items"    keys per thread per radix tile: 16 (4096-key tiles) or 32 (default)
+ *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
+ *                    largest the 32-bit keys hold; lower values exercise the wide mode)
  * Returns 0, or -1 for an unknown name or value.
  */
 SAGPU_API int32_t sagpu_set_option(const char *name, int64_t value);

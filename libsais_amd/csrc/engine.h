@@ -52,10 +52,10 @@ int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, 
 enum { RT_SMALL = 256 };   // groups up to this size are sorted inside one workgroup
 
 struct WsScratch {
-    void *tile_agg;   // [tiles_cap] 12-byte tile aggregates
+    void *tile_agg;   // [tiles_cap] 16-byte tile aggregates
     u64 tiles_cap;    // tiles of 4096 elements
-    u32 *total;       // [1]
-    u32 *base;        // [1] reserved output base
+    u64 *total;       // [1]
+    u64 *base;        // [1] reserved output base
 };
 
 // G: scratch of n entries (group head per sorted position, SA order).
@@ -64,15 +64,16 @@ int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits,
 // Batched GSA: keys = (string id << 32) | packed symbols.
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
-int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,
-                 hipStream_t s);
-int round_tile(const u32 *act_i, const u32 *act_g, const u32 *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
-               u32 *out_g, u32 *out_counter, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter, u64 big_cap,
-               u32 *LCP, hipStream_t s);
-int big_extract(const u32 *act_i, const u32 *key2, const u32 *bhead, const u32 *boff, u32 nb, u64 total, u64 *comp,
-                u32 *vals, hipStream_t s);
-int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, u32 *SA, u32 *ISA,
-             u32 *out_i, u32 *out_g, u32 *out_counter, u32 *LCP, hipStream_t s);
+// Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
+                 u32 *SA, hipStream_t s);
+int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
+               u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s);
+int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
+                u64 *comp, u32 *vals, hipStream_t s);
+int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
@@ -102,7 +103,7 @@ struct LcpScratch {
     u32 *queue_l;     // [cap] lcp matched so far
     u32 *queue_n;     // [1]
     u64 queue_cap;
-    u32 *tile_max;    // [tiles] per-tile max for the prefix-max scan
+    void *tile_max;   // [tiles] per-tile max for the prefix-max scan (u32 or u64)
     u64 tiles_cap;
 };
 
@@ -111,8 +112,10 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // PLCP from either PHI (ISA == nullptr) or ISA+SA (phi computed on the fly).
 // marks (SA order, with ISA+SA only): exact lcps or LCP_UNKNOWN | lower bound from
 // the construction; position i reads marks[ISA[i]] and only unknown ones compare.
+// V64 (n entries) is needed when PLCP[i] + i can reach 2^32 (n > 2^32 - 1 - n/2 is
+// not tracked: it is passed whenever n > INT32_MAX); otherwise PLCP is the scratch.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
-                const u32 *SA, u32 *PLCP, const u32 *marks, hipStream_t s);
+                const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);

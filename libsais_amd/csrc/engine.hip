@@ -118,6 +118,8 @@ Ctx *g_ctx = nullptr;
 bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
+// Texts longer than this use 64-bit rank keys and PLCP scan values.
+u64 g_wide_from = 0x7FFFFFFFull;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -360,81 +362,90 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     GET(d_lut, uint8_t, 256);
     SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
 
-    // Sort buffers: u32 keys for the initial sort, u64 (group, key2) for big groups.
-    GET(kb0, u64, n);
-    GET(kb1, u64, n);
-    GET(vb0, u32, n);
-    GET(vb1, u32, n);
-    GET(ai0, u32, n);
-    GET(ag0, u32, n);
-    GET(ai1, u32, n);
-    GET(ag1, u32, n);
-    GET(key2, u32, n);
+    // Buffer plan (each block is reused once its first use is over):
+    //   kA  [2n u32, 4n for batched u64 keys]: initial sort keys (ping-pong), then
+    //       the per-slot group heads, then the second active list
+    //   vA  [2n u32]: initial sort values (ping-pong), then key2 (u32 or u64)
+    //   aA  [2n u32]: first active list (suffix, group head)
+    // Groups too big for one workgroup get their own sort buffers per round.
+    const bool wide = n > g_wide_from;
+    GET(kA, u32, sid ? 4 * n : 2 * n);
+    GET(vA, u32, 2 * n);
+    GET(aA, u32, 2 * n);
     const u64 big_cap = n / (RT_SMALL + 1) + 16;
     GET(big_head, u32, big_cap);
     GET(big_g, u32, big_cap);
     GET(big_size, u32, big_cap);
-    GET(d_cnt, u32, 64);
-    GET(d_boff, u32, big_cap + 1);
+    GET(d_cnt, u64, 8);
+    GET(d_boff, u64, big_cap + 1);
     GET(d_bhead, u32, big_cap);
     GET(d_bslot, u32, big_cap);
-    u32 *act_i[2] = {ai0, ai1}, *act_g[2] = {ag0, ag1};
+    u64 *out_counter = d_cnt;
+    u32 *big_counter = (u32 *)(d_cnt + 4), *err = (u32 *)(d_cnt + 4) + 1;
+    u32 *act_i[2] = {aA, kA}, *act_g[2] = {aA + n, kA + n};
+    void *key2 = vA;
 
     WsScratch ws{};
     ws.tiles_cap = n / 4096 + 2;
     {
-        GET(tagg, u32, ws.tiles_cap * 3);
+        GET(tagg, u64, ws.tiles_cap * 2);
         ws.tile_agg = tagg;
-        ws.total = d_cnt + 8;
-        ws.base = d_cnt + 9;
+        ws.total = d_cnt + 2;
+        ws.base = d_cnt + 3;
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
 
     int alt = 0;
     if (!sid) {
+        u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
         {
             Span sp(c, F_INITIAL);
-            SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, (u32 *)kb0, vb0, s));
+            SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, k0, v0, s));
         }
-        SAGPU_TRY(sort_u32(c, (u32 *)kb0, vb0, (u32 *)kb1, vb1, n, 0, ksym * bits, &alt));
+        SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt));
         Span sp(c, F_INITIAL);
         // the unsorted key buffer is free now: it holds the per-slot group heads
-        SAGPU_TRY(group_init(ws, alt ? (u32 *)kb1 : (u32 *)kb0, alt ? vb1 : vb0, n, bits, ksym, gsa, dSA, dISA,
-                             act_i[0], act_g[0], dLCP, alt ? (u32 *)kb0 : (u32 *)kb1, s));
+        SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
+                             alt ? k0 : k1, s));
     } else {
         // batched GSA: (string id, symbols) keys, so every group stays inside one string
+        u64 *k0 = (u64 *)kA, *k1 = (u64 *)kA + n;
+        u32 *v0 = vA, *v1 = vA + n;
         {
             Span sp(c, F_INITIAL);
-            SAGPU_TRY(launch_pack_keys64_sid(dT, n, d_lut, bits, ksym, sid, kb0, vb0, s));
+            SAGPU_TRY(launch_pack_keys64_sid(dT, n, d_lut, bits, ksym, sid, k0, v0, s));
         }
-        SAGPU_TRY(sort_u64(c, kb0, vb0, kb1, vb1, n, 0, 32 + bits_for(nstrings ? nstrings - 1 : 0), &alt));
+        SAGPU_TRY(sort_u64(c, k0, v0, k1, v1, n, 0, 32 + bits_for(nstrings ? nstrings - 1 : 0), &alt));
         Span sp(c, F_INITIAL);
-        SAGPU_TRY(group_init64(ws, alt ? kb1 : kb0, alt ? vb1 : vb0, n, bits, ksym, dSA, dISA, act_i[0], act_g[0],
-                               dLCP, (u32 *)(alt ? kb0 : kb1), s));
+        SAGPU_TRY(group_init64(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, dSA, dISA, act_i[0], act_g[0], dLCP,
+                               (u32 *)(alt ? k0 : k1), s));
     }
-    u32 a = 0;
-    SAGPU_TRY(read_u32(c, ws.total, &a));
+    u64 a = 0;
+    SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    a = c.pinned[0];
 
     u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
-    std::vector<u32> bh, bg, bs, order, hoff, hhead, hslot;
+    std::vector<u32> bh, bg, bs, order, hhead, hslot;
+    std::vector<u64> hoff;
     while (a > 0) {
-        if (st.rounds < 64) st.active[st.rounds] = a;
+        if (st.rounds < 64) st.active[st.rounds] = (int64_t)a;
         st.rounds++;
         const int nxt = cur ^ 1;
         {
             Span sp(c, F_ROUNDS);
-            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, dSA, s));
-            SAGPU_TRY(launch_fill_u32(d_cnt, 3, 0, s));
-            SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, a, h, dSA, dISA, act_i[nxt], act_g[nxt], d_cnt,
-                                 big_head, big_g, big_size, d_cnt + 1, big_cap, dLCP, s));
+            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s));
+            SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
+            SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
+                                 out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s));
         }
-        SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 3 * sizeof(u32), hipMemcpyDeviceToHost, s));
+        SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
-        u32 out_n = ((u32 *)c.pinned)[0];
-        const u32 nb = ((u32 *)c.pinned)[1];
-        if (nb > big_cap || ((u32 *)c.pinned)[2] != 0 || out_n > a) {
+        u64 out_n = c.pinned[0];
+        const u32 nb = ((u32 *)(c.pinned + 4))[0];
+        if (nb > big_cap || ((u32 *)(c.pinned + 4))[1] != 0 || out_n > a) {
             sagpu_set_error("round bookkeeping inconsistent", __FILE__, __LINE__);
             return SAGPU_EDEVICE;
         }
@@ -455,12 +466,12 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             hslot.resize(nb);
             u64 total = 0;
             for (u32 q = 0; q < nb; q++) {
-                hoff[q] = (u32)total;
+                hoff[q] = total;
                 hhead[q] = bh[order[q]];
                 hslot[q] = bg[order[q]];
-                total += bs[order[q]];
+                total += (u64)bs[order[q]] + 1;  // sizes are stored minus one
             }
-            hoff[nb] = (u32)total;
+            hoff[nb] = total;
             for (u32 q = 0; q < nb; q++) {
                 u64 hq = hhead[q], sq = hoff[q + 1] - hoff[q];
                 if (sq <= RT_SMALL || hq + sq > a || (q + 1 < nb && hq + sq > hhead[q + 1])) total = ~0ull;
@@ -469,20 +480,26 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 sagpu_set_error("big-group list inconsistent", __FILE__, __LINE__);
                 return SAGPU_EDEVICE;
             }
-            SAGPU_CHECK(hipMemcpyAsync(d_boff, hoff.data(), (nb + 1) * 4, hipMemcpyHostToDevice, s));
+            SAGPU_CHECK(hipMemcpyAsync(d_boff, hoff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
             SAGPU_CHECK(hipMemcpyAsync(d_bhead, hhead.data(), nb * 4, hipMemcpyHostToDevice, s));
             SAGPU_CHECK(hipMemcpyAsync(d_bslot, hslot.data(), nb * 4, hipMemcpyHostToDevice, s));
+            Scope rsc(c);  // this round's big-group sort buffers
+            u64 *b0 = rsc.get<u64>(total), *b1 = rsc.get<u64>(total);
+            u32 *bv0 = rsc.get<u32>(total), *bv1 = rsc.get<u32>(total);
+            if (!b0 || !b1 || !bv0 || !bv1) return SAGPU_ENOMEM;
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_extract(act_i[cur], key2, d_bhead, d_boff, nb, total, kb0, vb0, s));
+                SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, b0, bv0, s));
             }
-            SAGPU_TRY(sort_u64(c, kb0, vb0, kb1, vb1, total, 0, 32 + bits_for(nb - 1), &alt));
+            SAGPU_TRY(sort_u64(c, b0, bv0, b1, bv1, total, 0, (wide ? 33 : 32) + bits_for(nb - 1), &alt));
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_emit(ws, alt ? kb1 : kb0, alt ? vb1 : vb0, d_bslot, total, h, dSA, dISA, act_i[nxt],
-                                   act_g[nxt], d_cnt, dLCP, s));
+                SAGPU_TRY(big_emit(ws, alt ? b1 : b0, alt ? bv1 : bv0, d_bslot, total, h, wide, dSA, dISA, act_i[nxt],
+                                   act_g[nxt], out_counter, dLCP, s));
             }
-            SAGPU_TRY(read_u32(c, d_cnt, &out_n));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, sizeof(u64), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipStreamSynchronize(s));
+            out_n = c.pinned[0];
             st.big_rounds++;
             st.big_elems += (int64_t)total;
         }
@@ -506,12 +523,17 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     GET(q, u32, ls.queue_cap);
     GET(ql, u32, ls.queue_cap);
     GET(qn, u32, 64);
-    GET(tm, u32, ls.tiles_cap);
+    GET(tm, u64, ls.tiles_cap);
     ls.queue = q;
     ls.queue_l = ql;
     ls.queue_n = qn;
     ls.tile_max = tm;
-    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, marks, c.stream);
+    u64 *V64 = nullptr;
+    if (n > g_wide_from) {  // PLCP[i] + i may reach 2^32
+        V64 = sc.get<u64>(n);
+        if (!V64) return SAGPU_ENOMEM;
+    }
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream);
 }
 
 // Copies a text into a pool buffer with 64 zero bytes of padding.
@@ -553,7 +575,8 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
     int rc;
     {
         Span total(c, F_TOTAL);
-        rc = sa_build(c, sc, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr);
+        Scope work(c);  // construction buffers go back to the pool before the caller's downloads
+        rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
@@ -562,7 +585,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (want_lcp) {
             {
                 Span sp(c, F_PLCP);
-                SAGPU_TRY(plcp_run(c, sc, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks));
+                SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks));
             }
             if (dLCP) {
                 Span sp(c, F_LCP);
@@ -647,8 +670,8 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx *c;
     SAGPU_TRY(ctx_get(&c));
-    if ((u64)n >= (1ull << 32)) {
-        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
     }
     stats_reset(*c);
@@ -674,8 +697,8 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx *c;
     SAGPU_TRY(ctx_get(&c));
-    if ((u64)n >= (1ull << 32)) {
-        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
     }
     stats_reset(*c);
@@ -699,8 +722,8 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx *c;
     SAGPU_TRY(ctx_get(&c));
-    if ((u64)n >= (1ull << 32)) {
-        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
     }
     stats_reset(*c);
@@ -724,8 +747,8 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA,
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx *c;
     SAGPU_TRY(ctx_get(&c));
-    if ((u64)n >= (1ull << 32)) {
-        sagpu_set_error("n >= 2^32 is not supported by this engine build", __FILE__, __LINE__);
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
     }
     Scope sc(*c);
@@ -823,6 +846,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "wide_from")) {
+        if (value < 0 || value > 0x7FFFFFFF) return SAGPU_EINVAL;
+        g_wide_from = (u64)value;
         return 0;
     }
     return SAGPU_EINVAL;

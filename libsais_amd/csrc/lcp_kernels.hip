@@ -38,10 +38,11 @@ __global__ __launch_bounds__(256) void k_phi(const u32 *__restrict__ SA, u64 n, 
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     // Entries outside [0, n) (an invalid SA from the caller) are skipped rather than
-    // written out of bounds.
+    // written out of bounds.  The first suffix has no predecessor: PHI points it at
+    // itself, a value no real predecessor takes and one that fits for n = 2^32.
     for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         u32 p = SA[j];
-        if (p < n) PHI[p] = j == 0 ? (u32)n : SA[j - 1];
+        if (p < n) PHI[p] = j == 0 ? p : SA[j - 1];
     }
 }
 
@@ -83,9 +84,10 @@ constexpr u64 PL_THREAD_LIMIT = 64;
 
 // Per position: decide reducibility, compare irreducible ones up to 64 bytes, queue
 // the rest.  Writes v[i] (PLCP[i] + i, or 0 for reducible positions) into PLCP.
+template <typename VT>
 __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                   const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
-                                                  const u32 *__restrict__ SA, u32 *__restrict__ V,
+                                                  const u32 *__restrict__ SA, VT *__restrict__ V,
                                                   u32 *__restrict__ queue, u32 *__restrict__ queue_l,
                                                   u32 *__restrict__ queue_n, u64 queue_cap,
                                                   const u32 *__restrict__ marks)
@@ -96,13 +98,14 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
         u64 k;
         if (PHI) {
             k = PHI[i];
+            if (k == i) k = n;  // first suffix in SA order
         } else {
             u32 r = ISA[i];
             if (marks) {
                 // construction already knows most lcps: read it where i sits in SA
                 u32 p = r < n ? marks[r] : 0u;
                 if (!(p & LCP_UNKNOWN)) {
-                    V[i] = (u32)(p + i);
+                    V[i] = (VT)(p + i);
                     continue;
                 }
                 lb = p & ~LCP_UNKNOWN;
@@ -116,22 +119,22 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
             irr = (a != b) || (gsa && a == 0);
         }
         if (!irr) { V[i] = 0; continue; }
-        if (k >= n) { V[i] = (u32)i; continue; }
+        if (k >= n) { V[i] = (VT)i; continue; }
         u64 m = n - (i > k ? i : k); // also bounds GSA matches (they stop earlier)
         bool done = false;
         u64 l = match_bytes(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
         if (done) {
-            V[i] = (u32)(l + i);
+            V[i] = (VT)(l + i);
         } else {
             u32 slot = atomicAdd(queue_n, 1u);
             if (slot < queue_cap) {
                 queue[slot] = (u32)i;
                 queue_l[slot] = (u32)l;
-                V[i] = (u32)(l + i); // provisional; overwritten by k_plcp_long
+                V[i] = (VT)(l + i); // provisional; overwritten by k_plcp_long
             } else {
                 // queue full: finish here (correct, only slower)
                 l = match_bytes(T, i, k, l, m, gsa, ~0ull >> 1, &done);
-                V[i] = (u32)(l + i);
+                V[i] = (VT)(l + i);
             }
         }
     }
@@ -139,9 +142,10 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
 
 // One workgroup per queued position: 256 lanes x 8 bytes per step, first mismatch
 // found with a workgroup min-reduction.
+template <typename VT>
 __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
-                                                   const u32 *__restrict__ SA, u32 *__restrict__ V,
+                                                   const u32 *__restrict__ SA, VT *__restrict__ V,
                                                    const u32 *__restrict__ queue, const u32 *__restrict__ queue_l,
                                                    const u32 *__restrict__ queue_n, u64 queue_cap)
 {
@@ -151,8 +155,10 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
     for (u32 q = blockIdx.x; q < cnt; q += gridDim.x) {
         const u64 i = queue[q];
         u64 k;
-        if (PHI) k = PHI[i];
-        else {
+        if (PHI) {
+            k = PHI[i];
+            if (k == i) k = n;
+        } else {
             u32 r = ISA[i];
             k = r == 0 ? n : SA[r - 1];
         }
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             u64 r = s_res;
             __syncthreads();
             if (r != ~0ull) {
-                if (threadIdx.x == 0) V[i] = (u32)(r + i);
+                if (threadIdx.x == 0) V[i] = (VT)(r + i);
                 break;
             }
             l += 256 * 8;
@@ -197,80 +203,99 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
     }
 }
 
-// Prefix max over V, then PLCP[i] = max - i (in place).
+// Prefix max over V, then PLCP[i] = max - i.  VT = u32 works in place (V is the
+// PLCP array); VT = u64 is used when PLCP[i] + i can reach 2^32.
 constexpr int MS_THREADS = 256, MS_ITEMS = 16, MS_TILE = MS_THREADS * MS_ITEMS;
 
-__global__ __launch_bounds__(MS_THREADS) void k_max_reduce(const u32 *__restrict__ V, u64 n, u32 *__restrict__ tmax)
+template <typename VT>
+__device__ __forceinline__ VT vmax(VT a, VT b) { return a > b ? a : b; }
+
+template <typename VT>
+__device__ __forceinline__ VT wave_incl_max_t(VT v)
 {
-    __shared__ u32 s_w[4];
+#pragma unroll
+    for (int d = 1; d < SAGPU_WAVE; d <<= 1) {
+        VT o = __shfl_up(v, d, SAGPU_WAVE);
+        if ((int)lane_id() >= d) v = vmax(v, o);
+    }
+    return v;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(MS_THREADS) void k_max_reduce(const VT *__restrict__ V, u64 n, VT *__restrict__ tmax)
+{
+    __shared__ VT s_w[4];
     const u64 base = (u64)blockIdx.x * MS_TILE;
-    u32 m = 0;
+    VT m = 0;
     for (int j = 0; j < MS_ITEMS; j++) {
         u64 e = base + (u64)j * MS_THREADS + threadIdx.x;
-        if (e < n) { u32 v = V[e]; m = m > v ? m : v; }
+        if (e < n) m = vmax(m, V[e]);
     }
-    for (int d = 32; d >= 1; d >>= 1) { u32 o = __shfl_xor(m, d, SAGPU_WAVE); m = m > o ? m : o; }
+    for (int d = 32; d >= 1; d >>= 1) m = vmax(m, (VT)__shfl_xor(m, d, SAGPU_WAVE));
     if (lane_id() == 0) s_w[threadIdx.x / SAGPU_WAVE] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        u32 r = s_w[0];
-        for (int w = 1; w < 4; w++) r = r > s_w[w] ? r : s_w[w];
+        VT r = s_w[0];
+        for (int w = 1; w < 4; w++) r = vmax(r, s_w[w]);
         tmax[blockIdx.x] = r;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_max_scan_tiles(u32 *tmax, u64 tiles)
+template <typename VT>
+__global__ __launch_bounds__(1024) void k_max_scan_tiles(VT *tmax, u64 tiles)
 {
-    __shared__ u32 s_w[16];
-    __shared__ u32 s_carry;
+    __shared__ VT s_w[16];
+    __shared__ VT s_carry;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
     for (u64 base = 0; base < tiles; base += 1024) {
         u64 i = base + threadIdx.x;
-        u32 v = i < tiles ? tmax[i] : 0u;
-        u32 inc = wave_incl_max(v);
+        VT v = i < tiles ? tmax[i] : (VT)0;
+        VT inc = wave_incl_max_t(v);
         if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
         __syncthreads();
-        u32 c = s_carry;
-        for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = c > s_w[q] ? c : s_w[q];
-        u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
-        if (lane_id() > 0) c = c > prev ? c : prev;
+        VT c = s_carry;
+        for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
+        VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
+        if (lane_id() > 0) c = vmax(c, prev);
         if (i < tiles) tmax[i] = c;
         __syncthreads();
         if (threadIdx.x == 1023) {
-            u32 t = s_carry;
-            for (int q = 0; q < 16; q++) t = t > s_w[q] ? t : s_w[q];
+            VT t = s_carry;
+            for (int q = 0; q < 16; q++) t = vmax(t, s_w[q]);
             s_carry = t;
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(MS_THREADS) void k_max_apply(u32 *__restrict__ V, u64 n, const u32 *__restrict__ tmax)
+template <typename VT>
+__global__ __launch_bounds__(MS_THREADS) void k_max_apply(const VT *V, u64 n, const VT *__restrict__ tmax,
+                                                          u32 *PLCP)
 {
-    __shared__ u32 s_w[4];
+    __shared__ VT s_w[4];
     // blocked layout: thread handles MS_ITEMS consecutive elements
     const u64 base = (u64)blockIdx.x * MS_TILE + (u64)threadIdx.x * MS_ITEMS;
-    u32 v[MS_ITEMS];
-    u32 m = 0;
+    VT v[MS_ITEMS];
+    VT m = 0;
 #pragma unroll
     for (int j = 0; j < MS_ITEMS; j++) {
         u64 e = base + j;
-        v[j] = e < n ? V[e] : 0u;
-        m = m > v[j] ? m : v[j];
+        v[j] = e < n ? V[e] : (VT)0;
+        m = vmax(m, v[j]);
     }
-    u32 inc = wave_incl_max(m);
+    VT inc = wave_incl_max_t(m);
     if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
     __syncthreads();
-    u32 c = tmax[blockIdx.x];
-    for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = c > s_w[q] ? c : s_w[q];
-    u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
-    if (lane_id() > 0) c = c > prev ? c : prev;
+    VT c = tmax[blockIdx.x];
+    for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
+    VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
+    if (lane_id() > 0) c = vmax(c, prev);
 #pragma unroll
     for (int j = 0; j < MS_ITEMS; j++) {
         u64 e = base + j;
-        c = c > v[j] ? c : v[j];
-        if (e < n) V[e] = (u32)(c - e);
+        c = vmax(c, v[j]);
+        if (e < n) PLCP[e] = (u32)(c - e);
     }
 }
 
@@ -323,24 +348,33 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
     return SAGPU_OK;
 }
 
-int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, const u32 *marks, hipStream_t s)
+template <typename VT>
+int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
+                  u32 *PLCP, VT *V, const u32 *marks, hipStream_t s)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {
         sagpu_set_error("plcp scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
+    VT *tmax = (VT *)ls.tile_max;
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
-    hipLaunchKernelGGL(k_plcp_irr, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP,
+    hipLaunchKernelGGL(k_plcp_irr<VT>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V,
                        ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
-    hipLaunchKernelGGL(k_plcp_long, dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, PLCP, ls.queue, ls.queue_l,
+    hipLaunchKernelGGL(k_plcp_long<VT>, dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
                        ls.queue_n, ls.queue_cap);
-    hipLaunchKernelGGL(k_max_reduce, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, PLCP, n, ls.tile_max);
-    hipLaunchKernelGGL(k_max_scan_tiles, dim3(1), dim3(1024), 0, s, ls.tile_max, tiles);
-    hipLaunchKernelGGL(k_max_apply, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, PLCP, n, ls.tile_max);
+    hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
+    hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
+    hipLaunchKernelGGL(k_max_apply<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax, PLCP);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
+}
+
+int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
+                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s)
+{
+    if (V64) return launch_plcp_t<u64>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s);
+    return launch_plcp_t<u32>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s);
 }
 
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)

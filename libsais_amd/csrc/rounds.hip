@@ -8,8 +8,9 @@
 //
 // Round with offset h (all group members share their first h symbols):
 //   k_round_gather   key2[k] = rank of i+h (or a length code past the end), and
-//                    SA[g] = end of the group in the list (SA slots of unsorted
-//                    groups are free until their suffixes are placed)
+//                    SA[g] = list index of the group's last member (SA slots of
+//                    unsorted groups are free until their suffixes are placed;
+//                    the last index, not the end, so 2^32 members still fit)
 //   k_round_tile     groups of <= RT_SMALL members whose first element lies in a
 //                    2048-wide stretch of the list: ranked by key2 inside LDS,
 //                    regrouped, ISA/SA written, survivors emitted
@@ -28,7 +29,8 @@ namespace {
 // max-scans (positions of the latest group starts).
 
 struct Agg {
-    u32 s, m1, m2;
+    u64 s;        // active count (u64: a text of 2^32 suffixes may have them all active)
+    u32 m1, m2;
 };
 
 __device__ __forceinline__ u32 umax(u32 a, u32 b) { return a > b ? a : b; }
@@ -41,7 +43,8 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
     const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
-    u32 s = 0, m1 = 0, m2 = 0;
+    u64 s = 0;
+    u32 m1 = 0, m2 = 0;
     for (int r = 0; r < WS_ROWS; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         if (e < f.n) {
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
 // Exclusive scan of tile aggregates (one workgroup).  Writes the grand total of
 // active elements and, when out_counter is given, reserves that many slots in the
 // output list: *base = atomicAdd(out_counter, total).
-__global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u32 *total, u32 *out_counter, u32 *base)
+__global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u64 *total, u64 *out_counter, u64 *base)
 {
     __shared__ Agg s_w[16];
     __shared__ Agg s_carry;
@@ -83,7 +86,8 @@ __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u3
     for (u64 b0 = 0; b0 < tiles; b0 += 1024) {
         u64 i = b0 + threadIdx.x;
         Agg v = i < tiles ? tagg[i] : Agg{0, 0, 0};
-        u32 is = wave_incl_sum(v.s), i1 = wave_incl_max(v.m1), i2 = wave_incl_max(v.m2);
+        u64 is = wave_incl_sum64(v.s);
+        u32 i1 = wave_incl_max(v.m1), i2 = wave_incl_max(v.m2);
         const u32 w = threadIdx.x / SAGPU_WAVE;
         if (lane_id() == SAGPU_WAVE - 1) s_w[w] = Agg{is, i1, i2};
         __syncthreads();
@@ -119,14 +123,15 @@ __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u3
 }
 
 template <class F>
-__global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restrict__ texcl, const u32 *__restrict__ base)
+__global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restrict__ texcl, const u64 *__restrict__ base)
 {
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
     const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
     bool act[WS_ROWS];
     u32 v1[WS_ROWS], v2[WS_ROWS];
-    u32 s = 0, m1 = 0, m2 = 0;
+    u64 s = 0;
+    u32 m1 = 0, m2 = 0;
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
@@ -150,14 +155,14 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
         c.m1 = umax(c.m1, s_w[q].m1);
         c.m2 = umax(c.m2, s_w[q].m2);
     }
-    const u32 ob = base ? *base : 0u;
+    const u64 ob = base ? *base : 0ull;
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         u32 is = wave_incl_sum((u32)act[r]);
         u32 i1 = umax(c.m1, wave_incl_max(v1[r]));
         u32 i2 = umax(c.m2, wave_incl_max(v2[r]));
-        if (e < f.n) f.emit(e, act[r], ob + c.s + is - (u32)act[r], i1, i2);
+        if (e < f.n) f.emit(e, act[r], ob + c.s + is - (u64)act[r], i1, i2);
         c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m1 = __shfl(i1, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m2 = __shfl(i2, SAGPU_WAVE - 1, SAGPU_WAVE);
@@ -211,7 +216,7 @@ struct InitF {
         }
         return l;
     }
-    __device__ void emit(u64 e, bool act, u32 o, u32 g, u32) const
+    __device__ void emit(u64 e, bool act, u64 o, u32 g, u32) const
     {
         u32 v = vals[e];
         G[e] = g;
@@ -246,11 +251,12 @@ struct InitF {
 
 // ------------------------------------------------------- big-group emission
 struct BigF {
-    const u64 *comp;     // (dense big-group index << 32) | key2, sorted
+    const u64 *comp;     // (dense big-group index << kb) | key2, sorted
     const u32 *vals;     // suffix index
     const u32 *gslot;    // first SA slot of each big group
     u64 n;
     u32 h;
+    int kb;              // key2 width: 32, or 33 for texts longer than 2^31 - 1
     u32 *SA, *ISA, *out_i, *out_g, *LCP;
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
@@ -260,16 +266,16 @@ struct BigF {
         u64 cn = e + 1 < n ? comp[e + 1] : ~0ull;
         bool head = (e == 0) || c != cp;
         bool tail = (e + 1 == n) || c != cn;
-        bool gch = (e == 0) || (c >> 32) != (cp >> 32);
+        bool gch = (e == 0) || (c >> kb) != (cp >> kb);
         act = !(head && tail);
         v1 = gch ? (u32)e : 0u;
         v2 = head ? (u32)e : 0u;
     }
-    __device__ void emit(u64 e, bool act, u32 o, u32 gstart, u32 hstart) const
+    __device__ void emit(u64 e, bool act, u64 o, u32 gstart, u32 hstart) const
     {
         u32 v = vals[e];
         u64 c = comp[e];
-        u32 g = gslot[(u32)(c >> 32)];
+        u32 g = gslot[(u32)(c >> kb)];
         u32 r = g + (hstart - gstart);
         ISA[v] = r;
         if (act) {
@@ -283,7 +289,7 @@ struct BigF {
 };
 
 template <class F>
-int ws_run(WsScratch &ws, const F &f, u64 n, u32 *out_counter, hipStream_t s)
+int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
 {
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
@@ -294,7 +300,7 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u32 *out_counter, hipStream_t s)
     hipLaunchKernelGGL(k_ws_reduce<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, out_counter, ws.base);
     hipLaunchKernelGGL(k_ws_emit<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg,
-                       out_counter ? ws.base : (const u32 *)nullptr);
+                       out_counter ? ws.base : (const u64 *)nullptr);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -303,7 +309,7 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u32 *out_counter, hipStream_t s)
 // 64 Mi-entry window: random 4-byte writes confined to 256 MiB are merged by the
 // memory-side cache instead of costing a partial-line write each.
 __global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals, const u32 *__restrict__ G, u64 n,
-                                                    u32 *__restrict__ ISA, u32 lo, u32 hi)
+                                                    u32 *__restrict__ ISA, u64 lo, u64 hi)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
@@ -313,36 +319,40 @@ __global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals
 }
 
 // ------------------------------------------------------------- round kernels
+template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                       u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
-                                                      u32 *__restrict__ key2, u32 *__restrict__ SA)
+                                                      KT *__restrict__ key2, u32 *__restrict__ SA)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < a; k += stride) {
         u32 i = act_i[k];
         u32 g = act_g[k];
         u64 ih = (u64)i + h;
-        key2[k] = ih < n ? (u32)(h + ISA[ih]) : (u32)(n - 1 - i);
-        if (k + 1 == a || act_g[k + 1] != g) SA[g] = (u32)(k + 1);
+        key2[k] = ih < n ? (KT)(h + ISA[ih]) : (KT)(n - 1 - i);
+        if (k + 1 == a || act_g[k + 1] != g) SA[g] = (u32)k;
     }
 }
 
 constexpr int RT_THREADS = 256, RT_RANGE = 2048, RT_CAP = RT_RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
 static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
 
+template <typename KT>
 __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
-                                                           const u32 *__restrict__ key2, u64 a, u32 h,
+                                                           const KT *__restrict__ key2, u64 a, u32 h,
                                                            u32 *__restrict__ SA, u32 *__restrict__ ISA,
                                                            u32 *__restrict__ out_i, u32 *__restrict__ out_g,
-                                                           u32 *__restrict__ out_counter, u32 *__restrict__ big_head,
-                                                           u32 *__restrict__ big_g, u32 *__restrict__ big_size,
-                                                           u32 *__restrict__ big_counter, u64 big_cap,
-                                                           u32 *__restrict__ LCP)
+                                                           u64 *__restrict__ out_counter, u32 *__restrict__ err,
+                                                           u32 *__restrict__ big_head, u32 *__restrict__ big_g,
+                                                           u32 *__restrict__ big_size, u32 *__restrict__ big_counter,
+                                                           u64 big_cap, u32 *__restrict__ LCP)
 {
-    __shared__ u32 s_i[RT_CAP], s_k[RT_CAP], s_g[RT_CAP], s_x0[RT_CAP];
+    __shared__ u32 s_i[RT_CAP], s_g[RT_CAP], s_x0[RT_CAP];
+    __shared__ KT s_k[RT_CAP];
     __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
     __shared__ uint8_t s_os[RT_CAP];
-    __shared__ u32 s_first, s_end, s_wsum[4], s_base;
+    __shared__ u32 s_first, s_end, s_wsum[4];
+    __shared__ u64 s_base;
 
     const u32 tid = threadIdx.x;
     const u64 lo = (u64)blockIdx.x * RT_RANGE;
@@ -358,8 +368,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     for (u64 k = lo + tid; k < hi; k += RT_THREADS) {
         u32 g = act_g[k];
         if (k == 0 || act_g[k - 1] != g) {
-            u32 end = SA[g];
-            u32 size = end - (u32)k;
+            const u64 end = (u64)SA[g] + 1;
+            const u64 size = end - k;
             if (size <= RT_SMALL) {
                 atomicMin(&s_first, (u32)(k - lo));
                 atomicMax(&s_end, (u32)(end - lo));
@@ -368,7 +378,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 if (slot < big_cap) {
                     big_head[slot] = (u32)k;
                     big_g[slot] = g;
-                    big_size[slot] = size;
+                    big_size[slot] = (u32)(size - 1);  // size - 1: a group may hold 2^32 members
                 }
             }
         }
@@ -378,7 +388,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     const u32 first = s_first, len = s_end - s_first;
     if (s_end < s_first || len > RT_CAP) {
         // inconsistent group bookkeeping: report instead of overrunning LDS
-        if (tid == 0) atomicOr(out_counter + 2, 1u);
+        if (tid == 0) atomicOr(err, 1u);
         return;
     }
     const u64 gfirst = lo + first;
@@ -422,12 +432,13 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     for (u32 x = tid; x < len; x += RT_THREADS) {
         const u32 x0 = s_x0[x];
         const u32 g = s_g[x];
-        const u32 gsz = SA[g] - (u32)(gfirst + x0); // group end was stored at its first slot
-        if (gsz > RT_SMALL) continue;              // big group: not ours
-        const u32 key = s_k[x];
+        const u64 gsz64 = (u64)SA[g] + 1 - (gfirst + x0);  // last index was stored at the first slot
+        if (gsz64 > RT_SMALL) continue;                    // big group: not ours
+        const u32 gsz = (u32)gsz64;
+        const KT key = s_k[x];
         u32 lt = 0, eq = 0, eqb = 0;
         for (u32 y = x0; y < x0 + gsz; y++) {
-            u32 ky = s_k[y];
+            KT ky = s_k[y];
             lt += ky < key;
             eq += ky == key;
             eqb += (ky == key) & (y < x);
@@ -464,10 +475,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         __syncthreads();
         if (tid == 0) {
             u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-            s_base = all ? atomicAdd(out_counter, all) : 0u;
+            s_base = all ? atomicAdd(out_counter, (u64)all) : 0ull;
         }
         __syncthreads();
-        u32 o = s_base + inc - tot;
+        u64 o = s_base + inc - tot;
         for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
@@ -483,8 +494,9 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
 
 // Copies the members of the big groups (sorted by list position) into the sort
 // buffers: comp = (dense group index << 32) | key2, val = suffix index.
-__global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act_i, const u32 *__restrict__ key2,
-                                                     const u32 *__restrict__ bhead, const u32 *__restrict__ boff,
+template <typename KT>
+__global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
+                                                     const u32 *__restrict__ bhead, const u64 *__restrict__ boff,
                                                      u32 nb, u64 total, u64 *__restrict__ comp, u32 *__restrict__ vals)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -496,7 +508,7 @@ __global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act
             else hi = mid - 1;
         }
         u64 k = (u64)bhead[lo] + (e - boff[lo]);
-        comp[e] = ((u64)lo << 32) | key2[k];
+        comp[e] = ((u64)lo << (sizeof(KT) == 4 ? 32 : 33)) | (u64)key2[k];
         vals[e] = act_i[k];
     }
 }
@@ -520,8 +532,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     const u64 win = 1ull << 26;
     for (u64 lo = 0; lo < n; lo += win) {
         u64 hi = lo + win < n ? lo + win : n;
-        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, (u32)lo,
-                           (u32)hi);
+        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi);
     }
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
@@ -539,39 +550,67 @@ int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bit
     return group_init_t<u64>(ws, keys, vals, n, bits, ksym, 1, SA, ISA, act_i, act_g, LCP, G, s);
 }
 
-int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, u32 *key2, u32 *SA,
-                 hipStream_t s)
+template <typename KT>
+int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
+                   hipStream_t s)
 {
-    hipLaunchKernelGGL(k_round_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA, n, h,
-                       key2, SA);
+    hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
+                       n, h, key2, SA);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int round_tile(const u32 *act_i, const u32 *act_g, const u32 *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
-               u32 *out_g, u32 *out_counter, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter, u64 big_cap,
-               u32 *LCP, hipStream_t s)
+template <typename KT>
+int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
+                 u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter,
+                 u64 big_cap, u32 *LCP, hipStream_t s)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    hipLaunchKernelGGL(k_round_tile, dim3(ceil_div_u(a, RT_RANGE)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, hh,
-                       SA, ISA, out_i, out_g, out_counter, big_head, big_g, big_size, big_counter, big_cap, LCP);
+    hipLaunchKernelGGL(k_round_tile<KT>, dim3(ceil_div_u(a, RT_RANGE)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a,
+                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,
+                       LCP);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int big_extract(const u32 *act_i, const u32 *key2, const u32 *bhead, const u32 *boff, u32 nb, u64 total, u64 *comp,
-                u32 *vals, hipStream_t s)
+template <typename KT>
+int big_extract_t(const u32 *act_i, const KT *key2, const u32 *bhead, const u64 *boff, u32 nb, u64 total, u64 *comp,
+                  u32 *vals, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_big_extract, dim3(grid_for(total, 256 * 4, 65535)), dim3(256), 0, s, act_i, key2, bhead, boff,
-                       nb, total, comp, vals);
+    hipLaunchKernelGGL(k_big_extract<KT>, dim3(grid_for(total, 256 * 4, 65535)), dim3(256), 0, s, act_i, key2, bhead,
+                       boff, nb, total, comp, vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, u32 *SA, u32 *ISA,
-             u32 *out_i, u32 *out_g, u32 *out_counter, u32 *LCP, hipStream_t s)
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
+                 u32 *SA, hipStream_t s)
+{
+    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s)
+                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s);
+}
+
+int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
+               u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s)
+{
+    return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s)
+                : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s);
+}
+
+int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
+                u64 *comp, u32 *vals, hipStream_t s)
+{
+    return wide ? big_extract_t<u64>(act_i, (const u64 *)key2, bhead, boff, nb, total, comp, vals, s)
+                : big_extract_t<u32>(act_i, (const u32 *)key2, bhead, boff, nb, total, comp, vals, s);
+}
+
+int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    BigF f{comp, vals, gslot, total, hh, SA, ISA, out_i, out_g, LCP};
+    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP};
     return ws_run(ws, f, total, out_counter, s);
 }

@@ -76,6 +76,11 @@ def ref():
         for name in ("libsais64_plcp", "libsais64_plcp_gsa", "libsais64_lcp"):
             getattr(r, name).argtypes = [_vp, _vp, _vp, _i64]
             getattr(r, name).restype = _i64
+        r.libsais64_omp.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64]
+        r.libsais64_omp.restype = _i64
+        for name in ("libsais64_plcp_omp", "libsais64_lcp_omp"):
+            getattr(r, name).argtypes = [_vp, _vp, _vp, _i64, _i64]
+            getattr(r, name).restype = _i64
         _r = r
     return _r
 
@@ -176,3 +181,47 @@ def ref_lcp(P, SA, threads=0):
     else:
         r.libsais_lcp(_p(P), _p(SA), _p(L), n)
     return L[:n]
+
+
+# ------------------------------------------------------- 64-bit (n > INT32_MAX)
+def bwt64(T, SA):
+    """Oracle BWT from an int64 SA: (U, primary)."""
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint8)
+    o = oracle()
+    o.oracle_bwt.argtypes = [_vp, _vp, _vp, _i64]
+    o.oracle_bwt.restype = _i64
+    idx = o.oracle_bwt(_p(T), _p(SA), _p(U), n)
+    return U[:n], idx
+
+
+def ref64_sa(T, threads):
+    n = len(T)
+    SA = np.empty(max(n, 1), np.int64)
+    f = np.zeros(256, np.int64)
+    rc = ref().libsais64_omp(_p(T), _p(SA), n, 0, _p(f), threads)
+    return rc, SA[:n], f
+
+
+def ref64_plcp(T, SA, threads):
+    n = len(T)
+    P = np.empty(max(n, 1), np.int64)
+    rc = ref().libsais64_plcp_omp(_p(T), _p(SA), _p(P), n, threads)
+    return rc, P[:n]
+
+
+def ref64_lcp(P, SA, threads):
+    n = len(SA)
+    L = np.empty(max(n, 1), np.int64)
+    rc = ref().libsais64_lcp_omp(_p(P), _p(SA), _p(L), n, threads)
+    return rc, L[:n]
+
+
+def chunked_equal(a, b, chunk=1 << 26):
+    """Array equality without an n-sized temporary."""
+    if len(a) != len(b):
+        return False
+    for o in range(0, len(a), chunk):
+        if not np.array_equal(a[o:o + chunk], b[o:o + chunk]):
+            return False
+    return True

@@ -1,0 +1,96 @@
+"""The 4 GiB libsais64 case: n = 2^32 random ACGT bytes through libsais64,
+libsais64_bwt, libsais64_plcp and libsais64_lcp, compared bit for bit with the
+reference libsais64 (OpenMP, 16 threads) and the oracle BWT on the same text.
+
+Needs about 150 GiB of host memory and a few minutes, so it runs only when
+SAGPU_C3D=1 is set:  SAGPU_C3D=1 python -u -m pytest -s tests/test_gpu_c3d.py
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import libsais_amd as la
+from libsais_amd import workloads
+from tests import oracle_lib as ol
+
+N = 1 << 32
+THREADS = 16
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow,
+              pytest.mark.skipif(os.environ.get("SAGPU_C3D") != "1",
+                                 reason="4 GiB case: set SAGPU_C3D=1 (about 150 GiB host memory)"),
+              pytest.mark.skipif(not ol.have_ref(), reason="compiled reference not built")]
+
+
+def _log(msg):
+    print(f"[c3d {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def text():
+    t = time.perf_counter()
+    T = workloads.iid(N, seed=2026, alphabet=b"ACGT")
+    _log(f"text n={N} generated in {time.perf_counter() - t:.1f}s")
+    return T
+
+
+@pytest.fixture(scope="module")
+def ref_sa(text):
+    t = time.perf_counter()
+    rc, SA, f = ol.ref64_sa(text, THREADS)
+    assert rc == 0
+    _log(f"reference libsais64_omp({THREADS}) {time.perf_counter() - t:.1f}s")
+    return SA, f
+
+
+@pytest.mark.timeout(1200)
+def test_c3d_sa(text, ref_sa):
+    SA_ref, f_ref = ref_sa
+    SA = np.empty(N, np.int64)
+    f = np.zeros(256, np.int64)
+    t = time.perf_counter()
+    assert la.libsais64(text, SA, N, 0, f) == 0, la.last_error()
+    _log(f"gpu libsais64 {time.perf_counter() - t:.1f}s (host buffers, int64 SA)")
+    assert np.array_equal(f, f_ref)
+    assert ol.chunked_equal(SA, SA_ref)
+    _log("SA identical")
+
+
+@pytest.mark.timeout(1200)
+def test_c3d_bwt(text, ref_sa):
+    SA_ref, _ = ref_sa
+    U = np.empty(N, np.uint8)
+    A = np.empty(1, np.int64)  # accepted, not used
+    t = time.perf_counter()
+    idx = la.libsais64_bwt(text, U, A, N, 0, None)
+    _log(f"gpu libsais64_bwt {time.perf_counter() - t:.1f}s primary={idx}")
+    t = time.perf_counter()
+    U_ref, idx_ref = ol.bwt64(text, SA_ref)
+    _log(f"oracle bwt {time.perf_counter() - t:.1f}s")
+    assert idx == idx_ref
+    assert ol.chunked_equal(U, U_ref)
+    _log("BWT identical")
+
+
+@pytest.mark.timeout(1200)
+def test_c3d_plcp_lcp(text, ref_sa):
+    SA_ref, _ = ref_sa
+    P = np.empty(N, np.int64)
+    t = time.perf_counter()
+    assert la.libsais64_plcp(text, SA_ref, P, N) == 0, la.last_error()
+    _log(f"gpu libsais64_plcp {time.perf_counter() - t:.1f}s")
+    rc, P_ref = ol.ref64_plcp(text, SA_ref, THREADS)
+    assert rc == 0
+    assert ol.chunked_equal(P, P_ref)
+    _log("PLCP identical")
+    del P
+    L = np.empty(N, np.int64)
+    t = time.perf_counter()
+    assert la.libsais64_lcp(P_ref, SA_ref, L, N) == 0, la.last_error()
+    _log(f"gpu libsais64_lcp {time.perf_counter() - t:.1f}s")
+    rc, L_ref = ol.ref64_lcp(P_ref, SA_ref, THREADS)
+    assert rc == 0
+    assert ol.chunked_equal(L, L_ref)
+    _log(f"LCP identical (max lcp {int(L_ref.max())})")
