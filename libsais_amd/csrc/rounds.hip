@@ -529,7 +529,8 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
 {
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP};
     SAGPU_TRY(ws_run(ws, f, n, nullptr, s));
-    const u64 win = 1ull << 26;
+    // 64 Mi-entry windows, but at most 8 passes over vals/G for very long texts
+    const u64 win = n > (8ull << 26) ? (n + 7) / 8 : 1ull << 26;
     for (u64 lo = 0; lo < n; lo += win) {
         u64 hi = lo + win < n ? lo + win : n;
         hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi);
