@@ -3,10 +3,10 @@
  *
  * Same prototypes, version macros and return-code conventions as libsais 2.10.4
  * (include/libsais.h of the reference), for the entry points this engine provides:
- * suffix array, generalized suffix array, BWT, PLCP (plain and GSA) and LCP.  The
- * reference's context (_ctx), OpenMP (_omp), integer-alphabet (_int), sampled-BWT
- * (_aux) and inverse-BWT (unbwt) functions are not provided, so a program that uses
- * them fails at link time instead of silently running something else.
+ * suffix array, generalized suffix array, BWT (plain and with auxiliary indexes),
+ * inverse BWT, PLCP (plain and GSA) and LCP.  The reference's context (_ctx),
+ * OpenMP (_omp) and integer-alphabet (_int) functions are not provided, so a program
+ * that uses them fails at link time instead of silently running something else.
  *
  * All buffers are caller-owned host memory; calls are synchronous and may be made
  * from several host threads at once.  Work runs on GPU 0.
@@ -61,6 +61,29 @@ LIBSAIS_API int32_t libsais_gsa(const uint8_t * T, int32_t * SA, int32_t n, int3
  * or -1 / -2.  Replaces reference include/libsais.h:182 (src/libsais.c:7097-7121).
  */
 LIBSAIS_API int32_t libsais_bwt(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq);
+
+/*
+ * BWT with auxiliary indexes: U as libsais_bwt, and I[k] = 1 + the SA slot of suffix
+ * k*r for k = 0..(n-1)/r (I[0] is the primary index).  r must be a power of two >= 2.
+ * Returns 0, or -1 / -2.  Replaces reference include/libsais.h:196
+ * (src/libsais.c:7123-7147, sampling at 4811).
+ */
+LIBSAIS_API int32_t libsais_bwt_aux(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq, int32_t r, int32_t * I);
+
+/*
+ * Inverse BWT: T holds the BWT (libsais_bwt layout) with primary index i; the original
+ * string is written to U (U may alias T).  A (n+1 entries) is accepted and not used;
+ * freq is accepted and not used (the counts are taken from T).  Returns 0 or -1.
+ * Replaces reference include/libsais.h:289 (src/libsais.c:8030-8033).
+ */
+LIBSAIS_API int32_t libsais_unbwt(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t i);
+
+/*
+ * Inverse BWT with the auxiliary indexes of libsais_bwt_aux (r = n takes I[0] as the
+ * primary index).  Returns 0 or -1.
+ * Replaces reference include/libsais.h:315 (src/libsais.c:8040-8056).
+ */
+LIBSAIS_API int32_t libsais_unbwt_aux(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t r, const int32_t * I);
 
 /*
  * Permuted LCP array: PLCP[i] = lcp of suffix i and the suffix preceding it in SA.

@@ -2,7 +2,8 @@
  * libsais64.h -- 64-bit drop-in header of the MI355X suffix-array engine.
  *
  * Same prototypes and conventions as libsais 2.10.4's include/libsais64.h for the
- * suffix array, GSA, BWT, PLCP and LCP entry points.  For n <= INT32_MAX the results
+ * suffix array, GSA, BWT (plain and auxiliary), inverse BWT, PLCP and LCP entry
+ * points.  For n <= INT32_MAX the results
  * equal the 32-bit functions' results widened to int64 (as the reference delegates,
  * src/libsais64.c:7071-7083).
  */
@@ -42,6 +43,15 @@ LIBSAIS64_API int64_t libsais64_gsa(const uint8_t * T, int64_t * SA, int64_t n, 
 
 /* Replaces reference include/libsais64.h:135 (src/libsais64.c:7133-7170). */
 LIBSAIS64_API int64_t libsais64_bwt(const uint8_t * T, uint8_t * U, int64_t * A, int64_t n, int64_t fs, int64_t * freq);
+
+/* Replaces reference include/libsais64.h:149 (src/libsais64.c:7172-7209). */
+LIBSAIS64_API int64_t libsais64_bwt_aux(const uint8_t * T, uint8_t * U, int64_t * A, int64_t n, int64_t fs, int64_t * freq, int64_t r, int64_t * I);
+
+/* Replaces reference include/libsais64.h:191 (src/libsais64.c:8034-8037).  n < 2^32. */
+LIBSAIS64_API int64_t libsais64_unbwt(const uint8_t * T, uint8_t * U, int64_t * A, int64_t n, const int64_t * freq, int64_t i);
+
+/* Replaces reference include/libsais64.h:204 (src/libsais64.c:8039-8064).  n < 2^32. */
+LIBSAIS64_API int64_t libsais64_unbwt_aux(const uint8_t * T, uint8_t * U, int64_t * A, int64_t n, const int64_t * freq, int64_t r, const int64_t * I);
 
 /* Replaces reference include/libsais64.h:243 (src/libsais64.c:8302-8318). */
 LIBSAIS64_API int64_t libsais64_plcp(const uint8_t * T, const int64_t * SA, int64_t * PLCP, int64_t n);

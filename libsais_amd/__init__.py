@@ -83,6 +83,9 @@ ABI_32 = [
     ("libsais", _i32, [_vp, _vp, _i32, _i32, _vp]),
     ("libsais_gsa", _i32, [_vp, _vp, _i32, _i32, _vp]),
     ("libsais_bwt", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais_bwt_aux", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]),
+    ("libsais_unbwt", _i32, [_vp, _vp, _vp, _i32, _vp, _i32]),
+    ("libsais_unbwt_aux", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _vp]),
     ("libsais_plcp", _i32, [_vp, _vp, _vp, _i32]),
     ("libsais_plcp_gsa", _i32, [_vp, _vp, _vp, _i32]),
     ("libsais_lcp", _i32, [_vp, _vp, _vp, _i32]),
@@ -91,6 +94,9 @@ ABI_64 = [
     ("libsais64", _i64, [_vp, _vp, _i64, _i64, _vp]),
     ("libsais64_gsa", _i64, [_vp, _vp, _i64, _i64, _vp]),
     ("libsais64_bwt", _i64, [_vp, _vp, _vp, _i64, _i64, _vp]),
+    ("libsais64_bwt_aux", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp]),
+    ("libsais64_unbwt", _i64, [_vp, _vp, _vp, _i64, _vp, _i64]),
+    ("libsais64_unbwt_aux", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _vp]),
     ("libsais64_plcp", _i64, [_vp, _vp, _vp, _i64]),
     ("libsais64_plcp_gsa", _i64, [_vp, _vp, _vp, _i64]),
     ("libsais64_lcp", _i64, [_vp, _vp, _vp, _i64]),
@@ -161,12 +167,18 @@ def _mk(name):
 libsais = _mk("libsais")
 libsais_gsa = _mk("libsais_gsa")
 libsais_bwt = _mk("libsais_bwt")
+libsais_bwt_aux = _mk("libsais_bwt_aux")
+libsais_unbwt = _mk("libsais_unbwt")
+libsais_unbwt_aux = _mk("libsais_unbwt_aux")
 libsais_plcp = _mk("libsais_plcp")
 libsais_plcp_gsa = _mk("libsais_plcp_gsa")
 libsais_lcp = _mk("libsais_lcp")
 libsais64 = _mk("libsais64")
 libsais64_gsa = _mk("libsais64_gsa")
 libsais64_bwt = _mk("libsais64_bwt")
+libsais64_bwt_aux = _mk("libsais64_bwt_aux")
+libsais64_unbwt = _mk("libsais64_unbwt")
+libsais64_unbwt_aux = _mk("libsais64_unbwt_aux")
 libsais64_plcp = _mk("libsais64_plcp")
 libsais64_plcp_gsa = _mk("libsais64_plcp_gsa")
 libsais64_lcp = _mk("libsais64_lcp")
@@ -201,6 +213,31 @@ def bwt(T: np.ndarray):
     if idx < 0:
         raise RuntimeError(f"libsais_bwt returned {idx}: {last_error()}")
     return U[:n], idx
+
+
+def bwt_aux(T: np.ndarray, r: int):
+    """(U, I) exactly as libsais_bwt_aux."""
+    T = np.ascontiguousarray(T, dtype=np.uint8)
+    n = len(T)
+    U = np.zeros(max(n, 1), dtype=np.uint8)
+    A = np.zeros(1, dtype=np.int32)
+    I = np.zeros((n - 1) // r + 1 if n > 0 else 1, dtype=np.int32)
+    rc = libsais_bwt_aux(T, U, A, n, 0, None, r, I)
+    if rc != 0:
+        raise RuntimeError(f"libsais_bwt_aux returned {rc}: {last_error()}")
+    return U[:n], I
+
+
+def unbwt(U: np.ndarray, primary: int):
+    """Original string from a BWT and its primary index (libsais_unbwt)."""
+    U = np.ascontiguousarray(U, dtype=np.uint8)
+    n = len(U)
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    A = np.zeros(1, dtype=np.int32)
+    rc = libsais_unbwt(U, out, A, n, None, primary)
+    if rc != 0:
+        raise RuntimeError(f"libsais_unbwt returned {rc}: {last_error()}")
+    return out[:n]
 
 
 def plcp(T: np.ndarray, SA: np.ndarray, gsa: bool = False):

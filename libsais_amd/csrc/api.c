@@ -19,6 +19,9 @@ HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_t n, int32_t gsa,
 HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int32_t width);
 HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa, int32_t width);
 HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA, void *LCP, int64_t n, int32_t width);
+HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int64_t r, void *I,
+                                  int32_t width);
+HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, int64_t n, int64_t r, const void *I, int32_t width);
 
 /* Engine codes below -2 (device failures) surface as -2, the only failure code the
  * reference defines besides -1. */
@@ -60,6 +63,46 @@ int32_t libsais_bwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t
         return n;
     }
     return (int32_t)map_rc(sagpu_host_bwt(T, U, n, freq, 4));
+}
+
+/* reference src/libsais.c:7123-7147 */
+int32_t libsais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t fs, int32_t *freq, int32_t r,
+                        int32_t *I)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (r < 2) || ((r & (r - 1)) != 0) ||
+        (I == NULL))
+        return -1;
+    if (n <= 1) {
+        if (freq != NULL) memset(freq, 0, 256 * sizeof(int32_t));
+        if (n == 1) { U[0] = T[0]; if (freq != NULL) freq[T[0]]++; }
+        I[0] = n;
+        return 0;
+    }
+    return (int32_t)map_rc(sagpu_host_bwt_aux(T, U, n, freq, r, I, 4));
+}
+
+/* reference src/libsais.c:8040-8056 (the freq table is not needed: counts come from T) */
+int32_t libsais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, const int32_t *freq, int32_t r,
+                          const int32_t *I)
+{
+    (void)freq;
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) ||
+        (I == NULL))
+        return -1;
+    if (n <= 1) {
+        if (I[0] != n) return -1;
+        if (n == 1) U[0] = T[0];
+        return 0;
+    }
+    for (int64_t t = 0; t <= (int64_t)(n - 1) / r; ++t)
+        if (I[t] <= 0 || I[t] > n) return -1;
+    return (int32_t)map_rc(sagpu_host_unbwt(T, U, n, r, I, 4));
+}
+
+/* reference src/libsais.c:8030-8033 */
+int32_t libsais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, const int32_t *freq, int32_t i)
+{
+    return libsais_unbwt_aux(T, U, A, n, freq, n, &i);
 }
 
 /* reference src/libsais.c:8363-8379 */
@@ -131,6 +174,46 @@ int64_t libsais64_bwt(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, int64
         return n;
     }
     return map_rc(sagpu_host_bwt(T, U, n, freq, 8));
+}
+
+/* reference src/libsais64.c:7172-7209 */
+int64_t libsais64_bwt_aux(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, int64_t fs, int64_t *freq, int64_t r,
+                          int64_t *I)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (r < 2) || ((r & (r - 1)) != 0) ||
+        (I == NULL))
+        return -1;
+    if (n <= 1) {
+        if (freq != NULL) memset(freq, 0, 256 * sizeof(int64_t));
+        if (n == 1) { U[0] = T[0]; if (freq != NULL) freq[T[0]]++; }
+        I[0] = n;
+        return 0;
+    }
+    return map_rc(sagpu_host_bwt_aux(T, U, n, freq, r, I, 8));
+}
+
+/* reference src/libsais64.c:8039-8064 */
+int64_t libsais64_unbwt_aux(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, const int64_t *freq, int64_t r,
+                            const int64_t *I)
+{
+    (void)freq;
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) ||
+        (I == NULL))
+        return -1;
+    if (n <= 1) {
+        if (I[0] != n) return -1;
+        if (n == 1) U[0] = T[0];
+        return 0;
+    }
+    for (int64_t t = 0; t <= (n - 1) / r; ++t)
+        if (I[t] <= 0 || I[t] > n) return -1;
+    return map_rc(sagpu_host_unbwt(T, U, n, r, I, 8));
+}
+
+/* reference src/libsais64.c:8034-8037 */
+int64_t libsais64_unbwt(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, const int64_t *freq, int64_t i)
+{
+    return libsais64_unbwt_aux(T, U, A, n, freq, n, &i);
 }
 
 /* reference src/libsais64.c:8302-8318 */

@@ -118,6 +118,16 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
                 const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
+// BWT auxiliary samples and inverse BWT (unbwt.hip).
+int launch_aux_samples(const u32 *ISA, u64 n, u64 r, u64 count, u32 *out, hipStream_t s);
+u64 unbwt_count_slots(u64 n);
+int launch_unbwt_succ(const uint8_t *U, u64 n, u64 p, u32 *counts, ScanScratch &sc, u32 *S, hipStream_t s);
+int launch_unbwt_aux_walk(const uint8_t *U, const u32 *S, u64 n, u64 p, u64 r, const u64 *rows, u64 chunks,
+                          uint8_t *out, u32 *err, hipStream_t s);
+int launch_unbwt_flags(u64 n, u32 *flags, hipStream_t s);
+int launch_unbwt_sample_total(const u32 *sid, u64 n, u32 *out, hipStream_t s);
+int launch_unbwt_rank(const uint8_t *U, const u32 *S, const u32 *sid, u64 n, u32 m, u32 *samp_q, u32 *nxt[2],
+                      u64 *dist[2], u64 *gap, uint8_t *out, u32 *err, hipStream_t s);
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 
 // Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).

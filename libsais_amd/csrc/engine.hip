@@ -555,13 +555,16 @@ void stats_reset(Ctx &c)
 
 // Fused device pipeline.  Returns 0 or the primary index when U is requested.
 int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *dSA, uint8_t *dU, u32 *dPLCP,
-                  u32 *dLCP, u64 *freq)
+                  u32 *dLCP, u64 *freq, u32 *dISA = nullptr)
 {
     if (!dSA) {
         dSA = sc.get<u32>(n);
         if (!dSA) return SAGPU_ENOMEM;
     }
-    GET(dISA, u32, n);
+    if (!dISA) {
+        dISA = sc.get<u32>(n);
+        if (!dISA) return SAGPU_ENOMEM;
+    }
     const bool want_lcp = dPLCP || dLCP;
     u32 *marks = dLCP;
     if (want_lcp && !marks) {
@@ -714,6 +717,110 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     store_freq(f, freq, width);
     return primary;
+}
+
+// BWT with auxiliary indexes: I[k] = 1 + SA slot of suffix k*r (libsais_bwt_aux).
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int64_t r,
+                                                   void *I, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    uint8_t *dU = sc.get<uint8_t>((u64)n);
+    u32 *dISA = sc.get<u32>((u64)n);
+    const u64 count = ((u64)n - 1) / (u64)r + 1;
+    u32 *dS = sc.get<u32>(count);
+    if (!dU || !dISA || !dS) return SAGPU_ENOMEM;
+    u64 f[256];
+    int64_t primary = build_dev(*c, sc, dT, (u64)n, 0, nullptr, dU, nullptr, nullptr, f, dISA);
+    if (primary < 0) return primary;
+    SAGPU_TRY(launch_aux_samples(dISA, (u64)n, (u64)r, count, dS, c->stream));
+    std::vector<u32> hs(count);
+    SAGPU_CHECK(hipMemcpyAsync(hs.data(), dS, count * 4, hipMemcpyDeviceToHost, c->stream));
+    SAGPU_CHECK(hipMemcpyAsync(U, dU, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    for (u64 k = 0; k < count; k++) {
+        if (width == 4) ((int32_t *)I)[k] = (int32_t)(hs[k] + 1);
+        else ((int64_t *)I)[k] = (int64_t)hs[k] + 1;
+    }
+    store_freq(f, freq, width);
+    return 0;
+}
+
+// Inverse BWT (libsais_unbwt / libsais_unbwt_aux).  T = the BWT, U = the output
+// text, I[0..(n-1)/r] already range-checked by the caller, r == n for plain unbwt.
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, int64_t n, int64_t r, const void *I,
+                                                 int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    const u64 N = (u64)n, chunks = (N - 1) / (u64)r + 1;
+    std::vector<u64> rows(chunks);
+    for (u64 k = 0; k < chunks; k++)
+        rows[k] = width == 4 ? (u64)((const int32_t *)I)[k] : (u64)((const int64_t *)I)[k];
+    Scope sc(*c);
+    hipStream_t s = c->stream;
+    GET(dB, uint8_t, N);
+    GET(dOut, uint8_t, N);
+    GET(S, u32, N);
+    GET(d_err, u32, 64);
+    const u64 slots = unbwt_count_slots(N);
+    GET(counts, u32, slots);
+    const u64 scap = (slots > N ? slots : N) / 4096 + 2;
+    GET(part, u32, scap);
+    ScanScratch ssc{part, scap};
+    SAGPU_CHECK(hipMemcpyAsync(dB, T, N, hipMemcpyHostToDevice, s));
+    SAGPU_CHECK(hipMemsetAsync(dOut, 0, N, s));
+    SAGPU_TRY(launch_fill_u32(d_err, 2, 0, s));
+    SAGPU_TRY(launch_unbwt_succ(dB, N, rows[0], counts, ssc, S, s));
+    if (r < n && chunks >= 32768) {
+        GET(d_rows, u64, chunks);
+        SAGPU_CHECK(hipMemcpyAsync(d_rows, rows.data(), chunks * 8, hipMemcpyHostToDevice, s));
+        SAGPU_TRY(launch_unbwt_aux_walk(dB, S, N, rows[0], (u64)r, d_rows, chunks, dOut, d_err, s));
+    } else {
+        GET(sid, u32, N);
+        SAGPU_TRY(launch_unbwt_flags(N, sid, s));
+        SAGPU_TRY(scan_exclusive_u32(ssc, sid, N, s));
+        SAGPU_TRY(launch_unbwt_sample_total(sid, N, d_err + 1, s));
+        u32 m = 0;
+        SAGPU_TRY(read_u32(*c, d_err + 1, &m));
+        if (m == 0 || m > N) {
+            sagpu_set_error("inverse BWT: sample count inconsistent", __FILE__, __LINE__);
+            return SAGPU_EDEVICE;
+        }
+        GET(samp_q, u32, m);
+        GET(nxt0, u32, m);
+        GET(nxt1, u32, m);
+        GET(dist0, u64, m);
+        GET(dist1, u64, m);
+        GET(gap, u64, m);
+        u32 *nxt[2] = {nxt0, nxt1};
+        u64 *dist[2] = {dist0, dist1};
+        SAGPU_TRY(launch_unbwt_rank(dB, S, sid, N, m, samp_q, nxt, dist, gap, dOut, d_err, s));
+    }
+    u32 err = 0;
+    SAGPU_TRY(read_u32(*c, d_err, &err));
+    if (err & 2u) {
+        sagpu_set_error("inverse BWT: a sample gap exceeded its step limit", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    SAGPU_CHECK(hipMemcpyAsync(U, dOut, N, hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    return 0;
 }
 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,

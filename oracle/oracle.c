@@ -210,6 +210,51 @@ int64_t oracle_bwt(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n)
     return j0 + 1;
 }
 
+int oracle_bwt_aux(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n, int64_t r, int64_t *aux)
+{
+    if (n <= 1) {
+        if (n == 1) U[0] = T[0];
+        aux[0] = n;
+        return 0;
+    }
+    oracle_bwt(T, SA, U, n);
+    /* libsais.c:4811: when suffix p (p % r == 0) is placed at SA slot j, the
+     * reference records the slot's 1-based position. */
+    for (I j = 0; j < n; j++)
+        if (SA[j] % r == 0) aux[SA[j] / r] = j + 1;
+    return 0;
+}
+
+/* Rows 0..n of the sorted rotations of T$: row 0 starts with $, and row x >= 1 is
+ * SA slot x-1.  The last column is L[0] = U[0], L[p] = $ and L[x] = U[x - (x > p)]
+ * otherwise.  LF(x) = 1 + (number of bytes < L[x]) + (occurrences of L[x] in
+ * L[0..x)), and walking LF from row 0 reads T backwards. */
+int oracle_unbwt(const uint8_t *U, uint8_t *out, int64_t n, int64_t p)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { out[0] = U[0]; return 0; }
+    I cnt[256] = {0}, C[256];
+    for (I q = 0; q < n; q++) cnt[U[q]]++;
+    I acc = 1;
+    for (int c = 0; c < 256; c++) { C[c] = acc; acc += cnt[c]; }
+    I *lf = (I *)malloc((size_t)(n + 1) * sizeof(I));
+    if (!lf) return -2;
+    I seen[256] = {0};
+    for (I x = 0; x <= n; x++) {
+        if (x == p) { lf[x] = 0; continue; }
+        uint8_t c = U[x - (x > p)];
+        lf[x] = C[c] + seen[c]++;
+    }
+    I x = 0;
+    for (I t = n - 1; t >= 0; t--) {
+        out[t] = U[x - (x > p)];
+        x = lf[x];
+        if (x == p) break;  /* only for an inconsistent (U, p): stop at the $ row */
+    }
+    free(lf);
+    return 0;
+}
+
 void oracle_plcp(const uint8_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int gsa)
 {
     if (n <= 0) return;

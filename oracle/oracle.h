@@ -14,6 +14,10 @@
  *   oracle_bwt         compiled reference libsais_bwt, banana/mississippi.
  *   oracle_plcp/lcp    compiled reference, direct per-pair comparison, closed forms.
  *   oracle_gsa_valid   compiled reference over all {0,1,2}-strings up to length 10.
+ *   oracle_bwt_aux     compiled reference libsais_bwt_aux (U and I) over sampling
+ *                      rates 2..n, hand-worked banana samples.
+ *   oracle_unbwt       round trip T -> oracle_bwt -> oracle_unbwt, the compiled
+ *                      reference libsais_unbwt / libsais_unbwt_aux, banana by hand.
  */
 #ifndef SAGPU_ORACLE_H
 #define SAGPU_ORACLE_H
@@ -38,6 +42,17 @@ int     oracle_sa(const uint8_t *T, int64_t *SA, int64_t n, int gsa);
 /* BWT from a suffix array (libsais.c:7110-7118): writes U[0..n-1], returns the
  * primary index (1 + row of suffix 0). */
 int64_t oracle_bwt(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n);
+
+/* BWT with auxiliary indexes (libsais.c:7123-7146 with the sampling of
+ * 4799-4819): U as oracle_bwt, and I[k] = 1 + row of suffix k*r for
+ * k = 0..(n-1)/r, so I[0] is the primary index.  Returns 0. */
+int     oracle_bwt_aux(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n, int64_t r, int64_t *I);
+
+/* Inverse BWT (libsais.c:8030-8055 unbwt / unbwt_aux output for a valid input):
+ * U is the libsais BWT layout with primary index p (1-based row of suffix 0);
+ * writes the original text to out[0..n-1] by the textbook LF walk from row 0.
+ * Returns 0, or -2 on allocation failure. */
+int     oracle_unbwt(const uint8_t *U, uint8_t *out, int64_t n, int64_t p);
 
 /* Phi + Kasai PLCP exactly as libsais.c:8116-8142 (phi), 8167-8190 (plcp) and
  * 8215-8238 (plcp_gsa, gsa=1). */

@@ -76,6 +76,12 @@ def ref():
         for name in ("libsais64_plcp", "libsais64_plcp_gsa", "libsais64_lcp"):
             getattr(r, name).argtypes = [_vp, _vp, _vp, _i64]
             getattr(r, name).restype = _i64
+        r.libsais_bwt_aux.argtypes = [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]
+        r.libsais_bwt_aux.restype = _i32
+        r.libsais_unbwt.argtypes = [_vp, _vp, _vp, _i32, _vp, _i32]
+        r.libsais_unbwt.restype = _i32
+        r.libsais_unbwt_aux.argtypes = [_vp, _vp, _vp, _i32, _vp, _i32, _vp]
+        r.libsais_unbwt_aux.restype = _i32
         r.libsais64_omp.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64]
         r.libsais64_omp.restype = _i64
         for name in ("libsais64_plcp_omp", "libsais64_lcp_omp"):
@@ -225,3 +231,54 @@ def chunked_equal(a, b, chunk=1 << 26):
         if not np.array_equal(a[o:o + chunk], b[o:o + chunk]):
             return False
     return True
+
+
+# ------------------------------------------------------------ BWT aux / inverse
+def bwt_aux(T, SA, r):
+    """Oracle (U, I) of libsais_bwt_aux from a suffix array."""
+    n = len(T)
+    o = oracle()
+    o.oracle_bwt_aux.argtypes = [_vp, _vp, _vp, _i64, _i64, _vp]
+    o.oracle_bwt_aux.restype = ctypes.c_int
+    U = np.zeros(max(n, 1), np.uint8)
+    I = np.zeros((n - 1) // r + 1 if n > 0 else 1, np.int64)
+    S64 = np.ascontiguousarray(SA, np.int64)
+    o.oracle_bwt_aux(_p(T), _p(S64), _p(U), n, r, _p(I))
+    return U[:n], I
+
+
+def unbwt(U, p):
+    """Oracle inverse BWT of (U, primary index)."""
+    n = len(U)
+    o = oracle()
+    o.oracle_unbwt.argtypes = [_vp, _vp, _i64, _i64]
+    o.oracle_unbwt.restype = ctypes.c_int
+    out = np.zeros(max(n, 1), np.uint8)
+    assert o.oracle_unbwt(_p(U), _p(out), n, p) == 0
+    return out[:n]
+
+
+def ref_bwt_aux(T, r):
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint8)
+    A = np.zeros(max(n, 1), np.int32)
+    I = np.zeros((n - 1) // r + 1 if n > 0 else 1, np.int32)
+    rc = ref().libsais_bwt_aux(_p(T), _p(U), _p(A), n, 0, None, r, _p(I))
+    return rc, U[:n], I
+
+
+def ref_unbwt(U, p):
+    n = len(U)
+    out = np.zeros(max(n, 1), np.uint8)
+    A = np.zeros(n + 1, np.int32)
+    rc = ref().libsais_unbwt(_p(U), _p(out), _p(A), n, None, p)
+    return rc, out[:n]
+
+
+def ref_unbwt_aux(U, r, I):
+    n = len(U)
+    out = np.zeros(max(n, 1), np.uint8)
+    A = np.zeros(n + 1, np.int32)
+    I32 = np.ascontiguousarray(I, np.int32)
+    rc = ref().libsais_unbwt_aux(_p(U), _p(out), _p(A), n, None, r, _p(I32))
+    return rc, out[:n]

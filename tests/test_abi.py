@@ -42,7 +42,8 @@ def test_exports_every_declared_symbol():
 def test_no_out_of_scope_exports():
     out = subprocess.run(["nm", "-D", "--defined-only", la.LIB_PATH], capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    for name in ("libsais_ctx", "libsais_omp", "libsais_int", "libsais_unbwt", "libsais_bwt_aux"):
+    for name in ("libsais_ctx", "libsais_omp", "libsais_int", "libsais_create_ctx", "libsais_unbwt_create_ctx",
+                 "libsais16", "libsais64_long"):
         assert name not in exported
 
 
@@ -167,3 +168,63 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(la, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(la.EngineNotBuilt):
         la.libsais(T5, np.zeros(5, np.int32), 5, 0, None)
+
+
+def _aux_calls(mod, T, U, A, I, n, fs, r, freq):
+    """bwt_aux / unbwt / unbwt_aux with the given arguments; the codes and outputs."""
+    return [
+        mod.libsais_bwt_aux(T, U, A, n, fs, freq, r, I),
+        mod.libsais_unbwt(T, U, A, n, None, int(I[0])),
+        mod.libsais_unbwt_aux(T, U, A, n, None, r, I),
+    ]
+
+
+def test_aux_and_unbwt_argument_checks():
+    U = np.zeros(8, np.uint8)
+    A = np.zeros(9, np.int32)
+    I = np.array([1, 2, 3], np.int32)
+    assert la.libsais_bwt_aux(T5, U, A, 5, 0, None, 3, I) == -1   # r not a power of two
+    assert la.libsais_bwt_aux(T5, U, A, 5, 0, None, 1, I) == -1   # r < 2
+    assert la.libsais_bwt_aux(T5, U, A, 5, 0, None, 2, None) == -1
+    assert la.libsais_unbwt_aux(T5, U, A, 5, None, 3, I) == -1
+    assert la.libsais_unbwt_aux(T5, U, None, 5, None, 2, I) == -1
+    bad = np.array([0, 2, 3], np.int32)                          # I[t] must lie in [1, n]
+    assert la.libsais_unbwt_aux(T5, U, A, 5, None, 2, bad) == -1
+    assert la.libsais_unbwt(T5, U, A, 5, None, 6) == -1
+    assert la.libsais_unbwt(T5, U, A, 5, None, 0) == -1
+    I64 = np.array([0, 2, 3], np.int64)
+    A64 = np.zeros(9, np.int64)
+    assert la.libsais64_unbwt_aux(T5, U, A64, 5, None, 2, I64) == -1
+    assert la.libsais64_bwt_aux(T5, U, A64, 5, 0, None, 6, I64) == -1
+    assert la.libsais64_unbwt(T5, U, A64, 5, None, -1) == -1
+
+
+@pytest.mark.skipif(not ol.have_ref(), reason="reference build oracle/_ref absent")
+def test_aux_edge_cases_match_reference():
+    r = ol.ref()
+
+    class Ref:
+        pass
+
+    ref = Ref()
+    for name in ("libsais_bwt_aux", "libsais_unbwt", "libsais_unbwt_aux"):
+        f = getattr(r, name)
+
+        def mk(f):
+            return lambda *a: f(*[x.ctypes.data if isinstance(x, np.ndarray) else x for x in a])
+
+        setattr(ref, name, mk(f))
+    for n in (-1, 0, 1):
+        for fs in (-1, 0):
+            for rr in (1, 2, 3, 4):
+                for i0 in (0, 1, 2):
+                    outs = []
+                    for mod in (la, ref):
+                        T = np.frombuffer(b"\x05\x06", np.uint8).copy()
+                        U = np.full(2, 77, np.uint8)
+                        A = np.zeros(3, np.int32)
+                        I = np.array([i0, 1], np.int32)
+                        freq = np.full(256, 9, np.int32)
+                        codes = _aux_calls(mod, T, U, A, I, n, fs, rr, freq)
+                        outs.append((codes, U.tolist(), I.tolist(), freq.tolist()))
+                    assert outs[0] == outs[1], (n, fs, rr, i0)

@@ -239,3 +239,42 @@ def test_gsa_sets_against_reference():
         assert rc == rc2 == 0
         assert np.array_equal(SA, SA_r)
         assert np.array_equal(ol.plcp(T, SA, gsa=True), ol.ref_plcp(T, SA_r, gsa=True))
+
+
+# ------------------------------------------------------------ BWT aux / inverse
+def test_bwt_aux_and_unbwt_by_hand():
+    T = np.frombuffer(b"banana", np.uint8).copy()
+    SA = np.array([5, 3, 1, 0, 4, 2], np.int64)
+    U, I = ol.bwt_aux(T, SA, 2)
+    assert bytes(U) == b"annbaa" and list(I) == [4, 6, 5]
+    assert bytes(ol.unbwt(U, 4)) == b"banana"
+
+
+def test_unbwt_round_trip_shapes():
+    rng = np.random.default_rng(21)
+    shapes = [workloads.iid(5000, seed=3, alphabet=b"ACGT"), np.full(777, 65, np.uint8),
+              workloads.periodic(4000, 17, 3), workloads.fibonacci(3000),
+              rng.integers(0, 256, 3000).astype(np.uint8), rng.integers(0, 2, 3000).astype(np.uint8)]
+    for T in shapes:
+        rc, SA = ol.sa(T)
+        U, p = ol.bwt(T, SA)
+        assert np.array_equal(ol.unbwt(U, p), T)
+
+
+@needs_ref
+@pytest.mark.parametrize("alphabet", [1, 2, 4, 256])
+def test_bwt_aux_and_unbwt_against_reference(alphabet):
+    rng = np.random.default_rng(100 + alphabet)
+    for n in (2, 3, 31, 1000, 20000):
+        T = rng.integers(0, alphabet, n).astype(np.uint8)
+        rc, SA = ol.sa(T)
+        for r in (2, 4, 64, 1 << 20):
+            rrc, rU, rI = ol.ref_bwt_aux(T, r)
+            assert rrc == 0
+            U, I = ol.bwt_aux(T, SA, r)
+            assert np.array_equal(U, rU) and np.array_equal(I, rI.astype(np.int64)), (n, r)
+            rc2, out = ol.ref_unbwt_aux(rU, r, rI)
+            assert rc2 == 0 and np.array_equal(out, T)
+        U, p = ol.bwt(T, SA)
+        rc3, out = ol.ref_unbwt(U, p)
+        assert rc3 == 0 and np.array_equal(out, ol.unbwt(U, p)) and np.array_equal(out, T)
