@@ -39,6 +39,40 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def copy_peak(lib, nbytes: int = 1 << 30, reps: int = 5):
+    """Device copy bandwidth (read + write bytes / time) of the engine's copy kernel."""
+    src = lib.sagpu_malloc(nbytes)
+    dst = lib.sagpu_malloc(nbytes)
+    if not src or not dst:
+        raise MemoryError("device allocation failed")
+    try:
+        lib.sagpu_memset(src, 1, nbytes)
+        assert lib.sagpu_memcpy_d2d(dst, src, nbytes) == 0
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            assert lib.sagpu_memcpy_d2d(dst, src, nbytes) == 0
+            best = min(best, time.perf_counter() - t0)
+    finally:
+        lib.sagpu_free(src)
+        lib.sagpu_free(dst)
+    return 2 * nbytes / best / 1e9
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per radix-pass launch from the committed PMC summary (rocprofv3
+    FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
+        return None
+    return d.get("radix_pass_bytes_per_launch")
+
+
 def a_r_from_lcp(L: np.ndarray, h0: int = 8):
     """a_r = #{j : max(LCP[j], LCP[j+1]) >= h0 * 2^(r-1)}, LCP[n] := 0."""
     n = len(L)
@@ -177,7 +211,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and serving sections")
-    ap.add_argument("--mode", choices=["main", "serving-full", "batched"], default="main")
+    ap.add_argument("--mode", choices=["main", "serving-full", "batched", "copy"], default="main")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="engine option (sagpu_set_option), repeatable")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -190,11 +226,20 @@ def main():
     from libsais_amd import workloads
 
     lib = la.load()
+    options = {}
+    for kv in args.set:
+        name, _, val = kv.partition("=")
+        if lib.sagpu_set_option(name.encode(), int(val)) != 0:
+            raise SystemExit(f"unknown option or value: {kv}")
+        options[name] = int(val)
     if args.mode == "serving-full":
         print(json.dumps({"serving": bench_serving(2000, 64 << 20)}), flush=True)
         return
     if args.mode == "batched":
         print(json.dumps({"batched": bench_batched()}), flush=True)
+        return
+    if args.mode == "copy":
+        print(json.dumps({"copy_GBs": round(copy_peak(lib), 1), "bytes": 1 << 30}), flush=True)
         return
     n = args.n
     t0 = time.perf_counter()
@@ -259,6 +304,9 @@ def main():
     bytes_per_elem = sweep_bytes / max(1, sweep_elems)
     achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
 
+    workload = "dna_like_256MiB_sa_bwt_plcp_lcp" if n == 1 << 28 else f"dna_like_{n}B_sa_bwt_plcp_lcp"
+    cpeak = copy_peak(lib)
+
     # pipeline yardstick from this run's LCP
     L = np.empty(n, np.int32)
     lib.sagpu_memcpy_d2h(L.ctypes.data, dL, 4 * n)
@@ -280,12 +328,13 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": "dna_like_256MiB_sa_bwt_plcp_lcp" if n == 1 << 28 else f"dna_like_{n}B_sa_bwt_plcp_lcp",
+            "workload": workload,
             "n": n,
             "seed": args.seed,
             "outputs": ["SA", "BWT+primary", "PLCP", "LCP"],
             "index": "int32",
             "inputs_resident": "HBM",
+            **({"options": options} if options else {}),
         },
         "roofline": {
             "bound": "hbm",
@@ -294,7 +343,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(workload),
+            "copy_peak_GBs": round(cpeak, 1),
+            "frac_of_copy": round(achieved / cpeak, 4),
             "launches": sweep_launch,
             "avg_launch_ms": round(sweep_ms / max(1, sweep_launch), 4),
             "algorithmic_bytes_per_elem": round(bytes_per_elem, 2),
@@ -321,7 +372,7 @@ def main():
         result["batched"] = bench_batched()
         log(f"batched: {result['batched']}")
         log("serving trace ...")
-        result["serving"] = bench_serving(300, 16 << 20)
+        result["serving"] = bench_serving(600, 64 << 20)
         log(f"serving: {result['serving']}")
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -37,6 +37,9 @@ SAGPU_API void    sagpu_free(void *d_ptr);
 SAGPU_API int32_t sagpu_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
 SAGPU_API int32_t sagpu_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
 SAGPU_API int32_t sagpu_memset(void *d_dst, int value, size_t bytes);
+/* Device-to-device copy by the engine's own copy kernel (16-byte vector loads and
+ * stores); also the copy-bandwidth yardstick of the benchmark. */
+SAGPU_API int32_t sagpu_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
 SAGPU_API int32_t sagpu_synchronize(void);
 
 /* Caps the engine's device working memory (bytes; 0 = unlimited).  Requests beyond
@@ -160,6 +163,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  * Tuning knobs (process-wide):
  *   "radix_variant"  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
  *   "radix_items"    keys per thread per radix tile: 16 (4096-key tiles) or 32 (default)
+ *   "bwt_variant"    0 = gather T[SA[j]-1] in SA order (default), 1 = scatter in text order
+ *   "isa_window_log2" ISA scatter window after the initial sort, 2^v entries (20..32, default 26)
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)
  * Returns 0, or -1 for an unknown name or value.

@@ -110,6 +110,7 @@ ABI_SAGPU = [
     ("sagpu_memcpy_h2d", _i32, [_vp, _vp, ctypes.c_size_t]),
     ("sagpu_memcpy_d2h", _i32, [_vp, _vp, ctypes.c_size_t]),
     ("sagpu_memset", _i32, [_vp, ctypes.c_int, ctypes.c_size_t]),
+    ("sagpu_memcpy_d2d", _i32, [_vp, _vp, ctypes.c_size_t]),
     ("sagpu_synchronize", _i32, []),
     ("sagpu_set_memory_limit", None, [ctypes.c_uint64]),
     ("sagpu_trim", None, []),

@@ -61,6 +61,8 @@ struct WsScratch {
 // G: scratch of n entries (group head per sorted position, SA order).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
+// ISA scatter window after the initial sort: 2^v entries per pass (v in [20, 32]).
+void set_isa_window_log2(int v);
 // Batched GSA: keys = (string id << 32) | packed symbols.
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
@@ -108,6 +110,9 @@ struct LcpScratch {
 };
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
+// The same BWT in text order: U[row of suffix i] = T[i-1], reading ISA and T
+// sequentially and scattering bytes.
+int launch_bwt_scatter(const uint8_t *T, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
 int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // PLCP from either PHI (ISA == nullptr) or ISA+SA (phi computed on the fly).
 // marks (SA order, with ISA+SA only): exact lcps or LCP_UNKNOWN | lower bound from
@@ -133,3 +138,4 @@ int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).
 int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
 int launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);
+int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s);

@@ -120,6 +120,7 @@ int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
 // Texts longer than this use 64-bit rank keys and PLCP scan values.
 u64 g_wide_from = 0x7FFFFFFFull;
+int g_bwt_variant = 0;  // 0: gather T[SA[j]-1] in SA order, 1: scatter in text order
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -583,7 +584,8 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
-            SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
+            if (g_bwt_variant == 1) SAGPU_TRY(launch_bwt_scatter(dTpad, dISA, n, dU, c.stream));
+            else SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
         }
         if (want_lcp) {
             {
@@ -919,6 +921,18 @@ extern "C" int32_t sagpu_memset(void *d, int value, size_t bytes)
     return hipMemset(d, value, bytes) == hipSuccess ? 0 : SAGPU_EDEVICE;
 }
 
+extern "C" int32_t sagpu_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes)
+{
+    if ((!d_dst || !d_src) && bytes) return SAGPU_EINVAL;
+    if (!bytes) return 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    SAGPU_TRY(launch_copy(d_dst, d_src, bytes, c->stream));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int32_t sagpu_synchronize(void)
 {
     return hipDeviceSynchronize() == hipSuccess ? 0 : SAGPU_EDEVICE;
@@ -953,6 +967,16 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "bwt_variant")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_bwt_variant = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "isa_window_log2")) {
+        if (value < 20 || value > 32) return SAGPU_EINVAL;
+        set_isa_window_log2((int)value);
         return 0;
     }
     if (!strcmp(name, "wide_from")) {

@@ -34,6 +34,23 @@ __global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, cons
     }
 }
 
+// Text order: suffix i (i >= 1) sits in row j = ISA[i], whose BWT symbol T[i-1] goes
+// to U[j + (j < j0)]; the primary row j0 = ISA[0] is dropped and U[0] = T[n-1].
+__global__ __launch_bounds__(256) void k_bwt_scatter(const uint8_t *__restrict__ T, const u32 *__restrict__ ISA,
+                                                     u64 n, uint8_t *__restrict__ U)
+{
+    const u64 j0 = ISA[0];
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i == 0) {
+            U[0] = T[n - 1];
+            continue;
+        }
+        const u64 j = ISA[i];
+        U[j + (j < j0)] = T[i - 1];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_phi(const u32 *__restrict__ SA, u64 n, u32 *__restrict__ PHI)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -314,6 +331,13 @@ __global__ __launch_bounds__(256) void k_lcp(const u32 *__restrict__ PLCP, const
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bwt, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_bwt_scatter(const uint8_t *T, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bwt_scatter, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, ISA, n, U);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

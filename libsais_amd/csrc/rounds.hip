@@ -277,7 +277,7 @@ struct BigF {
         u64 c = comp[e];
         u32 g = gslot[(u32)(c >> kb)];
         u32 r = g + (hstart - gstart);
-        ISA[v] = r;
+        if (hstart != gstart) ISA[v] = r;  // the first subgroup keeps the group's rank
         if (act) {
             out_i[o] = v;
             out_g[o] = r;
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         const u32 r = g + lt;
         s_oi[pos] = s_i[x];
         s_og[pos] = r;
-        s_os[pos] = eq > 1 ? 2 : 1;
+        s_os[pos] = (eq > 1 ? 2 : 1) | (lt == 0 ? 4 : 0);  // 4: rank unchanged, ISA already holds it
         if (LCP && lt > 0 && eqb == 0) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
     }
     __syncthreads();
@@ -455,8 +455,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     for (u32 x = tid; x < len; x += RT_THREADS) {
         u32 st = s_os[x];
         if (st == 0) continue;
-        ISA[s_oi[x]] = s_og[x];
-        if (st == 1) SA[s_og[x]] = s_oi[x];
+        if (!(st & 4)) ISA[s_oi[x]] = s_og[x];
+        if ((st & 3) == 1) SA[s_og[x]] = s_oi[x];
     }
 
     // 4. survivors, in sorted order, to the next active list
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
             u32 x = tid * RT_PER + j;
-            cnt[j] = (x < len && s_os[x] == 2) ? 1u : 0u;
+            cnt[j] = (x < len && (s_os[x] & 3) == 2) ? 1u : 0u;
             tot += cnt[j];
         }
         u32 inc = wave_incl_sum(tot);
@@ -523,14 +523,17 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 
 } // namespace
 
+int g_isa_window_log2 = 26;
+
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
 {
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP};
     SAGPU_TRY(ws_run(ws, f, n, nullptr, s));
-    // 64 Mi-entry windows, but at most 8 passes over vals/G for very long texts
-    const u64 win = n > (8ull << 26) ? (n + 7) / 8 : 1ull << 26;
+    // windows of 2^g_isa_window_log2 entries, but at most 8 passes over vals/G
+    const u64 w0 = 1ull << g_isa_window_log2;
+    const u64 win = n > 8 * w0 ? (n + 7) / 8 : w0;
     for (u64 lo = 0; lo < n; lo += win) {
         u64 hi = lo + win < n ? lo + win : n;
         hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi);
@@ -538,6 +541,8 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
+
+void set_isa_window_log2(int v) { g_isa_window_log2 = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)

@@ -177,6 +177,18 @@ __global__ void k_batch_primary(const u32 *__restrict__ ISA, const int64_t *__re
         prim[q] = ISA[off[q]] - (u32)off[q];
 }
 
+// Plain device copy, 16 bytes per lane per step (the copy-bandwidth yardstick).
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 count)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, u64 count)
+{
+    for (u64 i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(u32 *p, u64 count, u32 v)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -274,6 +286,21 @@ int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u32 *pri
 int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256 * 4, 16384)), dim3(256), 0, s, p, count, value);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s)
+{
+    const u64 vec = ((uintptr_t)dst % 16 == 0 && (uintptr_t)src % 16 == 0) ? bytes / 16 : 0;
+    if (vec) {
+        u64 blocks = (vec + 256 * 4 - 1) / (256 * 4);
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, vec);
+    }
+    if (bytes > vec * 16)
+        hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(256), 0, s, (const uint8_t *)src + vec * 16,
+                           (uint8_t *)dst + vec * 16, bytes - vec * 16);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

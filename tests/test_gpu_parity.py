@@ -197,3 +197,32 @@ def test_repeatable():
     a = la.suffix_array(T)
     b = la.suffix_array(T)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana"])
+def test_bwt_scatter_variant(name):
+    T = dict(CASES)[name]
+    lib = la.load()
+    assert lib.sagpu_set_option(b"bwt_variant", 1) == 0
+    try:
+        U_ref, idx_ref = ol.bwt(T, ol.sa(T)[1])
+        U = np.zeros(len(T), np.uint8)
+        A = np.zeros(len(T), np.int32)
+        assert la.libsais_bwt(T, U, A, len(T), 0, None) == idx_ref
+        assert np.array_equal(U, U_ref)
+        fSA, fU, fidx, _, _, _ = fused(T)
+        assert fidx == idx_ref and np.array_equal(fU, U_ref)
+    finally:
+        assert lib.sagpu_set_option(b"bwt_variant", 0) == 0
+
+
+@pytest.mark.parametrize("wlog", [20, 32])
+def test_isa_window_option(wlog):
+    T = workloads.dna_like(3 << 20, seed=17)
+    lib = la.load()
+    assert lib.sagpu_set_option(b"isa_window_log2", wlog) == 0
+    try:
+        rc, SA_ref = ol.sa(T)
+        assert np.array_equal(la.suffix_array(T), SA_ref)
+    finally:
+        assert lib.sagpu_set_option(b"isa_window_log2", 26) == 0
