@@ -1489,6 +1489,7 @@ extern "C" int32_t sagpu_run_job(sagpu_job *job)
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx *c;
     SAGPU_TRY(ctx_get(&c));
+    job->batched = 0;
     job->t_start = now_s();
     int rc = run_single(*c, job);
     job->t_done = now_s();

@@ -21,7 +21,7 @@ constexpr int RS_WAVES = RS_THREADS / SAGPU_WAVE;
 constexpr int LB_VAL_BITS = 40;
 constexpr u64 LB_VAL_MASK = (1ull << LB_VAL_BITS) - 1ull;
 constexpr u64 LB_AGG = 1ull, LB_PRE = 2ull;
-constexpr int LB_WIN = 8;
+constexpr int LB_WIN = 2;
 
 __device__ __forceinline__ u64 lb_pack(u32 epoch, u64 flag, u64 v)
 {
@@ -86,7 +86,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     __shared__ u64 s_gbase[256];
     __shared__ u32 s_wsum[RS_WAVES];
     __shared__ K s_keys[TILE_];
-    __shared__ uint8_t s_dig[TILE_];
 
     const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
     if (LOOKBACK) {
@@ -99,24 +98,33 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     const u64 tile = LOOKBACK ? (u64)s_tile : (u64)blockIdx.x;
     const u64 wbase = tile * TILE_ + (u64)w * WAVE_ITEMS_;
 
+    // Values are loaded only for the value phase, after the keys have left the
+    // registers: fewer live registers per wave, more waves per SIMD.
+    //
+    // Positions past n (only in the last tile) take the largest key.  They rank
+    // after every real key of the largest digit (position order is the tie-break),
+    // so they occupy the tile's last LDS slots [tile_n, TILE_), which are never
+    // written out; the only count they inflate is that digit's count in the last
+    // tile's own look-back entry, which no later tile reads.
     K key[ITEMS];
-    u32 val[ITEMS];
     u32 rank[ITEMS];
+    const bool full = (tile + 1) * (u64)TILE_ <= n;
+    if (full) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-        bool ok = pos < n;
-        key[j] = ok ? kin[pos] : (K)0;
-        if (HAS_VALS) val[j] = ok ? vin[pos] : 0u;
+        for (int j = 0; j < ITEMS; j++) key[j] = kin[wbase + (u64)j * SAGPU_WAVE + lane];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+            key[j] = pos < n ? kin[pos] : ~(K)0;
+        }
     }
 
     // Wave-level stable ranking: order inside the wave is (item j, lane).
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
-        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-        bool ok = pos < n;
         u32 d = (u32)(key[j] >> shift) & dmask;
-        u64 m = __ballot(ok);
+        u64 m = ~0ull;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             bool bit = (d >> b) & 1u;
@@ -124,10 +132,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
             m &= bit ? bb : ~bb;
         }
         u32 below = __popcll(m & lanemask_lt());
-        u32 old = 0;
-        if (ok) old = s_wcnt[w][d];
-        if (ok && below == 0) s_wcnt[w][d] = old + (u32)__popcll(m);
-        rank[j] = old + below;
+        u32 old = s_wcnt[w][d];
+        if (below == 0) s_wcnt[w][d] = old + (u32)__popcll(m);
+        // rank inside the wave (< 2^16) with the digit above it: one live register
+        // per item until the reorder instead of three (rank, digit, LDS address)
+        rank[j] = (old + below) | (d << 16);
+        // Materialise the rank here: otherwise the compiler sinks every item's rank
+        // arithmetic to its first use after the look-back and keeps digit, old count
+        // and peer mask of all items live (four registers per item instead of one).
+        asm volatile("" : "+v"(rank[j]));
     }
     __syncthreads();
 
@@ -188,37 +201,56 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     }
     __syncthreads();
 
-    // Reorder keys in LDS by digit.
+    // Reorder keys in LDS by digit (past-the-end keys land in [tile_n, TILE_)).
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
-        u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-        if (pos < n) {
-            u32 d = (u32)(key[j] >> shift) & dmask;
-            u32 lp = s_dstart[d] + s_wcnt[w][d] + rank[j];
-            s_keys[lp] = key[j];
-            s_dig[lp] = (uint8_t)d;
-            rank[j] = lp;
-        }
+        const u32 d = rank[j] >> 16;
+        const u32 lp = s_dstart[d] + s_wcnt[w][d] + (rank[j] & 0xFFFFu);
+        s_keys[lp] = key[j];
+        rank[j] = lp;
     }
     __syncthreads();
     const u64 tbase = tile * TILE_;
     const u32 tile_n = (u32)((n - tbase) < (u64)TILE_ ? (n - tbase) : (u64)TILE_);
-    for (u32 idx = tid; idx < tile_n; idx += RS_THREADS) {
-        u32 d = s_dig[idx];
-        kout[s_gbase[d] + (idx - s_dstart[d])] = s_keys[idx];
+    // Digits of the staged keys are recomputed and kept packed in registers for the
+    // value phase, so no LDS digit array is needed (one more workgroup per CU fits).
+    u32 dpack[ITEMS / 4];
+#pragma unroll
+    for (int q = 0; q < ITEMS / 4; q++) dpack[q] = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const u32 idx = tid + (u32)j * RS_THREADS;
+        if (idx < tile_n) {
+            const K kk = s_keys[idx];
+            const u32 d = (u32)(kk >> shift) & dmask;
+            dpack[j / 4] |= d << (8 * (j % 4));
+            kout[s_gbase[d] + (idx - s_dstart[d])] = kk;
+        }
+        // bound the LDS loads the scheduler hoists (each item holds four registers)
+        if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);
     }
     if (HAS_VALS) {
         __syncthreads();
         u32 *s_vals = reinterpret_cast<u32 *>(s_keys);
+        if (full) {
 #pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-            if (pos < n) s_vals[rank[j]] = val[j];
+            for (int j = 0; j < ITEMS; j++) s_vals[rank[j]] = vin[wbase + (u64)j * SAGPU_WAVE + lane];
+        } else {
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+                if (pos < n) s_vals[rank[j]] = vin[pos];
+            }
         }
         __syncthreads();
-        for (u32 idx = tid; idx < tile_n; idx += RS_THREADS) {
-            u32 d = s_dig[idx];
-            vout[s_gbase[d] + (idx - s_dstart[d])] = s_vals[idx];
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const u32 idx = tid + (u32)j * RS_THREADS;
+            if (idx < tile_n) {
+                const u32 d = (dpack[j / 4] >> (8 * (j % 4))) & 255u;
+                vout[s_gbase[d] + (idx - s_dstart[d])] = s_vals[idx];
+            }
+            if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }

@@ -12,7 +12,7 @@
 //                    unsorted groups are free until their suffixes are placed;
 //                    the last index, not the end, so 2^32 members still fit)
 //   k_round_tile     groups of <= RT_SMALL members whose first element lies in a
-//                    2048-wide stretch of the list: ranked by key2 inside LDS,
+//                    512-wide stretch of the list: ranked by key2 inside LDS,
 //                    regrouped, ISA/SA written, survivors emitted
 //   big path         larger groups: extracted, sorted by (group, key2) with the
 //                    radix sort, regrouped with a segmented scan (BigF)
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
     }
 }
 
-constexpr int RT_THREADS = 256, RT_RANGE = 2048, RT_CAP = RT_RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
+constexpr int RT_THREADS = 256, RT_RANGE = 512, RT_CAP = RT_RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
 static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
 
 template <typename KT>
