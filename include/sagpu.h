@@ -148,6 +148,9 @@ typedef struct sagpu_stats {
     double  ms_bwt;
     double  ms_plcp;
     double  ms_lcp;
+    double  ms_onesweep_u32;      /* the u32-key passes among ms_onesweep           */
+    int32_t onesweep_launches_u32;
+    int64_t sorted_bytes_u32;     /* algorithmic bytes of the u32-key passes        */
 } sagpu_stats;
 
 /*

@@ -72,6 +72,9 @@ class SagpuStats(ctypes.Structure):
         ("ms_bwt", ctypes.c_double),
         ("ms_plcp", ctypes.c_double),
         ("ms_lcp", ctypes.c_double),
+        ("ms_onesweep_u32", ctypes.c_double),
+        ("onesweep_launches_u32", ctypes.c_int32),
+        ("sorted_bytes_u32", ctypes.c_int64),
     ]
 
 

@@ -99,7 +99,7 @@ struct Timer {
     std::vector<Rec> recs;
 };
 
-enum Family { F_ONESWEEP = 0, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_COUNT };
+enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_COUNT };
 
 struct Ctx {
     int device = 0;
@@ -111,6 +111,7 @@ struct Ctx {
     size_t stage_cap = 0;
     Timer timer;
     sagpu_stats stats{};
+    bool sort_u32 = false;  // the radix passes being timed are u32-key passes
 };
 
 std::mutex g_mu;
@@ -234,6 +235,12 @@ void timer_collect(Ctx &c)
         (void)hipEventElapsedTime(&ms, r.a, r.b);
         switch (r.family) {
         case F_ONESWEEP: c.stats.ms_onesweep += ms; c.stats.onesweep_launches++; break;
+        case F_ONESWEEP32:
+            c.stats.ms_onesweep += ms;
+            c.stats.onesweep_launches++;
+            c.stats.ms_onesweep_u32 += ms;
+            c.stats.onesweep_launches_u32++;
+            break;
         case F_INITIAL: c.stats.ms_initial += ms; break;
         case F_ROUNDS: c.stats.ms_rounds += ms; break;
         case F_BWT: c.stats.ms_bwt += ms; break;
@@ -288,7 +295,7 @@ void onesweep_hook(void *user, int phase)
 {
     Ctx &c = *(Ctx *)user;
     static thread_local Span *open = nullptr;
-    if (phase == 0) open = new Span(c, F_ONESWEEP);
+    if (phase == 0) open = new Span(c, c.sort_u32 ? F_ONESWEEP32 : F_ONESWEEP);
     else { delete open; open = nullptr; }
 }
 
@@ -297,6 +304,7 @@ int sort_u64(Ctx &c, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, i
     int passes = 0;
     c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
     c.rs.hook_user = &c;
+    c.sort_u32 = false;
     int rc = radix_sort_u64(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
     c.stats.sort_passes += passes;
     c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
@@ -309,10 +317,13 @@ int sort_u32(Ctx &c, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, i
     int passes = 0;
     c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
     c.rs.hook_user = &c;
+    c.sort_u32 = true;
     int rc = radix_sort_u32(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
+    c.sort_u32 = false;
     c.stats.sort_passes += passes;
     c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
     c.stats.sorted_bytes += (int64_t)passes * (int64_t)n * 16;
+    c.stats.sorted_bytes_u32 += (int64_t)passes * (int64_t)n * 16;
     return rc;
 }
 
