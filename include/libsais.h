@@ -3,10 +3,12 @@
  *
  * Same prototypes, version macros and return-code conventions as libsais 2.10.4
  * (include/libsais.h of the reference), for the entry points this engine provides:
- * suffix array, generalized suffix array, BWT (plain and with auxiliary indexes),
- * inverse BWT, PLCP (plain and GSA) and LCP.  The reference's context (_ctx),
- * OpenMP (_omp) and integer-alphabet (_int) functions are not provided, so a program
- * that uses them fails at link time instead of silently running something else.
+ * suffix array (byte and integer alphabets), generalized suffix array, BWT (plain
+ * and with auxiliary indexes), inverse BWT, PLCP (byte, GSA and integer) and LCP,
+ * with the reference's context (_ctx) and OpenMP (_omp, declared when LIBSAIS_OPENMP
+ * is defined, as in the reference) forms.  Contexts are tokens here (the device
+ * buffers are pooled inside the engine) and the OpenMP thread count is checked but
+ * the work runs on the GPU.
  *
  * All buffers are caller-owned host memory; calls are synchronous and may be made
  * from several host threads at once.  Work runs on GPU 0.
@@ -56,6 +58,12 @@ LIBSAIS_API int32_t libsais(const uint8_t * T, int32_t * SA, int32_t n, int32_t 
 LIBSAIS_API int32_t libsais_gsa(const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq);
 
 /*
+ * Suffix array of an integer string (symbols in [0, k)).  T is not modified.
+ * Returns 0, -1 or -2.  Replaces reference include/libsais.h:107 (src/libsais.c:7050-7063).
+ */
+LIBSAIS_API int32_t libsais_int(int32_t * T, int32_t * SA, int32_t n, int32_t k, int32_t fs);
+
+/*
  * Burrows-Wheeler transform of T into U (U may alias T); A is temporary space of
  * n+fs entries whose content is unspecified on return.  Returns the primary index,
  * or -1 / -2.  Replaces reference include/libsais.h:182 (src/libsais.c:7097-7121).
@@ -102,6 +110,49 @@ LIBSAIS_API int32_t libsais_plcp_gsa(const uint8_t * T, const int32_t * SA, int3
  * Replaces reference include/libsais.h:398 (src/libsais.c:8417-8432).
  */
 LIBSAIS_API int32_t libsais_lcp(const int32_t * PLCP, const int32_t * SA, int32_t * LCP, int32_t n);
+
+/*
+ * PLCP of an integer string.  Replaces reference include/libsais.h:388
+ * (src/libsais.c:8399-8415).
+ */
+LIBSAIS_API int32_t libsais_plcp_int(const int32_t * T, const int32_t * SA, int32_t * PLCP, int32_t n);
+
+/*
+ * Contexts (reference include/libsais.h:57, 73, 119, 131, 209, 224, 261, 277, 302, 329): a context
+ * is a token, NULL on failure; the _ctx calls return -1 for a NULL context and
+ * otherwise behave as the context-free calls.
+ */
+LIBSAIS_API void * libsais_create_ctx(void);
+LIBSAIS_API void libsais_free_ctx(void * ctx);
+LIBSAIS_API int32_t libsais_ctx(const void * ctx, const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq);
+LIBSAIS_API int32_t libsais_gsa_ctx(const void * ctx, const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq);
+LIBSAIS_API int32_t libsais_bwt_ctx(const void * ctx, const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq);
+LIBSAIS_API int32_t libsais_bwt_aux_ctx(const void * ctx, const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq, int32_t r, int32_t * I);
+LIBSAIS_API void * libsais_unbwt_create_ctx(void);
+LIBSAIS_API void libsais_unbwt_free_ctx(void * ctx);
+LIBSAIS_API int32_t libsais_unbwt_ctx(const void * ctx, const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t i);
+LIBSAIS_API int32_t libsais_unbwt_aux_ctx(const void * ctx, const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t r, const int32_t * I);
+
+#if defined(LIBSAIS_OPENMP)
+/*
+ * OpenMP forms (reference include/libsais.h:66, 144, 156, 169, 238, 253, 270, 343,
+ * 357, 410, 421, 432, 443): `threads` < 0 is rejected with -1 (NULL for the context
+ * constructors); otherwise the call behaves as its single-threaded form.
+ */
+LIBSAIS_API void * libsais_create_ctx_omp(int32_t threads);
+LIBSAIS_API int32_t libsais_omp(const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq, int32_t threads);
+LIBSAIS_API int32_t libsais_gsa_omp(const uint8_t * T, int32_t * SA, int32_t n, int32_t fs, int32_t * freq, int32_t threads);
+LIBSAIS_API int32_t libsais_int_omp(int32_t * T, int32_t * SA, int32_t n, int32_t k, int32_t fs, int32_t threads);
+LIBSAIS_API int32_t libsais_bwt_omp(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq, int32_t threads);
+LIBSAIS_API int32_t libsais_bwt_aux_omp(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, int32_t fs, int32_t * freq, int32_t r, int32_t * I, int32_t threads);
+LIBSAIS_API void * libsais_unbwt_create_ctx_omp(int32_t threads);
+LIBSAIS_API int32_t libsais_unbwt_omp(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t i, int32_t threads);
+LIBSAIS_API int32_t libsais_unbwt_aux_omp(const uint8_t * T, uint8_t * U, int32_t * A, int32_t n, const int32_t * freq, int32_t r, const int32_t * I, int32_t threads);
+LIBSAIS_API int32_t libsais_plcp_omp(const uint8_t * T, const int32_t * SA, int32_t * PLCP, int32_t n, int32_t threads);
+LIBSAIS_API int32_t libsais_plcp_gsa_omp(const uint8_t * T, const int32_t * SA, int32_t * PLCP, int32_t n, int32_t threads);
+LIBSAIS_API int32_t libsais_plcp_int_omp(const int32_t * T, const int32_t * SA, int32_t * PLCP, int32_t n, int32_t threads);
+LIBSAIS_API int32_t libsais_lcp_omp(const int32_t * PLCP, const int32_t * SA, int32_t * LCP, int32_t n, int32_t threads);
+#endif
 
 #ifdef __cplusplus
 }

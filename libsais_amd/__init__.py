@@ -93,6 +93,33 @@ ABI_32 = [
     ("libsais_plcp_gsa", _i32, [_vp, _vp, _vp, _i32]),
     ("libsais_lcp", _i32, [_vp, _vp, _vp, _i32]),
 ]
+ABI_32 += [
+    ("libsais_int", _i32, [_vp, _vp, _i32, _i32, _i32]),
+    ("libsais_plcp_int", _i32, [_vp, _vp, _vp, _i32]),
+    ("libsais_create_ctx", _vp, []),
+    ("libsais_free_ctx", None, [_vp]),
+    ("libsais_ctx", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais_gsa_ctx", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais_bwt_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais_bwt_aux_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]),
+    ("libsais_unbwt_create_ctx", _vp, []),
+    ("libsais_unbwt_free_ctx", None, [_vp]),
+    ("libsais_unbwt_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32]),
+    ("libsais_unbwt_aux_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp]),
+    ("libsais_create_ctx_omp", _vp, [_i32]),
+    ("libsais_omp", _i32, [_vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais_gsa_omp", _i32, [_vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais_int_omp", _i32, [_vp, _vp, _i32, _i32, _i32, _i32]),
+    ("libsais_bwt_omp", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais_bwt_aux_omp", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp, _i32]),
+    ("libsais_unbwt_create_ctx_omp", _vp, [_i32]),
+    ("libsais_unbwt_omp", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _i32]),
+    ("libsais_unbwt_aux_omp", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32]),
+    ("libsais_plcp_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+    ("libsais_plcp_gsa_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+    ("libsais_plcp_int_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+    ("libsais_lcp_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+]
 ABI_64 = [
     ("libsais64", _i64, [_vp, _vp, _i64, _i64, _vp]),
     ("libsais64_gsa", _i64, [_vp, _vp, _i64, _i64, _vp]),
@@ -103,6 +130,19 @@ ABI_64 = [
     ("libsais64_plcp", _i64, [_vp, _vp, _vp, _i64]),
     ("libsais64_plcp_gsa", _i64, [_vp, _vp, _vp, _i64]),
     ("libsais64_lcp", _i64, [_vp, _vp, _vp, _i64]),
+]
+ABI_64 += [
+    ("libsais64_long", _i64, [_vp, _vp, _i64, _i64, _i64]),
+    ("libsais64_omp", _i64, [_vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais64_gsa_omp", _i64, [_vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais64_long_omp", _i64, [_vp, _vp, _i64, _i64, _i64, _i64]),
+    ("libsais64_bwt_omp", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais64_bwt_aux_omp", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64]),
+    ("libsais64_unbwt_omp", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _i64]),
+    ("libsais64_unbwt_aux_omp", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64]),
+    ("libsais64_plcp_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
+    ("libsais64_plcp_gsa_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
+    ("libsais64_lcp_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
 ]
 ABI_SAGPU = [
     ("sagpu_init", _i32, []),
@@ -186,6 +226,12 @@ libsais64_unbwt_aux = _mk("libsais64_unbwt_aux")
 libsais64_plcp = _mk("libsais64_plcp")
 libsais64_plcp_gsa = _mk("libsais64_plcp_gsa")
 libsais64_lcp = _mk("libsais64_lcp")
+
+# every other exported entry point under its C name (contexts, OpenMP forms,
+# integer alphabets)
+for _name, _res, _args in ABI_32 + ABI_64:
+    if _name not in globals():
+        globals()[_name] = _mk(_name)
 
 
 def last_error() -> str:

@@ -7,10 +7,12 @@
  * success, -1 invalid arguments, -2 when device memory cannot be obtained or the
  * device fails (the reference's allocation-failure code).
  */
+#define LIBSAIS_OPENMP 1 /* declare the _omp forms, which this library defines */
 #include "../../include/libsais.h"
 #include "../../include/libsais64.h"
 
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define HIDDEN __attribute__((visibility("hidden")))
@@ -247,4 +249,298 @@ int64_t libsais64_lcp(const int64_t *PLCP, const int64_t *SA, int64_t *LCP, int6
         return 0;
     }
     return map_rc(sagpu_host_lcp(PLCP, SA, LCP, n, 8));
+}
+
+/* --------------------------------------------- integer alphabet (32 / 64) -- */
+
+HIDDEN int64_t sagpu_host_sa_int(const void *T, void *SA, int64_t n, int32_t width);
+HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void *SA, void *PLCP, int64_t n, int32_t width);
+
+/* reference src/libsais.c:7050-7063.  T is not modified (the reference restores it). */
+int32_t libsais_int(int32_t *T, int32_t *SA, int32_t n, int32_t k, int32_t fs)
+{
+    (void)k;
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0)) return -1;
+    if (n <= 1) {
+        if (n == 1) SA[0] = 0;
+        return 0;
+    }
+    return (int32_t)map_rc(sagpu_host_sa_int(T, SA, n, 4));
+}
+
+/* reference src/libsais64.c:7118-7131 */
+int64_t libsais64_long(int64_t *T, int64_t *SA, int64_t n, int64_t k, int64_t fs)
+{
+    (void)k;
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0)) return -1;
+    if (n <= 1) {
+        if (n == 1) SA[0] = 0;
+        return 0;
+    }
+    return map_rc(sagpu_host_sa_int(T, SA, n, 8));
+}
+
+/* reference src/libsais.c:8399-8415 */
+int32_t libsais_plcp_int(const int32_t *T, const int32_t *SA, int32_t *PLCP, int32_t n)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0)) return -1;
+    if (n <= 1) {
+        if (n == 1) PLCP[0] = 0;
+        return 0;
+    }
+    return (int32_t)map_rc(sagpu_host_plcp_int(T, SA, PLCP, n, 4));
+}
+
+/* ------------------------------------------------------------- contexts -- */
+/* The reference's contexts hold reusable host buffers (src/libsais.c:7008-7016,
+ * 8020-8028).  Here the device buffers are pooled inside the engine, so a context
+ * is only a token the caller must pass back; the calls behave as the context-free
+ * ones after the same checks. */
+
+typedef struct { uint32_t magic; } sagpu_ctx_token;
+#define SAGPU_CTX_MAGIC 0x5341474Bu
+
+static void *ctx_new(void)
+{
+    sagpu_ctx_token *t = (sagpu_ctx_token *)malloc(sizeof *t);
+    if (t) t->magic = SAGPU_CTX_MAGIC;
+    return t;
+}
+
+static void ctx_free(void *ctx)
+{
+    if (ctx) {
+        ((sagpu_ctx_token *)ctx)->magic = 0;
+        free(ctx);
+    }
+}
+
+/* reference src/libsais.c:7008-7011 */
+void *libsais_create_ctx(void) { return ctx_new(); }
+
+/* reference src/libsais.c:7201-7209 */
+void *libsais_create_ctx_omp(int32_t threads) { return threads < 0 ? NULL : ctx_new(); }
+
+/* reference src/libsais.c:7013-7016 */
+void libsais_free_ctx(void *ctx) { ctx_free(ctx); }
+
+/* reference src/libsais.c:8020-8023 */
+void *libsais_unbwt_create_ctx(void) { return ctx_new(); }
+
+/* reference src/libsais.c:8078-8086 */
+void *libsais_unbwt_create_ctx_omp(int32_t threads) { return threads < 0 ? NULL : ctx_new(); }
+
+/* reference src/libsais.c:8025-8028 */
+void libsais_unbwt_free_ctx(void *ctx) { ctx_free(ctx); }
+
+/* reference src/libsais.c:7065-7079 */
+int32_t libsais_ctx(const void *ctx, const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
+{
+    if (ctx == NULL) return -1;
+    return libsais(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais.c:7081-7095 */
+int32_t libsais_gsa_ctx(const void *ctx, const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
+{
+    if (ctx == NULL) return -1;
+    return libsais_gsa(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais.c:7148-7172 */
+int32_t libsais_bwt_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t fs,
+                        int32_t *freq)
+{
+    if (ctx == NULL) return -1;
+    return libsais_bwt(T, U, A, n, fs, freq);
+}
+
+/* reference src/libsais.c:7174-7197 */
+int32_t libsais_bwt_aux_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t fs,
+                            int32_t *freq, int32_t r, int32_t *I)
+{
+    if (ctx == NULL) return -1;
+    return libsais_bwt_aux(T, U, A, n, fs, freq, r, I);
+}
+
+/* reference src/libsais.c:8035-8038 */
+int32_t libsais_unbwt_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, const int32_t *freq,
+                          int32_t i)
+{
+    return libsais_unbwt_aux_ctx(ctx, T, U, A, n, freq, n, &i);
+}
+
+/* reference src/libsais.c:8058-8074 */
+int32_t libsais_unbwt_aux_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32_t *A, int32_t n,
+                              const int32_t *freq, int32_t r, const int32_t *I)
+{
+    if (ctx == NULL) return -1;
+    return libsais_unbwt_aux(T, U, A, n, freq, r, I);
+}
+
+/* ------------------------------------------------------- OpenMP entries -- */
+/* The reference's _omp functions take a thread count (0 = OpenMP default, < 0
+ * rejected); the work here runs on the GPU, so the count is only checked. */
+
+/* reference src/libsais.c:7211-7228 */
+int32_t libsais_omp(const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais.c:7230-7247 */
+int32_t libsais_gsa_omp(const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (n > 0 && T[n - 1] != 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais_gsa(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais.c:7249-7265 */
+int32_t libsais_int_omp(int32_t *T, int32_t *SA, int32_t n, int32_t k, int32_t fs, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais_int(T, SA, n, k, fs);
+}
+
+/* reference src/libsais.c:7267-7294 */
+int32_t libsais_bwt_omp(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t fs, int32_t *freq,
+                        int32_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais_bwt(T, U, A, n, fs, freq);
+}
+
+/* reference src/libsais.c:7296-7322 */
+int32_t libsais_bwt_aux_omp(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t fs, int32_t *freq,
+                            int32_t r, int32_t *I, int32_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (r < 2) || ((r & (r - 1)) != 0) ||
+        (I == NULL) || (threads < 0))
+        return -1;
+    return libsais_bwt_aux(T, U, A, n, fs, freq, r, I);
+}
+
+/* reference src/libsais.c:8088-8091 */
+int32_t libsais_unbwt_omp(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, const int32_t *freq, int32_t i,
+                          int32_t threads)
+{
+    return libsais_unbwt_aux_omp(T, U, A, n, freq, n, &i, threads);
+}
+
+/* reference src/libsais.c:8093-8112 */
+int32_t libsais_unbwt_aux_omp(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, const int32_t *freq, int32_t r,
+                              const int32_t *I, int32_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) ||
+        (I == NULL) || (threads < 0))
+        return -1;
+    return libsais_unbwt_aux(T, U, A, n, freq, r, I);
+}
+
+/* reference src/libsais.c:8436-8455 */
+int32_t libsais_plcp_omp(const uint8_t *T, const int32_t *SA, int32_t *PLCP, int32_t n, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0) || (threads < 0)) return -1;
+    return libsais_plcp(T, SA, PLCP, n);
+}
+
+/* reference src/libsais.c:8457-8476 */
+int32_t libsais_plcp_gsa_omp(const uint8_t *T, const int32_t *SA, int32_t *PLCP, int32_t n, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0) || (n > 0 && T[n - 1] != 0) || (threads < 0))
+        return -1;
+    return libsais_plcp_gsa(T, SA, PLCP, n);
+}
+
+/* reference src/libsais.c:8478-8497 */
+int32_t libsais_plcp_int_omp(const int32_t *T, const int32_t *SA, int32_t *PLCP, int32_t n, int32_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0) || (threads < 0)) return -1;
+    return libsais_plcp_int(T, SA, PLCP, n);
+}
+
+/* reference src/libsais.c:8499-8517 */
+int32_t libsais_lcp_omp(const int32_t *PLCP, const int32_t *SA, int32_t *LCP, int32_t n, int32_t threads)
+{
+    if ((PLCP == NULL) || (SA == NULL) || (LCP == NULL) || (n < 0) || (threads < 0)) return -1;
+    return libsais_lcp(PLCP, SA, LCP, n);
+}
+
+/* reference src/libsais64.c:7213-7244 */
+int64_t libsais64_omp(const uint8_t *T, int64_t *SA, int64_t n, int64_t fs, int64_t *freq, int64_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais64(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais64.c:7246-7277 */
+int64_t libsais64_gsa_omp(const uint8_t *T, int64_t *SA, int64_t n, int64_t fs, int64_t *freq, int64_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (n > 0 && T[n - 1] != 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais64_gsa(T, SA, n, fs, freq);
+}
+
+/* reference src/libsais64.c:7279-7295 */
+int64_t libsais64_long_omp(int64_t *T, int64_t *SA, int64_t n, int64_t k, int64_t fs, int64_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais64_long(T, SA, n, k, fs);
+}
+
+/* reference src/libsais64.c:7297-7337 */
+int64_t libsais64_bwt_omp(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, int64_t fs, int64_t *freq,
+                          int64_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (threads < 0)) return -1;
+    return libsais64_bwt(T, U, A, n, fs, freq);
+}
+
+/* reference src/libsais64.c:7339-7379 */
+int64_t libsais64_bwt_aux_omp(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, int64_t fs, int64_t *freq,
+                              int64_t r, int64_t *I, int64_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || (fs < 0) || (r < 2) || ((r & (r - 1)) != 0) ||
+        (I == NULL) || (threads < 0))
+        return -1;
+    return libsais64_bwt_aux(T, U, A, n, fs, freq, r, I);
+}
+
+/* reference src/libsais64.c:8067-8070 */
+int64_t libsais64_unbwt_omp(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, const int64_t *freq, int64_t i,
+                            int64_t threads)
+{
+    return libsais64_unbwt_aux_omp(T, U, A, n, freq, n, &i, threads);
+}
+
+/* reference src/libsais64.c:8072-8099 */
+int64_t libsais64_unbwt_aux_omp(const uint8_t *T, uint8_t *U, int64_t *A, int64_t n, const int64_t *freq, int64_t r,
+                                const int64_t *I, int64_t threads)
+{
+    if ((T == NULL) || (U == NULL) || (A == NULL) || (n < 0) || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) ||
+        (I == NULL) || (threads < 0))
+        return -1;
+    return libsais64_unbwt_aux(T, U, A, n, freq, r, I);
+}
+
+/* reference src/libsais64.c:8357-8376 */
+int64_t libsais64_plcp_omp(const uint8_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int64_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0) || (threads < 0)) return -1;
+    return libsais64_plcp(T, SA, PLCP, n);
+}
+
+/* reference src/libsais64.c:8378-8397 */
+int64_t libsais64_plcp_gsa_omp(const uint8_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int64_t threads)
+{
+    if ((T == NULL) || (SA == NULL) || (PLCP == NULL) || (n < 0) || (n > 0 && T[n - 1] != 0) || (threads < 0))
+        return -1;
+    return libsais64_plcp_gsa(T, SA, PLCP, n);
+}
+
+/* reference src/libsais64.c:8399-8417 */
+int64_t libsais64_lcp_omp(const int64_t *PLCP, const int64_t *SA, int64_t *LCP, int64_t n, int64_t threads)
+{
+    if ((PLCP == NULL) || (SA == NULL) || (LCP == NULL) || (n < 0) || (threads < 0)) return -1;
+    return libsais64_lcp(PLCP, SA, LCP, n);
 }

@@ -120,7 +120,7 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // V64 (n entries) is needed when PLCP[i] + i can reach 2^32 (n > 2^32 - 1 - n/2 is
 // not tracked: it is passed whenever n > INT32_MAX); otherwise PLCP is the scratch.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
-                const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s);
+                const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes = 1);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // BWT auxiliary samples and inverse BWT (unbwt.hip).
@@ -139,3 +139,10 @@ int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
 int launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);
 int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s);
+// Integer alphabet (libsais_int / libsais64_long): T is int32 (width 4) or int64.
+int launch_pack_int(const u32 *S, u64 n, int bits, int k, u32 *keys, u32 *vals, hipStream_t s);
+int launch_max_sym(const void *T, int width, u64 n, u64 *out, hipStream_t s);
+int launch_sym_narrow(const void *T, int width, u64 n, u32 *S, hipStream_t s);
+int launch_sym_keys64(const void *T, int width, u64 n, u64 *keys, u32 *vals, hipStream_t s);
+int launch_run_flags64(const u64 *keys, u64 n, u32 *flags, hipStream_t s);
+int launch_dense_scatter64(const u64 *keys, const u32 *vals, const u32 *excl, u64 n, u32 *S, hipStream_t s);

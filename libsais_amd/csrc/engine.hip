@@ -335,16 +335,12 @@ int read_u32(Ctx &c, const u32 *d, u32 *out)
     return SAGPU_OK;
 }
 
-// Suffix array of the padded device text dT[0..n) (n >= 2, n <= 2^31 - 1).  On
-// return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
-// optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
-// previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
-int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
-             const u32 *sid = nullptr, u32 nstrings = 0)
+// Byte alphabet of a text: freq (optional, host), the GSA validity test, and the
+// dense symbol codes (device LUT) with their width and the symbols per 32-bit key.
+int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *freq_out, int *bits_out, int *ksym_out,
+                  uint8_t **lut_out)
 {
     hipStream_t s = c.stream;
-    sagpu_stats &st = c.stats;
-
     GET(d_hist, u64, 256 + 8);
     u32 *d_bad = (u32 *)(d_hist + 256);
     u64 freq[256];
@@ -366,13 +362,35 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         lut_h[ch] = (uint8_t)sigma;
         if (freq[ch]) sigma++;
     }
-    const int bits = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
-    const int ksym = 32 / bits;
+    *bits_out = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
+    *ksym_out = 32 / *bits_out;
+    GET(d_lut, uint8_t, 256);
+    SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
+    *lut_out = d_lut;
+    return SAGPU_OK;
+}
+
+// Suffix array of the padded device text dT[0..n) (n >= 2, n <= 2^31 - 1).  On
+// return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
+// optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
+// previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
+int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
+             const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0)
+{
+    hipStream_t s = c.stream;
+    sagpu_stats &st = c.stats;
+    int bits, ksym;
+    uint8_t *d_lut = nullptr;
+    if (dSym) {
+        // integer alphabet: symbols already below 2^sym_bits
+        bits = sym_bits < 1 ? 1 : sym_bits;
+        ksym = 32 / bits;
+    } else {
+        SAGPU_TRY(byte_alphabet(c, sc, dT, n, gsa, freq_out, &bits, &ksym, &d_lut));
+    }
     st.initial_symbols = ksym;
     st.bits_per_symbol = bits;
 
-    GET(d_lut, uint8_t, 256);
-    SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
 
     // Buffer plan (each block is reused once its first use is over):
     //   kA  [2n u32, 4n for batched u64 keys]: initial sort keys (ping-pong), then
@@ -412,7 +430,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
         {
             Span sp(c, F_INITIAL);
-            SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, k0, v0, s));
+            if (dSym) SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, k0, v0, s));
+            else SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, k0, v0, s));
         }
         SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt));
         Span sp(c, F_INITIAL);
@@ -527,7 +546,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 }
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-             u32 *PLCP, const u32 *marks)
+             u32 *PLCP, const u32 *marks, int sym_bytes = 1)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -545,7 +564,7 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
         V64 = sc.get<u64>(n);
         if (!V64) return SAGPU_ENOMEM;
     }
-    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream);
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes);
 }
 
 // Copies a text into a pool buffer with 64 zero bytes of padding.
@@ -730,6 +749,102 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     store_freq(f, freq, width);
     return primary;
+}
+
+// Integer alphabet (libsais_int, libsais64_long): T holds int32 (width 4) or int64
+// (width 8) symbols, expected in [0, k).  Symbols below 2^16 are packed as they
+// are; wider ones are replaced by their dense ranks first (one sort by value), so
+// the initial keys still hold as many symbols as the alphabet allows.
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa_int(const void *T, void *SA, int64_t n, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    const u64 N = (u64)n;
+    hipStream_t s = c->stream;
+    Scope sc(*c);
+    GET(dSym, u32, N);
+    int bits = 1;
+    {
+        Scope in(*c);  // the uploaded symbols and ranking buffers go back before the build
+        uint8_t *dTin = in.get<uint8_t>(N * (u64)width);
+        u64 *d_max = in.get<u64>(1);
+        if (!dTin || !d_max) return SAGPU_ENOMEM;
+        SAGPU_CHECK(hipMemcpyAsync(dTin, T, N * (u64)width, hipMemcpyHostToDevice, s));
+        SAGPU_TRY(launch_fill_u64(d_max, 1, 0, s));
+        SAGPU_TRY(launch_max_sym(dTin, width, N, d_max, s));
+        SAGPU_CHECK(hipMemcpyAsync(c->pinned, d_max, 8, hipMemcpyDeviceToHost, s));
+        SAGPU_CHECK(hipStreamSynchronize(s));
+        u64 maxv = c->pinned[0];
+        if (maxv < (1ull << 16)) {
+            SAGPU_TRY(launch_sym_narrow(dTin, width, N, dSym, s));
+            bits = maxv ? bits_for(maxv) : 1;
+        } else {
+            u64 *k0 = in.get<u64>(N), *k1 = in.get<u64>(N);
+            u32 *v0 = in.get<u32>(N), *v1 = in.get<u32>(N), *flags = in.get<u32>(N);
+            const u64 pcap = N / 4096 + 2;
+            u32 *part = in.get<u32>(pcap);
+            if (!k0 || !k1 || !v0 || !v1 || !flags || !part) return SAGPU_ENOMEM;
+            SAGPU_TRY(launch_sym_keys64(dTin, width, N, k0, v0, s));
+            SAGPU_TRY(ensure_radix_scratch(*c, in, N));
+            int alt = 0;
+            SAGPU_TRY(sort_u64(*c, k0, v0, k1, v1, N, 0, bits_for(maxv), &alt));
+            const u64 *sk = alt ? k1 : k0;
+            const u32 *sv = alt ? v1 : v0;
+            SAGPU_TRY(launch_run_flags64(sk, N, flags, s));
+            ScanScratch ssc{part, pcap};
+            SAGPU_TRY(scan_exclusive_u32(ssc, flags, N, s));
+            SAGPU_TRY(launch_dense_scatter64(sk, sv, flags, N, dSym, s));
+            SAGPU_TRY(launch_fill_u64(d_max, 1, 0, s));
+            SAGPU_TRY(launch_max_sym(dSym, 4, N, d_max, s));
+            SAGPU_CHECK(hipMemcpyAsync(c->pinned, d_max, 8, hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipStreamSynchronize(s));
+            const u64 maxrank = c->pinned[0];
+            bits = maxrank ? bits_for(maxrank) : 1;
+        }
+    }
+    GET(dSA, u32, N);
+    GET(dISA, u32, N);
+    {
+        Scope work(*c);
+        SAGPU_TRY(sa_build(*c, work, nullptr, N, 0, dSA, dISA, nullptr, nullptr, nullptr, 0, dSym, bits));
+    }
+    SAGPU_TRY(download_idx(*c, sc, dSA, SA, N, width));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// PLCP of an integer string (libsais_plcp_int): int32 symbols compared 4 bytes at a
+// time by the byte kernels.
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void *SA, void *PLCP, int64_t n,
+                                                    int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 4 * (u64)n, true, &dT));
+    u32 *dSA;
+    SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
+    u32 *dPHI = sc.get<u32>((u64)n);
+    u32 *dP = sc.get<u32>((u64)n);
+    if (!dPHI || !dP) return SAGPU_ENOMEM;
+    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, 0, dPHI, nullptr, nullptr, dP, nullptr, 4));
+    SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 // BWT with auxiliary indexes: I[k] = 1 + SA slot of suffix k*r (libsais_bwt_aux).

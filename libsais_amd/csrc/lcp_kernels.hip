@@ -101,7 +101,9 @@ constexpr u64 PL_THREAD_LIMIT = 64;
 
 // Per position: decide reducibility, compare irreducible ones up to 64 bytes, queue
 // the rest.  Writes v[i] (PLCP[i] + i, or 0 for reducible positions) into PLCP.
-template <typename VT>
+// W = bytes per symbol: 1 for byte strings, 4 for int32 strings (libsais_plcp_int),
+// whose symbols are compared as W-byte groups of the same byte comparison.
+template <typename VT, int W>
 __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                   const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                   const u32 *__restrict__ SA, VT *__restrict__ V,
@@ -132,14 +134,18 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
         if (k > n) k = n;
         bool irr = (i == 0) || (k == 0) || (k >= n);
         if (!irr) {
-            uint8_t a = T[i - 1], b = T[k - 1];
-            irr = (a != b) || (gsa && a == 0);
+            if (W == 1) {
+                uint8_t a = T[i - 1], b = T[k - 1];
+                irr = (a != b) || (gsa && a == 0);
+            } else {
+                irr = *(const u32 *)(T + W * (i - 1)) != *(const u32 *)(T + W * (k - 1));
+            }
         }
         if (!irr) { V[i] = 0; continue; }
         if (k >= n) { V[i] = (VT)i; continue; }
         u64 m = n - (i > k ? i : k); // also bounds GSA matches (they stop earlier)
         bool done = false;
-        u64 l = match_bytes(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
+        u64 l = match_bytes(T, W * i, W * k, W * (lb < m ? lb : m), W * m, gsa, W * PL_THREAD_LIMIT, &done) / W;
         if (done) {
             V[i] = (VT)(l + i);
         } else {
@@ -150,7 +156,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                 V[i] = (VT)(l + i); // provisional; overwritten by k_plcp_long
             } else {
                 // queue full: finish here (correct, only slower)
-                l = match_bytes(T, i, k, l, m, gsa, ~0ull >> 1, &done);
+                l = match_bytes(T, W * i, W * k, W * l, W * m, gsa, ~0ull >> 1, &done) / W;
                 V[i] = (VT)(l + i);
             }
         }
@@ -158,8 +164,8 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
 }
 
 // One workgroup per queued position: 256 lanes x 8 bytes per step, first mismatch
-// found with a workgroup min-reduction.
-template <typename VT>
+// found with a workgroup min-reduction (byte offsets; W bytes per symbol).
+template <typename VT, int W>
 __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                    const u32 *__restrict__ SA, VT *__restrict__ V,
@@ -180,13 +186,13 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             k = r == 0 ? n : SA[r - 1];
         }
         if (k > n) k = n;
-        const u64 m = n - (i > k ? i : k);
-        u64 l = queue_l[q];
+        const u64 m = W * (n - (i > k ? i : k));
+        u64 l = W * (u64)queue_l[q];
         for (;;) {
             u64 off = l + (u64)threadIdx.x * 8;
             u64 first = ~0ull;
             if (off < m) {
-                u64 x = load8u(T + i + off), y = load8u(T + k + off);
+                u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
                 u64 stop = x ^ y;
                 if (gsa) stop |= (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
                 if (stop) {
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             u64 r = s_res;
             __syncthreads();
             if (r != ~0ull) {
-                if (threadIdx.x == 0) V[i] = (VT)(r + i);
+                if (threadIdx.x == 0) V[i] = (VT)(r / W + i);
                 break;
             }
             l += 256 * 8;
@@ -372,7 +378,7 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
     return SAGPU_OK;
 }
 
-template <typename VT>
+template <typename VT, int W>
 int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
                   u32 *PLCP, VT *V, const u32 *marks, hipStream_t s)
 {
@@ -383,9 +389,9 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     }
     VT *tmax = (VT *)ls.tile_max;
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
-    hipLaunchKernelGGL(k_plcp_irr<VT>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V,
+    hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V,
                        ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
-    hipLaunchKernelGGL(k_plcp_long<VT>, dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
+    hipLaunchKernelGGL((k_plcp_long<VT, W>), dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
                        ls.queue_n, ls.queue_cap);
     hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
     hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
@@ -395,10 +401,18 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
 }
 
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s)
+                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes)
 {
-    if (V64) return launch_plcp_t<u64>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s);
-    return launch_plcp_t<u32>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s);
+    if (sym_bytes == 4) {
+        if (gsa || marks || !PHI) {
+            sagpu_set_error("integer PLCP takes the Phi form only", __FILE__, __LINE__);
+            return SAGPU_EINVAL;
+        }
+        if (V64) return launch_plcp_t<u64, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, V64, nullptr, s);
+        return launch_plcp_t<u32, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, PLCP, nullptr, s);
+    }
+    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s);
+    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s);
 }
 
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)

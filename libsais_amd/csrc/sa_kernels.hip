@@ -201,6 +201,78 @@ __global__ void k_fill_u64(u64 *p, u64 count, u64 v)
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
+// Integer alphabet: packed keys of `k` symbols of `bits` bits each read from a u32
+// symbol array (values < 2^bits).  Past the end reads 0, as for bytes.
+__global__ __launch_bounds__(PK_THREADS) void k_pack_int(const u32 *__restrict__ S, u64 n, int bits, int k,
+                                                         u32 *__restrict__ keys, u32 *__restrict__ vals)
+{
+    __shared__ u32 s_c[PK_TILE + PK_HALO];
+    const u64 base = (u64)blockIdx.x * PK_TILE;
+    for (int i = threadIdx.x; i < PK_TILE + PK_HALO; i += PK_THREADS) {
+        u64 p = base + i;
+        s_c[i] = p < n ? S[p] : 0u;
+    }
+    __syncthreads();
+    for (int j = 0; j < PK_ITEMS; j++) {
+        int li = j * PK_THREADS + threadIdx.x;
+        u64 p = base + li;
+        if (p >= n) break;
+        u32 key = 0;
+        for (int t = 0; t < k; t++) key = (bits >= 32 ? 0u : key << bits) | s_c[li + t];
+        keys[p] = key;
+        vals[p] = (u32)p;
+    }
+}
+
+// Largest symbol value (as unsigned), for the symbol width.
+template <typename ST>
+__global__ __launch_bounds__(256) void k_max_sym(const ST *__restrict__ T, u64 n, u64 *__restrict__ out)
+{
+    u64 m = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 v = sizeof(ST) == 4 ? (u64)(u32)T[i] : (u64)T[i];
+        m = v > m ? v : m;
+    }
+    m = wave_incl_max64(m);
+    if (lane_id() == SAGPU_WAVE - 1 && m) atomicMax((unsigned long long *)out, (unsigned long long)m);
+}
+
+// Symbols as sort keys (u32 narrowing when the values are known to fit, or u64)
+// with their positions for dense ranking (vals may be null).
+template <typename ST, typename K>
+__global__ __launch_bounds__(256) void k_sym_keys(const ST *__restrict__ T, u64 n, K *__restrict__ keys,
+                                                  u32 *__restrict__ vals)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        keys[i] = sizeof(ST) == 4 ? (K)(u32)T[i] : (K)(u64)T[i];
+        if (vals) vals[i] = (u32)i;
+    }
+}
+
+// After sorting (value, position): flag[e] = 1 where a new value starts.
+template <typename K>
+__global__ __launch_bounds__(256) void k_run_flags(const K *__restrict__ keys, u64 n, u32 *__restrict__ flags)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+        flags[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1u : 0u;
+}
+
+// S[position] = dense rank (number of distinct smaller values), from the exclusive
+// scan of the run flags.
+template <typename K>
+__global__ __launch_bounds__(256) void k_dense_scatter(const K *__restrict__ keys, const u32 *__restrict__ vals,
+                                                       const u32 *__restrict__ excl, u64 n, u32 *__restrict__ S)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const u32 f = (e == 0 || keys[e] != keys[e - 1]) ? 1u : 0u;
+        S[vals[e]] = excl[e] + f - 1;
+    }
+}
+
 unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 {
     u64 b = (work + per_block - 1) / per_block;
@@ -228,6 +300,68 @@ int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, in
     }
     hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
                        keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_pack_int(const u32 *S, u64 n, int bits, int k, u32 *keys, u32 *vals, hipStream_t s)
+{
+    if (k > PK_HALO || k * bits > 32 || k < 1) {
+        sagpu_set_error("pack_int: bad symbol count", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    hipLaunchKernelGGL(k_pack_int, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, S, n, bits, k, keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_max_sym(const void *T, int width, u64 n, u64 *out, hipStream_t s)
+{
+    if (width == 4)
+        hipLaunchKernelGGL(k_max_sym<int32_t>, dim3(grid_for(n, 256 * 8, 4096)), dim3(256), 0, s, (const int32_t *)T,
+                           n, out);
+    else
+        hipLaunchKernelGGL(k_max_sym<int64_t>, dim3(grid_for(n, 256 * 8, 4096)), dim3(256), 0, s, (const int64_t *)T,
+                           n, out);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_sym_narrow(const void *T, int width, u64 n, u32 *S, hipStream_t s)
+{
+    if (width == 4)
+        hipLaunchKernelGGL((k_sym_keys<int32_t, u32>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s,
+                           (const int32_t *)T, n, S, (u32 *)nullptr);
+    else
+        hipLaunchKernelGGL((k_sym_keys<int64_t, u32>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s,
+                           (const int64_t *)T, n, S, (u32 *)nullptr);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_sym_keys64(const void *T, int width, u64 n, u64 *keys, u32 *vals, hipStream_t s)
+{
+    if (width == 4)
+        hipLaunchKernelGGL((k_sym_keys<int32_t, u64>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s,
+                           (const int32_t *)T, n, keys, vals);
+    else
+        hipLaunchKernelGGL((k_sym_keys<int64_t, u64>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s,
+                           (const int64_t *)T, n, keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_run_flags64(const u64 *keys, u64 n, u32 *flags, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_run_flags<u64>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, keys, n, flags);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_dense_scatter64(const u64 *keys, const u32 *vals, const u32 *excl, u64 n, u32 *S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dense_scatter<u64>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, keys, vals, excl, n,
+                       S);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -164,6 +164,60 @@ int oracle_sa(const uint8_t *T, int64_t *SA, int64_t n, int gsa)
     return rc;
 }
 
+/* Integer alphabet (libsais_int, libsais64_long): the order of suffixes depends
+ * only on the order of the symbol values, so the values are replaced by their dense
+ * ranks first and the same SA-IS runs on those. */
+static int cmp_i64(const void *a, const void *b)
+{
+    I x = *(const I *)a, y = *(const I *)b;
+    return (x > y) - (x < y);
+}
+
+int oracle_sa_int(const int64_t *T, int64_t *SA, int64_t n)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { SA[0] = 0; return 0; }
+    I *v = (I *)malloc((size_t)n * sizeof(I));
+    I *s = (I *)malloc((size_t)n * sizeof(I));
+    if (!v || !s) { free(v); free(s); return -2; }
+    memcpy(v, T, (size_t)n * sizeof(I));
+    qsort(v, (size_t)n, sizeof(I), cmp_i64);
+    I sigma = 0;
+    for (I i = 0; i < n; i++)
+        if (i == 0 || v[i] != v[i - 1]) v[sigma++] = v[i];
+    for (I i = 0; i < n; i++) {
+        I lo = 0, hi = sigma - 1;
+        while (lo < hi) {
+            I mid = (lo + hi) / 2;
+            if (v[mid] < T[i]) lo = mid + 1;
+            else hi = mid;
+        }
+        s[i] = lo;
+    }
+    free(v);
+    int rc = sais(s, SA, n, sigma);
+    free(s);
+    return rc;
+}
+
+/* libsais.c:8399-8415 with compute_plcp_int (8263-8286): Phi, then Kasai over
+ * integer symbols. */
+void oracle_plcp_int(const int64_t *T, const int64_t *SA, int64_t *PLCP, int64_t n)
+{
+    if (n <= 0) return;
+    if (n == 1) { PLCP[0] = 0; return; }
+    I k = n;
+    for (I i = 0; i < n; i++) { PLCP[SA[i]] = k; k = SA[i]; }
+    I l = 0;
+    for (I i = 0; i < n; i++) {
+        k = PLCP[i];
+        I m = n - (i > k ? i : k);
+        while (l < m && T[i + l] == T[k + l]) l++;
+        PLCP[i] = l;
+        l -= (l != 0);
+    }
+}
+
 /* ------------------------------------------------------- SA verification ---- */
 
 int oracle_check_sa(const uint8_t *T, const int64_t *SA, int64_t n, int gsa)

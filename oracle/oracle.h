@@ -16,6 +16,9 @@
  *   oracle_gsa_valid   compiled reference over all {0,1,2}-strings up to length 10.
  *   oracle_bwt_aux     compiled reference libsais_bwt_aux (U and I) over sampling
  *                      rates 2..n, hand-worked banana samples.
+ *   oracle_sa_int      brute force over small integer strings, the compiled
+ *                      reference libsais_int / libsais64_long.
+ *   oracle_plcp_int    the compiled reference libsais_plcp_int, direct comparison.
  *   oracle_unbwt       round trip T -> oracle_bwt -> oracle_unbwt, the compiled
  *                      reference libsais_unbwt / libsais_unbwt_aux, banana by hand.
  */
@@ -42,6 +45,13 @@ int     oracle_sa(const uint8_t *T, int64_t *SA, int64_t n, int gsa);
 /* BWT from a suffix array (libsais.c:7110-7118): writes U[0..n-1], returns the
  * primary index (1 + row of suffix 0). */
 int64_t oracle_bwt(const uint8_t *T, const int64_t *SA, uint8_t *U, int64_t n);
+
+/* Suffix array of an integer string (libsais_int / libsais64_long, any values:
+ * suffixes compare by symbol value, a proper prefix first).  Returns 0 or -2. */
+int     oracle_sa_int(const int64_t *T, int64_t *SA, int64_t n);
+
+/* PLCP of an integer string (libsais.c:8399-8415, 8263-8286). */
+void    oracle_plcp_int(const int64_t *T, const int64_t *SA, int64_t *PLCP, int64_t n);
 
 /* BWT with auxiliary indexes (libsais.c:7123-7146 with the sampling of
  * 4799-4819): U as oracle_bwt, and I[k] = 1 + row of suffix k*r for

@@ -82,6 +82,12 @@ def ref():
         r.libsais_unbwt.restype = _i32
         r.libsais_unbwt_aux.argtypes = [_vp, _vp, _vp, _i32, _vp, _i32, _vp]
         r.libsais_unbwt_aux.restype = _i32
+        r.libsais_int.argtypes = [_vp, _vp, _i32, _i32, _i32]
+        r.libsais_int.restype = _i32
+        r.libsais_plcp_int.argtypes = [_vp, _vp, _vp, _i32]
+        r.libsais_plcp_int.restype = _i32
+        r.libsais64_long.argtypes = [_vp, _vp, _i64, _i64, _i64]
+        r.libsais64_long.restype = _i64
         r.libsais64_omp.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64]
         r.libsais64_omp.restype = _i64
         for name in ("libsais64_plcp_omp", "libsais64_lcp_omp"):
@@ -282,3 +288,53 @@ def ref_unbwt_aux(U, r, I):
     I32 = np.ascontiguousarray(I, np.int32)
     rc = ref().libsais_unbwt_aux(_p(U), _p(out), _p(A), n, None, r, _p(I32))
     return rc, out[:n]
+
+
+# ------------------------------------------------------------ integer alphabet
+def sa_int(T):
+    """Oracle SA of an integer string (int64 result)."""
+    T64 = np.ascontiguousarray(T, np.int64)
+    n = len(T64)
+    SA = np.zeros(max(n, 1), np.int64)
+    o = oracle()
+    o.oracle_sa_int.argtypes = [_vp, _vp, _i64]
+    o.oracle_sa_int.restype = ctypes.c_int
+    assert o.oracle_sa_int(_p(T64), _p(SA), n) == 0
+    return SA[:n]
+
+
+def plcp_int(T, SA):
+    T64 = np.ascontiguousarray(T, np.int64)
+    S64 = np.ascontiguousarray(SA, np.int64)
+    n = len(T64)
+    P = np.zeros(max(n, 1), np.int64)
+    o = oracle()
+    o.oracle_plcp_int.argtypes = [_vp, _vp, _vp, _i64]
+    o.oracle_plcp_int.restype = None
+    o.oracle_plcp_int(_p(T64), _p(S64), _p(P), n)
+    return P[:n]
+
+
+def ref_sa_int(T, k):
+    T32 = np.ascontiguousarray(T, np.int32).copy()
+    n = len(T32)
+    SA = np.zeros(max(n, 1), np.int32)
+    rc = ref().libsais_int(_p(T32), _p(SA), n, k, 0)
+    return rc, SA[:n], T32
+
+
+def ref_sa_long(T, k):
+    T64 = np.ascontiguousarray(T, np.int64).copy()
+    n = len(T64)
+    SA = np.zeros(max(n, 1), np.int64)
+    rc = ref().libsais64_long(_p(T64), _p(SA), n, k, 0)
+    return rc, SA[:n], T64
+
+
+def ref_plcp_int(T, SA):
+    T32 = np.ascontiguousarray(T, np.int32)
+    S32 = np.ascontiguousarray(SA, np.int32)
+    n = len(T32)
+    P = np.zeros(max(n, 1), np.int32)
+    rc = ref().libsais_plcp_int(_p(T32), _p(S32), _p(P), n)
+    return rc, P[:n]

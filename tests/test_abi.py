@@ -42,9 +42,9 @@ def test_exports_every_declared_symbol():
 def test_no_out_of_scope_exports():
     out = subprocess.run(["nm", "-D", "--defined-only", la.LIB_PATH], capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    for name in ("libsais_ctx", "libsais_omp", "libsais_int", "libsais_create_ctx", "libsais_unbwt_create_ctx",
-                 "libsais16", "libsais64_long"):
-        assert name not in exported
+    # the 16-bit-alphabet libraries (libsais16.h, libsais16x64.h) are not provided
+    assert not any(name.startswith("libsais16") for name in exported)
+    assert all(name.startswith(("libsais", "sagpu")) for name in exported)
 
 
 def test_python_binding_table_matches_headers():
@@ -228,3 +228,33 @@ def test_aux_edge_cases_match_reference():
                         codes = _aux_calls(mod, T, U, A, I, n, fs, rr, freq)
                         outs.append((codes, U.tolist(), I.tolist(), freq.tolist()))
                     assert outs[0] == outs[1], (n, fs, rr, i0)
+
+
+def test_context_and_omp_forms_check_like_the_reference():
+    T = np.frombuffer(b"abcab", np.uint8).copy()
+    SA = np.zeros(5, np.int32)
+    ctx = la.libsais_create_ctx()
+    assert ctx
+    assert la.libsais_ctx(None, T, SA, 5, 0, None) == -1
+    assert la.libsais_ctx(ctx, None, SA, 5, 0, None) == -1
+    assert la.libsais_ctx(ctx, T, SA, 1, 0, None) == 0 and SA[0] == 0
+    la.libsais_free_ctx(ctx)
+    la.libsais_free_ctx(None)
+    assert la.libsais_create_ctx_omp(-1) is None
+    c2 = la.libsais_create_ctx_omp(0)
+    assert c2
+    la.libsais_free_ctx(c2)
+    u = la.libsais_unbwt_create_ctx()
+    assert u
+    la.libsais_unbwt_free_ctx(u)
+    assert la.libsais_omp(T, SA, 5, 0, None, -1) == -1
+    assert la.libsais_int_omp(np.zeros(5, np.int32), SA, 5, 3, 0, -1) == -1
+    assert la.libsais_int(np.zeros(5, np.int32), SA, -1, 3, 0) == -1
+    I = np.array([1], np.int32)
+    U = np.zeros(5, np.uint8)
+    A = np.zeros(6, np.int32)
+    assert la.libsais_unbwt_aux_omp(T, U, A, 5, None, 2, I, -2) == -1
+    assert la.libsais_plcp_omp(T, SA, SA, 5, -1) == -1
+    S64 = np.zeros(5, np.int64)
+    assert la.libsais64_long(np.zeros(1, np.int64), S64, 1, 5, 0) == 0 and S64[0] == 0
+    assert la.libsais64_omp(T, S64, 5, 0, None, -1) == -1

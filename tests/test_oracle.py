@@ -278,3 +278,37 @@ def test_bwt_aux_and_unbwt_against_reference(alphabet):
         U, p = ol.bwt(T, SA)
         rc3, out = ol.ref_unbwt(U, p)
         assert rc3 == 0 and np.array_equal(out, ol.unbwt(U, p)) and np.array_equal(out, T)
+
+
+# ------------------------------------------------------------ integer alphabet
+def test_int_oracle_brute_force():
+    rng = np.random.default_rng(31)
+    for n in range(1, 40):
+        for _ in range(5):
+            k = int(rng.integers(1, 6))
+            T = rng.integers(0, k, n) * 1000 - 7  # any values: only their order matters
+            SA = ol.sa_int(T)
+            brute = sorted(range(n), key=lambda i: list(T[i:]))
+            assert list(SA) == brute
+            P = ol.plcp_int(T, SA)
+            for j in range(1, n):
+                a, b = SA[j - 1], SA[j]
+                l = 0
+                while a + l < n and b + l < n and T[a + l] == T[b + l]:
+                    l += 1
+                assert P[b] == l
+
+
+@needs_ref
+@pytest.mark.parametrize("k", [1, 2, 3, 100, 70000, 1 << 22])
+def test_int_oracle_against_reference(k):
+    rng = np.random.default_rng(k)
+    for n in (2, 5, 1000, 30000):
+        T = rng.integers(0, k, n).astype(np.int32)
+        rc, SA, Tafter = ol.ref_sa_int(T, k)
+        assert rc == 0 and np.array_equal(Tafter, T)  # restored by the reference
+        assert np.array_equal(ol.sa_int(T), SA.astype(np.int64))
+        rc, P = ol.ref_plcp_int(T, SA)
+        assert rc == 0 and np.array_equal(ol.plcp_int(T, SA), P.astype(np.int64))
+        rc, SA64, _ = ol.ref_sa_long(T.astype(np.int64), k)
+        assert rc == 0 and np.array_equal(SA64, SA.astype(np.int64))
