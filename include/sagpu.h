@@ -165,7 +165,10 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
 /*
  * Tuning knobs (process-wide):
  *   "radix_variant"  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
- *   "radix_items"    keys per thread per radix tile: 16 (4096-key tiles) or 32 (default)
+ *   "radix_items"    keys per thread per radix tile: 8, 16 or 32 (2048-, 4096- or 8192-key
+ *                    tiles; default 32)
+ *   "tile_range"     list entries per workgroup of the doubling-round tile kernel: 256, 512
+ *                    (default) or 1024
  *   "bwt_variant"    0 = gather T[SA[j]-1] in SA order (default), 1 = scatter in text order
  *   "isa_window_log2" ISA scatter window after the initial sort, 2^v entries (20..32, default 26)
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the

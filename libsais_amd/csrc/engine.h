@@ -37,7 +37,7 @@ struct RadixScratch {
 
 enum { RADIX_ONESWEEP = 0, RADIX_REDUCE_THEN_SCAN = 1 };
 
-enum { RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS };
+enum { RS_THREADS = 256, RS_ITEMS = 8, RS_TILE = RS_THREADS * RS_ITEMS };  // smallest tile: sizes the scratch
 
 int radix_sort_u64(RadixScratch &rs, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit,
                    int end_bit, hipStream_t s, int *result_in_alt, int *passes_run);
@@ -63,6 +63,8 @@ int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits,
                u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
 // ISA scatter window after the initial sort: 2^v entries per pass (v in [20, 32]).
 void set_isa_window_log2(int v);
+// List entries per workgroup of the tile round kernel: 256, 512 (default) or 1024.
+void set_tile_range(int v);
 // Batched GSA: keys = (string id << 32) | packed symbols.
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);

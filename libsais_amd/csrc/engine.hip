@@ -1084,7 +1084,7 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!name) return SAGPU_EINVAL;
     std::lock_guard<std::mutex> lk(g_mu);
     if (!strcmp(name, "radix_items")) {
-        if (value != 16 && value != 32) return SAGPU_EINVAL;
+        if (value != 8 && value != 16 && value != 32) return SAGPU_EINVAL;
         g_radix_items = (int)value;
         if (g_ctx) g_ctx->rs.items = g_radix_items;
         return 0;
@@ -1098,6 +1098,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "bwt_variant")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_bwt_variant = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "tile_range")) {
+        if (value != 256 && value != 512 && value != 1024) return SAGPU_EINVAL;
+        set_tile_range((int)value);
         return 0;
     }
     if (!strcmp(name, "isa_window_log2")) {

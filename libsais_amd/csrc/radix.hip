@@ -286,7 +286,7 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     if (passes_run) *passes_run = 0;
     if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
     const int npass = (end_bit - begin_bit + 7) / 8;
-    const int items = rs.items == 32 ? 32 : 16;
+    const int items = rs.items == 32 ? 32 : rs.items == 8 ? 8 : 16;
     const u64 tile_keys = (u64)RS_THREADS * items;
     const u64 tiles = (n + tile_keys - 1) / tile_keys;
     if (tiles > rs.lb_tiles || npass > 8 || (rs.variant == RADIX_REDUCE_THEN_SCAN && tiles * 256 > rs.counts_cap)) {
@@ -323,7 +323,10 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         const int shift = begin_bit + 8 * p;
         const u32 dmask = end_bit - shift >= 8 ? 255u : ((1u << (end_bit - shift)) - 1u);
         if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
-            if (items == 32)
+            if (items == 8)
+                hipLaunchKernelGGL((k_rts_upsweep<K, 8>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                   shift, dmask, rs.counts, tiles);
+            else if (items == 32)
                 hipLaunchKernelGGL((k_rts_upsweep<K, 32>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
                                    shift, dmask, rs.counts, tiles);
             else
@@ -341,11 +344,11 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
                        n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles)
         if (v0) {
-            if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else RS_LAUNCH(true, true, 16); }
-            else { if (items == 32) RS_LAUNCH(true, false, 32); else RS_LAUNCH(true, false, 16); }
+            if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
+            else { if (items == 32) RS_LAUNCH(true, false, 32); else if (items == 8) RS_LAUNCH(true, false, 8); else RS_LAUNCH(true, false, 16); }
         } else {
-            if (lb) { if (items == 32) RS_LAUNCH(false, true, 32); else RS_LAUNCH(false, true, 16); }
-            else { if (items == 32) RS_LAUNCH(false, false, 32); else RS_LAUNCH(false, false, 16); }
+            if (lb) { if (items == 32) RS_LAUNCH(false, true, 32); else if (items == 8) RS_LAUNCH(false, true, 8); else RS_LAUNCH(false, true, 16); }
+            else { if (items == 32) RS_LAUNCH(false, false, 32); else if (items == 8) RS_LAUNCH(false, false, 8); else RS_LAUNCH(false, false, 16); }
         }
 #undef RS_LAUNCH
         SAGPU_CHECK(hipGetLastError());

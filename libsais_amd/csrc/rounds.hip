@@ -12,7 +12,7 @@
 //                    unsorted groups are free until their suffixes are placed;
 //                    the last index, not the end, so 2^32 members still fit)
 //   k_round_tile     groups of <= RT_SMALL members whose first element lies in a
-//                    512-wide stretch of the list: ranked by key2 inside LDS,
+//                    512-wide stretch of the list (tunable): ranked by key2 inside LDS,
 //                    regrouped, ISA/SA written, survivors emitted
 //   big path         larger groups: extracted, sorted by (group, key2) with the
 //                    radix sort, regrouped with a segmented scan (BigF)
@@ -334,10 +334,11 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
     }
 }
 
-constexpr int RT_THREADS = 256, RT_RANGE = 512, RT_CAP = RT_RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
-static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
+constexpr int RT_THREADS = 256;
 
-template <typename KT>
+// RANGE = list entries whose group heads one workgroup takes (256, 512 or 1024; the
+// "tile_range" option): smaller ranges use less LDS and fit more workgroups per CU.
+template <typename KT, int RANGE>
 __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                            const KT *__restrict__ key2, u64 a, u32 h,
                                                            u32 *__restrict__ SA, u32 *__restrict__ ISA,
@@ -347,6 +348,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                                                            u32 *__restrict__ big_size, u32 *__restrict__ big_counter,
                                                            u64 big_cap, u32 *__restrict__ LCP)
 {
+    constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
+    static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
     __shared__ u32 s_i[RT_CAP], s_g[RT_CAP], s_x0[RT_CAP];
     __shared__ KT s_k[RT_CAP];
     __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
@@ -355,8 +358,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     __shared__ u64 s_base;
 
     const u32 tid = threadIdx.x;
-    const u64 lo = (u64)blockIdx.x * RT_RANGE;
-    const u64 hi = lo + RT_RANGE < a ? lo + RT_RANGE : a;
+    const u64 lo = (u64)blockIdx.x * RANGE;
+    const u64 hi = lo + RANGE < a ? lo + RANGE : a;
     if (tid == 0) {
         s_first = 0xFFFFFFFFu;
         s_end = 0;
@@ -524,6 +527,7 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 } // namespace
 
 int g_isa_window_log2 = 26;
+int g_tile_range = 512;
 
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
@@ -543,6 +547,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
 }
 
 void set_isa_window_log2(int v) { g_isa_window_log2 = v; }
+void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
@@ -572,9 +577,14 @@ int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 
                  u64 big_cap, u32 *LCP, hipStream_t s)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    hipLaunchKernelGGL(k_round_tile<KT>, dim3(ceil_div_u(a, RT_RANGE)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a,
-                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,
-                       LCP);
+#define RT_LAUNCH(R)                                                                                              \
+    hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, \
+                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,  \
+                       LCP)
+    if (g_tile_range == 256) RT_LAUNCH(256);
+    else if (g_tile_range == 1024) RT_LAUNCH(1024);
+    else RT_LAUNCH(512);
+#undef RT_LAUNCH
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
