@@ -117,9 +117,23 @@ def cpu_baseline(T: np.ndarray, reps: int):
             times.append(t3 - t0)
             log(f"cpu reference: sa {t1 - t0:.2f}s bwt {t2 - t1:.2f}s plcp+lcp {t3 - t2:.2f}s")
         t = float(np.median(times))
+        # thread sweep on a 32 MiB prefix (the reference saturates near the memory
+        # channel count), single-threaded included
+        m = min(n, 1 << 25)
+        S = np.ascontiguousarray(T[:m])
+        sweep = {}
+        for th in sorted({1, 8, cores}):
+            t0 = time.perf_counter()
+            rc, SA_s, _ = ol.ref_sa(S, threads=th)
+            ol.ref_bwt(S, threads=th)
+            P_s = ol.ref_plcp(S, SA_s, threads=th)
+            ol.ref_lcp(P_s, SA_s, threads=th)
+            sweep[str(th)] = round(m / (time.perf_counter() - t0) / 1e6, 2)
+        log(f"cpu thread sweep on {m} bytes: {sweep}")
         return {"value": round(n / t / 1e6, 2), "unit": "MB/s", "cores": cores, "kind": "reference",
                 "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp, "
-                          f"median of {reps}"}
+                          f"median of {reps}",
+                "thread_sweep_MBps": {"sample_bytes": m, **sweep}}
     # oracle port, single thread, on a bounded prefix
     m = min(n, 1 << 24)
     S = np.ascontiguousarray(T[:m])
@@ -389,6 +403,24 @@ def main():
     }
     for p in (dT, dSA, dU, dP, dL):
         lib.sagpu_free(p)
+
+    # the libsais API on host buffers (PCIe transfers included): libsais + libsais_bwt
+    # + libsais_plcp + libsais_lcp, as a caller of the reference would run them
+    SAh = np.empty(n, np.int32)
+    Uh = np.empty(n, np.uint8)
+    Ph = np.empty(n, np.int32)
+    Lh = np.empty(n, np.int32)
+    Ah = np.empty(1, np.int32)
+    la.libsais(T, SAh, n, 0, None)
+    t0 = time.perf_counter()
+    assert la.libsais(T, SAh, n, 0, None) == 0
+    assert la.libsais_bwt(T, Uh, Ah, n, 0, None) > 0
+    assert la.libsais_plcp(T, SAh, Ph, n) == 0
+    assert la.libsais_lcp(Ph, SAh, Lh, n) == 0
+    t_api = time.perf_counter() - t0
+    result["host_api"] = {"calls": "libsais + libsais_bwt + libsais_plcp + libsais_lcp on host buffers",
+                          "seconds": round(t_api, 3), "MBps": round(n / t_api / 1e6, 1)}
+    del SAh, Uh, Ph, Lh
 
     if not args.no_extras:
         log("batched GSA ...")

@@ -144,6 +144,63 @@ ABI_64 += [
     ("libsais64_plcp_gsa_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
     ("libsais64_lcp_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
 ]
+# 16-bit alphabet (libsais16.h / libsais16x64.h)
+ABI_16 = [
+    ("libsais16_create_ctx", _vp, []),
+    ("libsais16_free_ctx", None, [_vp]),
+    ("libsais16", _i32, [_vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_gsa", _i32, [_vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_int", _i32, [_vp, _vp, _i32, _i32, _i32]),
+    ("libsais16_ctx", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_gsa_ctx", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_bwt", _i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_bwt_aux", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]),
+    ("libsais16_bwt_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
+    ("libsais16_bwt_aux_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]),
+    ("libsais16_unbwt_create_ctx", _vp, []),
+    ("libsais16_unbwt_free_ctx", None, [_vp]),
+    ("libsais16_unbwt", _i32, [_vp, _vp, _vp, _i32, _vp, _i32]),
+    ("libsais16_unbwt_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32]),
+    ("libsais16_unbwt_aux", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _vp]),
+    ("libsais16_unbwt_aux_ctx", _i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp]),
+    ("libsais16_plcp", _i32, [_vp, _vp, _vp, _i32]),
+    ("libsais16_plcp_gsa", _i32, [_vp, _vp, _vp, _i32]),
+    ("libsais16_lcp", _i32, [_vp, _vp, _vp, _i32]),
+    ("libsais16_create_ctx_omp", _vp, [_i32]),
+    ("libsais16_omp", _i32, [_vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais16_gsa_omp", _i32, [_vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais16_int_omp", _i32, [_vp, _vp, _i32, _i32, _i32, _i32]),
+    ("libsais16_bwt_omp", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32]),
+    ("libsais16_bwt_aux_omp", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp, _i32]),
+    ("libsais16_unbwt_create_ctx_omp", _vp, [_i32]),
+    ("libsais16_unbwt_omp", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _i32]),
+    ("libsais16_unbwt_aux_omp", _i32, [_vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32]),
+    ("libsais16_plcp_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+    ("libsais16_plcp_gsa_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+    ("libsais16_lcp_omp", _i32, [_vp, _vp, _vp, _i32, _i32]),
+]
+ABI_16X64 = [
+    ("libsais16x64", _i64, [_vp, _vp, _i64, _i64, _vp]),
+    ("libsais16x64_gsa", _i64, [_vp, _vp, _i64, _i64, _vp]),
+    ("libsais16x64_long", _i64, [_vp, _vp, _i64, _i64, _i64]),
+    ("libsais16x64_bwt", _i64, [_vp, _vp, _vp, _i64, _i64, _vp]),
+    ("libsais16x64_bwt_aux", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp]),
+    ("libsais16x64_unbwt", _i64, [_vp, _vp, _vp, _i64, _vp, _i64]),
+    ("libsais16x64_unbwt_aux", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _vp]),
+    ("libsais16x64_plcp", _i64, [_vp, _vp, _vp, _i64]),
+    ("libsais16x64_plcp_gsa", _i64, [_vp, _vp, _vp, _i64]),
+    ("libsais16x64_lcp", _i64, [_vp, _vp, _vp, _i64]),
+    ("libsais16x64_omp", _i64, [_vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais16x64_gsa_omp", _i64, [_vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais16x64_long_omp", _i64, [_vp, _vp, _i64, _i64, _i64, _i64]),
+    ("libsais16x64_bwt_omp", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64]),
+    ("libsais16x64_bwt_aux_omp", _i64, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64]),
+    ("libsais16x64_unbwt_omp", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _i64]),
+    ("libsais16x64_unbwt_aux_omp", _i64, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64]),
+    ("libsais16x64_plcp_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
+    ("libsais16x64_plcp_gsa_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
+    ("libsais16x64_lcp_omp", _i64, [_vp, _vp, _vp, _i64, _i64]),
+]
 ABI_SAGPU = [
     ("sagpu_init", _i32, []),
     ("sagpu_shutdown", None, []),
@@ -176,7 +233,7 @@ ABI_SAGPU = [
     ("sagpu_set_timing", None, [_i32]),
     ("sagpu_get_stats", _i32, [ctypes.POINTER(SagpuStats)]),
 ]
-ABI = ABI_32 + ABI_64 + ABI_SAGPU
+ABI = ABI_32 + ABI_64 + ABI_16 + ABI_16X64 + ABI_SAGPU
 
 
 def _declare(lib):
@@ -228,8 +285,8 @@ libsais64_plcp_gsa = _mk("libsais64_plcp_gsa")
 libsais64_lcp = _mk("libsais64_lcp")
 
 # every other exported entry point under its C name (contexts, OpenMP forms,
-# integer alphabets)
-for _name, _res, _args in ABI_32 + ABI_64:
+# integer alphabets, 16-bit alphabet)
+for _name, _res, _args in ABI_32 + ABI_64 + ABI_16 + ABI_16X64:
     if _name not in globals():
         globals()[_name] = _mk(_name)
 

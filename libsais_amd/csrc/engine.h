@@ -112,6 +112,7 @@ struct LcpScratch {
 };
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
+int launch_bwt16(const uint16_t *T, const u32 *SA, const u32 *ISA, u64 n, uint16_t *U, hipStream_t s);
 // The same BWT in text order: U[row of suffix i] = T[i-1], reading ISA and T
 // sequentially and scattering bytes.
 int launch_bwt_scatter(const uint8_t *T, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
@@ -129,12 +130,16 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
 int launch_aux_samples(const u32 *ISA, u64 n, u64 r, u64 count, u32 *out, hipStream_t s);
 u64 unbwt_count_slots(u64 n);
 int launch_unbwt_succ(const uint8_t *U, u64 n, u64 p, u32 *counts, ScanScratch &sc, u32 *S, hipStream_t s);
-int launch_unbwt_aux_walk(const uint8_t *U, const u32 *S, u64 n, u64 p, u64 r, const u64 *rows, u64 chunks,
-                          uint8_t *out, u32 *err, hipStream_t s);
+int launch_unbwt_aux_walk(const void *U, int sym_bytes, const u32 *S, u64 n, u64 p, u64 r, const u64 *rows,
+                          u64 chunks, void *out, u32 *err, hipStream_t s);
+// 16-bit symbols: sort keys for the stable rank, then the successor array from the
+// sorted positions.
+int launch_unbwt_succ16(const uint16_t *U, u64 n, u64 p, u32 *keys, u32 *vals, u32 *S, hipStream_t s);
+int launch_unbwt_succ_sorted(const u32 *V, u64 n, u64 p, u32 *S, hipStream_t s);
 int launch_unbwt_flags(u64 n, u32 *flags, hipStream_t s);
 int launch_unbwt_sample_total(const u32 *sid, u64 n, u32 *out, hipStream_t s);
-int launch_unbwt_rank(const uint8_t *U, const u32 *S, const u32 *sid, u64 n, u32 m, u32 *samp_q, u32 *nxt[2],
-                      u64 *dist[2], u64 *gap, uint8_t *out, u32 *err, hipStream_t s);
+int launch_unbwt_rank(const void *U, int sym_bytes, const u32 *S, const u32 *sid, u64 n, u32 m, u32 *samp_q,
+                      u32 *nxt[2], u64 *dist[2], u64 *gap, void *out, u32 *err, hipStream_t s);
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 
 // Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).
@@ -142,7 +147,10 @@ int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
 int launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);
 int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s);
 // Integer alphabet (libsais_int / libsais64_long): T is int32 (width 4) or int64.
-int launch_pack_int(const u32 *S, u64 n, int bits, int k, u32 *keys, u32 *vals, hipStream_t s);
+int launch_pack_int(const u32 *S, u64 n, int bits, int k, int gsa, u32 *keys, u32 *vals, hipStream_t s);
+// 16-bit alphabet: symbol histogram (+ GSA validity count) and dense coding.
+int launch_hist16(const uint16_t *T, u64 n, int gsa, u64 *hist65536, u32 *bad, hipStream_t s);
+int launch_map16(const uint16_t *T, u64 n, const u32 *lut, u32 *S, hipStream_t s);
 int launch_max_sym(const void *T, int width, u64 n, u64 *out, hipStream_t s);
 int launch_sym_narrow(const void *T, int width, u64 n, u32 *S, hipStream_t s);
 int launch_sym_keys64(const void *T, int width, u64 n, u64 *keys, u32 *vals, hipStream_t s);

@@ -430,7 +430,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
         {
             Span sp(c, F_INITIAL);
-            if (dSym) SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, k0, v0, s));
+            if (dSym) SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, gsa, k0, v0, s));
             else SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, k0, v0, s));
         }
         SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt));
@@ -883,49 +883,58 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U,
     return 0;
 }
 
-// Inverse BWT (libsais_unbwt / libsais_unbwt_aux).  T = the BWT, U = the output
-// text, I[0..(n-1)/r] already range-checked by the caller, r == n for plain unbwt.
-extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, int64_t n, int64_t r, const void *I,
-                                                 int32_t width)
+// Inverse BWT (libsais_unbwt / libsais_unbwt_aux and the 16-bit forms).  T = the
+// BWT (sym_bytes per symbol), U = the output text, I[0..(n-1)/r] already
+// range-checked by the caller, r == n for plain unbwt.
+int unbwt_host(Ctx &c, const void *T, void *U, u64 N, u64 r, const void *I, int width, int sym_bytes)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
-    if ((u64)n >= (1ull << 32)) {
-        sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
-        return SAGPU_ENOMEM;
-    }
-    stats_reset(*c);
-    const u64 N = (u64)n, chunks = (N - 1) / (u64)r + 1;
+    const u64 chunks = (N - 1) / r + 1;
     std::vector<u64> rows(chunks);
     for (u64 k = 0; k < chunks; k++)
         rows[k] = width == 4 ? (u64)((const int32_t *)I)[k] : (u64)((const int64_t *)I)[k];
-    Scope sc(*c);
-    hipStream_t s = c->stream;
-    GET(dB, uint8_t, N);
-    GET(dOut, uint8_t, N);
+    Scope sc(c);
+    hipStream_t s = c.stream;
+    const u64 bytes = N * (u64)sym_bytes;
+    GET(dB, uint8_t, bytes);
+    GET(dOut, uint8_t, bytes);
     GET(S, u32, N);
     GET(d_err, u32, 64);
-    const u64 slots = unbwt_count_slots(N);
-    GET(counts, u32, slots);
-    const u64 scap = (slots > N ? slots : N) / 4096 + 2;
-    GET(part, u32, scap);
-    ScanScratch ssc{part, scap};
-    SAGPU_CHECK(hipMemcpyAsync(dB, T, N, hipMemcpyHostToDevice, s));
-    SAGPU_CHECK(hipMemsetAsync(dOut, 0, N, s));
+    SAGPU_CHECK(hipMemcpyAsync(dB, T, bytes, hipMemcpyHostToDevice, s));
+    SAGPU_CHECK(hipMemsetAsync(dOut, 0, bytes, s));
     SAGPU_TRY(launch_fill_u32(d_err, 2, 0, s));
-    SAGPU_TRY(launch_unbwt_succ(dB, N, rows[0], counts, ssc, S, s));
-    if (r < n && chunks >= 32768) {
+    const u64 pcap = N / 4096 + 2;
+    if (sym_bytes == 1) {
+        const u64 slots = unbwt_count_slots(N);
+        GET(counts, u32, slots);
+        const u64 scap = slots / 4096 + 2;
+        GET(part, u32, scap);
+        ScanScratch ssc{part, scap};
+        SAGPU_TRY(launch_unbwt_succ(dB, N, rows[0], counts, ssc, S, s));
+    } else {
+        // 16-bit symbols: stable sort of (symbol, position); slot k holds LF row k + 1
+        Scope rs(c);
+        u32 *k0 = rs.get<u32>(N), *k1 = rs.get<u32>(N), *v0 = rs.get<u32>(N), *v1 = rs.get<u32>(N);
+        if (!k0 || !k1 || !v0 || !v1) return SAGPU_ENOMEM;
+        SAGPU_TRY(launch_unbwt_succ16((const uint16_t *)dB, N, rows[0], k0, v0, S, s));
+        SAGPU_TRY(ensure_radix_scratch(c, rs, N));
+        int alt = 0;
+        SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, N, 0, 16, &alt));
+        SAGPU_TRY(launch_unbwt_succ_sorted(alt ? v1 : v0, N, rows[0], S, s));
+        SAGPU_CHECK(hipStreamSynchronize(s));
+    }
+    if (r < N && chunks >= 32768) {
         GET(d_rows, u64, chunks);
         SAGPU_CHECK(hipMemcpyAsync(d_rows, rows.data(), chunks * 8, hipMemcpyHostToDevice, s));
-        SAGPU_TRY(launch_unbwt_aux_walk(dB, S, N, rows[0], (u64)r, d_rows, chunks, dOut, d_err, s));
+        SAGPU_TRY(launch_unbwt_aux_walk(dB, sym_bytes, S, N, rows[0], r, d_rows, chunks, dOut, d_err, s));
     } else {
+        GET(part, u32, pcap);
+        ScanScratch ssc{part, pcap};
         GET(sid, u32, N);
         SAGPU_TRY(launch_unbwt_flags(N, sid, s));
         SAGPU_TRY(scan_exclusive_u32(ssc, sid, N, s));
         SAGPU_TRY(launch_unbwt_sample_total(sid, N, d_err + 1, s));
         u32 m = 0;
-        SAGPU_TRY(read_u32(*c, d_err + 1, &m));
+        SAGPU_TRY(read_u32(c, d_err + 1, &m));
         if (m == 0 || m > N) {
             sagpu_set_error("inverse BWT: sample count inconsistent", __FILE__, __LINE__);
             return SAGPU_EDEVICE;
@@ -938,17 +947,181 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, i
         GET(gap, u64, m);
         u32 *nxt[2] = {nxt0, nxt1};
         u64 *dist[2] = {dist0, dist1};
-        SAGPU_TRY(launch_unbwt_rank(dB, S, sid, N, m, samp_q, nxt, dist, gap, dOut, d_err, s));
+        SAGPU_TRY(launch_unbwt_rank(dB, sym_bytes, S, sid, N, m, samp_q, nxt, dist, gap, dOut, d_err, s));
     }
     u32 err = 0;
-    SAGPU_TRY(read_u32(*c, d_err, &err));
+    SAGPU_TRY(read_u32(c, d_err, &err));
     if (err & 2u) {
         sagpu_set_error("inverse BWT: a sample gap exceeded its step limit", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
-    SAGPU_CHECK(hipMemcpyAsync(U, dOut, N, hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipMemcpyAsync(U, dOut, bytes, hipMemcpyDeviceToHost, s));
     SAGPU_CHECK(hipStreamSynchronize(s));
     return 0;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, int64_t n, int64_t r, const void *I,
+                                                 int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    return unbwt_host(*c, T, U, (u64)n, (u64)r, I, width, 1);
+}
+
+// ------------------------------------------------------------ 16-bit alphabet
+// (libsais16 / libsais16x64): the 65536-bin histogram gives freq, the GSA check and
+// dense symbol codes; from there the integer-mode pipeline builds SA and ISA.
+int sa16_build(Ctx &c, Scope &sc, const uint16_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq)
+{
+    hipStream_t s = c.stream;
+    GET(d_hist, u64, 65536 + 8);
+    u32 *d_bad = (u32 *)(d_hist + 65536);
+    SAGPU_TRY(launch_fill_u64(d_hist, 65536 + 8, 0, s));
+    SAGPU_TRY(launch_hist16(dT, n, gsa, d_hist, d_bad, s));
+    std::vector<u64> h(65536 + 1);
+    SAGPU_CHECK(hipMemcpyAsync(h.data(), d_hist, (65536 + 1) * sizeof(u64), hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    if (freq) memcpy(freq, h.data(), 65536 * sizeof(u64));
+    if (gsa && (u32)h[65536] != 0) return SAGPU_EINVAL;
+    std::vector<u32> lut(65536);
+    u32 sigma = 0;
+    for (u32 v = 0; v < 65536; v++) {
+        lut[v] = sigma;
+        if (h[v]) sigma++;
+    }
+    const int bits = sigma <= 2 ? 1 : bits_for((u64)sigma - 1);
+    GET(d_lut, u32, 65536);
+    SAGPU_CHECK(hipMemcpyAsync(d_lut, lut.data(), 65536 * 4, hipMemcpyHostToDevice, s));
+    GET(dSym, u32, n);
+    SAGPU_TRY(launch_map16(dT, n, d_lut, dSym, s));
+    Scope work(c);
+    SAGPU_TRY(sa_build(c, work, nullptr, n, gsa, dSA, dISA, nullptr, nullptr, nullptr, 0, dSym, bits));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    return SAGPU_OK;
+}
+
+void store_freq16(const u64 *f, void *freq, int width)
+{
+    if (!freq) return;
+    for (u32 v = 0; v < 65536; v++) {
+        if (width == 4) ((int32_t *)freq)[v] = (int32_t)f[v];
+        else ((int64_t *)freq)[v] = (int64_t)f[v];
+    }
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa16(const uint16_t *T, void *SA, int64_t n, int32_t gsa, void *freq,
+                                                int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    const u64 N = (u64)n;
+    Scope sc(*c);
+    GET(dT, uint16_t, N);
+    SAGPU_CHECK(hipMemcpyAsync(dT, T, 2 * N, hipMemcpyHostToDevice, c->stream));
+    GET(dSA, u32, N);
+    GET(dISA, u32, N);
+    std::vector<u64> f(65536);
+    SAGPU_TRY(sa16_build(*c, sc, dT, N, gsa, dSA, dISA, f.data()));
+    SAGPU_TRY(download_idx(*c, sc, dSA, SA, N, width));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    store_freq16(f.data(), freq, width);
+    return 0;
+}
+
+// BWT of 16-bit symbols; r > 0 also writes the auxiliary indexes I (libsais16_bwt_aux)
+// and returns 0, r == 0 returns the primary index (libsais16_bwt).
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U, int64_t n, void *freq, int64_t r,
+                                                 void *I, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    const u64 N = (u64)n;
+    hipStream_t s = c->stream;
+    Scope sc(*c);
+    GET(dT, uint16_t, N);
+    SAGPU_CHECK(hipMemcpyAsync(dT, T, 2 * N, hipMemcpyHostToDevice, s));
+    GET(dSA, u32, N);
+    GET(dISA, u32, N);
+    GET(dU, uint16_t, N);
+    std::vector<u64> f(65536);
+    SAGPU_TRY(sa16_build(*c, sc, dT, N, 0, dSA, dISA, f.data()));
+    SAGPU_TRY(launch_bwt16(dT, dSA, dISA, N, dU, s));
+    u32 j0 = 0;
+    SAGPU_TRY(read_u32(*c, dISA, &j0));
+    if (r > 0) {
+        const u64 count = (N - 1) / (u64)r + 1;
+        GET(dS, u32, count);
+        SAGPU_TRY(launch_aux_samples(dISA, N, (u64)r, count, dS, s));
+        std::vector<u32> hs(count);
+        SAGPU_CHECK(hipMemcpyAsync(hs.data(), dS, count * 4, hipMemcpyDeviceToHost, s));
+        SAGPU_CHECK(hipStreamSynchronize(s));
+        for (u64 k = 0; k < count; k++) {
+            if (width == 4) ((int32_t *)I)[k] = (int32_t)(hs[k] + 1);
+            else ((int64_t *)I)[k] = (int64_t)hs[k] + 1;
+        }
+    }
+    SAGPU_CHECK(hipMemcpyAsync(U, dU, 2 * N, hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipStreamSynchronize(s));
+    store_freq16(f.data(), freq, width);
+    return r > 0 ? 0 : (int64_t)j0 + 1;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp16(const uint16_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,
+                                                  int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n > (1ull << 32)) {
+        sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    Scope sc(*c);
+    uint8_t *dT;
+    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 2 * (u64)n, true, &dT));
+    u32 *dSA;
+    SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
+    u32 *dPHI = sc.get<u32>((u64)n);
+    u32 *dP = sc.get<u32>((u64)n);
+    if (!dPHI || !dP) return SAGPU_ENOMEM;
+    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, nullptr, 2));
+    SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
+    SAGPU_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt16(const uint16_t *T, uint16_t *U, int64_t n, int64_t r,
+                                                   const void *I, int32_t width)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx *c;
+    SAGPU_TRY(ctx_get(&c));
+    if ((u64)n >= (1ull << 32)) {
+        sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
+        return SAGPU_ENOMEM;
+    }
+    stats_reset(*c);
+    return unbwt_host(*c, T, U, (u64)n, (u64)r, I, width, 2);
 }
 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,

@@ -22,15 +22,16 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 
 // BWT with the libsais layout: U[0] = T[n-1]; row j (j != j0, SA[j0] == 0) goes to
 // U[j + (j < j0)].  j0 = ISA[0].
-__global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, const u32 *__restrict__ SA,
-                                             const u32 *__restrict__ ISA, u64 n, uint8_t *__restrict__ U)
+template <typename ST>
+__global__ __launch_bounds__(256) void k_bwt(const ST *__restrict__ T, const u32 *__restrict__ SA,
+                                             const u32 *__restrict__ ISA, u64 n, ST *__restrict__ U)
 {
     const u64 j0 = ISA[0];
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         u32 s = SA[j];
         if (j == j0) U[0] = T[n - 1];
-        else U[j + (j < j0)] = (s - 1u < n) ? T[s - 1] : (uint8_t)0;
+        else U[j + (j < j0)] = (s - 1u < n) ? T[s - 1] : (ST)0;
     }
 }
 
@@ -76,8 +77,18 @@ __device__ __forceinline__ u64 load8u(const uint8_t *p)
     return (lo >> sh) | (hi << (64 - sh));
 }
 
+// Lanes of x that hold a zero symbol (W-byte symbols, W = 1 or 2; for W = 4 GSA is not
+// used).  Only the lowest flagged lane is exact, which is the one used.
+template <int W>
+__device__ __forceinline__ u64 zero_lanes(u64 x)
+{
+    if (W == 2) return (x - 0x0001000100010001ull) & ~x & 0x8000800080008000ull;
+    return (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+}
+
 // Compare up to `limit` more bytes from offset l.  Returns the new l; *done is set
 // when the comparison ended (mismatch, separator, or bound).
+template <int W>
 __device__ __forceinline__ u64 match_bytes(const uint8_t *T, u64 i, u64 k, u64 l, u64 m, int gsa, u64 limit,
                                            bool *done)
 {
@@ -86,7 +97,7 @@ __device__ __forceinline__ u64 match_bytes(const uint8_t *T, u64 i, u64 k, u64 l
         if (l >= m) { *done = true; return m; }
         u64 x = load8u(T + i + l), y = load8u(T + k + l);
         u64 stop = x ^ y;
-        if (gsa) stop |= (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+        if (gsa) stop |= zero_lanes<W>(x);
         if (stop) {
             l += (u64)(__builtin_ctzll(stop) >> 3);
             *done = true;
@@ -137,6 +148,9 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
             if (W == 1) {
                 uint8_t a = T[i - 1], b = T[k - 1];
                 irr = (a != b) || (gsa && a == 0);
+            } else if (W == 2) {
+                uint16_t a = *(const uint16_t *)(T + 2 * (i - 1)), b = *(const uint16_t *)(T + 2 * (k - 1));
+                irr = (a != b) || (gsa && a == 0);
             } else {
                 irr = *(const u32 *)(T + W * (i - 1)) != *(const u32 *)(T + W * (k - 1));
             }
@@ -145,7 +159,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
         if (k >= n) { V[i] = (VT)i; continue; }
         u64 m = n - (i > k ? i : k); // also bounds GSA matches (they stop earlier)
         bool done = false;
-        u64 l = match_bytes(T, W * i, W * k, W * (lb < m ? lb : m), W * m, gsa, W * PL_THREAD_LIMIT, &done) / W;
+        u64 l = match_bytes<W>(T, W * i, W * k, W * (lb < m ? lb : m), W * m, gsa, W * PL_THREAD_LIMIT, &done) / W;
         if (done) {
             V[i] = (VT)(l + i);
         } else {
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                 V[i] = (VT)(l + i); // provisional; overwritten by k_plcp_long
             } else {
                 // queue full: finish here (correct, only slower)
-                l = match_bytes(T, W * i, W * k, W * l, W * m, gsa, ~0ull >> 1, &done) / W;
+                l = match_bytes<W>(T, W * i, W * k, W * l, W * m, gsa, ~0ull >> 1, &done) / W;
                 V[i] = (VT)(l + i);
             }
         }
@@ -194,7 +208,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             if (off < m) {
                 u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
                 u64 stop = x ^ y;
-                if (gsa) stop |= (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+                if (gsa) stop |= zero_lanes<W>(x);
                 if (stop) {
                     first = off + (u64)(__builtin_ctzll(stop) >> 3);
                     if (first > m) first = m;
@@ -336,7 +350,14 @@ __global__ __launch_bounds__(256) void k_lcp(const u32 *__restrict__ PLCP, const
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bwt, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
+    hipLaunchKernelGGL(k_bwt<uint8_t>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_bwt16(const uint16_t *T, const u32 *SA, const u32 *ISA, u64 n, uint16_t *U, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bwt<uint16_t>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -403,6 +424,14 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
                 u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes)
 {
+    if (sym_bytes == 2) {
+        if (marks || !PHI) {
+            sagpu_set_error("16-bit PLCP takes the Phi form only", __FILE__, __LINE__);
+            return SAGPU_EINVAL;
+        }
+        if (V64) return launch_plcp_t<u64, 2>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, nullptr, s);
+        return launch_plcp_t<u32, 2>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, nullptr, s);
+    }
     if (sym_bytes == 4) {
         if (gsa || marks || !PHI) {
             sagpu_set_error("integer PLCP takes the Phi form only", __FILE__, __LINE__);

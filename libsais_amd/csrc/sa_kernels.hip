@@ -203,7 +203,7 @@ __global__ void k_fill_u64(u64 *p, u64 count, u64 v)
 
 // Integer alphabet: packed keys of `k` symbols of `bits` bits each read from a u32
 // symbol array (values < 2^bits).  Past the end reads 0, as for bytes.
-__global__ __launch_bounds__(PK_THREADS) void k_pack_int(const u32 *__restrict__ S, u64 n, int bits, int k,
+__global__ __launch_bounds__(PK_THREADS) void k_pack_int(const u32 *__restrict__ S, u64 n, int bits, int k, int gsa,
                                                          u32 *__restrict__ keys, u32 *__restrict__ vals)
 {
     __shared__ u32 s_c[PK_TILE + PK_HALO];
@@ -218,7 +218,15 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_int(const u32 *__restrict__
         u64 p = base + li;
         if (p >= n) break;
         u32 key = 0;
-        for (int t = 0; t < k; t++) key = (bits >= 32 ? 0u : key << bits) | s_c[li + t];
+        bool sep = false;
+        for (int t = 0; t < k; t++) {
+            u32 c = s_c[li + t];
+            if (gsa) {  // code 0 is the separator: everything after the first one reads 0
+                if (sep) c = 0;
+                sep = sep || c == 0;
+            }
+            key = (bits >= 32 ? 0u : key << bits) | c;
+        }
         keys[p] = key;
         vals[p] = (u32)p;
     }
@@ -273,6 +281,41 @@ __global__ __launch_bounds__(256) void k_dense_scatter(const K *__restrict__ key
     }
 }
 
+// 65536-bin histogram of 16-bit symbols.  Lanes holding the same symbol are
+// grouped with a ballot peer mask, so each distinct symbol of a wave costs one global
+// atomic.  In GSA mode it also counts the invalid positions (a 0 first, or two
+// adjacent 0s).
+__global__ __launch_bounds__(256) void k_hist16(const uint16_t *__restrict__ T, u64 n, int gsa, u64 *__restrict__ hist,
+                                                u32 *__restrict__ bad)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 lane = lane_id();
+    u32 nbad = 0;
+    for (u64 base = (u64)blockIdx.x * blockDim.x + threadIdx.x - lane; base < n; base += stride) {
+        const u64 i = base + lane;
+        const bool ok = i < n;
+        const u32 v = ok ? T[i] : 0u;
+        if (gsa && ok && v == 0 && (i == 0 || T[i - 1] == 0)) nbad++;
+        u64 m = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            const bool bit = (v >> b) & 1u;
+            const u64 bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        if (ok && __popcll(m & lanemask_lt()) == 0) atomicAdd((unsigned long long *)&hist[v], (unsigned long long)__popcll(m));
+    }
+    if (gsa && nbad) atomicAdd(bad, nbad);
+}
+
+// Dense codes of 16-bit symbols through a 65536-entry table.
+__global__ __launch_bounds__(256) void k_map16(const uint16_t *__restrict__ T, u64 n, const u32 *__restrict__ lut,
+                                               u32 *__restrict__ S)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) S[i] = lut[T[i]];
+}
+
 unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 {
     u64 b = (work + per_block - 1) / per_block;
@@ -304,13 +347,29 @@ int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, in
     return SAGPU_OK;
 }
 
-int launch_pack_int(const u32 *S, u64 n, int bits, int k, u32 *keys, u32 *vals, hipStream_t s)
+int launch_hist16(const uint16_t *T, u64 n, int gsa, u64 *hist65536, u32 *bad, hipStream_t s)
+{
+    // the loop steps a whole wave at a time, so the grid is a multiple of 64 threads
+    hipLaunchKernelGGL(k_hist16, dim3(grid_for(n, 256 * 16, 4096)), dim3(256), 0, s, T, n, gsa, hist65536, bad);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_map16(const uint16_t *T, u64 n, const u32 *lut, u32 *S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_map16, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, lut, S);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_pack_int(const u32 *S, u64 n, int bits, int k, int gsa, u32 *keys, u32 *vals, hipStream_t s)
 {
     if (k > PK_HALO || k * bits > 32 || k < 1) {
         sagpu_set_error("pack_int: bad symbol count", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
-    hipLaunchKernelGGL(k_pack_int, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, S, n, bits, k, keys, vals);
+    hipLaunchKernelGGL(k_pack_int, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, S, n, bits, k, gsa, keys,
+                       vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -120,9 +120,10 @@ __global__ __launch_bounds__(UB_THREADS) void k_ub_succ(const uint8_t *__restric
 
 // Aux decoding: one thread per chunk [c*r, min((c+1)*r, n)), walking back from the
 // row of suffix (c+1)*r (row 0 for the last chunk).
-__global__ __launch_bounds__(256) void k_ub_walk_aux(const uint8_t *__restrict__ U, const u32 *__restrict__ S, u64 n,
+template <typename ST>
+__global__ __launch_bounds__(256) void k_ub_walk_aux(const ST *__restrict__ U, const u32 *__restrict__ S, u64 n,
                                                      u64 p, u64 r, const u64 *__restrict__ rows, u64 chunks,
-                                                     uint8_t *__restrict__ out, u32 *__restrict__ err)
+                                                     ST *__restrict__ out, u32 *__restrict__ err)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 c = (u64)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += stride) {
@@ -199,10 +200,11 @@ __global__ __launch_bounds__(256) void k_ub_jump(const u32 *__restrict__ nxt, co
 // Samples on the path from position 0 (their pointer reached the end) write their
 // gap: step t of the path emits out[n-1-t], and sample id starts at step
 // dist[0] - dist[id].
-__global__ __launch_bounds__(256) void k_ub_lr_emit(const uint8_t *__restrict__ U, const u32 *__restrict__ S,
+template <typename ST>
+__global__ __launch_bounds__(256) void k_ub_lr_emit(const ST *__restrict__ U, const u32 *__restrict__ S,
                                                     const u32 *__restrict__ samp_q, u32 m, const u32 *__restrict__ nxt,
                                                     const u64 *__restrict__ dist, const u64 *__restrict__ gap, u64 n,
-                                                    uint8_t *__restrict__ out)
+                                                    ST *__restrict__ out)
 {
     const u64 total = dist[0];
     for (u32 id = blockIdx.x * blockDim.x + threadIdx.x; id < m; id += gridDim.x * blockDim.x) {
@@ -216,6 +218,27 @@ __global__ __launch_bounds__(256) void k_ub_lr_emit(const uint8_t *__restrict__ 
             out[n - 1 - t] = U[q];
             if (j + 1 < len) q = S[q];
         }
+    }
+}
+
+// Successor array from a stable sort of (symbol, position): the position sorted to
+// slot k has LF row k + 1 (16-bit symbols, where per-tile counting tables do not fit).
+__global__ __launch_bounds__(256) void k_ub_succ_sorted(const u32 *__restrict__ V, u64 n, u64 p, u32 *__restrict__ S)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const u64 x = k + 1;
+        S[V[k]] = x == p ? UB_END : (u32)(x - (x > p));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ub_keys16(const uint16_t *__restrict__ U, u64 n, u32 *__restrict__ keys,
+                                                   u32 *__restrict__ vals)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        keys[q] = U[q];
+        vals[q] = (u32)q;
     }
 }
 
@@ -261,11 +284,15 @@ int launch_unbwt_succ(const uint8_t *U, u64 n, u64 p, u32 *counts, ScanScratch &
     return SAGPU_OK;
 }
 
-int launch_unbwt_aux_walk(const uint8_t *U, const u32 *S, u64 n, u64 p, u64 r, const u64 *rows, u64 chunks,
-                          uint8_t *out, u32 *err, hipStream_t s)
+int launch_unbwt_aux_walk(const void *U, int sym_bytes, const u32 *S, u64 n, u64 p, u64 r, const u64 *rows,
+                          u64 chunks, void *out, u32 *err, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ub_walk_aux, dim3(grid_of(chunks, 256, 65535)), dim3(256), 0, s, U, S, n, p, r, rows, chunks,
-                       out, err);
+    if (sym_bytes == 2)
+        hipLaunchKernelGGL(k_ub_walk_aux<uint16_t>, dim3(grid_of(chunks, 256, 65535)), dim3(256), 0, s,
+                           (const uint16_t *)U, S, n, p, r, rows, chunks, (uint16_t *)out, err);
+    else
+        hipLaunchKernelGGL(k_ub_walk_aux<uint8_t>, dim3(grid_of(chunks, 256, 65535)), dim3(256), 0, s,
+                           (const uint8_t *)U, S, n, p, r, rows, chunks, (uint8_t *)out, err);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -284,8 +311,24 @@ int launch_unbwt_sample_total(const u32 *sid, u64 n, u32 *out, hipStream_t s)
     return SAGPU_OK;
 }
 
-int launch_unbwt_rank(const uint8_t *U, const u32 *S, const u32 *sid, u64 n, u32 m, u32 *samp_q, u32 *nxt[2],
-                      u64 *dist[2], u64 *gap, uint8_t *out, u32 *err, hipStream_t s)
+int launch_unbwt_succ16(const uint16_t *U, u64 n, u64 p, u32 *keys, u32 *vals, u32 *S, hipStream_t s)
+{
+    (void)p;
+    (void)S;
+    hipLaunchKernelGGL(k_ub_keys16, dim3(grid_of(n, 256 * 8, 65535)), dim3(256), 0, s, U, n, keys, vals);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_unbwt_succ_sorted(const u32 *V, u64 n, u64 p, u32 *S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ub_succ_sorted, dim3(grid_of(n, 256 * 8, 65535)), dim3(256), 0, s, V, n, p, S);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_unbwt_rank(const void *U, int sym_bytes, const u32 *S, const u32 *sid, u64 n, u32 m, u32 *samp_q,
+                      u32 *nxt[2], u64 *dist[2], u64 *gap, void *out, u32 *err, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ub_collect, dim3(grid_of(n, 256 * 8, 65535)), dim3(256), 0, s, n, sid, samp_q);
     const unsigned g = grid_of(m, 256, 65535);
@@ -298,7 +341,12 @@ int launch_unbwt_rank(const uint8_t *U, const u32 *S, const u32 *sid, u64 n, u32
         hipLaunchKernelGGL(k_ub_jump, dim3(g), dim3(256), 0, s, nxt[cur], dist[cur], m, nxt[cur ^ 1], dist[cur ^ 1]);
         cur ^= 1;
     }
-    hipLaunchKernelGGL(k_ub_lr_emit, dim3(g), dim3(256), 0, s, U, S, samp_q, m, nxt[cur], dist[cur], gap, n, out);
+    if (sym_bytes == 2)
+        hipLaunchKernelGGL(k_ub_lr_emit<uint16_t>, dim3(g), dim3(256), 0, s, (const uint16_t *)U, S, samp_q, m,
+                           nxt[cur], dist[cur], gap, n, (uint16_t *)out);
+    else
+        hipLaunchKernelGGL(k_ub_lr_emit<uint8_t>, dim3(g), dim3(256), 0, s, (const uint8_t *)U, S, samp_q, m,
+                           nxt[cur], dist[cur], gap, n, (uint8_t *)out);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -218,6 +218,104 @@ void oracle_plcp_int(const int64_t *T, const int64_t *SA, int64_t *PLCP, int64_t
     }
 }
 
+/* 16-bit alphabet (libsais16.c): the same algorithms over uint16_t symbols.  GSA
+ * validity uses the same rule as bytes (libsais16.c keeps the 8-bit checks with a
+ * 16-bit zero separator). */
+int oracle_gsa_valid16(const uint16_t *T, int64_t n)
+{
+    if (n > 0 && T[n - 1] != 0) return 0;
+    if (n <= 1) return 1;
+    if (T[0] == 0) return 0;
+    for (I i = 0; i + 1 < n; i++)
+        if (T[i] == 0 && T[i + 1] == 0) return 0;
+    return 1;
+}
+
+int oracle_sa16(const uint16_t *T, int64_t *SA, int64_t n, int gsa)
+{
+    if (n <= 0) return 0;
+    if (gsa && !oracle_gsa_valid16(T, n)) return -1;
+    if (n == 1) { SA[0] = 0; return 0; }
+    I *s = (I *)malloc((size_t)n * sizeof(I));
+    if (!s) return -2;
+    I K = 65536;
+    if (gsa) {
+        I z = 0;
+        for (I i = 0; i < n; i++)
+            if (T[i] == 0) s[i] = z++;
+        for (I i = 0; i < n; i++)
+            if (T[i] != 0) s[i] = z + T[i];
+        K = z + 65536;
+    } else {
+        for (I i = 0; i < n; i++) s[i] = T[i];
+    }
+    int rc = sais(s, SA, n, K);
+    free(s);
+    return rc;
+}
+
+int64_t oracle_bwt16(const uint16_t *T, const int64_t *SA, uint16_t *U, int64_t n)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { U[0] = T[0]; return 1; }
+    I j0 = -1;
+    for (I j = 0; j < n; j++)
+        if (SA[j] == 0) { j0 = j; break; }
+    U[0] = T[n - 1];
+    for (I j = 0; j < n; j++) {
+        if (j == j0) continue;
+        U[j + (j < j0)] = T[SA[j] - 1];
+    }
+    return j0 + 1;
+}
+
+void oracle_plcp16(const uint16_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int gsa)
+{
+    if (n <= 0) return;
+    if (n == 1) { PLCP[0] = 0; return; }
+    I k = n;
+    for (I i = 0; i < n; i++) { PLCP[SA[i]] = k; k = SA[i]; }
+    I l = 0;
+    for (I i = 0; i < n; i++) {
+        k = PLCP[i];
+        I m = n - (i > k ? i : k);
+        if (!gsa) {
+            while (l < m && T[i + l] == T[k + l]) l++;
+        } else {
+            while (l < m && T[i + l] == T[k + l] && T[i + l] != 0) l++;
+        }
+        PLCP[i] = l;
+        l -= (l != 0);
+    }
+}
+
+int oracle_unbwt16(const uint16_t *U, uint16_t *out, int64_t n, int64_t p)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { out[0] = U[0]; return 0; }
+    I *cnt = (I *)calloc(65536, sizeof(I));
+    I *C = (I *)malloc(65536 * sizeof(I));
+    I *seen = (I *)calloc(65536, sizeof(I));
+    I *lf = (I *)malloc((size_t)(n + 1) * sizeof(I));
+    if (!cnt || !C || !seen || !lf) { free(cnt); free(C); free(seen); free(lf); return -2; }
+    for (I q = 0; q < n; q++) cnt[U[q]]++;
+    I acc = 1;
+    for (I c = 0; c < 65536; c++) { C[c] = acc; acc += cnt[c]; }
+    for (I x = 0; x <= n; x++) {
+        if (x == p) { lf[x] = 0; continue; }
+        uint16_t c = U[x - (x > p)];
+        lf[x] = C[c] + seen[c]++;
+    }
+    I x = 0;
+    for (I t = n - 1; t >= 0; t--) {
+        out[t] = U[x - (x > p)];
+        x = lf[x];
+        if (x == p) break;
+    }
+    free(cnt); free(C); free(seen); free(lf);
+    return 0;
+}
+
 /* ------------------------------------------------------- SA verification ---- */
 
 int oracle_check_sa(const uint8_t *T, const int64_t *SA, int64_t n, int gsa)

@@ -19,6 +19,8 @@
  *   oracle_sa_int      brute force over small integer strings, the compiled
  *                      reference libsais_int / libsais64_long.
  *   oracle_plcp_int    the compiled reference libsais_plcp_int, direct comparison.
+ *   oracle_*16         the compiled reference libsais16_* on random 16-bit strings
+ *                      (small and large alphabets, GSA sets), brute force on tiny ones.
  *   oracle_unbwt       round trip T -> oracle_bwt -> oracle_unbwt, the compiled
  *                      reference libsais_unbwt / libsais_unbwt_aux, banana by hand.
  */
@@ -52,6 +54,14 @@ int     oracle_sa_int(const int64_t *T, int64_t *SA, int64_t n);
 
 /* PLCP of an integer string (libsais.c:8399-8415, 8263-8286). */
 void    oracle_plcp_int(const int64_t *T, const int64_t *SA, int64_t *PLCP, int64_t n);
+
+/* 16-bit alphabet (libsais16.c): SA / GSA, BWT, PLCP / PLCP-GSA and inverse BWT over
+ * uint16_t symbols, with the same conventions as the byte functions. */
+int     oracle_gsa_valid16(const uint16_t *T, int64_t n);
+int     oracle_sa16(const uint16_t *T, int64_t *SA, int64_t n, int gsa);
+int64_t oracle_bwt16(const uint16_t *T, const int64_t *SA, uint16_t *U, int64_t n);
+void    oracle_plcp16(const uint16_t *T, const int64_t *SA, int64_t *PLCP, int64_t n, int gsa);
+int     oracle_unbwt16(const uint16_t *U, uint16_t *out, int64_t n, int64_t p);
 
 /* BWT with auxiliary indexes (libsais.c:7123-7146 with the sampling of
  * 4799-4819): U as oracle_bwt, and I[k] = 1 + row of suffix k*r for

@@ -88,6 +88,18 @@ def ref():
         r.libsais_plcp_int.restype = _i32
         r.libsais64_long.argtypes = [_vp, _vp, _i64, _i64, _i64]
         r.libsais64_long.restype = _i64
+        for name in ("libsais16", "libsais16_gsa"):
+            getattr(r, name).argtypes = [_vp, _vp, _i32, _i32, _vp]
+            getattr(r, name).restype = _i32
+        r.libsais16_bwt.argtypes = [_vp, _vp, _vp, _i32, _i32, _vp]
+        r.libsais16_bwt.restype = _i32
+        r.libsais16_bwt_aux.argtypes = [_vp, _vp, _vp, _i32, _i32, _vp, _i32, _vp]
+        r.libsais16_bwt_aux.restype = _i32
+        r.libsais16_unbwt.argtypes = [_vp, _vp, _vp, _i32, _vp, _i32]
+        r.libsais16_unbwt.restype = _i32
+        for name in ("libsais16_plcp", "libsais16_plcp_gsa"):
+            getattr(r, name).argtypes = [_vp, _vp, _vp, _i32]
+            getattr(r, name).restype = _i32
         r.libsais64_omp.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64]
         r.libsais64_omp.restype = _i64
         for name in ("libsais64_plcp_omp", "libsais64_lcp_omp"):
@@ -338,3 +350,103 @@ def ref_plcp_int(T, SA):
     P = np.zeros(max(n, 1), np.int32)
     rc = ref().libsais_plcp_int(_p(T32), _p(S32), _p(P), n)
     return rc, P[:n]
+
+
+# ------------------------------------------------------------ 16-bit alphabet
+def _o16():
+    o = oracle()
+    o.oracle_sa16.argtypes = [_vp, _vp, _i64, ctypes.c_int]
+    o.oracle_sa16.restype = ctypes.c_int
+    o.oracle_bwt16.argtypes = [_vp, _vp, _vp, _i64]
+    o.oracle_bwt16.restype = _i64
+    o.oracle_plcp16.argtypes = [_vp, _vp, _vp, _i64, ctypes.c_int]
+    o.oracle_plcp16.restype = None
+    o.oracle_unbwt16.argtypes = [_vp, _vp, _i64, _i64]
+    o.oracle_unbwt16.restype = ctypes.c_int
+    return o
+
+
+def sa16(T, gsa=False):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    SA = np.zeros(max(n, 1), np.int64)
+    rc = _o16().oracle_sa16(_p(T), _p(SA), n, int(gsa))
+    return rc, SA[:n]
+
+
+def bwt16(T, SA):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint16)
+    idx = _o16().oracle_bwt16(_p(T), _p(np.ascontiguousarray(SA, np.int64)), _p(U), n)
+    return U[:n], idx
+
+
+def plcp16(T, SA, gsa=False):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    P = np.zeros(max(n, 1), np.int64)
+    _o16().oracle_plcp16(_p(T), _p(np.ascontiguousarray(SA, np.int64)), _p(P), n, int(gsa))
+    return P[:n]
+
+
+def unbwt16(U, p):
+    U = np.ascontiguousarray(U, np.uint16)
+    n = len(U)
+    out = np.zeros(max(n, 1), np.uint16)
+    assert _o16().oracle_unbwt16(_p(U), _p(out), n, p) == 0
+    return out[:n]
+
+
+def ref_sa16(T, gsa=False):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    SA = np.zeros(max(n, 1), np.int32)
+    f = np.zeros(65536, np.int32)
+    rc = (ref().libsais16_gsa if gsa else ref().libsais16)(_p(T), _p(SA), n, 0, _p(f))
+    return rc, SA[:n], f
+
+
+def ref_bwt16(T):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    U = np.zeros(max(n, 1), np.uint16)
+    A = np.zeros(max(n, 1), np.int32)
+    idx = ref().libsais16_bwt(_p(T), _p(U), _p(A), n, 0, None)
+    return U[:n], idx
+
+
+def ref_plcp16(T, SA, gsa=False):
+    T = np.ascontiguousarray(T, np.uint16)
+    n = len(T)
+    P = np.zeros(max(n, 1), np.int32)
+    (ref().libsais16_plcp_gsa if gsa else ref().libsais16_plcp)(_p(T), _p(np.ascontiguousarray(SA, np.int32)),
+                                                                _p(P), n)
+    return P[:n]
+
+
+def ref_unbwt16(U, p):
+    U = np.ascontiguousarray(U, np.uint16)
+    n = len(U)
+    out = np.zeros(max(n, 1), np.uint16)
+    A = np.zeros(n + 1, np.int32)
+    rc = ref().libsais16_unbwt(_p(U), _p(out), _p(A), n, None, p)
+    return rc, out[:n]
+
+
+def aux_samples(SA, r):
+    """I[t] = 1 + row of suffix t*r: the auxiliary samples of the bwt_aux forms, which
+    depend on the suffix array only (any symbol width)."""
+    isa = np.empty(len(SA), np.int64)
+    isa[np.asarray(SA, np.int64)] = np.arange(len(SA))
+    return isa[0::r] + 1
+
+
+def gsa_set16(rng, count, maxlen, k):
+    """A valid 16-bit GSA input: `count` non-empty strings over 1..k-1, each followed by 0."""
+    parts = []
+    for _ in range(count):
+        m = int(rng.integers(1, maxlen + 1))
+        parts.append(rng.integers(1, k, m).astype(np.uint16))
+        parts.append(np.zeros(1, np.uint16))
+    return np.concatenate(parts)

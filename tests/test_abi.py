@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     names = set()
-    for h in ("libsais.h", "libsais64.h", "sagpu.h"):
+    for h in ("libsais.h", "libsais64.h", "libsais16.h", "libsais16x64.h", "sagpu.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         names |= set(re.findall(r"\b((?:libsais|sagpu)\w*)\s*\(", src))
     return names
@@ -42,9 +42,9 @@ def test_exports_every_declared_symbol():
 def test_no_out_of_scope_exports():
     out = subprocess.run(["nm", "-D", "--defined-only", la.LIB_PATH], capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    # the 16-bit-alphabet libraries (libsais16.h, libsais16x64.h) are not provided
-    assert not any(name.startswith("libsais16") for name in exported)
     assert all(name.startswith(("libsais", "sagpu")) for name in exported)
+    # nothing exported beyond what the headers declare
+    assert exported <= header_functions()
 
 
 def test_python_binding_table_matches_headers():
@@ -258,3 +258,98 @@ def test_context_and_omp_forms_check_like_the_reference():
     S64 = np.zeros(5, np.int64)
     assert la.libsais64_long(np.zeros(1, np.int64), S64, 1, 5, 0) == 0 and S64[0] == 0
     assert la.libsais64_omp(T, S64, 5, 0, None, -1) == -1
+
+
+# ------------------------------------------------------------ 16-bit alphabet
+def test_16bit_argument_checks():
+    T = np.array([5, 9, 5, 9, 5], np.uint16)
+    SA = np.zeros(5, np.int32)
+    U = np.zeros(5, np.uint16)
+    A = np.zeros(6, np.int32)
+    I = np.array([1, 2, 3], np.int32)
+    assert la.libsais16(None, SA, 5, 0, None) == -1
+    assert la.libsais16(T, SA, -1, 0, None) == -1
+    assert la.libsais16(T, SA, 5, -1, None) == -1
+    assert la.libsais16_gsa(T, SA, 5, 0, None) == -1            # T[n-1] != 0
+    assert la.libsais16_bwt(T, U, None, 5, 0, None) == -1
+    assert la.libsais16_bwt_aux(T, U, A, 5, 0, None, 3, I) == -1  # r not a power of two
+    assert la.libsais16_unbwt_aux(T, U, A, 5, None, 2, np.array([0, 2, 3], np.int32)) == -1
+    assert la.libsais16_unbwt(T, U, A, 5, None, 6) == -1
+    assert la.libsais16_plcp_gsa(T, SA, SA, 5) == -1
+    assert la.libsais16_lcp(None, SA, SA, 5) == -1
+    assert la.libsais16_ctx(None, T, SA, 5, 0, None) == -1
+    assert la.libsais16_omp(T, SA, 5, 0, None, -1) == -1
+    assert la.libsais16_create_ctx_omp(-1) is None
+    c = la.libsais16_create_ctx()
+    assert c
+    la.libsais16_free_ctx(c)
+    S64 = np.zeros(5, np.int64)
+    assert la.libsais16x64(T, S64, 5, -1, None) == -1
+    assert la.libsais16x64_gsa(T, S64, 5, 0, None) == -1
+    assert la.libsais16x64_bwt_aux(T, U, S64, 5, 0, None, 6, S64) == -1
+    assert la.libsais16x64_lcp_omp(S64, S64, S64, 5, -1) == -1
+
+
+def test_16bit_small_n_cases():
+    T = np.array([40000, 0], np.uint16)
+    for n in (0, 1):
+        SA = np.full(2, 99, np.int32)
+        freq = np.full(65536, 5, np.int32)
+        assert la.libsais16(T, SA, n, 0, freq) == 0
+        expect = np.zeros(65536, np.int32)
+        expect[40000] = n
+        assert np.array_equal(freq, expect)
+        U = np.full(2, 9, np.uint16)
+        A = np.zeros(2, np.int32)
+        assert la.libsais16_bwt(T, U, A, n, 0, None) == n
+        if n == 1:
+            assert SA[0] == 0 and U[0] == 40000
+        I = np.array([7], np.int32)
+        assert la.libsais16_bwt_aux(T, U, A, n, 0, None, 4, I) == 0 and I[0] == n
+        f64 = np.full(65536, 3, np.int64)
+        S64 = np.full(2, 9, np.int64)
+        assert la.libsais16x64(T, S64, n, 0, f64) == 0 and f64.sum() == n
+
+
+@pytest.mark.skipif(not ol.have_ref(), reason="reference build oracle/_ref absent")
+def test_16bit_edge_cases_match_reference():
+    r = ol.ref()
+    table = {name: (res, args) for name, res, args in la.ABI_16 + la.ABI_16X64}
+
+    def ref_fn(name):
+        f = getattr(r, name)
+        f.restype, f.argtypes = table[name]
+        return lambda *a: f(*[x.ctypes.data if isinstance(x, np.ndarray) else x for x in a])
+
+    def calls(get, it, T, SA, U, A, P, L, I, n, fs, rr, i0, freq):
+        return [
+            get(f"libsais16{it}")(T, SA, n, fs, freq),
+            get(f"libsais16{it}_gsa")(T, SA, n, fs, freq),
+            get(f"libsais16{it}_bwt")(T, U, A, n, fs, freq),
+            get(f"libsais16{it}_bwt_aux")(T, U, A, n, fs, freq, rr, I),
+            get(f"libsais16{it}_unbwt")(T, U, A, n, None, i0),
+            get(f"libsais16{it}_unbwt_aux")(T, U, A, n, None, rr, I),
+            get(f"libsais16{it}_plcp")(T, SA, P, n),
+            get(f"libsais16{it}_plcp_gsa")(T, SA, P, n),
+            get(f"libsais16{it}_lcp")(P, SA, L, n),
+        ]
+
+    for it, dt in (("", np.int32), ("x64", np.int64)):
+        for n in (-1, 0, 1):
+            for fs in (-1, 0):
+                for tv in ((5, 0), (5, 6)):
+                    for rr, i0 in ((2, 1), (3, 0), (4, 2)):
+                        outs = []
+                        for get in (lambda name: getattr(la, name), ref_fn):
+                            T = np.array(tv, np.uint16)
+                            SA = np.array([1, 0], dt)
+                            U = np.full(2, 77, np.uint16)
+                            A = np.zeros(3, dt)
+                            P = np.array([3, 4], dt)
+                            L = np.full(2, 77, dt)
+                            I = np.array([i0, 1], dt)
+                            freq = np.full(65536, 9, dt)
+                            codes = calls(get, it, T, SA, U, A, P, L, I, n, fs, rr, i0, freq)
+                            outs.append((codes, SA.tolist(), U.tolist(), P.tolist(), L.tolist(), I.tolist(),
+                                         freq[:8].tolist(), int(freq.sum())))
+                        assert outs[0] == outs[1], (it, n, fs, tv, rr, i0)

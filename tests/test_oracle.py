@@ -250,6 +250,16 @@ def test_bwt_aux_and_unbwt_by_hand():
     assert bytes(ol.unbwt(U, 4)) == b"banana"
 
 
+def test_aux_samples_match_oracle_bwt_aux():
+    """The width-independent sample formula used by the 16-bit tests agrees with the
+    byte oracle's bwt_aux (itself pinned against the reference below)."""
+    for n in (1, 2, 7, 5000):
+        T = np.random.default_rng(n).integers(0, 4, n).astype(np.uint8)
+        rc, SA = ol.sa(T)
+        for r in (2, 8, 1 << 12):
+            assert np.array_equal(ol.bwt_aux(T, SA, r)[1], ol.aux_samples(SA, r))
+
+
 def test_unbwt_round_trip_shapes():
     rng = np.random.default_rng(21)
     shapes = [workloads.iid(5000, seed=3, alphabet=b"ACGT"), np.full(777, 65, np.uint8),
@@ -312,3 +322,37 @@ def test_int_oracle_against_reference(k):
         assert rc == 0 and np.array_equal(ol.plcp_int(T, SA), P.astype(np.int64))
         rc, SA64, _ = ol.ref_sa_long(T.astype(np.int64), k)
         assert rc == 0 and np.array_equal(SA64, SA.astype(np.int64))
+
+
+# ------------------------------------------------------------ 16-bit alphabet
+def test_16bit_oracle_brute_force():
+    rng = np.random.default_rng(61)
+    for n in range(1, 30):
+        T = (rng.integers(0, 3, n) * 30000 + 1).astype(np.uint16)
+        rc, SA = ol.sa16(T)
+        assert rc == 0 and list(SA) == sorted(range(n), key=lambda i: list(T[i:]))
+        U, p = ol.bwt16(T, SA)
+        assert np.array_equal(ol.unbwt16(U, p), T)
+
+
+@needs_ref
+@pytest.mark.parametrize("k", [2, 4, 1000, 65536])
+def test_16bit_oracle_against_reference(k):
+    rng = np.random.default_rng(k + 5)
+    for n in (2, 7, 5000, 40000):
+        T = rng.integers(0, k, n).astype(np.uint16)
+        rc, SA_r, f = ol.ref_sa16(T)
+        rc2, SA = ol.sa16(T)
+        assert rc == 0 and rc2 == 0 and np.array_equal(SA, SA_r.astype(np.int64))
+        assert np.array_equal(f, np.bincount(T, minlength=65536).astype(np.int32))
+        U_r, p_r = ol.ref_bwt16(T)
+        U, p = ol.bwt16(T, SA)
+        assert p == p_r and np.array_equal(U, U_r)
+        assert np.array_equal(ol.plcp16(T, SA), ol.ref_plcp16(T, SA_r).astype(np.int64))
+        rc3, out = ol.ref_unbwt16(U_r, p_r)
+        assert rc3 == 0 and np.array_equal(out, T) and np.array_equal(ol.unbwt16(U, p), T)
+    G = ol.gsa_set16(rng, 50, 300, k if k > 2 else 3)
+    rc, SA_r, _ = ol.ref_sa16(G, gsa=True)
+    rc2, SA = ol.sa16(G, gsa=True)
+    assert rc == 0 and rc2 == 0 and np.array_equal(SA, SA_r.astype(np.int64))
+    assert np.array_equal(ol.plcp16(G, SA, gsa=True), ol.ref_plcp16(G, SA_r, gsa=True).astype(np.int64))
