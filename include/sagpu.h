@@ -169,7 +169,10 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    tiles; default 32)
  *   "tile_range"     list entries per workgroup of the doubling-round tile kernel: 256, 512
  *                    (default) or 1024
- *   "bwt_variant"    0 = gather T[SA[j]-1] in SA order (default), 1 = scatter in text order
+ *   "bwt_variant"    0 = seeded (default): the initial sort carries each suffix's BWT symbol
+ *                    beside its index where both fit in 32 bits, and the text is read only for
+ *                    the rows the rounds place; otherwise as 2.  1 = scatter in text order,
+ *                    2 = gather T[SA[j]-1] in SA order
  *   "isa_window_log2" ISA scatter window after the initial sort, 2^v entries (20..32, default 26)
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)

@@ -58,9 +58,20 @@ struct WsScratch {
     u64 *base;        // [1] reserved output base
 };
 
+// BWT seeding: the initial sort's values carry (suffix << cb) | code of the symbol
+// before the suffix (launch_pack_keys32 with cb > 0).  group_init then stores that
+// code per SA slot (codes, n bytes) and a bitmap of the slots still in groups
+// (abits, n/64 words); launch_bwt_seeded finishes the BWT after the rounds,
+// reading the text only at those slots.  inv: code -> byte (256 bytes, device).
+struct BwtSeed {
+    int cb;
+    uint8_t *codes;
+    u64 *abits;
+    const uint8_t *inv;
+};
 // G: scratch of n entries (group head per sorted position, SA order).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed = nullptr);
 // ISA scatter window after the initial sort: 2^v entries per pass (v in [20, 32]).
 void set_isa_window_log2(int v);
 // List entries per workgroup of the tile round kernel: 256, 512 (default) or 1024.
@@ -81,8 +92,9 @@ int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, 
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);
-int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
-                       hipStream_t s);
+// vals[p] = p, or (p << cb) | code of T[p-1] with cb > 0 (n - 1 < 2^(32 - cb)).
+int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, int cb, u32 *keys,
+                       u32 *vals, hipStream_t s);
 // Batched GSA: sid[p] = number of 0 bytes before p (string id); keys get sid in
 // their high half.  launch_string_ids needs a ScanScratch for n entries.
 int launch_string_ids(const uint8_t *T, u64 n, u32 *sid, ScanScratch &sc, hipStream_t s);
@@ -112,6 +124,8 @@ struct LcpScratch {
 };
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
+int launch_bwt_seeded(const uint8_t *T, const u32 *SA, const u32 *ISA, const BwtSeed &seed, u64 n, uint8_t *U,
+                      hipStream_t s);
 int launch_bwt16(const uint16_t *T, const u32 *SA, const u32 *ISA, u64 n, uint16_t *U, hipStream_t s);
 // The same BWT in text order: U[row of suffix i] = T[i-1], reading ISA and T
 // sequentially and scattering bytes.

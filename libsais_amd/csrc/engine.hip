@@ -121,7 +121,9 @@ int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
 // Texts longer than this use 64-bit rank keys and PLCP scan values.
 u64 g_wide_from = 0x7FFFFFFFull;
-int g_bwt_variant = 0;  // 0: gather T[SA[j]-1] in SA order, 1: scatter in text order
+// 0: seeded from the initial sort where the symbol code fits beside the suffix
+// index, else gather T[SA[j]-1] in SA order; 1: scatter in text order; 2: gather
+int g_bwt_variant = 0;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -338,7 +340,7 @@ int read_u32(Ctx &c, const u32 *d, u32 *out)
 // Byte alphabet of a text: freq (optional, host), the GSA validity test, and the
 // dense symbol codes (device LUT) with their width and the symbols per 32-bit key.
 int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *freq_out, int *bits_out, int *ksym_out,
-                  uint8_t **lut_out)
+                  uint8_t **lut_out, uint8_t **inv_out = nullptr)
 {
     hipStream_t s = c.stream;
     GET(d_hist, u64, 256 + 8);
@@ -367,6 +369,15 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
     GET(d_lut, uint8_t, 256);
     SAGPU_CHECK(hipMemcpyAsync(d_lut, lut_h, 256, hipMemcpyHostToDevice, s));
     *lut_out = d_lut;
+    if (inv_out) {  // code -> byte
+        uint8_t *inv_h = lut_h + 256;
+        memset(inv_h, 0, 256);
+        for (int ch = 0; ch < 256; ch++)
+            if (freq[ch]) inv_h[lut_h[ch]] = (uint8_t)ch;
+        GET(d_inv, uint8_t, 256);
+        SAGPU_CHECK(hipMemcpyAsync(d_inv, inv_h, 256, hipMemcpyHostToDevice, s));
+        *inv_out = d_inv;
+    }
     return SAGPU_OK;
 }
 
@@ -374,20 +385,33 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
 // return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
 // optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
 // previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
+//
+// seed (optional, byte texts without GSA): when the BWT symbol's code fits beside
+// the suffix index in the sort values, the build leaves BWT seeds there for
+// launch_bwt_seeded (seed->cb > 0 on return); otherwise seed->cb stays 0.
 int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
-             const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0)
+             const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0,
+             BwtSeed *seed = nullptr)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
     int bits, ksym;
-    uint8_t *d_lut = nullptr;
+    uint8_t *d_lut = nullptr, *d_inv = nullptr;
+    const bool want_seed = seed && !dSym && !sid && !gsa;
+    if (seed) *seed = BwtSeed{};
     if (dSym) {
         // integer alphabet: symbols already below 2^sym_bits
         bits = sym_bits < 1 ? 1 : sym_bits;
         ksym = 32 / bits;
     } else {
-        SAGPU_TRY(byte_alphabet(c, sc, dT, n, gsa, freq_out, &bits, &ksym, &d_lut));
+        SAGPU_TRY(byte_alphabet(c, sc, dT, n, gsa, freq_out, &bits, &ksym, &d_lut, want_seed ? &d_inv : nullptr));
     }
+    if (want_seed && ((n - 1) >> (32 - bits)) == 0) {
+        GET(codes, uint8_t, n);
+        GET(abits, u64, (n + 63) / 64);
+        *seed = BwtSeed{bits, codes, abits, d_inv};
+    }
+    const int cb = seed ? seed->cb : 0;
     st.initial_symbols = ksym;
     st.bits_per_symbol = bits;
 
@@ -431,13 +455,13 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         {
             Span sp(c, F_INITIAL);
             if (dSym) SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, gsa, k0, v0, s));
-            else SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, k0, v0, s));
+            else SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
         }
         SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt));
         Span sp(c, F_INITIAL);
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
-                             alt ? k0 : k1, s));
+                             alt ? k0 : k1, s, cb ? seed : nullptr));
     } else {
         // batched GSA: (string id, symbols) keys, so every group stays inside one string
         u64 *k0 = (u64 *)kA, *k1 = (u64 *)kA + n;
@@ -610,11 +634,14 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
     {
         Span total(c, F_TOTAL);
         Scope work(c);  // construction buffers go back to the pool before the caller's downloads
-        rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr);
+        BwtSeed seed{};
+        rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr, nullptr, 0, nullptr, 0,
+                      dU && g_bwt_variant == 0 ? &seed : nullptr);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
-            if (g_bwt_variant == 1) SAGPU_TRY(launch_bwt_scatter(dTpad, dISA, n, dU, c.stream));
+            if (seed.cb) SAGPU_TRY(launch_bwt_seeded(dTpad, dSA, dISA, seed, n, dU, c.stream));
+            else if (g_bwt_variant == 1) SAGPU_TRY(launch_bwt_scatter(dTpad, dISA, n, dU, c.stream));
             else SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
         }
         if (want_lcp) {
@@ -1269,7 +1296,7 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         return 0;
     }
     if (!strcmp(name, "bwt_variant")) {
-        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        if (value < 0 || value > 2) return SAGPU_EINVAL;
         g_bwt_variant = (int)value;
         return 0;
     }

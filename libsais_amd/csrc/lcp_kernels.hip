@@ -35,6 +35,46 @@ __global__ __launch_bounds__(256) void k_bwt(const ST *__restrict__ T, const u32
     }
 }
 
+// BWT from the seeds left by the initial grouping: output byte p comes from SA slot
+// j = p - 1 (p <= j0) or j = p (p > j0), U[0] = T[n-1].  Slots that were alone
+// after the initial sort already hold their symbol's code (codes[j]); only the
+// slots flagged in abits read the text at SA[j] - 1.  Four output bytes per lane.
+__global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ T, const u32 *__restrict__ SA,
+                                                    const u32 *__restrict__ ISA, const uint8_t *__restrict__ codes,
+                                                    const u64 *__restrict__ abits, const uint8_t *__restrict__ inv,
+                                                    u64 n, uint8_t *__restrict__ U, int aligned)
+{
+    __shared__ uint8_t s_inv[256];
+    s_inv[threadIdx.x] = inv[threadIdx.x];
+    __syncthreads();
+    const u64 j0 = ISA[0];
+    const u64 quads = (n + 3) / 4;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
+        const u64 p0 = 4 * q;
+        u32 word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const u64 p = p0 + k;
+            if (p >= n) break;
+            uint8_t b;
+            if (p == 0) {
+                b = T[n - 1];
+            } else {
+                const u64 j = p <= j0 ? p - 1 : p;
+                const bool grouped = (abits[j >> 6] >> (j & 63)) & 1;
+                b = grouped ? T[SA[j] - 1u] : s_inv[codes[j]];
+            }
+            word |= (u32)b << (8 * k);
+        }
+        if (aligned && p0 + 4 <= n) {
+            *reinterpret_cast<u32 *>(U + p0) = word;
+        } else {
+            for (int k = 0; k < 4 && p0 + k < n; k++) U[p0 + k] = (uint8_t)(word >> (8 * k));
+        }
+    }
+}
+
 // Text order: suffix i (i >= 1) sits in row j = ISA[i], whose BWT symbol T[i-1] goes
 // to U[j + (j < j0)]; the primary row j0 = ISA[0] is dropped and U[0] = T[n-1].
 __global__ __launch_bounds__(256) void k_bwt_scatter(const uint8_t *__restrict__ T, const u32 *__restrict__ ISA,
@@ -306,33 +346,69 @@ __global__ __launch_bounds__(1024) void k_max_scan_tiles(VT *tmax, u64 tiles)
     }
 }
 
+// Elements 4*lane .. 4*lane+3 of a 256-element row: one 16-byte (u32) or two
+// 16-byte (u64) loads per lane, or element loads (0 past n) in the last tile.
+template <typename VT>
+__device__ __forceinline__ void load4(const VT *V, u64 e0, u64 n, bool full, VT (&x)[4])
+{
+    if (full) {
+        if constexpr (sizeof(VT) == 4) {
+            uint4 q = *reinterpret_cast<const uint4 *>(V + e0);
+            x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
+        } else {
+            ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(V + e0);
+            ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(V + e0 + 2);
+            x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = e0 + k < n ? V[e0 + k] : (VT)0;
+    }
+}
+
+// Each wave owns 1024 consecutive elements of the 4096-element tile, as 4 rows of
+// 256 (4 consecutive per lane), so loads and stores are 16-byte and coalesced.
 template <typename VT>
 __global__ __launch_bounds__(MS_THREADS) void k_max_apply(const VT *V, u64 n, const VT *__restrict__ tmax,
                                                           u32 *PLCP)
 {
-    __shared__ VT s_w[4];
-    // blocked layout: thread handles MS_ITEMS consecutive elements
-    const u64 base = (u64)blockIdx.x * MS_TILE + (u64)threadIdx.x * MS_ITEMS;
-    VT v[MS_ITEMS];
-    VT m = 0;
+    constexpr int ROWS = MS_TILE / (4 * 256);   // rows of 256 per wave
+    __shared__ VT s_w[MS_THREADS / SAGPU_WAVE];
+    const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * MS_TILE + (u64)w * (MS_TILE / (MS_THREADS / SAGPU_WAVE));
+    const bool full = wbase + (u64)ROWS * 256 <= n;
+    VT v[ROWS][4], inc[ROWS];
+    VT tot = 0;
 #pragma unroll
-    for (int j = 0; j < MS_ITEMS; j++) {
-        u64 e = base + j;
-        v[j] = e < n ? V[e] : (VT)0;
-        m = vmax(m, v[j]);
+    for (int r = 0; r < ROWS; r++) {
+        load4(V, wbase + (u64)r * 256 + 4 * lane, n, full, v[r]);
+        VT m = vmax(vmax(v[r][0], v[r][1]), vmax(v[r][2], v[r][3]));
+        inc[r] = wave_incl_max_t(m);
+        tot = vmax(tot, (VT)__shfl(inc[r], SAGPU_WAVE - 1, SAGPU_WAVE));
     }
-    VT inc = wave_incl_max_t(m);
-    if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
+    if (lane == 0) s_w[w] = tot;
     __syncthreads();
     VT c = tmax[blockIdx.x];
-    for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
-    VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
-    if (lane_id() > 0) c = vmax(c, prev);
+    for (u32 q = 0; q < w; q++) c = vmax(c, s_w[q]);
 #pragma unroll
-    for (int j = 0; j < MS_ITEMS; j++) {
-        u64 e = base + j;
-        c = vmax(c, v[j]);
-        if (e < n) PLCP[e] = (u32)(c - e);
+    for (int r = 0; r < ROWS; r++) {
+        const u64 e0 = wbase + (u64)r * 256 + 4 * lane;
+        VT prev = __shfl_up(inc[r], 1, SAGPU_WAVE);
+        VT cl = lane > 0 ? vmax(c, prev) : c;
+        u32 o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cl = vmax(cl, v[r][k]);
+            o[k] = (u32)(cl - (VT)(e0 + k));
+        }
+        if (full) {
+            *reinterpret_cast<uint4 *>(PLCP + e0) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (e0 + k < n) PLCP[e0 + k] = o[k];
+        }
+        c = vmax(c, (VT)__shfl(inc[r], SAGPU_WAVE - 1, SAGPU_WAVE));
     }
 }
 
@@ -351,6 +427,16 @@ __global__ __launch_bounds__(256) void k_lcp(const u32 *__restrict__ PLCP, const
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bwt<uint8_t>, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, SA, ISA, n, U);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_bwt_seeded(const uint8_t *T, const u32 *SA, const u32 *ISA, const BwtSeed &seed, u64 n, uint8_t *U,
+                      hipStream_t s)
+{
+    const int aligned = ((uintptr_t)U & 3) == 0;
+    hipLaunchKernelGGL(k_bwt_seeded, dim3(grid_for((n + 3) / 4, 256 * 4, 65535)), dim3(256), 0, s, T, SA, ISA,
+                       seed.codes, seed.abits, seed.inv, n, U, aligned);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -26,7 +26,9 @@ namespace {
 // A tile of 4096 elements is split into four 1024-element wave spans, each walked
 // as 16 rows of 64 consecutive elements, so every load and store of a row is
 // coalesced.  The scan carries one sum (output index of active elements) and two
-// max-scans (positions of the latest group starts).
+// max-scans (positions of the latest group starts).  Functors are called by every
+// lane of a row (lane l holds element row_base + l), also past the end, so they can
+// take neighbour values from the adjacent lanes.
 
 struct Agg {
     u64 s;        // active count (u64: a text of 2^32 suffixes may have them all active)
@@ -47,14 +49,12 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
     u32 m1 = 0, m2 = 0;
     for (int r = 0; r < WS_ROWS; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
-        if (e < f.n) {
-            bool act;
-            u32 v1, v2;
-            f.scan_inputs(e, act, v1, v2);
-            s += act;
-            m1 = umax(m1, v1);
-            m2 = umax(m2, v2);
-        }
+        bool act;
+        u32 v1, v2;
+        f.scan_inputs(e, act, v1, v2);   // every lane calls (rows may shuffle); e >= n yields nothing
+        s += act;
+        m1 = umax(m1, v1);
+        m2 = umax(m2, v2);
     }
     for (int d = 32; d >= 1; d >>= 1) {
         s += __shfl_xor(s, d, SAGPU_WAVE);
@@ -135,9 +135,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
-        act[r] = false;
-        v1[r] = v2[r] = 0;
-        if (e < f.n) f.scan_inputs(e, act[r], v1[r], v2[r]);
+        f.scan_inputs(e, act[r], v1[r], v2[r]);
         s += act[r];
         m1 = umax(m1, v1[r]);
         m2 = umax(m2, v2[r]);
@@ -162,7 +160,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
         u32 is = wave_incl_sum((u32)act[r]);
         u32 i1 = umax(c.m1, wave_incl_max(v1[r]));
         u32 i2 = umax(c.m2, wave_incl_max(v2[r]));
-        if (e < f.n) f.emit(e, act[r], ob + c.s + is - (u64)act[r], i1, i2);
+        f.emit(e, act[r], ob + c.s + is - (u64)act[r], i1, i2);   // every lane calls; e >= n writes nothing
         c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m1 = __shfl(i1, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m2 = __shfl(i2, SAGPU_WAVE - 1, SAGPU_WAVE);
@@ -181,16 +179,33 @@ struct InitF {
     u32 sepmask;     // GSA: low symbol slot; keys with it zero contain a separator
     int bits, ksym, gsa;
     u32 *SA, *G, *act_i, *act_g, *LCP;   // G[e] = group head (ISA is scattered later)
+    // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
+    // it.  codes[e] receives that code and bit e of abits is set when slot e is
+    // still in a group (its occupant, and so its BWT symbol, is decided later).
+    int cb;
+    uint8_t *codes;
+    u64 *abits;
 
     __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
+    // Key of element e and of its left neighbour: one load per element, the
+    // neighbour from the adjacent lane (lane 0 loads its own).
+    __device__ void key_pair(u64 e, K &k, K &kp) const
+    {
+        k = e < n ? keys[e] : K(0);
+        kp = __shfl_up(k, 1, SAGPU_WAVE);
+        if (lane_id() == 0 && e > 0 && e <= n) kp = keys[e - 1];
+    }
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
-        K k = keys[e];
-        bool s = sep((u32)k);
-        bool head = (e == 0) || s || keys[e - 1] != k;
-        bool tail = (e + 1 == n) || s || keys[e + 1] != k;
-        act = !(head && tail);
-        v1 = head ? (u32)e : 0u;
+        K k, kp;
+        key_pair(e, k, kp);
+        K kn = __shfl_down(k, 1, SAGPU_WAVE);
+        if (lane_id() == SAGPU_WAVE - 1 && e + 1 < n) kn = keys[e + 1];
+        const bool s = sep((u32)k);
+        const bool head = (e == 0) || s || kp != k;
+        const bool tail = (e + 1 >= n) || s || kn != k;
+        act = e < n && !(head && tail);
+        v1 = e < n && head ? (u32)e : 0u;
         v2 = 0;
     }
     // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
@@ -218,20 +233,30 @@ struct InitF {
     }
     __device__ void emit(u64 e, bool act, u64 o, u32 g, u32) const
     {
-        u32 v = vals[e];
+        K kk = 0, kkp = 0;
+        if (LCP) key_pair(e, kk, kkp);   // all lanes: neighbour exchange
+        if (abits) {
+            const u64 m = __ballot(act && e < n);   // rows are 64-aligned: lane 0 holds e % 64 == 0
+            if (lane_id() == 0 && e < n) abits[e >> 6] = m;
+        }
+        if (e >= n) return;
+        const u32 vv = vals[e];
+        const u32 v = vv >> cb;
+        if (codes) codes[e] = (uint8_t)(vv & ((1u << cb) - 1u));
         G[e] = g;
+        // every slot is written (full lines); slots of active elements are
+        // overwritten when their suffixes are placed
+        SA[e] = v;
         if (act) {
             act_i[o] = v;
             act_g[o] = g;
-        } else {
-            SA[e] = v;
         }
         if (LCP) {
             u32 out = 0;
             if (e > 0) {
                 // a pair from two different strings (batched GSA) always has a
                 // separator suffix second, whose lcp is 0 by the separator rule
-                u32 k = (u32)keys[e], kp = (u32)keys[e - 1];
+                u32 k = (u32)kk, kp = (u32)kkp;
                 u32 l = key_lcp(k, kp);
                 if (l >= (u32)ksym && !sep(k)) {
                     out = LCP_UNKNOWN | (u32)ksym;
@@ -261,6 +286,11 @@ struct BigF {
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
+        if (e >= n) {
+            act = false;
+            v1 = v2 = 0;
+            return;
+        }
         u64 c = comp[e];
         u64 cp = e > 0 ? comp[e - 1] : ~0ull;
         u64 cn = e + 1 < n ? comp[e + 1] : ~0ull;
@@ -273,6 +303,7 @@ struct BigF {
     }
     __device__ void emit(u64 e, bool act, u64 o, u32 gstart, u32 hstart) const
     {
+        if (e >= n) return;
         u32 v = vals[e];
         u64 c = comp[e];
         u32 g = gslot[(u32)(c >> kb)];
@@ -309,11 +340,11 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
 // 64 Mi-entry window: random 4-byte writes confined to 256 MiB are merged by the
 // memory-side cache instead of costing a partial-line write each.
 __global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals, const u32 *__restrict__ G, u64 n,
-                                                    u32 *__restrict__ ISA, u64 lo, u64 hi)
+                                                    u32 *__restrict__ ISA, u64 lo, u64 hi, int cb)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-        u32 v = vals[e];
+        u32 v = vals[e] >> cb;
         if (v >= lo && v < hi) ISA[v] = G[e];
     }
 }
@@ -531,16 +562,19 @@ int g_tile_range = 512;
 
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed = nullptr)
 {
-    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP};
+    const int cb = seed ? seed->cb : 0;
+    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP,
+               cb, seed ? seed->codes : nullptr, seed ? seed->abits : nullptr};
     SAGPU_TRY(ws_run(ws, f, n, nullptr, s));
     // windows of 2^g_isa_window_log2 entries, but at most 8 passes over vals/G
     const u64 w0 = 1ull << g_isa_window_log2;
     const u64 win = n > 8 * w0 ? (n + 7) / 8 : w0;
     for (u64 lo = 0; lo < n; lo += win) {
         u64 hi = lo + win < n ? lo + win : n;
-        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi);
+        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi,
+                           cb);
     }
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
@@ -550,9 +584,9 @@ void set_isa_window_log2(int v) { g_isa_window_log2 = v; }
 void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed)
 {
-    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, G, s);
+    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, G, s, seed);
 }
 
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,

@@ -54,18 +54,22 @@ constexpr int PK_THREADS = 256, PK_ITEMS = 16, PK_TILE = PK_THREADS * PK_ITEMS, 
 // rounds, where a suffix shorter than the offset ranks by its length.  In GSA mode
 // (separator = code 0, letters >= 1) every symbol after the first separator is
 // forced to 0, so keys that contain a separator end in a zero slot.
+// keys[p] = codes of T[p .. p+k) (first symbol high), vals[p] = p, or with cb > 0
+// (p << cb) | code of T[p-1]: the BWT symbol of suffix p travels with it through
+// the initial sort.
 __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__restrict__ T, u64 n,
                                                             const uint8_t *__restrict__ lut, int bits, int k,
-                                                            int gsa, u32 *__restrict__ keys, u32 *__restrict__ vals)
+                                                            int gsa, int cb, u32 *__restrict__ keys,
+                                                            u32 *__restrict__ vals)
 {
     __shared__ uint8_t s_lut[256];
-    __shared__ uint8_t s_c[PK_TILE + PK_HALO];
+    __shared__ uint8_t s_c[PK_TILE + PK_HALO + 1];   // s_c[i] = code of T[base - 1 + i]
     s_lut[threadIdx.x] = lut[threadIdx.x];
     __syncthreads();
     const u64 base = (u64)blockIdx.x * PK_TILE;
-    for (int i = threadIdx.x; i < PK_TILE + PK_HALO; i += PK_THREADS) {
-        u64 p = base + i;
-        s_c[i] = p < n ? s_lut[T[p]] : 0;
+    for (int i = threadIdx.x; i < PK_TILE + PK_HALO + 1; i += PK_THREADS) {
+        u64 p = base + i;   // position base - 1 + i, shifted by one
+        s_c[i] = (p >= 1 && p - 1 < n) ? s_lut[T[p - 1]] : 0;
     }
     __syncthreads();
 #pragma unroll 4
@@ -76,7 +80,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__res
         u32 key = 0;
         bool sep = false;
         for (int t = 0; t < k; t++) {
-            u32 c = s_c[li + t];
+            u32 c = s_c[li + 1 + t];
             if (gsa) {
                 if (sep) c = 0;
                 sep = sep || c == 0;
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__res
             key = (key << bits) | c;
         }
         keys[p] = key;
-        vals[p] = (u32)p;
+        vals[p] = cb ? ((u32)p << cb) | (u32)s_c[li] : (u32)p;
     }
 }
 
@@ -334,15 +338,15 @@ int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_ba
     return SAGPU_OK;
 }
 
-int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, u32 *keys, u32 *vals,
-                       hipStream_t s)
+int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, int k, int gsa, int cb, u32 *keys,
+                       u32 *vals, hipStream_t s)
 {
-    if (k > PK_HALO || k * bits > 32) {
+    if (k > PK_HALO || k * bits > 32 || (cb && (u64)(n - 1) >> (32 - cb))) {
         sagpu_set_error("pack32: bad symbol count", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
     hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
-                       keys, vals);
+                       cb, keys, vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -199,11 +199,14 @@ def test_repeatable():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana"])
-def test_bwt_scatter_variant(name):
+def test_bwt_variants(name, variant):
+    """Scatter (1) and plain gather (2) against the oracle; the default (0, seeded
+    from the initial sort) is what every other BWT test runs."""
     T = dict(CASES)[name]
     lib = la.load()
-    assert lib.sagpu_set_option(b"bwt_variant", 1) == 0
+    assert lib.sagpu_set_option(b"bwt_variant", variant) == 0
     try:
         U_ref, idx_ref = ol.bwt(T, ol.sa(T)[1])
         U = np.zeros(len(T), np.uint8)
@@ -214,6 +217,29 @@ def test_bwt_scatter_variant(name):
         assert fidx == idx_ref and np.array_equal(fU, U_ref)
     finally:
         assert lib.sagpu_set_option(b"bwt_variant", 0) == 0
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 4097, 100001])
+def test_bwt_unaligned_output(n):
+    """sagpu_build32 with a BWT output that is not 4-byte aligned."""
+    T = workloads.iid(n, seed=n + 3, alphabet=b"ACGT")
+    U_ref, idx_ref = ol.bwt(T, ol.sa(T)[1])
+    lib = la.load()
+    dT = lib.sagpu_malloc(n)
+    dSA = lib.sagpu_malloc(4 * n)
+    dUb = lib.sagpu_malloc(n + 8)
+    try:
+        assert lib.sagpu_memcpy_h2d(dT, T.ctypes.data, n) == 0
+        assert lib.sagpu_memset(dUb, 0x5A, n + 8) == 0
+        r = lib.sagpu_build32(dT, n, 0, dSA, dUb + 1, None, None, None)
+        assert r == idx_ref, la.last_error()
+        out = np.empty(n + 8, np.uint8)
+        lib.sagpu_memcpy_d2h(out.ctypes.data, dUb, n + 8)
+        assert np.array_equal(out[1:n + 1], U_ref)
+        assert out[0] == 0x5A and np.all(out[n + 1:] == 0x5A)   # nothing written outside
+    finally:
+        for p in (dT, dSA, dUb):
+            lib.sagpu_free(p)
 
 
 @pytest.mark.parametrize("wlog", [20, 32])
