@@ -173,6 +173,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    beside its index where both fit in 32 bits, and the text is read only for
  *                    the rows the rounds place; otherwise as 2.  1 = scatter in text order,
  *                    2 = gather T[SA[j]-1] in SA order
+ *   "plcp_pairs"     1 = the fused PLCP pass reads one 8-byte pair word per position, built in
+ *                    SA order (default); 0 = it gathers the mark, SA[j-1] and T[k-1] separately
  *   "isa_window_log2" ISA scatter window after the initial sort, 2^v entries (20..32, default 26)
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)

@@ -136,8 +136,11 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // the construction; position i reads marks[ISA[i]] and only unknown ones compare.
 // V64 (n entries) is needed when PLCP[i] + i can reach 2^32 (n > 2^32 - 1 - n/2 is
 // not tracked: it is passed whenever n > INT32_MAX); otherwise PLCP is the scratch.
+// pairs (n entries, byte texts with marks only): scratch for the per-slot pair words
+// that replace the mark, SA[j-1] and T[k-1] gathers; U (optional) is the finished BWT.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
-                const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes = 1);
+                const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes = 1,
+                u64 *pairs = nullptr, const uint8_t *U = nullptr);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // BWT auxiliary samples and inverse BWT (unbwt.hip).

@@ -124,6 +124,9 @@ u64 g_wide_from = 0x7FFFFFFFull;
 // 0: seeded from the initial sort where the symbol code fits beside the suffix
 // index, else gather T[SA[j]-1] in SA order; 1: scatter in text order; 2: gather
 int g_bwt_variant = 0;
+// 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
+// SA and text gathered separately
+int g_plcp_pairs = 1;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -570,7 +573,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 }
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-             u32 *PLCP, const u32 *marks, int sym_bytes = 1)
+             u32 *PLCP, const u32 *marks, int sym_bytes = 1, const uint8_t *dU = nullptr)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -588,7 +591,12 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
         V64 = sc.get<u64>(n);
         if (!V64) return SAGPU_ENOMEM;
     }
-    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes);
+    u64 *pairs = nullptr;
+    if (marks && sym_bytes == 1 && g_plcp_pairs) {
+        pairs = sc.get<u64>(n);
+        if (!pairs) return SAGPU_ENOMEM;
+    }
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes, pairs, dU);
 }
 
 // Copies a text into a pool buffer with 64 zero bytes of padding.
@@ -647,7 +655,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (want_lcp) {
             {
                 Span sp(c, F_PLCP);
-                SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks));
+                SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks, 1, dU));
             }
             if (dLCP) {
                 Span sp(c, F_LCP);
@@ -1293,6 +1301,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "plcp_pairs")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_plcp_pairs = (int)value;
         return 0;
     }
     if (!strcmp(name, "bwt_variant")) {

@@ -160,15 +160,36 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                                                   const u32 *__restrict__ SA, VT *__restrict__ V,
                                                   u32 *__restrict__ queue, u32 *__restrict__ queue_l,
                                                   u32 *__restrict__ queue_n, u64 queue_cap,
-                                                  const u32 *__restrict__ marks)
+                                                  const u32 *__restrict__ marks, const u64 *__restrict__ pairs)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 lb = 0;
         u64 k;
+        bool tested = false;   // reducibility already decided (pairs form)
         if (PHI) {
             k = PHI[i];
             if (k == i) k = n;  // first suffix in SA order
+        } else if (pairs) {
+            // one 8-byte read at i's SA slot: the known lcp, or the predecessor,
+            // its preceding symbol and a lower bound (see k_plcp_pairs)
+            const u32 r = ISA[i];
+            k = n;
+            if (r != 0) {
+                const u64 w = pairs[r];
+                if (!(w >> 63)) {
+                    V[i] = (VT)((u64)(u32)w + i);
+                    continue;
+                }
+                k = (u32)w;
+                const u32 lg = (u32)(w >> 58) & 31u;
+                lb = lg ? (1ull << (lg - 1)) : 0;
+                if (i > 0 && k > 0) {
+                    const uint8_t a = T[i - 1], b = (uint8_t)(w >> 32);
+                    if (a == b && !(gsa && a == 0)) { V[i] = 0; continue; }
+                }
+            }
+            tested = true;
         } else {
             u32 r = ISA[i];
             if (marks) {
@@ -183,7 +204,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
             k = r == 0 ? n : SA[r - 1];
         }
         if (k > n) k = n;
-        bool irr = (i == 0) || (k == 0) || (k >= n);
+        bool irr = tested || (i == 0) || (k == 0) || (k >= n);
         if (!irr) {
             if (W == 1) {
                 uint8_t a = T[i - 1], b = T[k - 1];
@@ -214,6 +235,37 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                 V[i] = (VT)(l + i);
             }
         }
+    }
+}
+
+// Per SA slot j >= 1, what the PLCP pass needs about the pair (SA[j-1], SA[j]):
+//   known lcp:  the value (bit 63 clear)
+//   unknown:    bit 63 | (floor(log2 bound) + 1, or 0) << 58 | T[k-1] << 32 | k,
+//               k = SA[j-1]; T[k-1] is the BWT symbol of slot j-1, taken from U (the
+//               BWT in output layout) when given.
+// Built in SA order with sequential reads, so the text-order pass gathers one
+// 8-byte word per position instead of the mark, SA[j-1] and T[k-1] separately.
+__global__ __launch_bounds__(256) void k_plcp_pairs(const u32 *__restrict__ SA, const u32 *__restrict__ marks,
+                                                    const uint8_t *__restrict__ U, const uint8_t *__restrict__ T,
+                                                    const u32 *__restrict__ ISA, u64 n, u64 *__restrict__ pairs)
+{
+    const u64 j0 = ISA[0];
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const u32 m = j ? marks[j] : 0u;
+        u64 w = m;
+        if (m & LCP_UNKNOWN) {
+            const u32 k = SA[j - 1];
+            const u32 lb = m & ~LCP_UNKNOWN;
+            const u32 lg = lb ? 32u - (u32)__clz(lb) : 0u;
+            uint8_t b = 0;
+            if (k != 0) {
+                const u64 q = j - 1;
+                b = U ? U[q + (q < j0)] : T[k - 1];
+            }
+            w = (1ull << 63) | ((u64)lg << 58) | ((u64)b << 32) | (u64)k;
+        }
+        pairs[j] = w;
     }
 }
 
@@ -487,7 +539,7 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
 
 template <typename VT, int W>
 int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                  u32 *PLCP, VT *V, const u32 *marks, hipStream_t s)
+                  u32 *PLCP, VT *V, const u32 *marks, hipStream_t s, const u64 *pairs = nullptr)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {
@@ -497,7 +549,7 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     VT *tmax = (VT *)ls.tile_max;
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
     hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V,
-                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
+                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks, pairs);
     hipLaunchKernelGGL((k_plcp_long<VT, W>), dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
                        ls.queue_n, ls.queue_cap);
     hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
@@ -508,7 +560,7 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
 }
 
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes)
+                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes, u64 *pairs, const uint8_t *U)
 {
     if (sym_bytes == 2) {
         if (marks || !PHI) {
@@ -526,8 +578,14 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
         if (V64) return launch_plcp_t<u64, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, V64, nullptr, s);
         return launch_plcp_t<u32, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, PLCP, nullptr, s);
     }
-    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s);
-    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s);
+    if (marks && pairs) {
+        hipLaunchKernelGGL(k_plcp_pairs, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, SA, marks, U, T, ISA, n,
+                           pairs);
+    } else {
+        pairs = nullptr;
+    }
+    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s, pairs);
+    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s, pairs);
 }
 
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)

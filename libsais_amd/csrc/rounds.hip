@@ -187,26 +187,14 @@ struct InitF {
     u64 *abits;
 
     __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
-    // Key of element e and of its left neighbour: one load per element, the
-    // neighbour from the adjacent lane (lane 0 loads its own).
-    __device__ void key_pair(u64 e, K &k, K &kp) const
+    // Group flags of element e from its key and its neighbours' keys.
+    __device__ void flags(u64 e, K k, K kp, K kn, bool &act, u32 &v1) const
     {
-        k = e < n ? keys[e] : K(0);
-        kp = __shfl_up(k, 1, SAGPU_WAVE);
-        if (lane_id() == 0 && e > 0 && e <= n) kp = keys[e - 1];
-    }
-    __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
-    {
-        K k, kp;
-        key_pair(e, k, kp);
-        K kn = __shfl_down(k, 1, SAGPU_WAVE);
-        if (lane_id() == SAGPU_WAVE - 1 && e + 1 < n) kn = keys[e + 1];
         const bool s = sep((u32)k);
         const bool head = (e == 0) || s || kp != k;
         const bool tail = (e + 1 >= n) || s || kn != k;
         act = e < n && !(head && tail);
         v1 = e < n && head ? (u32)e : 0u;
-        v2 = 0;
     }
     // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
     __device__ u32 pad_tail(u32 k) const
@@ -231,48 +219,162 @@ struct InitF {
         }
         return l;
     }
-    __device__ void emit(u64 e, bool act, u64 o, u32 g, u32) const
+    // LCP mark of slot e >= 1: the lcp with slot e-1 when the keys decide it, else
+    // LCP_UNKNOWN | lower bound.  A pair from two different strings (batched GSA)
+    // always has a separator suffix second, whose lcp is 0 by the separator rule.
+    __device__ u32 lcp_mark(u32 k, u32 kp) const
     {
-        K kk = 0, kkp = 0;
-        if (LCP) key_pair(e, kk, kkp);   // all lanes: neighbour exchange
-        if (abits) {
-            const u64 m = __ballot(act && e < n);   // rows are 64-aligned: lane 0 holds e % 64 == 0
-            if (lane_id() == 0 && e < n) abits[e >> 6] = m;
-        }
-        if (e >= n) return;
-        const u32 vv = vals[e];
-        const u32 v = vv >> cb;
-        if (codes) codes[e] = (uint8_t)(vv & ((1u << cb) - 1u));
-        G[e] = g;
-        // every slot is written (full lines); slots of active elements are
-        // overwritten when their suffixes are placed
-        SA[e] = v;
-        if (act) {
-            act_i[o] = v;
-            act_g[o] = g;
-        }
-        if (LCP) {
-            u32 out = 0;
-            if (e > 0) {
-                // a pair from two different strings (batched GSA) always has a
-                // separator suffix second, whose lcp is 0 by the separator rule
-                u32 k = (u32)kk, kp = (u32)kkp;
-                u32 l = key_lcp(k, kp);
-                if (l >= (u32)ksym && !sep(k)) {
-                    out = LCP_UNKNOWN | (u32)ksym;
-                } else if (!gsa && (pad_tail(k) + l >= (u32)ksym || pad_tail(kp) + l >= (u32)ksym)) {
-                    // One of the two groups may hold a suffix that ends within the
-                    // first l symbols (its key is padded with code 0 there); its true
-                    // lcp is then shorter, so leave it to the LCP pass with bound l.
-                    out = LCP_UNKNOWN | l;
-                } else {
-                    out = l;
-                }
-            }
-            LCP[e] = out;
-        }
+        const u32 l = key_lcp(k, kp);
+        if (l >= (u32)ksym && !sep(k)) return LCP_UNKNOWN | (u32)ksym;
+        // One of the two groups may hold a suffix that ends within the first l
+        // symbols (its key is padded with code 0 there); its true lcp is then
+        // shorter, so leave it to the LCP pass with bound l.
+        if (!gsa && (pad_tail(k) + l >= (u32)ksym || pad_tail(kp) + l >= (u32)ksym)) return LCP_UNKNOWN | l;
+        return l;
     }
 };
+
+// The keys of one wave's 1024-element span (16 rows of 64) in registers, and the
+// keys just before and after the span.  Neighbours come from lane shuffles, so
+// every key is loaded once and all loads of a span are in flight together.
+template <typename K>
+struct SpanKeys {
+    K k[WS_ROWS];
+    K before, after;
+
+    __device__ void load(const K *keys, u64 n, u64 wbase)
+    {
+        const u32 lane = lane_id();
+#pragma unroll
+        for (int r = 0; r < WS_ROWS; r++) {
+            const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+            k[r] = e < n ? keys[e] : K(0);
+        }
+        before = (wbase > 0 && wbase - 1 < n) ? keys[wbase - 1] : K(0);
+        after = wbase + WS_SPAN < n ? keys[wbase + WS_SPAN] : K(0);
+    }
+    // Left and right neighbour keys of row r (r is a compile-time constant in the
+    // unrolled callers).
+    __device__ void neighbours(int r, K &kp, K &kn) const
+    {
+        const u32 lane = lane_id();
+        const K up = __shfl_up(k[r], 1, SAGPU_WAVE);
+        const K last = r > 0 ? __shfl(k[r - 1], SAGPU_WAVE - 1, SAGPU_WAVE) : before;
+        kp = lane ? up : last;
+        const K down = __shfl_down(k[r], 1, SAGPU_WAVE);
+        const K first = r + 1 < WS_ROWS ? __shfl(k[r + 1], 0, SAGPU_WAVE) : after;
+        kn = lane + 1 < SAGPU_WAVE ? down : first;
+    }
+};
+
+// Initial grouping, pass 1: per tile, the number of grouped elements and the
+// position of the last group start.
+template <typename K>
+__global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__restrict__ tagg)
+{
+    __shared__ Agg s_w[WS_WAVES];
+    const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    SpanKeys<K> sk;
+    sk.load(f.keys, f.n, wbase);
+    u64 s = 0;
+    u32 m1 = 0;
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        K kp, kn;
+        sk.neighbours(r, kp, kn);
+        bool act;
+        u32 v1;
+        f.flags(wbase + (u64)r * SAGPU_WAVE + lane, sk.k[r], kp, kn, act, v1);
+        s += act;
+        m1 = umax(m1, v1);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d, SAGPU_WAVE);
+        m1 = umax(m1, __shfl_xor(m1, d, SAGPU_WAVE));
+    }
+    if (lane == 0) s_w[w] = Agg{s, m1, 0};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Agg t{0, 0, 0};
+        for (int q = 0; q < WS_WAVES; q++) {
+            t.s += s_w[q].s;
+            t.m1 = umax(t.m1, s_w[q].m1);
+        }
+        tagg[blockIdx.x] = t;
+    }
+}
+
+// Initial grouping, pass 2 (after k_ws_scan_tiles): group heads G, singletons into
+// SA, grouped elements into the active list, LCP marks, BWT seeds.
+template <typename K>
+__global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg *__restrict__ texcl)
+{
+    __shared__ Agg s_w[WS_WAVES];
+    const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    const u64 n = f.n;
+    SpanKeys<K> sk;
+    sk.load(f.keys, n, wbase);
+    u32 vv[WS_ROWS];
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        vv[r] = e < n ? f.vals[e] : 0u;
+    }
+    bool act[WS_ROWS];
+    u32 v1[WS_ROWS];
+    u64 s = 0;
+    u32 m1 = 0;
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        K kp, kn;
+        sk.neighbours(r, kp, kn);
+        f.flags(wbase + (u64)r * SAGPU_WAVE + lane, sk.k[r], kp, kn, act[r], v1[r]);
+        s += act[r];
+        m1 = umax(m1, v1[r]);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d, SAGPU_WAVE);
+        m1 = umax(m1, __shfl_xor(m1, d, SAGPU_WAVE));
+    }
+    if (lane == 0) s_w[w] = Agg{s, m1, 0};
+    __syncthreads();
+    Agg c = texcl[blockIdx.x];
+    for (u32 q = 0; q < w; q++) {
+        c.s += s_w[q].s;
+        c.m1 = umax(c.m1, s_w[q].m1);
+    }
+    const u32 cmask = (1u << f.cb) - 1u;
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        const u32 is = wave_incl_sum((u32)act[r]);
+        const u32 g = umax(c.m1, wave_incl_max(v1[r]));
+        const u64 o = c.s + is - (u64)act[r];
+        if (f.abits) {
+            const u64 m = __ballot(act[r]);   // rows are 64-aligned
+            if (lane == 0 && e < n) f.abits[e >> 6] = m;
+        }
+        K kp = 0, kn;
+        if (f.LCP) sk.neighbours(r, kp, kn);
+        if (e < n) {
+            const u32 v = vv[r] >> f.cb;
+            if (f.codes) f.codes[e] = (uint8_t)(vv[r] & cmask);
+            f.G[e] = g;
+            // every slot is written (full lines); slots of grouped elements are
+            // overwritten when their suffixes are placed
+            f.SA[e] = v;
+            if (act[r]) {
+                f.act_i[o] = v;
+                f.act_g[o] = g;
+            }
+            if (f.LCP) f.LCP[e] = e > 0 ? f.lcp_mark((u32)sk.k[r], (u32)kp) : 0u;
+        }
+        c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
+        c.m1 = __shfl(g, SAGPU_WAVE - 1, SAGPU_WAVE);
+    }
+}
 
 // ------------------------------------------------------- big-group emission
 struct BigF {
@@ -567,7 +669,15 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     const int cb = seed ? seed->cb : 0;
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP,
                cb, seed ? seed->codes : nullptr, seed ? seed->abits : nullptr};
-    SAGPU_TRY(ws_run(ws, f, n, nullptr, s));
+    const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
+    if (tiles > ws.tiles_cap) {
+        sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
+    hipLaunchKernelGGL(k_init_reduce<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
+    hipLaunchKernelGGL(k_init_emit<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
     // windows of 2^g_isa_window_log2 entries, but at most 8 passes over vals/G
     const u64 w0 = 1ull << g_isa_window_log2;
     const u64 win = n > 8 * w0 ? (n + 7) / 8 : w0;

@@ -219,6 +219,23 @@ def test_bwt_variants(name, variant):
         assert lib.sagpu_set_option(b"bwt_variant", 0) == 0
 
 
+@pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana"])
+def test_plcp_without_pair_words(name):
+    """The fused PLCP pass with separate gathers (plcp_pairs = 0) against the oracle;
+    the default pair-word form is what the other fused tests run."""
+    T = dict(CASES)[name]
+    lib = la.load()
+    assert lib.sagpu_set_option(b"plcp_pairs", 0) == 0
+    try:
+        rc, SA_ref = ol.sa(T)
+        P_ref = ol.plcp(T, SA_ref)
+        fSA, fU, fidx, fP, fL, _ = fused(T)
+        assert np.array_equal(fSA, SA_ref) and np.array_equal(fP, P_ref)
+        assert np.array_equal(fL, ol.lcp(P_ref, SA_ref))
+    finally:
+        assert lib.sagpu_set_option(b"plcp_pairs", 1) == 0
+
+
 @pytest.mark.parametrize("n", [2, 3, 5, 4097, 100001])
 def test_bwt_unaligned_output(n):
     """sagpu_build32 with a BWT output that is not 4-byte aligned."""
