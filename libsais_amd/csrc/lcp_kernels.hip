@@ -52,19 +52,25 @@ __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ 
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
         const u64 p0 = 4 * q;
+        u64 j[4];
+        bool grouped[4];
+        uint8_t code[4];
+        u32 sa[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const u64 p = p0 + k;
+            j[k] = p <= j0 ? p - 1 : p;   // p = 0 is U[0]
+            const bool ok = p >= 1 && p < n;
+            grouped[k] = ok && ((abits[j[k] >> 6] >> (j[k] & 63)) & 1);
+            code[k] = ok ? codes[j[k]] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) sa[k] = grouped[k] ? SA[j[k]] : 1u;
         u32 word = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const u64 p = p0 + k;
-            if (p >= n) break;
-            uint8_t b;
-            if (p == 0) {
-                b = T[n - 1];
-            } else {
-                const u64 j = p <= j0 ? p - 1 : p;
-                const bool grouped = (abits[j >> 6] >> (j & 63)) & 1;
-                b = grouped ? T[SA[j] - 1u] : s_inv[codes[j]];
-            }
+            uint8_t b = p == 0 ? T[n - 1] : grouped[k] ? T[sa[k] - 1u] : s_inv[code[k]];
             word |= (u32)b << (8 * k);
         }
         if (aligned && p0 + 4 <= n) {
@@ -160,36 +166,15 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
                                                   const u32 *__restrict__ SA, VT *__restrict__ V,
                                                   u32 *__restrict__ queue, u32 *__restrict__ queue_l,
                                                   u32 *__restrict__ queue_n, u64 queue_cap,
-                                                  const u32 *__restrict__ marks, const u64 *__restrict__ pairs)
+                                                  const u32 *__restrict__ marks)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 lb = 0;
         u64 k;
-        bool tested = false;   // reducibility already decided (pairs form)
         if (PHI) {
             k = PHI[i];
             if (k == i) k = n;  // first suffix in SA order
-        } else if (pairs) {
-            // one 8-byte read at i's SA slot: the known lcp, or the predecessor,
-            // its preceding symbol and a lower bound (see k_plcp_pairs)
-            const u32 r = ISA[i];
-            k = n;
-            if (r != 0) {
-                const u64 w = pairs[r];
-                if (!(w >> 63)) {
-                    V[i] = (VT)((u64)(u32)w + i);
-                    continue;
-                }
-                k = (u32)w;
-                const u32 lg = (u32)(w >> 58) & 31u;
-                lb = lg ? (1ull << (lg - 1)) : 0;
-                if (i > 0 && k > 0) {
-                    const uint8_t a = T[i - 1], b = (uint8_t)(w >> 32);
-                    if (a == b && !(gsa && a == 0)) { V[i] = 0; continue; }
-                }
-            }
-            tested = true;
         } else {
             u32 r = ISA[i];
             if (marks) {
@@ -204,7 +189,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
             k = r == 0 ? n : SA[r - 1];
         }
         if (k > n) k = n;
-        bool irr = tested || (i == 0) || (k == 0) || (k >= n);
+        bool irr = (i == 0) || (k == 0) || (k >= n);
         if (!irr) {
             if (W == 1) {
                 uint8_t a = T[i - 1], b = T[k - 1];
@@ -245,27 +230,111 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
 //               BWT in output layout) when given.
 // Built in SA order with sequential reads, so the text-order pass gathers one
 // 8-byte word per position instead of the mark, SA[j-1] and T[k-1] separately.
+// Byte texts, pair-word form (k_plcp_pairs): per position one ISA read, one 8-byte
+// pair read and the symbol before it; known lcps and reducible positions finish
+// there, irreducible ones compare as k_plcp_irr.  PI positions per thread and step
+// with their loads issued together.
+constexpr int PI = 4;
+template <typename VT>
+__global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restrict__ T, u64 n, int gsa,
+                                                        const u32 *__restrict__ ISA, const u64 *__restrict__ pairs,
+                                                        VT *__restrict__ V, u32 *__restrict__ queue,
+                                                        u32 *__restrict__ queue_l, u32 *__restrict__ queue_n,
+                                                        u64 queue_cap)
+{
+    const u64 chunk = 256ull * PI;
+    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
+        u32 r[PI];
+        u64 w[PI];
+        uint8_t a[PI];
+#pragma unroll
+        for (int q = 0; q < PI; q++) {
+            const u64 i = base + (u64)q * 256 + threadIdx.x;
+            r[q] = i < n ? ISA[i] : 0u;
+            a[q] = (i > 0 && i < n) ? T[i - 1] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < PI; q++) w[q] = r[q] ? pairs[r[q]] : 0ull;
+#pragma unroll
+        for (int q = 0; q < PI; q++) {
+            const u64 i = base + (u64)q * 256 + threadIdx.x;
+            if (i >= n) continue;
+            if (r[q] == 0) {   // first suffix in SA order: lcp 0
+                V[i] = (VT)i;
+                continue;
+            }
+            if (!(w[q] >> 63)) {
+                V[i] = (VT)((u64)(u32)w[q] + i);
+                continue;
+            }
+            const u64 k = (u32)w[q];
+            const u32 lg = (u32)(w[q] >> 58) & 31u;
+            const u64 lb = lg ? (1ull << (lg - 1)) : 0;
+            if (i > 0 && k > 0 && a[q] == (uint8_t)(w[q] >> 32) && !(gsa && a[q] == 0)) {
+                V[i] = 0;   // reducible: PLCP[i] = PLCP[i-1] - 1
+                continue;
+            }
+            const u64 m = n - (i > k ? i : k);
+            bool done = false;
+            u64 l = match_bytes<1>(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
+            if (done) {
+                V[i] = (VT)(l + i);
+            } else {
+                u32 slot = atomicAdd(queue_n, 1u);
+                if (slot < queue_cap) {
+                    queue[slot] = (u32)i;
+                    queue_l[slot] = (u32)l;
+                    V[i] = (VT)(l + i);   // provisional; overwritten by k_plcp_long
+                } else {
+                    l = match_bytes<1>(T, i, k, l, m, gsa, ~0ull >> 1, &done);
+                    V[i] = (VT)(l + i);
+                }
+            }
+        }
+    }
+}
+
+constexpr int PP = 8;   // slots per thread and step
 __global__ __launch_bounds__(256) void k_plcp_pairs(const u32 *__restrict__ SA, const u32 *__restrict__ marks,
                                                     const uint8_t *__restrict__ U, const uint8_t *__restrict__ T,
                                                     const u32 *__restrict__ ISA, u64 n, u64 *__restrict__ pairs)
 {
     const u64 j0 = ISA[0];
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        const u32 m = j ? marks[j] : 0u;
-        u64 w = m;
-        if (m & LCP_UNKNOWN) {
-            const u32 k = SA[j - 1];
-            const u32 lb = m & ~LCP_UNKNOWN;
-            const u32 lg = lb ? 32u - (u32)__clz(lb) : 0u;
-            uint8_t b = 0;
-            if (k != 0) {
-                const u64 q = j - 1;
-                b = U ? U[q + (q < j0)] : T[k - 1];
-            }
-            w = (1ull << 63) | ((u64)lg << 58) | ((u64)b << 32) | (u64)k;
+    const u64 chunk = 256ull * PP;
+    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
+        u32 m[PP], k[PP];
+        uint8_t b[PP];
+#pragma unroll
+        for (int q = 0; q < PP; q++) {
+            const u64 j = base + (u64)q * 256 + threadIdx.x;
+            m[q] = (j > 0 && j < n) ? marks[j] : 0u;
         }
-        pairs[j] = w;
+#pragma unroll
+        for (int q = 0; q < PP; q++) {
+            const u64 j = base + (u64)q * 256 + threadIdx.x;
+            k[q] = (m[q] & LCP_UNKNOWN) ? SA[j - 1] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < PP; q++) {
+            const u64 j = base + (u64)q * 256 + threadIdx.x;
+            b[q] = 0;
+            if ((m[q] & LCP_UNKNOWN) && k[q] != 0) {
+                const u64 r = j - 1;
+                b[q] = U ? U[r + (r < j0)] : T[k[q] - 1];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PP; q++) {
+            const u64 j = base + (u64)q * 256 + threadIdx.x;
+            if (j >= n) continue;
+            u64 w = m[q];
+            if (m[q] & LCP_UNKNOWN) {
+                const u32 lb = m[q] & ~LCP_UNKNOWN;
+                const u32 lg = lb ? 32u - (u32)__clz(lb) : 0u;
+                w = (1ull << 63) | ((u64)lg << 58) | ((u64)b[q] << 32) | (u64)k[q];
+            }
+            pairs[j] = w;
+        }
     }
 }
 
@@ -516,14 +585,29 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s)
 
 namespace {
 
+constexpr int LB = 8;   // entries per thread and step of the gather kernels below
 __global__ __launch_bounds__(256) void k_lcp_fix(const u32 *__restrict__ PLCP, const u32 *__restrict__ SA, u64 n,
                                                  u32 *__restrict__ LCP)
 {
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        if (LCP[j] & LCP_UNKNOWN) {
-            u32 p = SA[j];
-            LCP[j] = p < n ? PLCP[p] : 0u;
+    const u64 chunk = 256ull * LB;
+    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
+        u32 m[LB], p[LB], v[LB];
+#pragma unroll
+        for (int k = 0; k < LB; k++) {
+            const u64 j = base + (u64)k * 256 + threadIdx.x;
+            m[k] = j < n ? LCP[j] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < LB; k++) {
+            const u64 j = base + (u64)k * 256 + threadIdx.x;
+            p[k] = (m[k] & LCP_UNKNOWN) ? SA[j] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < LB; k++) v[k] = ((m[k] & LCP_UNKNOWN) && p[k] < n) ? PLCP[p[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < LB; k++) {
+            const u64 j = base + (u64)k * 256 + threadIdx.x;
+            if (m[k] & LCP_UNKNOWN) LCP[j] = v[k];
         }
     }
 }
@@ -532,7 +616,7 @@ __global__ __launch_bounds__(256) void k_lcp_fix(const u32 *__restrict__ PLCP, c
 
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lcp_fix, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, PLCP, SA, n, LCP);
+    hipLaunchKernelGGL(k_lcp_fix, dim3(grid_for(n, 256 * LB, 65535)), dim3(256), 0, s, PLCP, SA, n, LCP);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -548,8 +632,14 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     }
     VT *tmax = (VT *)ls.tile_max;
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
-    hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V,
-                       ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks, pairs);
+    if (pairs) {
+        if constexpr (W == 1)
+            hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3(grid_for(n, 256 * PI, 65535)), dim3(256), 0, s, T, n, gsa,
+                               ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap);
+    } else {
+        hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA,
+                           SA, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
+    }
     hipLaunchKernelGGL((k_plcp_long<VT, W>), dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
                        ls.queue_n, ls.queue_cap);
     hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
@@ -579,7 +669,7 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
         return launch_plcp_t<u32, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, PLCP, nullptr, s);
     }
     if (marks && pairs) {
-        hipLaunchKernelGGL(k_plcp_pairs, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, SA, marks, U, T, ISA, n,
+        hipLaunchKernelGGL(k_plcp_pairs, dim3(grid_for(n, 256 * PP, 65535)), dim3(256), 0, s, SA, marks, U, T, ISA, n,
                            pairs);
     } else {
         pairs = nullptr;

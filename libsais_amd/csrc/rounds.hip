@@ -438,32 +438,66 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
     return SAGPU_OK;
 }
 
-// ISA[vals[e]] = G[e] for the destinations inside [lo, hi).  Called once per
+// ISA[vals[e] >> cb] = G[e] for the destinations inside [lo, hi).  Called once per
 // 64 Mi-entry window: random 4-byte writes confined to 256 MiB are merged by the
-// memory-side cache instead of costing a partial-line write each.
+// memory-side cache instead of costing a partial-line write each.  Each thread
+// takes IW elements per step with all their loads issued before any store.
+constexpr int IW = 8;
 __global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals, const u32 *__restrict__ G, u64 n,
                                                     u32 *__restrict__ ISA, u64 lo, u64 hi, int cb)
 {
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-        u32 v = vals[e] >> cb;
-        if (v >= lo && v < hi) ISA[v] = G[e];
+    const u64 chunk = 256ull * IW;
+    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
+        u32 v[IW];
+        bool in[IW];
+#pragma unroll
+        for (int k = 0; k < IW; k++) {
+            const u64 e = base + (u64)k * 256 + threadIdx.x;
+            v[k] = e < n ? vals[e] >> cb : 0u;
+            in[k] = e < n && v[k] >= lo && v[k] < hi;
+        }
+        u32 g[IW];
+#pragma unroll
+        for (int k = 0; k < IW; k++) g[k] = in[k] ? G[base + (u64)k * 256 + threadIdx.x] : 0u;
+#pragma unroll
+        for (int k = 0; k < IW; k++)
+            if (in[k]) ISA[v[k]] = g[k];
     }
 }
 
 // ------------------------------------------------------------- round kernels
+// IG list entries per thread and step, all loads (the random ISA reads included)
+// in flight before the stores.
+constexpr int IG = 4;
 template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                       u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
                                                       KT *__restrict__ key2, u32 *__restrict__ SA)
 {
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < a; k += stride) {
-        u32 i = act_i[k];
-        u32 g = act_g[k];
-        u64 ih = (u64)i + h;
-        key2[k] = ih < n ? (KT)(h + ISA[ih]) : (KT)(n - 1 - i);
-        if (k + 1 == a || act_g[k + 1] != g) SA[g] = (u32)k;
+    const u64 chunk = 256ull * IG;
+    for (u64 base = (u64)blockIdx.x * chunk; base < a; base += (u64)gridDim.x * chunk) {
+        u32 i[IG], g[IG], gn[IG], r[IG];
+#pragma unroll
+        for (int q = 0; q < IG; q++) {
+            const u64 k = base + (u64)q * 256 + threadIdx.x;
+            i[q] = k < a ? act_i[k] : 0u;
+            g[q] = k < a ? act_g[k] : 0u;
+            gn[q] = k + 1 < a ? act_g[k + 1] : ~g[q];
+        }
+#pragma unroll
+        for (int q = 0; q < IG; q++) {
+            const u64 k = base + (u64)q * 256 + threadIdx.x;
+            const u64 ih = (u64)i[q] + h;
+            r[q] = (k < a && ih < n) ? ISA[ih] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < IG; q++) {
+            const u64 k = base + (u64)q * 256 + threadIdx.x;
+            if (k >= a) continue;
+            const u64 ih = (u64)i[q] + h;
+            key2[k] = ih < n ? (KT)(h + r[q]) : (KT)(n - 1 - i[q]);
+            if (gn[q] != g[q]) SA[g[q]] = (u32)k;
+        }
     }
 }
 
@@ -683,7 +717,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     const u64 win = n > 8 * w0 ? (n + 7) / 8 : w0;
     for (u64 lo = 0; lo < n; lo += win) {
         u64 hi = lo + win < n ? lo + win : n;
-        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi,
+        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * IW, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi,
                            cb);
     }
     SAGPU_CHECK(hipGetLastError());
@@ -709,7 +743,7 @@ template <typename KT>
 int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
                    hipStream_t s)
 {
-    hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
+    hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
                        n, h, key2, SA);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
