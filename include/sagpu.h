@@ -175,7 +175,9 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    2 = gather T[SA[j]-1] in SA order
  *   "plcp_pairs"     1 = the fused PLCP pass reads one 8-byte pair word per position, built in
  *                    SA order (default); 0 = it gathers the mark, SA[j-1] and T[k-1] separately
- *   "isa_window_log2" ISA scatter window after the initial sort, 2^v entries (20..32, default 26)
+ *   "isa_window_log2" ISA scatter after the initial sort: (suffix, rank) pairs are bucketed by
+ *                    suffix into buckets of 2^v (10..32, default 25; at most 64 buckets) and
+ *                    scattered bucket after bucket
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)
  * Returns 0, or -1 for an unknown name or value.

@@ -56,6 +56,7 @@ struct WsScratch {
     u64 tiles_cap;    // tiles of 4096 elements
     u64 *total;       // [1]
     u64 *base;        // [1] reserved output base
+    u32 *bcur;        // [64] ISA bucket cursors (initial grouping)
 };
 
 // BWT seeding: the initial sort's values carry (suffix << cb) | code of the symbol
@@ -69,16 +70,18 @@ struct BwtSeed {
     u64 *abits;
     const uint8_t *inv;
 };
-// G: scratch of n entries (group head per sorted position, SA order).
+// PV, PG: scratch of n entries each ((suffix, group head) pairs bucketed for the
+// ISA scatter).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed = nullptr);
-// ISA scatter window after the initial sort: 2^v entries per pass (v in [20, 32]).
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr);
+// ISA scatter after the initial sort: buckets of 2^v suffixes (v in [10, 32]; at most
+// 64 buckets, larger ones for longer texts).
 void set_isa_window_log2(int v);
 // List entries per workgroup of the tile round kernel: 256, 512 (default) or 1024.
 void set_tile_range(int v);
 // Batched GSA: keys = (string id << 32) | packed symbols.
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
-                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s);
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s);
 // Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
                  u32 *SA, hipStream_t s);

@@ -449,6 +449,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         ws.tile_agg = tagg;
         ws.total = d_cnt + 2;
         ws.base = d_cnt + 3;
+        GET(bcur, u32, 64);
+        ws.bcur = bcur;
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
 
@@ -464,7 +466,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         Span sp(c, F_INITIAL);
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
-                             alt ? k0 : k1, s, cb ? seed : nullptr));
+                             alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr));
     } else {
         // batched GSA: (string id, symbols) keys, so every group stays inside one string
         u64 *k0 = (u64 *)kA, *k1 = (u64 *)kA + n;
@@ -476,7 +478,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         SAGPU_TRY(sort_u64(c, k0, v0, k1, v1, n, 0, 32 + bits_for(nstrings ? nstrings - 1 : 0), &alt));
         Span sp(c, F_INITIAL);
         SAGPU_TRY(group_init64(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, dSA, dISA, act_i[0], act_g[0], dLCP,
-                               (u32 *)(alt ? k0 : k1), s));
+                               (u32 *)(alt ? k0 : k1), (u32 *)(alt ? k0 : k1) + n, s));
     }
     u64 a = 0;
     SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));
@@ -1319,7 +1321,7 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         return 0;
     }
     if (!strcmp(name, "isa_window_log2")) {
-        if (value < 20 || value > 32) return SAGPU_EINVAL;
+        if (value < 10 || value > 32) return SAGPU_EINVAL;
         set_isa_window_log2((int)value);
         return 0;
     }

@@ -178,7 +178,12 @@ struct InitF {
     u64 n;
     u32 sepmask;     // GSA: low symbol slot; keys with it zero contain a separator
     int bits, ksym, gsa;
-    u32 *SA, *G, *act_i, *act_g, *LCP;   // G[e] = group head (ISA is scattered later)
+    u32 *SA, *act_i, *act_g, *LCP;
+    // (suffix, group head) pairs for the ISA scatter, bucketed by suffix >> wlog:
+    // bucket b fills PV/PG[b << wlog, ...) (it holds exactly the suffixes of that
+    // range); bcur[b] counts the entries taken so far
+    u32 *PV, *PG, *bcur;
+    int wlog;
     // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
     // it.  codes[e] receives that code and bit e of abits is set when slot e is
     // still in a group (its occupant, and so its BWT symbol, is decided later).
@@ -305,15 +310,21 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__r
     }
 }
 
-// Initial grouping, pass 2 (after k_ws_scan_tiles): group heads G, singletons into
-// SA, grouped elements into the active list, LCP marks, BWT seeds.
+// Initial grouping, pass 2 (after k_ws_scan_tiles): singletons into SA, grouped
+// elements into the active list, LCP marks, BWT seeds, and the (suffix, group head)
+// pairs into their ISA buckets.
+constexpr int IB_MAX = 64;   // ISA buckets
 template <typename K>
 __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg *__restrict__ texcl)
 {
     __shared__ Agg s_w[WS_WAVES];
+    __shared__ u32 s_cnt[IB_MAX];
+    __shared__ u64 s_base[IB_MAX];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
     const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
     const u64 n = f.n;
+    const u32 nb = (u32)(((n - 1) >> f.wlog) + 1);
+    if (threadIdx.x < IB_MAX) s_cnt[threadIdx.x] = 0;
     SpanKeys<K> sk;
     sk.load(f.keys, n, wbase);
     u32 vv[WS_ROWS];
@@ -322,17 +333,26 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         vv[r] = e < n ? f.vals[e] : 0u;
     }
-    bool act[WS_ROWS];
-    u32 v1[WS_ROWS];
+    __syncthreads();
+    // st[r]: bit 31 grouped, bit 30 group start, low bits the rank inside the
+    // workgroup's share of the element's ISA bucket
+    u32 st[WS_ROWS];
     u64 s = 0;
     u32 m1 = 0;
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         K kp, kn;
         sk.neighbours(r, kp, kn);
-        f.flags(wbase + (u64)r * SAGPU_WAVE + lane, sk.k[r], kp, kn, act[r], v1[r]);
-        s += act[r];
-        m1 = umax(m1, v1[r]);
+        bool act;
+        u32 v1;
+        f.flags(e, sk.k[r], kp, kn, act, v1);
+        s += act;
+        m1 = umax(m1, v1);
+        u32 rank = 0;
+        // u64 shifts: wlog may be 32 (one bucket), which a 32-bit shift cannot express
+        if (e < n) rank = atomicAdd(&s_cnt[(u32)((u64)(vv[r] >> f.cb) >> f.wlog)], 1u);
+        st[r] = rank | (act ? 1u << 31 : 0u) | (e < n && v1 == (u32)e && (e == 0 || v1 != 0) ? 1u << 30 : 0u);
     }
     for (int d = 32; d >= 1; d >>= 1) {
         s += __shfl_xor(s, d, SAGPU_WAVE);
@@ -340,20 +360,24 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
     }
     if (lane == 0) s_w[w] = Agg{s, m1, 0};
     __syncthreads();
+    if (threadIdx.x < nb && s_cnt[threadIdx.x])
+        s_base[threadIdx.x] = ((u64)threadIdx.x << f.wlog) + atomicAdd(&f.bcur[threadIdx.x], s_cnt[threadIdx.x]);
     Agg c = texcl[blockIdx.x];
     for (u32 q = 0; q < w; q++) {
         c.s += s_w[q].s;
         c.m1 = umax(c.m1, s_w[q].m1);
     }
+    __syncthreads();
     const u32 cmask = (1u << f.cb) - 1u;
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
-        const u32 is = wave_incl_sum((u32)act[r]);
-        const u32 g = umax(c.m1, wave_incl_max(v1[r]));
-        const u64 o = c.s + is - (u64)act[r];
+        const bool act = st[r] >> 31;
+        const u32 is = wave_incl_sum((u32)act);
+        const u32 g = umax(c.m1, wave_incl_max((st[r] >> 30) & 1u ? (u32)e : 0u));
+        const u64 o = c.s + is - (u64)act;
         if (f.abits) {
-            const u64 m = __ballot(act[r]);   // rows are 64-aligned
+            const u64 m = __ballot(act);   // rows are 64-aligned
             if (lane == 0 && e < n) f.abits[e >> 6] = m;
         }
         K kp = 0, kn;
@@ -361,11 +385,13 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         if (e < n) {
             const u32 v = vv[r] >> f.cb;
             if (f.codes) f.codes[e] = (uint8_t)(vv[r] & cmask);
-            f.G[e] = g;
+            const u64 slot = s_base[(u32)((u64)v >> f.wlog)] + (st[r] & 0x3fffffffu);
+            f.PV[slot] = v;
+            f.PG[slot] = g;
             // every slot is written (full lines); slots of grouped elements are
             // overwritten when their suffixes are placed
             f.SA[e] = v;
-            if (act[r]) {
+            if (act) {
                 f.act_i[o] = v;
                 f.act_g[o] = g;
             }
@@ -373,6 +399,30 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         }
         c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m1 = __shfl(g, SAGPU_WAVE - 1, SAGPU_WAVE);
+    }
+}
+
+// ISA[PV[e]] = PG[e] over the bucketed pairs, in order: the workgroups in flight at
+// any time work inside one or two buckets, so the random 4-byte writes of the whole
+// device land in a stretch of ISA that the memory-side cache holds and merges.
+// Measured at 2^28 entries: 2^25-entry buckets (128 MiB of ISA) 4.0 ms, 2^22-2^24
+// 5.4-6.5 ms, 2^26-2^27 8-10 ms, one bucket per XCD 9.8 ms.
+constexpr int IW = 8;
+__global__ __launch_bounds__(256) void k_isa_scatter(const u32 *__restrict__ PV, const u32 *__restrict__ PG, u64 n,
+                                                     u32 *__restrict__ ISA)
+{
+    const u64 chunk = 256ull * IW;
+    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
+        u32 v[IW], g[IW];
+#pragma unroll
+        for (int k = 0; k < IW; k++) {
+            const u64 e = base + (u64)k * 256 + threadIdx.x;
+            v[k] = e < n ? PV[e] : 0u;
+            g[k] = e < n ? PG[e] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < IW; k++)
+            if (base + (u64)k * 256 + threadIdx.x < n) ISA[v[k]] = g[k];
     }
 }
 
@@ -436,33 +486,6 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
                        out_counter ? ws.base : (const u64 *)nullptr);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
-}
-
-// ISA[vals[e] >> cb] = G[e] for the destinations inside [lo, hi).  Called once per
-// 64 Mi-entry window: random 4-byte writes confined to 256 MiB are merged by the
-// memory-side cache instead of costing a partial-line write each.  Each thread
-// takes IW elements per step with all their loads issued before any store.
-constexpr int IW = 8;
-__global__ __launch_bounds__(256) void k_isa_window(const u32 *__restrict__ vals, const u32 *__restrict__ G, u64 n,
-                                                    u32 *__restrict__ ISA, u64 lo, u64 hi, int cb)
-{
-    const u64 chunk = 256ull * IW;
-    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
-        u32 v[IW];
-        bool in[IW];
-#pragma unroll
-        for (int k = 0; k < IW; k++) {
-            const u64 e = base + (u64)k * 256 + threadIdx.x;
-            v[k] = e < n ? vals[e] >> cb : 0u;
-            in[k] = e < n && v[k] >= lo && v[k] < hi;
-        }
-        u32 g[IW];
-#pragma unroll
-        for (int k = 0; k < IW; k++) g[k] = in[k] ? G[base + (u64)k * 256 + threadIdx.x] : 0u;
-#pragma unroll
-        for (int k = 0; k < IW; k++)
-            if (in[k]) ISA[v[k]] = g[k];
-    }
 }
 
 // ------------------------------------------------------------- round kernels
@@ -693,15 +716,18 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 
 } // namespace
 
-int g_isa_window_log2 = 26;
+int g_isa_window_log2 = 25;
 int g_tile_range = 512;
 
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed = nullptr)
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr)
 {
     const int cb = seed ? seed->cb : 0;
-    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, G, act_i, act_g, LCP,
+    // ISA buckets of 2^wlog suffixes, at most IB_MAX of them
+    int wlog = g_isa_window_log2;
+    while ((((n - 1) >> wlog) + 1) > (u64)IB_MAX) wlog++;
+    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
                cb, seed ? seed->codes : nullptr, seed ? seed->abits : nullptr};
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
@@ -709,17 +735,11 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
         return SAGPU_EDEVICE;
     }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
+    SAGPU_TRY(launch_fill_u32(ws.bcur, IB_MAX, 0, s));
     hipLaunchKernelGGL(k_init_reduce<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
     hipLaunchKernelGGL(k_init_emit<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
-    // windows of 2^g_isa_window_log2 entries, but at most 8 passes over vals/G
-    const u64 w0 = 1ull << g_isa_window_log2;
-    const u64 win = n > 8 * w0 ? (n + 7) / 8 : w0;
-    for (u64 lo = 0; lo < n; lo += win) {
-        u64 hi = lo + win < n ? lo + win : n;
-        hipLaunchKernelGGL(k_isa_window, dim3(grid_for(n, 256 * IW, 65535)), dim3(256), 0, s, vals, G, n, ISA, lo, hi,
-                           cb);
-    }
+    hipLaunchKernelGGL(k_isa_scatter, dim3(grid_for(n, 256 * IW, 65535)), dim3(256), 0, s, PV, PG, n, ISA);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -728,15 +748,15 @@ void set_isa_window_log2(int v) { g_isa_window_log2 = v; }
 void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s, const BwtSeed *seed)
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed)
 {
-    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, G, s, seed);
+    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, PV, PG, s, seed);
 }
 
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
-                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *G, hipStream_t s)
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s)
 {
-    return group_init_t<u64>(ws, keys, vals, n, bits, ksym, 1, SA, ISA, act_i, act_g, LCP, G, s);
+    return group_init_t<u64>(ws, keys, vals, n, bits, ksym, 1, SA, ISA, act_i, act_g, LCP, PV, PG, s);
 }
 
 template <typename KT>

@@ -259,7 +259,7 @@ def test_bwt_unaligned_output(n):
             lib.sagpu_free(p)
 
 
-@pytest.mark.parametrize("wlog", [20, 32])
+@pytest.mark.parametrize("wlog", [12, 20, 32])
 def test_isa_window_option(wlog):
     T = workloads.dna_like(3 << 20, seed=17)
     lib = la.load()
@@ -268,4 +268,4 @@ def test_isa_window_option(wlog):
         rc, SA_ref = ol.sa(T)
         assert np.array_equal(la.suffix_array(T), SA_ref)
     finally:
-        assert lib.sagpu_set_option(b"isa_window_log2", 26) == 0
+        assert lib.sagpu_set_option(b"isa_window_log2", 25) == 0
