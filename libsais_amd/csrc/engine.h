@@ -15,6 +15,17 @@ struct ScanScratch {
 int scan_exclusive_u32(ScanScratch &sc, u32 *data, u64 count, hipStream_t s);
 
 // ------------------------------------------------------------------ radix sort
+// Keys and values of the initial sort computed from a packed code stream instead of
+// read from arrays: key(p) = the kbits stream bits from bit p*bits (codes of T[p..],
+// first symbol highest), value(p) = p, or (p << cb) | code of T[p-1] when cb > 0.
+// pw holds the stream MSB-first in u32 words, zero past the text (pack_stream_words).
+struct KeyGen {
+    const u32 *pw;
+    int bits, kbits, cb;
+};
+u64 pack_stream_words(u64 n, int bits);
+int launch_pack_stream(const uint8_t *T, u64 n, const uint8_t *lut, int bits, u32 *pw, hipStream_t s);
+
 // Onesweep LSD radix sort, 8-bit digits, stable.  Keys are u64 or u32, values u32.
 // The result ends in (k0,v0) or in (k1,v1); *result_in_alt says which.
 struct RadixScratch {
@@ -33,6 +44,11 @@ struct RadixScratch {
     u32 *counts;      // [counts_cap] per-tile digit counts (reduce-then-scan)
     u64 counts_cap;
     ScanScratch scan;
+    // u32 sorts only: when set, the input (k0, v0) is not read; the first pass
+    // generates it (and a sort whose passes are all skipped writes it to k0, v0).
+    // gen_pass is 1 while that pass is bracketed by the hook.
+    const KeyGen *gen;
+    int gen_pass;
 };
 
 enum { RADIX_ONESWEEP = 0, RADIX_REDUCE_THEN_SCAN = 1 };

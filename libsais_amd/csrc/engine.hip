@@ -124,6 +124,9 @@ u64 g_wide_from = 0x7FFFFFFFull;
 // 0: seeded from the initial sort where the symbol code fits beside the suffix
 // index, else gather T[SA[j]-1] in SA order; 1: scatter in text order; 2: gather
 int g_bwt_variant = 0;
+// 1: byte texts (no GSA) sort keys generated from a packed code stream inside the
+// first radix pass; 0: keys and values written by k_pack_keys32 first
+int g_key_stream = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
 // SA and text gathered separately
 int g_plcp_pairs = 1;
@@ -300,7 +303,7 @@ void onesweep_hook(void *user, int phase)
 {
     Ctx &c = *(Ctx *)user;
     static thread_local Span *open = nullptr;
-    if (phase == 0) open = new Span(c, c.sort_u32 ? F_ONESWEEP32 : F_ONESWEEP);
+    if (phase == 0) open = new Span(c, c.sort_u32 && !c.rs.gen_pass ? F_ONESWEEP32 : F_ONESWEEP);
     else { delete open; open = nullptr; }
 }
 
@@ -327,8 +330,13 @@ int sort_u32(Ctx &c, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, i
     c.sort_u32 = false;
     c.stats.sort_passes += passes;
     c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
-    c.stats.sorted_bytes += (int64_t)passes * (int64_t)n * 16;
-    c.stats.sorted_bytes_u32 += (int64_t)passes * (int64_t)n * 16;
+    // a generating first pass reads the code stream and writes 8 bytes per key; it
+    // is counted with all passes, not with the array-to-array u32 passes
+    const bool gen = c.rs.gen && c.rs.variant != RADIX_REDUCE_THEN_SCAN && passes > 0;
+    const int plain = passes - (gen ? 1 : 0);
+    c.stats.sorted_bytes += (int64_t)plain * (int64_t)n * 16;
+    if (gen) c.stats.sorted_bytes += (int64_t)n * 8 + (int64_t)(n * (u64)c.rs.gen->bits / 8);
+    c.stats.sorted_bytes_u32 += (int64_t)plain * (int64_t)n * 16;
     return rc;
 }
 
@@ -457,12 +465,27 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     int alt = 0;
     if (!sid) {
         u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
+        KeyGen gen{};
+        bool use_gen = false;
         {
             Span sp(c, F_INITIAL);
-            if (dSym) SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, gsa, k0, v0, s));
-            else SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
+            if (dSym) {
+                SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, gsa, k0, v0, s));
+            } else if (!gsa && g_key_stream) {
+                // keys and values are generated by the sort's first pass from a
+                // packed code stream (n * bits / 8 bytes) instead of written here
+                GET(pw, u32, pack_stream_words(n, bits));
+                SAGPU_TRY(launch_pack_stream(dT, n, d_lut, bits, pw, s));
+                gen = KeyGen{pw, bits, ksym * bits, cb};
+                use_gen = true;
+            } else {
+                SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
+            }
         }
-        SAGPU_TRY(sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt));
+        c.rs.gen = use_gen ? &gen : nullptr;
+        const int src = sort_u32(c, k0, v0, k1, v1, n, 0, ksym * bits, &alt);
+        c.rs.gen = nullptr;
+        SAGPU_TRY(src);
         Span sp(c, F_INITIAL);
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
@@ -1303,6 +1326,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "key_stream")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_key_stream = (int)value;
         return 0;
     }
     if (!strcmp(name, "plcp_pairs")) {

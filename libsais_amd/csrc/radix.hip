@@ -28,8 +28,23 @@ __device__ __forceinline__ u64 lb_pack(u32 epoch, u64 flag, u64 v)
     return ((u64)epoch << 42) | (flag << LB_VAL_BITS) | v;
 }
 
-template <typename K>
-__global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, u64 n, int begin_bit,
+// Stream bits [bitpos, bitpos + nbits) (nbits <= 32) as an integer.
+__device__ __forceinline__ u32 stream_bits(const u32 *__restrict__ pw, u64 bitpos, int nbits)
+{
+    const u64 w = bitpos >> 5;
+    const u64 x = ((u64)pw[w] << 32) | (u64)pw[w + 1];
+    return (u32)((x << (bitpos & 31)) >> (64 - nbits));
+}
+__device__ __forceinline__ u32 gen_key(const KeyGen &g, u64 p) { return stream_bits(g.pw, p * (u64)g.bits, g.kbits); }
+__device__ __forceinline__ u32 gen_val(const KeyGen &g, u64 p)
+{
+    if (!g.cb) return (u32)p;
+    const u32 prev = p ? stream_bits(g.pw, (p - 1) * (u64)g.bits, g.bits) : 0u;
+    return ((u32)p << g.cb) | prev;
+}
+
+template <typename K, bool GEN>
+__global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, KeyGen gen, u64 n, int begin_bit,
                                                     int end_bit, int npass, u64 *__restrict__ hist)
 {
     __shared__ u32 s_h[8][256];
@@ -37,7 +52,9 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
     __syncthreads();
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        K k = keys[i];
+        K k;
+        if constexpr (GEN) k = (K)gen_key(gen, i);
+        else k = keys[i];
         for (int p = 0; p < npass; p++) {
             const int sh = begin_bit + 8 * p;
             const u32 dm = end_bit - sh >= 8 ? 255u : ((1u << (end_bit - sh)) - 1u);
@@ -71,12 +88,14 @@ __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
 // One radix pass over a tile of RS_TILE keys.  LOOKBACK = true: onesweep (tile
 // ticket, aggregate publish, decoupled look-back).  LOOKBACK = false: the tile's
 // global digit offsets were produced by an upsweep + scan (reduce-then-scan).
-template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS>
+// GEN: the input keys and values are generated (KeyGen) instead of read.
+template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS, bool GEN = false>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
                                                            K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
                                                            int shift, u32 dmask, const u64 *__restrict__ offsets,
                                                            u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
-                                                           u32 epoch, const u32 *__restrict__ scanned, u64 tiles)
+                                                           u32 epoch, const u32 *__restrict__ scanned, u64 tiles,
+                                                           KeyGen gen)
 {
     constexpr int TILE_ = RS_THREADS * ITEMS;
     constexpr int WAVE_ITEMS_ = SAGPU_WAVE * ITEMS;
@@ -109,7 +128,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     K key[ITEMS];
     u32 rank[ITEMS];
     const bool full = (tile + 1) * (u64)TILE_ <= n;
-    if (full) {
+    if constexpr (GEN) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+            key[j] = pos < n ? (K)gen_key(gen, pos) : ~(K)0;
+        }
+    } else if (full) {
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) key[j] = kin[wbase + (u64)j * SAGPU_WAVE + lane];
     } else {
@@ -232,7 +257,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     if (HAS_VALS) {
         __syncthreads();
         u32 *s_vals = reinterpret_cast<u32 *>(s_keys);
-        if (full) {
+        if constexpr (GEN) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+                if (pos < n) s_vals[rank[j]] = gen_val(gen, pos);
+            }
+        } else if (full) {
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) s_vals[rank[j]] = vin[wbase + (u64)j * SAGPU_WAVE + lane];
         } else {
@@ -278,6 +309,37 @@ __global__ __launch_bounds__(RS_THREADS) void k_rts_upsweep(const K *__restrict_
     counts[(u64)tid * tiles + blockIdx.x] = c;
 }
 
+// A sort whose passes were all skipped still returns its input in (k0, v0).
+__global__ __launch_bounds__(256) void k_gen_write(KeyGen gen, u64 n, u32 *__restrict__ keys, u32 *__restrict__ vals)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        keys[p] = gen_key(gen, p);
+        vals[p] = gen_val(gen, p);
+    }
+}
+
+// Word w of the MSB-first code stream: codes of the symbols overlapping stream bits
+// [32w, 32w + 32); symbols at or past n are code 0.
+__global__ __launch_bounds__(256) void k_pack_stream(const uint8_t *__restrict__ T, u64 n,
+                                                     const uint8_t *__restrict__ lut, int bits, u32 *__restrict__ pw,
+                                                     u64 words)
+{
+    __shared__ uint8_t s_lut[256];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        const u64 b0 = w * 32, b1 = b0 + 32;
+        u64 acc = 0;   // stream bits [first symbol start, ...) accumulated MSB-first
+        const u64 s0 = b0 / (u64)bits, s1 = (b1 - 1) / (u64)bits;
+        for (u64 q = s0; q <= s1; q++) acc = (acc << bits) | (q < n ? (u64)s_lut[T[q]] : 0ull);
+        // acc holds bits [s0*bits, (s1+1)*bits); keep [b0, b1)
+        const u64 end = (s1 + 1) * (u64)bits;
+        pw[w] = (u32)(acc >> (end - b1));
+    }
+}
+
 template <typename K>
 int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
                     hipStream_t s, int *result_in_alt, int *passes_run)
@@ -294,11 +356,27 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         return SAGPU_EDEVICE;
     }
 
+    const KeyGen *gen = sizeof(K) == 4 ? rs.gen : nullptr;
+    const KeyGen g0 = gen ? *gen : KeyGen{};
+    if (gen && rs.variant == RADIX_REDUCE_THEN_SCAN) {
+        // the upsweep reads its keys: write the generated input out first
+        if constexpr (sizeof(K) == 4) {
+            hipLaunchKernelGGL(k_gen_write, dim3((unsigned)((n + 2047) / 2048 < 65535 ? (n + 2047) / 2048 : 65535)),
+                               dim3(256), 0, s, g0, n, (u32 *)k0, v0);
+            SAGPU_CHECK(hipGetLastError());
+        }
+        gen = nullptr;
+    }
     SAGPU_TRY(launch_fill_u64(rs.hist, 16 * 256, 0, s));
     {
         int blocks = (int)((n + 256 * 64 - 1) / (256 * 64));
         if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(blocks), dim3(256), 0, s, k0, n, begin_bit, end_bit, npass, rs.hist);
+        if (gen)
+            hipLaunchKernelGGL((k_radix_hist<K, true>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit, end_bit,
+                               npass, rs.hist);
+        else
+            hipLaunchKernelGGL((k_radix_hist<K, false>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit, end_bit,
+                               npass, rs.hist);
         hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass);
         SAGPU_CHECK(hipGetLastError());
     }
@@ -334,6 +412,8 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
                                    shift, dmask, rs.counts, tiles);
             SAGPU_TRY(scan_exclusive_u32(rs.scan, rs.counts, tiles * 256, s));
         }
+        const bool gen_now = gen && ran == 0;
+        rs.gen_pass = gen_now;
         if (rs.hook) rs.hook(rs.hook_user, 0);
         const bool lb = rs.variant != RADIX_REDUCE_THEN_SCAN;
         const u64 *offs = lb ? rs.hist + 8 * 256 + p * 256 : nullptr;
@@ -342,8 +422,16 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         const u32 *cnts = lb ? nullptr : rs.counts;
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
-                       n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles)
-        if (v0) {
+                       n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
+#define RS_LAUNCH_GEN(LB, IT)                                                                                      \
+    hipLaunchKernelGGL((k_radix_pass<K, true, LB, IT, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, \
+                       ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
+        if (gen_now) {
+            if constexpr (sizeof(K) == 4) {
+                if (lb) { if (items == 32) RS_LAUNCH_GEN(true, 32); else if (items == 8) RS_LAUNCH_GEN(true, 8); else RS_LAUNCH_GEN(true, 16); }
+                else { if (items == 32) RS_LAUNCH_GEN(false, 32); else if (items == 8) RS_LAUNCH_GEN(false, 8); else RS_LAUNCH_GEN(false, 16); }
+            }
+        } else if (v0) {
             if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
             else { if (items == 32) RS_LAUNCH(true, false, 32); else if (items == 8) RS_LAUNCH(true, false, 8); else RS_LAUNCH(true, false, 16); }
         } else {
@@ -351,12 +439,21 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
             else { if (items == 32) RS_LAUNCH(false, false, 32); else if (items == 8) RS_LAUNCH(false, false, 8); else RS_LAUNCH(false, false, 16); }
         }
 #undef RS_LAUNCH
+#undef RS_LAUNCH_GEN
         SAGPU_CHECK(hipGetLastError());
         if (rs.hook) rs.hook(rs.hook_user, 1);
+        rs.gen_pass = 0;
         K *tk = ki; ki = ko; ko = tk;
         u32 *tv = vi; vi = vo; vo = tv;
         alt ^= 1;
         ran++;
+    }
+    if (gen && ran == 0) {
+        if constexpr (sizeof(K) == 4) {
+            hipLaunchKernelGGL(k_gen_write, dim3((unsigned)((n + 2047) / 2048 < 65535 ? (n + 2047) / 2048 : 65535)),
+                               dim3(256), 0, s, g0, n, (u32 *)k0, v0);
+            SAGPU_CHECK(hipGetLastError());
+        }
     }
     *result_in_alt = alt;
     if (passes_run) *passes_run = ran;
@@ -364,6 +461,18 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
 }
 
 } // namespace
+
+u64 pack_stream_words(u64 n, int bits) { return (n * (u64)bits + 31) / 32 + 2; }
+
+int launch_pack_stream(const uint8_t *T, u64 n, const uint8_t *lut, int bits, u32 *pw, hipStream_t s)
+{
+    const u64 words = pack_stream_words(n, bits);
+    u64 blocks = (words + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(k_pack_stream, dim3((unsigned)blocks), dim3(256), 0, s, T, n, lut, bits, pw, words);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
 
 int radix_sort_u64(RadixScratch &rs, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
                    hipStream_t s, int *result_in_alt, int *passes_run)

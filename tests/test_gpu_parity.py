@@ -236,6 +236,24 @@ def test_plcp_without_pair_words(name):
         assert lib.sagpu_set_option(b"plcp_pairs", 1) == 0
 
 
+@pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana"])
+def test_keys_written_before_the_sort(name):
+    """key_stream = 0: the initial keys are written by their own kernel instead of
+    being generated inside the first radix pass (the default every other test runs)."""
+    T = dict(CASES)[name]
+    lib = la.load()
+    assert lib.sagpu_set_option(b"key_stream", 0) == 0
+    try:
+        rc, SA_ref = ol.sa(T)
+        P_ref = ol.plcp(T, SA_ref)
+        U_ref, idx_ref = ol.bwt(T, SA_ref)
+        fSA, fU, fidx, fP, fL, _ = fused(T)
+        assert np.array_equal(fSA, SA_ref) and np.array_equal(fP, P_ref)
+        assert fidx == idx_ref and np.array_equal(fU, U_ref)
+    finally:
+        assert lib.sagpu_set_option(b"key_stream", 1) == 0
+
+
 @pytest.mark.parametrize("n", [2, 3, 5, 4097, 100001])
 def test_bwt_unaligned_output(n):
     """sagpu_build32 with a BWT output that is not 4-byte aligned."""
