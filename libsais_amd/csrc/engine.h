@@ -21,6 +21,7 @@ int scan_exclusive_u32(ScanScratch &sc, u32 *data, u64 count, hipStream_t s);
 // pw holds the stream MSB-first in u32 words, zero past the text (pack_stream_words).
 struct KeyGen {
     const u32 *pw;
+    u64 words;        // pack_stream_words(n, bits)
     int bits, kbits, cb;
 };
 u64 pack_stream_words(u64 n, int bits);

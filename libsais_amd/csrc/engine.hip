@@ -476,7 +476,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 // packed code stream (n * bits / 8 bytes) instead of written here
                 GET(pw, u32, pack_stream_words(n, bits));
                 SAGPU_TRY(launch_pack_stream(dT, n, d_lut, bits, pw, s));
-                gen = KeyGen{pw, bits, ksym * bits, cb};
+                gen = KeyGen{pw, pack_stream_words(n, bits), bits, ksym * bits, cb};
                 use_gen = true;
             } else {
                 SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));

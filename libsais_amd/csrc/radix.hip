@@ -127,12 +127,39 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     // tile's own look-back entry, which no later tile reads.
     K key[ITEMS];
     u32 rank[ITEMS];
+    u32 prevc[GEN ? ITEMS / 4 : 1];   // GEN: code of T[pos-1] per item, 8 bits each
     const bool full = (tile + 1) * (u64)TILE_ <= n;
     if constexpr (GEN) {
+        // The tile's stretch of the code stream (from one symbol before the tile)
+        // goes to LDS with coalesced loads; keys and preceding codes are cut from
+        // it.  The stretch aliases s_keys, which is first written after ranking.
+        static_assert(sizeof(K) == 4, "generated keys are 32-bit");
+        u32 *s_st = reinterpret_cast<u32 *>(s_keys);
+        const u64 bits = (u64)gen.bits;
+        const u64 bit0 = tile * (u64)TILE_ * bits;
+        const u64 word0 = bit0 >= bits ? (bit0 - bits) >> 5 : 0;
+        const u32 nw = (u32)(((bit0 + (u64)TILE_ * bits + 64) >> 5) - word0 + 1);
+        for (u32 i = tid; i < nw; i += RS_THREADS) s_st[i] = word0 + i < gen.words ? gen.pw[word0 + i] : 0u;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < ITEMS / 4; q++) prevc[q] = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
-            u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-            key[j] = pos < n ? (K)gen_key(gen, pos) : ~(K)0;
+            const u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+            const u64 lb = pos * bits - word0 * 32;
+            const u32 lw = (u32)(lb >> 5), off = (u32)(lb & 31);
+            const u64 x = ((u64)s_st[lw] << 32) | (u64)s_st[lw + 1];
+            key[j] = pos < n ? (K)((x << off) >> (64 - gen.kbits)) : ~(K)0;
+            if (gen.cb && pos > 0) {
+                const u64 pb = lb - bits;
+                const u32 pw0 = (u32)(pb >> 5), poff = (u32)(pb & 31);
+                const u64 y = ((u64)s_st[pw0] << 32) | (u64)s_st[pw0 + 1];
+                prevc[j / 4] |= (u32)((y << poff) >> (64 - bits)) << (8 * (j % 4));
+            }
+            // keep the scheduler from hoisting every item's LDS reads and 64-bit
+            // temporaries (they would double the registers and halve occupancy)
+            asm volatile("" : "+v"(key[j]));
+            if (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
     } else if (full) {
 #pragma unroll
@@ -261,7 +288,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
                 u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
-                if (pos < n) s_vals[rank[j]] = gen_val(gen, pos);
+                const u32 pc = (prevc[j / 4] >> (8 * (j % 4))) & 255u;
+                if (pos < n) s_vals[rank[j]] = gen.cb ? ((u32)pos << gen.cb) | pc : (u32)pos;
             }
         } else if (full) {
 #pragma unroll
@@ -329,6 +357,31 @@ __global__ __launch_bounds__(256) void k_pack_stream(const uint8_t *__restrict__
     s_lut[threadIdx.x] = lut[threadIdx.x];
     __syncthreads();
     const u64 stride = (u64)gridDim.x * blockDim.x;
+    if (32 % bits == 0) {
+        // whole symbols per word: 32 / bits consecutive bytes, read as 4-byte groups
+        const int per = 32 / bits;
+        for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+            const u64 q0 = w * (u64)per;
+            u32 acc = 0;
+            for (int g = 0; g < per; g += 4) {
+                const u64 q = q0 + g;
+                u32 b4;
+                if (q + 4 <= n) {
+                    b4 = *reinterpret_cast<const u32 *>(T + q);   // q is a multiple of 4
+                } else {
+                    b4 = 0;
+                    for (int t = 0; t < 4; t++)
+                        if (q + t < n) b4 |= (u32)T[q + t] << (8 * t);
+                }
+                for (int t = 0; t < 4 && g + t < per; t++) {
+                    const u32 c = q + t < n ? (u32)s_lut[(b4 >> (8 * t)) & 255u] : 0u;
+                    acc = (acc << bits) | c;
+                }
+            }
+            pw[w] = acc;
+        }
+        return;
+    }
     for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
         const u64 b0 = w * 32, b1 = b0 + 32;
         u64 acc = 0;   // stream bits [first symbol start, ...) accumulated MSB-first
@@ -423,13 +476,16 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
                        n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
-#define RS_LAUNCH_GEN(LB, IT)                                                                                      \
-    hipLaunchKernelGGL((k_radix_pass<K, true, LB, IT, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, \
-                       ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
+        // The generating pass uses tiles of at most 4096 keys: cutting keys from the
+        // staged stream needs more registers per item than loading them.
+#define RS_LAUNCH_GEN(IT)                                                                                          \
+    hipLaunchKernelGGL((k_radix_pass<K, true, true, IT, true>), dim3((unsigned)gtiles), dim3(RS_THREADS), 0, s, ki,  \
+                       vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, gtiles, g0)
         if (gen_now) {
             if constexpr (sizeof(K) == 4) {
-                if (lb) { if (items == 32) RS_LAUNCH_GEN(true, 32); else if (items == 8) RS_LAUNCH_GEN(true, 8); else RS_LAUNCH_GEN(true, 16); }
-                else { if (items == 32) RS_LAUNCH_GEN(false, 32); else if (items == 8) RS_LAUNCH_GEN(false, 8); else RS_LAUNCH_GEN(false, 16); }
+                const int git = items == 8 ? 8 : 16;
+                const u64 gtiles = (n + (u64)RS_THREADS * git - 1) / ((u64)RS_THREADS * git);
+                if (git == 8) RS_LAUNCH_GEN(8); else RS_LAUNCH_GEN(16);
             }
         } else if (v0) {
             if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
