@@ -471,7 +471,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             Span sp(c, F_INITIAL);
             if (dSym) {
                 SAGPU_TRY(launch_pack_int(dSym, n, bits, ksym, gsa, k0, v0, s));
-            } else if (!gsa && g_key_stream) {
+            } else if (!gsa && g_key_stream && n <= (1ull << 31)) {
                 // keys and values are generated by the sort's first pass from a
                 // packed code stream (n * bits / 8 bytes) instead of written here
                 GET(pw, u32, pack_stream_words(n, bits));

@@ -285,11 +285,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
         __syncthreads();
         u32 *s_vals = reinterpret_cast<u32 *>(s_keys);
         if constexpr (GEN) {
+            // positions fit 32 bits here (values carry them); recomputed per item
+            const u32 wb = (u32)wbase, nn = (u32)(n - 1);
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
-                u64 pos = wbase + (u64)j * SAGPU_WAVE + lane;
+                const u32 pos = wb + (u32)j * SAGPU_WAVE + lane;
                 const u32 pc = (prevc[j / 4] >> (8 * (j % 4))) & 255u;
-                if (pos < n) s_vals[rank[j]] = gen.cb ? ((u32)pos << gen.cb) | pc : (u32)pos;
+                if (pos <= nn) s_vals[rank[j]] = (pos << gen.cb) | pc;
             }
         } else if (full) {
 #pragma unroll
@@ -476,16 +478,14 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
                        n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
-        // The generating pass uses tiles of at most 4096 keys: cutting keys from the
-        // staged stream needs more registers per item than loading them.
 #define RS_LAUNCH_GEN(IT)                                                                                          \
     hipLaunchKernelGGL((k_radix_pass<K, true, true, IT, true>), dim3((unsigned)gtiles), dim3(RS_THREADS), 0, s, ki,  \
                        vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, gtiles, g0)
         if (gen_now) {
             if constexpr (sizeof(K) == 4) {
-                const int git = items == 8 ? 8 : 16;
+                const int git = items;
                 const u64 gtiles = (n + (u64)RS_THREADS * git - 1) / ((u64)RS_THREADS * git);
-                if (git == 8) RS_LAUNCH_GEN(8); else RS_LAUNCH_GEN(16);
+                if (git == 8) RS_LAUNCH_GEN(8); else if (git == 16) RS_LAUNCH_GEN(16); else RS_LAUNCH_GEN(32);
             }
         } else if (v0) {
             if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
