@@ -167,6 +167,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *   "radix_variant"  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
  *   "radix_items"    keys per thread per radix tile: 8, 16 or 32 (2048-, 4096- or 8192-key
  *                    tiles; default 32)
+ *   "radix_threads"  onesweep workgroup size with 32 keys per thread: 256 (8192-key tiles) or
+ *                    512 (16384-key tiles, default; 32-bit keys only)
  *   "tile_range"     list entries per workgroup of the doubling-round tile kernel: 256, 512
  *                    (default) or 1024
  *   "bwt_variant"    0 = seeded (default): the initial sort carries each suffix's BWT symbol

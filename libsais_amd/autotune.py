@@ -5,6 +5,7 @@ The knobs are process-wide options of the engine (sagpu_set_option):
   radix_items    keys per thread of a radix tile: 8, 16, 32 (tiles of 2048, 4096,
                  8192 keys)
   radix_variant  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
+  radix_threads  onesweep workgroup size with 32 keys per thread: 256 or 512
   tile_range     list entries per workgroup of the doubling-round tile kernel:
                  256, 512, 1024
 
@@ -35,9 +36,10 @@ from . import SAGPU_FLAG_GSA, last_error, load as _load_lib
 GRID = {
     "radix_items": (8, 16, 32),
     "radix_variant": (0, 1),
+    "radix_threads": (256, 512),
     "tile_range": (256, 512, 1024),
 }
-DEFAULT = {"radix_items": 32, "radix_variant": 0, "tile_range": 512}
+DEFAULT = {"radix_items": 32, "radix_variant": 0, "radix_threads": 512, "tile_range": 512}
 
 
 def apply(cfg: dict) -> None:

@@ -41,7 +41,8 @@ struct RadixScratch {
     void (*hook)(void *user, int phase);
     void *hook_user;
     int variant;      // RADIX_ONESWEEP or RADIX_REDUCE_THEN_SCAN
-    int items;        // keys per thread per tile: 16 or 32 (tile = 256 * items)
+    int items;        // keys per thread per tile: 8, 16 or 32 (tile = threads * items)
+    int threads;      // onesweep workgroup size with 32 items: 256 or 512
     u32 *counts;      // [counts_cap] per-tile digit counts (reduce-then-scan)
     u64 counts_cap;
     ScanScratch scan;

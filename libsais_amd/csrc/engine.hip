@@ -119,6 +119,7 @@ Ctx *g_ctx = nullptr;
 bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
+int g_radix_threads = 512;
 // Texts longer than this use 64-bit rank keys and PLCP scan values.
 u64 g_wide_from = 0x7FFFFFFFull;
 // 0: seeded from the initial sort where the symbol code fits beside the suffix
@@ -148,6 +149,7 @@ int ctx_create(Ctx **out)
     c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
     c->rs.variant = g_radix_variant;
     c->rs.items = g_radix_items;
+    c->rs.threads = g_radix_threads;
     *out = c;
     return SAGPU_OK;
 }
@@ -1320,6 +1322,12 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != 8 && value != 16 && value != 32) return SAGPU_EINVAL;
         g_radix_items = (int)value;
         if (g_ctx) g_ctx->rs.items = g_radix_items;
+        return 0;
+    }
+    if (!strcmp(name, "radix_threads")) {
+        if (value != 256 && value != 512) return SAGPU_EINVAL;
+        g_radix_threads = (int)value;
+        if (g_ctx) g_ctx->rs.threads = g_radix_threads;
         return 0;
     }
     if (!strcmp(name, "radix_variant")) {

@@ -89,29 +89,31 @@ __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
 // ticket, aggregate publish, decoupled look-back).  LOOKBACK = false: the tile's
 // global digit offsets were produced by an upsweep + scan (reduce-then-scan).
 // GEN: the input keys and values are generated (KeyGen) instead of read.
-template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS, bool GEN = false>
-__global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
+template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS, bool GEN = false, int NT = RS_THREADS>
+__global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
                                                            K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
                                                            int shift, u32 dmask, const u64 *__restrict__ offsets,
                                                            u64 *__restrict__ lookback, u32 *__restrict__ tile_ctr,
                                                            u32 epoch, const u32 *__restrict__ scanned, u64 tiles,
                                                            KeyGen gen)
 {
-    constexpr int TILE_ = RS_THREADS * ITEMS;
+    // NT threads (256 or 512) per workgroup; the 256 digits are handled by the
+    // first 256 threads.
+    constexpr int NW = NT / SAGPU_WAVE;
+    constexpr int TILE_ = NT * ITEMS;
     constexpr int WAVE_ITEMS_ = SAGPU_WAVE * ITEMS;
     __shared__ u32 s_tile;
-    __shared__ u32 s_wcnt[RS_WAVES][256];
+    __shared__ u32 s_wcnt[NW][256];
     __shared__ u32 s_dstart[256];
     __shared__ u64 s_gbase[256];
-    __shared__ u32 s_wsum[RS_WAVES];
+    __shared__ u32 s_wsum[256 / SAGPU_WAVE];
     __shared__ K s_keys[TILE_];
 
     const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
     if (LOOKBACK) {
         if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
     }
-#pragma unroll
-    for (int q = 0; q < RS_WAVES; q++) s_wcnt[q][tid] = 0;
+    for (u32 i = tid; i < (u32)NW * 256; i += NT) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
 
     const u64 tile = LOOKBACK ? (u64)s_tile : (u64)blockIdx.x;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
         const u64 bit0 = tile * (u64)TILE_ * bits;
         const u64 word0 = bit0 >= bits ? (bit0 - bits) >> 5 : 0;
         const u32 nw = (u32)(((bit0 + (u64)TILE_ * bits + 64) >> 5) - word0 + 1);
-        for (u32 i = tid; i < nw; i += RS_THREADS) s_st[i] = word0 + i < gen.words ? gen.pw[word0 + i] : 0u;
+        for (u32 i = tid; i < nw; i += NT) s_st[i] = word0 + i < gen.words ? gen.pw[word0 + i] : 0u;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < ITEMS / 4; q++) prevc[q] = 0;
@@ -196,17 +198,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     }
     __syncthreads();
 
-    // Per digit: wave offsets, tile count, global offset.
-    {
+    // Per digit (threads 0..255): wave offsets, tile count, global offset.
+    u32 run = 0, inc = 0;
+    if (tid < 256) {
         const u32 d = tid;
-        u32 run = 0;
 #pragma unroll
-        for (int q = 0; q < RS_WAVES; q++) {
+        for (int q = 0; q < NW; q++) {
             u32 c = s_wcnt[q][d];
             s_wcnt[q][d] = run;
             run += c;
         }
-        u32 inc = wave_incl_sum(run);
+        inc = wave_incl_sum(run);
         if (lane == SAGPU_WAVE - 1) s_wsum[w] = inc;
 
         u64 gbase;
@@ -246,10 +248,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
             gbase = scanned[(u64)d * tiles + tile];
         }
         s_gbase[d] = gbase;
-        __syncthreads();
+    }
+    __syncthreads();
+    if (tid < 256) {
         u32 carry = 0;
         for (u32 q = 0; q < w; q++) carry += s_wsum[q];
-        s_dstart[d] = carry + inc - run;
+        s_dstart[tid] = carry + inc - run;
     }
     __syncthreads();
 
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
     for (int q = 0; q < ITEMS / 4; q++) dpack[q] = 0;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
-        const u32 idx = tid + (u32)j * RS_THREADS;
+        const u32 idx = tid + (u32)j * NT;
         if (idx < tile_n) {
             const K kk = s_keys[idx];
             const u32 d = (u32)(kk >> shift) & dmask;
@@ -306,7 +310,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_pass(const K *__restrict__
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
-            const u32 idx = tid + (u32)j * RS_THREADS;
+            const u32 idx = tid + (u32)j * NT;
             if (idx < tile_n) {
                 const u32 d = (dpack[j / 4] >> (8 * (j % 4))) & 255u;
                 vout[s_gbase[d] + (idx - s_dstart[d])] = s_vals[idx];
@@ -475,13 +479,26 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         u64 *lbk = lb ? rs.lookback : nullptr;
         u32 *tctr = lb ? rs.tile_ctr + p : nullptr;
         const u32 *cnts = lb ? nullptr : rs.counts;
+        // onesweep passes may use 512-thread workgroups (16384-key tiles with 32
+        // items: twice as long digit runs per tile, same waves per CU)
+        const int nt = (sizeof(K) == 4 && lb && rs.threads == 512 && items == 32) ? 512 : RS_THREADS;
+        const u64 ptiles = (n + (u64)nt * items - 1) / ((u64)nt * items);
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
                        n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
+#define RS_LAUNCH512(HV, GN)                                                                                      \
+    hipLaunchKernelGGL((k_radix_pass<K, HV, true, 32, GN, 512>), dim3((unsigned)ptiles), dim3(512), 0, s, ki, vi, ko, \
+                       vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, ptiles, g0)
 #define RS_LAUNCH_GEN(IT)                                                                                          \
     hipLaunchKernelGGL((k_radix_pass<K, true, true, IT, true>), dim3((unsigned)gtiles), dim3(RS_THREADS), 0, s, ki,  \
                        vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, gtiles, g0)
-        if (gen_now) {
+        if (nt == 512) {
+            if constexpr (sizeof(K) == 4) {   // 16384 u32 keys: 64 KB of LDS staging
+                if (gen_now) RS_LAUNCH512(true, true);
+                else if (v0) RS_LAUNCH512(true, false);
+                else RS_LAUNCH512(false, false);
+            }
+        } else if (gen_now) {
             if constexpr (sizeof(K) == 4) {
                 const int git = items;
                 const u64 gtiles = (n + (u64)RS_THREADS * git - 1) / ((u64)RS_THREADS * git);
@@ -495,6 +512,7 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
             else { if (items == 32) RS_LAUNCH(false, false, 32); else if (items == 8) RS_LAUNCH(false, false, 8); else RS_LAUNCH(false, false, 16); }
         }
 #undef RS_LAUNCH
+#undef RS_LAUNCH512
 #undef RS_LAUNCH_GEN
         SAGPU_CHECK(hipGetLastError());
         if (rs.hook) rs.hook(rs.hook_user, 1);

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 def test_tune_text_grid_identical_outputs():
     T = workloads.dna_like(1 << 20, seed=21)
     res = autotune.tune_text(T, reps=1)
-    assert len(res["results"]) == 18 and all(r["identical"] for r in res["results"])
+    assert len(res["results"]) == 36 and all(r["identical"] for r in res["results"])
     assert res["best"] in [{**autotune.DEFAULT, **r["config"]} for r in res["results"]]
     # one non-default point against the oracle
     autotune.apply({"radix_items": 8, "radix_variant": 1, "tile_range": 256})
@@ -44,5 +44,5 @@ def test_tune_batched_small():
 
 def test_knob_validation():
     lib = la.load()
-    for name, bad in ((b"radix_items", 12), (b"tile_range", 300), (b"radix_variant", 2)):
+    for name, bad in ((b"radix_items", 12), (b"tile_range", 300), (b"radix_variant", 2), (b"radix_threads", 384)):
         assert lib.sagpu_set_option(name, bad) == -1

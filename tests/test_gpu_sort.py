@@ -9,12 +9,15 @@ import libsais_amd as la
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[0, 1], ids=["onesweep", "reduce_then_scan"])
+@pytest.fixture(params=[(0, 256), (0, 512), (1, 256)], ids=["onesweep", "onesweep_512", "reduce_then_scan"])
 def variant(request):
     lib = la.load()
-    assert lib.sagpu_set_option(b"radix_variant", request.param) == 0
+    v, threads = request.param
+    assert lib.sagpu_set_option(b"radix_variant", v) == 0
+    assert lib.sagpu_set_option(b"radix_threads", threads) == 0
     yield request.param
     lib.sagpu_set_option(b"radix_variant", 0)
+    lib.sagpu_set_option(b"radix_threads", 512)
 
 
 def _dev(arr):
