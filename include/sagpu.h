@@ -177,6 +177,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    2 = gather T[SA[j]-1] in SA order
  *   "key_stream"     1 = byte texts (no GSA): the initial sort's keys are generated inside its
  *                    first pass from a packed code stream (default); 0 = written first
+ *   "raw_round1"     1 = with key_stream, the first doubling round orders each group by the
+ *                    packed key that follows (same order as its rank; default); 0 = by rank
  *   "plcp_pairs"     1 = the fused PLCP pass reads one 8-byte pair word per position, built in
  *                    SA order (default); 0 = it gathers the mark, SA[j-1] and T[k-1] separately
  *   "isa_window_log2" ISA scatter after the initial sort: (suffix, rank) pairs are bucketed by

@@ -100,9 +100,17 @@ void set_tile_range(int v);
 // Batched GSA: keys = (string id << 32) | packed symbols.
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s);
+// First round from the code stream: key2 = packed key of i + h (h = symbols per key)
+// for every group except those holding one of the suffixes shorter than h (their
+// group heads in special[0..nspecial)), which keep key2 = h + ISA[i+h] / n-1-i.
+struct RawKey1 {
+    const u32 *pw;    // null: rank form everywhere
+    int bits, kbits, nspecial;
+    u32 special[32];
+};
 // Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s);
+                 u32 *SA, hipStream_t s, const RawKey1 *rk = nullptr);
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
                u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s);

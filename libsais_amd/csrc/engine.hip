@@ -128,6 +128,9 @@ int g_bwt_variant = 0;
 // 1: byte texts (no GSA) sort keys generated from a packed code stream inside the
 // first radix pass; 0: keys and values written by k_pack_keys32 first
 int g_key_stream = 1;
+// 1: with a code stream, the first round orders groups by the packed key of i + h
+// instead of its rank ISA[i + h] (the same order, read from a smaller array)
+int g_raw_round1 = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
 // SA and text gathered separately
 int g_plcp_pairs = 1;
@@ -465,6 +468,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
 
     int alt = 0;
+    RawKey1 rk1{};   // first round from the code stream (when there is one)
     if (!sid) {
         u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
         KeyGen gen{};
@@ -492,6 +496,18 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
                              alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr));
+        if (use_gen && g_raw_round1) {
+            // group heads of the suffixes shorter than ksym: their groups keep ranks
+            const u64 cnt = n < (u64)ksym ? n : (u64)ksym;
+            u32 *hs = (u32 *)(c.pinned + 6144);   // past the histogram and LUT staging
+            SAGPU_CHECK(hipMemcpyAsync(hs, dISA + (n - cnt), cnt * sizeof(u32), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipStreamSynchronize(s));
+            rk1.pw = gen.pw;
+            rk1.bits = bits;
+            rk1.kbits = ksym * bits;
+            rk1.nspecial = (int)cnt;
+            for (u64 t = 0; t < cnt; t++) rk1.special[t] = hs[t];
+        }
     } else {
         // batched GSA: (string id, symbols) keys, so every group stays inside one string
         u64 *k0 = (u64 *)kA, *k1 = (u64 *)kA + n;
@@ -521,7 +537,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         const int nxt = cur ^ 1;
         {
             Span sp(c, F_ROUNDS);
-            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s));
+            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s,
+                                   st.rounds == 1 && rk1.pw ? &rk1 : nullptr));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s));
@@ -1334,6 +1351,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "raw_round1")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_raw_round1 = (int)value;
         return 0;
     }
     if (!strcmp(name, "key_stream")) {

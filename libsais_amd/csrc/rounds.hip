@@ -495,11 +495,12 @@ constexpr int IG = 4;
 template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                       u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
-                                                      KT *__restrict__ key2, u32 *__restrict__ SA)
+                                                      KT *__restrict__ key2, u32 *__restrict__ SA, RawKey1 rk)
 {
     const u64 chunk = 256ull * IG;
     for (u64 base = (u64)blockIdx.x * chunk; base < a; base += (u64)gridDim.x * chunk) {
         u32 i[IG], g[IG], gn[IG], r[IG];
+        bool raw[IG];
 #pragma unroll
         for (int q = 0; q < IG; q++) {
             const u64 k = base + (u64)q * 256 + threadIdx.x;
@@ -511,14 +512,30 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
         for (int q = 0; q < IG; q++) {
             const u64 k = base + (u64)q * 256 + threadIdx.x;
             const u64 ih = (u64)i[q] + h;
-            r[q] = (k < a && ih < n) ? ISA[ih] : 0u;
+            // round 1 with a code stream: the packed key of i + h orders the group as
+            // its rank does (equal keys, equal ranks), read from the 2-bits-per-symbol
+            // stream instead of the 4-byte ISA; groups holding a suffix shorter than
+            // h keep the rank form
+            raw[q] = false;
+            if (rk.pw) {
+                raw[q] = true;
+                for (int t = 0; t < rk.nspecial; t++) raw[q] = raw[q] && g[q] != rk.special[t];
+            }
+            if (k < a && raw[q]) {
+                const u64 bitpos = ih * (u64)rk.bits;
+                const u64 w = bitpos >> 5;
+                const u64 x = ((u64)rk.pw[w] << 32) | (u64)rk.pw[w + 1];
+                r[q] = (u32)((x << (bitpos & 31)) >> (64 - rk.kbits));
+            } else {
+                r[q] = (k < a && ih < n) ? ISA[ih] : 0u;
+            }
         }
 #pragma unroll
         for (int q = 0; q < IG; q++) {
             const u64 k = base + (u64)q * 256 + threadIdx.x;
             if (k >= a) continue;
             const u64 ih = (u64)i[q] + h;
-            key2[k] = ih < n ? (KT)(h + r[q]) : (KT)(n - 1 - i[q]);
+            key2[k] = raw[q] ? (KT)r[q] : ih < n ? (KT)(h + r[q]) : (KT)(n - 1 - i[q]);
             if (gn[q] != g[q]) SA[g[q]] = (u32)k;
         }
     }
@@ -761,10 +778,10 @@ int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bit
 
 template <typename KT>
 int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
-                   hipStream_t s)
+                   hipStream_t s, const RawKey1 &rk)
 {
     hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
-                       n, h, key2, SA);
+                       n, h, key2, SA, rk);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -798,10 +815,11 @@ int big_extract_t(const u32 *act_i, const KT *key2, const u32 *bhead, const u64 
 }
 
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s)
+                 u32 *SA, hipStream_t s, const RawKey1 *rk)
 {
-    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s)
-                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s);
+    const RawKey1 r = rk ? *rk : RawKey1{};
+    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s, r)
+                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s, r);
 }
 
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
