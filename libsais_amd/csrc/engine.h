@@ -87,6 +87,8 @@ struct BwtSeed {
     uint8_t *codes;
     u64 *abits;
     const uint8_t *inv;
+    const u32 *pw;    // packed code stream of the text (KeyGen), or null: grouped slots
+                      // then read the code of T[SA[j]-1] there instead of the text
 };
 // PV, PG: scratch of n entries each ((suffix, group head) pairs bucketed for the
 // ISA scatter).

@@ -425,7 +425,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     if (want_seed && ((n - 1) >> (32 - bits)) == 0) {
         GET(codes, uint8_t, n);
         GET(abits, u64, (n + 63) / 64);
-        *seed = BwtSeed{bits, codes, abits, d_inv};
+        *seed = BwtSeed{bits, codes, abits, d_inv, nullptr};
     }
     const int cb = seed ? seed->cb : 0;
     st.initial_symbols = ksym;
@@ -484,6 +484,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 SAGPU_TRY(launch_pack_stream(dT, n, d_lut, bits, pw, s));
                 gen = KeyGen{pw, pack_stream_words(n, bits), bits, ksym * bits, cb};
                 use_gen = true;
+                if (cb) seed->pw = pw;
             } else {
                 SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
             }

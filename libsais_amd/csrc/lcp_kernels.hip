@@ -42,7 +42,8 @@ __global__ __launch_bounds__(256) void k_bwt(const ST *__restrict__ T, const u32
 __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ T, const u32 *__restrict__ SA,
                                                     const u32 *__restrict__ ISA, const uint8_t *__restrict__ codes,
                                                     const u64 *__restrict__ abits, const uint8_t *__restrict__ inv,
-                                                    u64 n, uint8_t *__restrict__ U, int aligned)
+                                                    const u32 *__restrict__ pw, int bits, u64 n,
+                                                    uint8_t *__restrict__ U, int aligned)
 {
     __shared__ uint8_t s_inv[256];
     s_inv[threadIdx.x] = inv[threadIdx.x];
@@ -70,7 +71,18 @@ __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ 
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const u64 p = p0 + k;
-            uint8_t b = p == 0 ? T[n - 1] : grouped[k] ? T[sa[k] - 1u] : s_inv[code[k]];
+            uint8_t b;
+            if (p == 0) {
+                b = T[n - 1];
+            } else if (grouped[k] && pw) {
+                // the code of T[sa - 1] from the packed stream (a smaller array to gather from)
+                const u64 bitpos = (u64)(sa[k] - 1u) * (u64)bits;
+                const u64 w = bitpos >> 5;
+                const u64 x = ((u64)pw[w] << 32) | (u64)pw[w + 1];
+                b = s_inv[(u32)((x << (bitpos & 31)) >> (64 - bits))];
+            } else {
+                b = grouped[k] ? T[sa[k] - 1u] : s_inv[code[k]];
+            }
             word |= (u32)b << (8 * k);
         }
         if (aligned && p0 + 4 <= n) {
@@ -557,7 +569,7 @@ int launch_bwt_seeded(const uint8_t *T, const u32 *SA, const u32 *ISA, const Bwt
 {
     const int aligned = ((uintptr_t)U & 3) == 0;
     hipLaunchKernelGGL(k_bwt_seeded, dim3(grid_for((n + 3) / 4, 256 * 4, 65535)), dim3(256), 0, s, T, SA, ISA,
-                       seed.codes, seed.abits, seed.inv, n, U, aligned);
+                       seed.codes, seed.abits, seed.inv, seed.pw, seed.cb, n, U, aligned);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
