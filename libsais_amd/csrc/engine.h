@@ -82,6 +82,11 @@ struct WsScratch {
 // code per SA slot (codes, n bytes) and a bitmap of the slots still in groups
 // (abits, n/64 words); launch_bwt_seeded finishes the BWT after the rounds,
 // reading the text only at those slots.  inv: code -> byte (256 bytes, device).
+// The packed code stream of a byte text (KeyGen), kept for the later stages.
+struct CodeStream {
+    const u32 *pw;
+    int bits;
+};
 struct BwtSeed {
     int cb;
     uint8_t *codes;
@@ -169,9 +174,11 @@ int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
 // not tracked: it is passed whenever n > INT32_MAX); otherwise PLCP is the scratch.
 // pairs (n entries, byte texts with marks only): scratch for the per-slot pair words
 // that replace the mark, SA[j-1] and T[k-1] gathers; U (optional) is the finished BWT.
+// pw/bits (optional): the text's packed code stream, which the pair-word pass
+// compares instead of the bytes.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
                 const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes = 1,
-                u64 *pairs = nullptr, const uint8_t *U = nullptr);
+                u64 *pairs = nullptr, const uint8_t *U = nullptr, const u32 *pw = nullptr, int bits = 0);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // BWT auxiliary samples and inverse BWT (unbwt.hip).

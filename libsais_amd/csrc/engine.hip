@@ -131,6 +131,8 @@ int g_key_stream = 1;
 // 1: with a code stream, the first round orders groups by the packed key of i + h
 // instead of its rank ISA[i + h] (the same order, read from a smaller array)
 int g_raw_round1 = 1;
+// 1: the PLCP pass compares suffixes on the code stream when there is one
+int g_stream_compare = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
 // SA and text gathered separately
 int g_plcp_pairs = 1;
@@ -407,7 +409,7 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
 // launch_bwt_seeded (seed->cb > 0 on return); otherwise seed->cb stays 0.
 int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
              const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0,
-             BwtSeed *seed = nullptr)
+             BwtSeed *seed = nullptr, CodeStream *cs = nullptr)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
@@ -485,6 +487,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 gen = KeyGen{pw, pack_stream_words(n, bits), bits, ksym * bits, cb};
                 use_gen = true;
                 if (cb) seed->pw = pw;
+                if (cs) *cs = CodeStream{pw, bits};
             } else {
                 SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
             }
@@ -618,7 +621,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 }
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-             u32 *PLCP, const u32 *marks, int sym_bytes = 1, const uint8_t *dU = nullptr)
+             u32 *PLCP, const u32 *marks, int sym_bytes = 1, const uint8_t *dU = nullptr,
+             const CodeStream *cs = nullptr)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -641,7 +645,8 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
         pairs = sc.get<u64>(n);
         if (!pairs) return SAGPU_ENOMEM;
     }
-    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes, pairs, dU);
+    return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes, pairs, dU,
+                       cs && g_stream_compare ? cs->pw : nullptr, cs ? cs->bits : 0);
 }
 
 // Copies a text into a pool buffer with 64 zero bytes of padding.
@@ -688,8 +693,9 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         Span total(c, F_TOTAL);
         Scope work(c);  // construction buffers go back to the pool before the caller's downloads
         BwtSeed seed{};
+        CodeStream cs{};
         rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr, nullptr, 0, nullptr, 0,
-                      dU && g_bwt_variant == 0 ? &seed : nullptr);
+                      dU && g_bwt_variant == 0 ? &seed : nullptr, &cs);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
@@ -700,7 +706,8 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (want_lcp) {
             {
                 Span sp(c, F_PLCP);
-                SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks, 1, dU));
+                SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks, 1, dU,
+                                   cs.pw ? &cs : nullptr));
             }
             if (dLCP) {
                 Span sp(c, F_LCP);
@@ -1352,6 +1359,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "stream_compare")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_stream_compare = (int)value;
         return 0;
     }
     if (!strcmp(name, "raw_round1")) {

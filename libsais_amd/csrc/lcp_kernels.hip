@@ -242,6 +242,37 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
 //               BWT in output layout) when given.
 // Built in SA order with sequential reads, so the text-order pass gathers one
 // 8-byte word per position instead of the mark, SA[j-1] and T[k-1] separately.
+// 64 stream bits from bit `bitpos` (the stream has a zero word past every window).
+__device__ __forceinline__ u64 stream_window(const u32 *__restrict__ pw, u64 bitpos)
+{
+    const u64 w = bitpos >> 5;
+    const u32 off = (u32)(bitpos & 31);
+    const u64 hi = ((u64)pw[w] << 32) | (u64)pw[w + 1];
+    return off ? (hi << off) | ((u64)pw[w + 2] >> (32 - off)) : hi;
+}
+
+// lcp of suffixes i and k on the packed code stream, from l (a known lower bound)
+// up to m (the shorter suffix's length): 64 / bits symbols per step.  Stops after
+// `limit` symbols with *done = false, as match_bytes.
+__device__ __forceinline__ u64 match_stream(const u32 *__restrict__ pw, int bits, u64 i, u64 k, u64 l, u64 m,
+                                            u64 limit, bool *done)
+{
+    const u32 per = 64u / (u32)bits;
+    const u64 keep = ~0ull << (64 - per * (u32)bits);
+    const u64 stop_at = l + limit;
+    while (true) {
+        if (l >= m) { *done = true; return m; }
+        const u64 d = (stream_window(pw, (i + l) * (u64)bits) ^ stream_window(pw, (k + l) * (u64)bits)) & keep;
+        if (d) {
+            l += (u64)(__clzll(d) / bits);
+            *done = true;
+            return l < m ? l : m;
+        }
+        l += per;
+        if (l >= stop_at) { *done = (l >= m); return l < m ? l : m; }
+    }
+}
+
 // Byte texts, pair-word form (k_plcp_pairs): per position one ISA read, one 8-byte
 // pair read and the symbol before it; known lcps and reducible positions finish
 // there, irreducible ones compare as k_plcp_irr.  PI positions per thread and step
@@ -252,7 +283,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
                                                         const u32 *__restrict__ ISA, const u64 *__restrict__ pairs,
                                                         VT *__restrict__ V, u32 *__restrict__ queue,
                                                         u32 *__restrict__ queue_l, u32 *__restrict__ queue_n,
-                                                        u64 queue_cap)
+                                                        u64 queue_cap, const u32 *__restrict__ pw, int bits)
 {
     const u64 chunk = 256ull * PI;
     for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
@@ -288,7 +319,9 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
             }
             const u64 m = n - (i > k ? i : k);
             bool done = false;
-            u64 l = match_bytes<1>(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
+            // with the code stream (no GSA), 64 / bits symbols per step from a smaller array
+            u64 l = pw ? match_stream(pw, bits, i, k, lb < m ? lb : m, m, PL_THREAD_LIMIT, &done)
+                       : match_bytes<1>(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
             if (done) {
                 V[i] = (VT)(l + i);
             } else {
@@ -635,7 +668,8 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
 
 template <typename VT, int W>
 int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                  u32 *PLCP, VT *V, const u32 *marks, hipStream_t s, const u64 *pairs = nullptr)
+                  u32 *PLCP, VT *V, const u32 *marks, hipStream_t s, const u64 *pairs = nullptr,
+                  const u32 *pw = nullptr, int bits = 0)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {
@@ -647,7 +681,8 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     if (pairs) {
         if constexpr (W == 1)
             hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3(grid_for(n, 256 * PI, 65535)), dim3(256), 0, s, T, n, gsa,
-                               ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap);
+                               ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, gsa ? nullptr : pw,
+                               bits);
     } else {
         hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA,
                            SA, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
@@ -662,7 +697,8 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
 }
 
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
-                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes, u64 *pairs, const uint8_t *U)
+                u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes, u64 *pairs, const uint8_t *U,
+                const u32 *pw, int bits)
 {
     if (sym_bytes == 2) {
         if (marks || !PHI) {
@@ -686,8 +722,8 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
     } else {
         pairs = nullptr;
     }
-    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s, pairs);
-    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s, pairs);
+    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s, pairs, pw, bits);
+    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s, pairs, pw, bits);
 }
 
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)

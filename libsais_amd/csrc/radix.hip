@@ -536,7 +536,8 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
 
 } // namespace
 
-u64 pack_stream_words(u64 n, int bits) { return (n * (u64)bits + 31) / 32 + 2; }
+// two words cut keys past the end; a third lets 64-bit windows start at any symbol
+u64 pack_stream_words(u64 n, int bits) { return (n * (u64)bits + 31) / 32 + 3; }
 
 int launch_pack_stream(const uint8_t *T, u64 n, const uint8_t *lut, int bits, u32 *pw, hipStream_t s)
 {

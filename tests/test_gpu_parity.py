@@ -255,12 +255,14 @@ def test_keys_written_before_the_sort(name):
 
 
 @pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana", "acgt17"])
-def test_first_round_by_rank(name):
+@pytest.mark.parametrize("option", [b"raw_round1", b"stream_compare"])
+def test_stream_shortcuts_off(name, option):
     """raw_round1 = 0: the first doubling round orders groups by rank (the default
-    orders them by the packed key that follows, read from the code stream)."""
+    orders them by the packed key that follows, read from the code stream);
+    stream_compare = 0: the PLCP pass compares bytes instead of stream symbols."""
     T = dict(CASES)[name]
     lib = la.load()
-    assert lib.sagpu_set_option(b"raw_round1", 0) == 0
+    assert lib.sagpu_set_option(option, 0) == 0
     try:
         rc, SA_ref = ol.sa(T)
         P_ref = ol.plcp(T, SA_ref)
@@ -268,7 +270,7 @@ def test_first_round_by_rank(name):
         assert np.array_equal(fSA, SA_ref) and np.array_equal(fP, P_ref)
         assert np.array_equal(fL, ol.lcp(P_ref, SA_ref))
     finally:
-        assert lib.sagpu_set_option(b"raw_round1", 1) == 0
+        assert lib.sagpu_set_option(option, 1) == 0
 
 
 @pytest.mark.parametrize("n", [2, 3, 5, 4097, 100001])
