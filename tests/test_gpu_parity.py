@@ -46,6 +46,12 @@ def _cases():
     for k in (1, 2, 3, 5, 20, 255, 256):
         n = 50000
         cases.append((f"bytes_k{k}", rng.integers(0, k, n).astype(np.uint8)))
+    # every symbol width of the packed code stream (bits 4, 6, 7, 8) at radix tile
+    # boundaries (8192- and 16384-key tiles)
+    for k, n in ((16, 16383), (16, 16385), (40, 32768), (100, 16384), (129, 8193)):
+        cases.append((f"bytes_k{k}_n{n}", rng.integers(0, k, n).astype(np.uint8)))
+    for n in (16383, 16384, 16385, 49153):
+        cases.append((f"acgt{n}", workloads.iid(n, seed=n, alphabet=b"ACGT")))
     cases.append(("bytes_with_zeros", rng.integers(0, 3, 30000).astype(np.uint8)))
     cases.append(("all_a_100", np.full(100, 65, np.uint8)))
     cases.append(("all_a_70000", np.full(70000, 65, np.uint8)))
