@@ -242,6 +242,12 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
 //               BWT in output layout) when given.
 // Built in SA order with sequential reads, so the text-order pass gathers one
 // 8-byte word per position instead of the mark, SA[j-1] and T[k-1] separately.
+// Prefix max over V (PLCP[i] + i) in tiles of MS_TILE positions.
+constexpr int MS_THREADS = 256, MS_ITEMS = 16, MS_TILE = MS_THREADS * MS_ITEMS;
+
+template <typename VT>
+__device__ __forceinline__ VT vmax(VT a, VT b) { return a > b ? a : b; }
+
 // 64 stream bits from bit `bitpos` (the stream has a zero word past every window).
 __device__ __forceinline__ u64 stream_window(const u32 *__restrict__ pw, u64 bitpos)
 {
@@ -278,64 +284,77 @@ __device__ __forceinline__ u64 match_stream(const u32 *__restrict__ pw, int bits
 // there, irreducible ones compare as k_plcp_irr.  PI positions per thread and step
 // with their loads issued together.
 constexpr int PI = 4;
+// Each workgroup takes whole MS_TILE tiles (PI rows of 256 per step) and writes the
+// tile's max of v to tmax, so the prefix max needs no separate reduce pass
+// (k_plcp_long raises the entries of the positions it finishes).
 template <typename VT>
 __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                         const u32 *__restrict__ ISA, const u64 *__restrict__ pairs,
                                                         VT *__restrict__ V, u32 *__restrict__ queue,
                                                         u32 *__restrict__ queue_l, u32 *__restrict__ queue_n,
-                                                        u64 queue_cap, const u32 *__restrict__ pw, int bits)
+                                                        u64 queue_cap, const u32 *__restrict__ pw, int bits,
+                                                        VT *__restrict__ tmax)
 {
-    const u64 chunk = 256ull * PI;
-    for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
-        u32 r[PI];
-        u64 w[PI];
-        uint8_t a[PI];
+    constexpr u64 STEP = 256ull * PI;
+    static_assert(MS_TILE % STEP == 0, "tile must be whole steps");
+    __shared__ VT s_m[4];
+    const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
+    for (u64 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        VT mx = 0;
+        for (u64 base = tile * MS_TILE; base < (tile + 1) * MS_TILE && base < n; base += STEP) {
+            u32 r[PI];
+            u64 w[PI];
+            uint8_t a[PI];
 #pragma unroll
-        for (int q = 0; q < PI; q++) {
-            const u64 i = base + (u64)q * 256 + threadIdx.x;
-            r[q] = i < n ? ISA[i] : 0u;
-            a[q] = (i > 0 && i < n) ? T[i - 1] : 0;
-        }
+            for (int q = 0; q < PI; q++) {
+                const u64 i = base + (u64)q * 256 + threadIdx.x;
+                r[q] = i < n ? ISA[i] : 0u;
+                a[q] = (i > 0 && i < n) ? T[i - 1] : 0;
+            }
 #pragma unroll
-        for (int q = 0; q < PI; q++) w[q] = r[q] ? pairs[r[q]] : 0ull;
+            for (int q = 0; q < PI; q++) w[q] = r[q] ? pairs[r[q]] : 0ull;
 #pragma unroll
-        for (int q = 0; q < PI; q++) {
-            const u64 i = base + (u64)q * 256 + threadIdx.x;
-            if (i >= n) continue;
-            if (r[q] == 0) {   // first suffix in SA order: lcp 0
-                V[i] = (VT)i;
-                continue;
-            }
-            if (!(w[q] >> 63)) {
-                V[i] = (VT)((u64)(u32)w[q] + i);
-                continue;
-            }
-            const u64 k = (u32)w[q];
-            const u32 lg = (u32)(w[q] >> 58) & 31u;
-            const u64 lb = lg ? (1ull << (lg - 1)) : 0;
-            if (i > 0 && k > 0 && a[q] == (uint8_t)(w[q] >> 32) && !(gsa && a[q] == 0)) {
-                V[i] = 0;   // reducible: PLCP[i] = PLCP[i-1] - 1
-                continue;
-            }
-            const u64 m = n - (i > k ? i : k);
-            bool done = false;
-            // with the code stream (no GSA), 64 / bits symbols per step from a smaller array
-            u64 l = pw ? match_stream(pw, bits, i, k, lb < m ? lb : m, m, PL_THREAD_LIMIT, &done)
-                       : match_bytes<1>(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
-            if (done) {
-                V[i] = (VT)(l + i);
-            } else {
-                u32 slot = atomicAdd(queue_n, 1u);
-                if (slot < queue_cap) {
-                    queue[slot] = (u32)i;
-                    queue_l[slot] = (u32)l;
-                    V[i] = (VT)(l + i);   // provisional; overwritten by k_plcp_long
+            for (int q = 0; q < PI; q++) {
+                const u64 i = base + (u64)q * 256 + threadIdx.x;
+                if (i >= n) continue;
+                VT v;
+                if (r[q] == 0) {   // first suffix in SA order: lcp 0
+                    v = (VT)i;
+                } else if (!(w[q] >> 63)) {
+                    v = (VT)((u64)(u32)w[q] + i);
                 } else {
-                    l = match_bytes<1>(T, i, k, l, m, gsa, ~0ull >> 1, &done);
-                    V[i] = (VT)(l + i);
+                    const u64 k = (u32)w[q];
+                    const u32 lg = (u32)(w[q] >> 58) & 31u;
+                    const u64 lb = lg ? (1ull << (lg - 1)) : 0;
+                    if (i > 0 && k > 0 && a[q] == (uint8_t)(w[q] >> 32) && !(gsa && a[q] == 0)) {
+                        v = 0;   // reducible: PLCP[i] = PLCP[i-1] - 1
+                    } else {
+                        const u64 m = n - (i > k ? i : k);
+                        bool done = false;
+                        // with the code stream (no GSA), 64 / bits symbols per step
+                        u64 l = pw ? match_stream(pw, bits, i, k, lb < m ? lb : m, m, PL_THREAD_LIMIT, &done)
+                                   : match_bytes<1>(T, i, k, lb < m ? lb : m, m, gsa, PL_THREAD_LIMIT, &done);
+                        if (!done) {
+                            u32 slot = atomicAdd(queue_n, 1u);
+                            if (slot < queue_cap) {
+                                queue[slot] = (u32)i;   // provisional value; k_plcp_long finishes it
+                                queue_l[slot] = (u32)l;
+                            } else {
+                                l = match_bytes<1>(T, i, k, l, m, gsa, ~0ull >> 1, &done);
+                            }
+                        }
+                        v = (VT)(l + i);
+                    }
                 }
+                V[i] = v;
+                mx = vmax(mx, v);
             }
         }
+        for (int d = 32; d >= 1; d >>= 1) mx = vmax(mx, (VT)__shfl_xor(mx, d, SAGPU_WAVE));
+        if (lane_id() == 0) s_m[threadIdx.x / SAGPU_WAVE] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) tmax[tile] = vmax(vmax(s_m[0], s_m[1]), vmax(s_m[2], s_m[3]));
+        __syncthreads();
     }
 }
 
@@ -390,7 +409,8 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                    const u32 *__restrict__ SA, VT *__restrict__ V,
                                                    const u32 *__restrict__ queue, const u32 *__restrict__ queue_l,
-                                                   const u32 *__restrict__ queue_n, u64 queue_cap)
+                                                   const u32 *__restrict__ queue_n, u64 queue_cap,
+                                                   VT *__restrict__ tmax)
 {
     __shared__ u64 s_min[4];
     __shared__ u64 s_res;
@@ -438,7 +458,11 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             u64 r = s_res;
             __syncthreads();
             if (r != ~0ull) {
-                if (threadIdx.x == 0) V[i] = (VT)(r / W + i);
+                if (threadIdx.x == 0) {
+                    V[i] = (VT)(r / W + i);
+                    // tile maxima written by the pass that queued i (pair-word form)
+                    if (tmax) atomicMax(&tmax[i / MS_TILE], (VT)(r / W + i));
+                }
                 break;
             }
             l += 256 * 8;
@@ -448,10 +472,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
 
 // Prefix max over V, then PLCP[i] = max - i.  VT = u32 works in place (V is the
 // PLCP array); VT = u64 is used when PLCP[i] + i can reach 2^32.
-constexpr int MS_THREADS = 256, MS_ITEMS = 16, MS_TILE = MS_THREADS * MS_ITEMS;
 
-template <typename VT>
-__device__ __forceinline__ VT vmax(VT a, VT b) { return a > b ? a : b; }
 
 template <typename VT>
 __device__ __forceinline__ VT wave_incl_max_t(VT v)
@@ -678,18 +699,19 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     }
     VT *tmax = (VT *)ls.tile_max;
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
-    if (pairs) {
+    const bool fused_max = pairs && W == 1;   // the pair-word pass writes the tile maxima itself
+    if (fused_max) {
         if constexpr (W == 1)
-            hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3(grid_for(n, 256 * PI, 65535)), dim3(256), 0, s, T, n, gsa,
-                               ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, gsa ? nullptr : pw,
-                               bits);
+            hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3((unsigned)(tiles < 65535 ? tiles : 65535)), dim3(256), 0, s,
+                               T, n, gsa, ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap,
+                               gsa ? nullptr : pw, bits, tmax);
     } else {
         hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA,
                            SA, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
     }
     hipLaunchKernelGGL((k_plcp_long<VT, W>), dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
-                       ls.queue_n, ls.queue_cap);
-    hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
+                       ls.queue_n, ls.queue_cap, fused_max ? tmax : (VT *)nullptr);
+    if (!fused_max) hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
     hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
     hipLaunchKernelGGL(k_max_apply<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax, PLCP);
     SAGPU_CHECK(hipGetLastError());
