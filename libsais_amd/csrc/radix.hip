@@ -47,9 +47,14 @@ template <typename K, bool GEN>
 __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, KeyGen gen, u64 n, int begin_bit,
                                                     int end_bit, int npass, u64 *__restrict__ hist)
 {
-    __shared__ u32 s_h[8][256];
-    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&s_h[0][0])[i] = 0;
+    // 32-bit keys (at most 4 passes): 4 copies per digit position, picked by lane % 4,
+    // so skewed digits (DNA) contend less on the same LDS words
+    constexpr int C = sizeof(K) == 4 ? 4 : 1;
+    constexpr int NP = sizeof(K) == 4 ? 4 : 8;
+    __shared__ u32 s_hc[NP * C][256];
+    for (int i = threadIdx.x; i < NP * C * 256; i += blockDim.x) (&s_hc[0][0])[i] = 0;
     __syncthreads();
+    const u32 copy = threadIdx.x & (C - 1);
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         K k;
@@ -59,12 +64,14 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
             const int sh = begin_bit + 8 * p;
             const u32 dm = end_bit - sh >= 8 ? 255u : ((1u << (end_bit - sh)) - 1u);
             u32 d = (u32)(k >> sh) & dm;
-            atomicAdd(&s_h[p][d], 1u);
+            atomicAdd(&s_hc[p * C + copy][d], 1u);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < npass * 256; i += blockDim.x) {
-        u32 v = (&s_h[0][0])[i];
+        const int p = i / 256, d = i % 256;
+        u32 v = 0;
+        for (int c = 0; c < C; c++) v += s_hc[p * C + c][d];
         if (v) atomicAdd(&hist[i], (u64)v);
     }
 }

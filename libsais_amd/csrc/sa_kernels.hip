@@ -10,10 +10,12 @@ namespace {
 __global__ __launch_bounds__(256) void k_histogram(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                    u64 *__restrict__ hist, u32 *__restrict__ bad)
 {
-    __shared__ u32 s_h[4][256];
-    for (int i = threadIdx.x; i < 4 * 256; i += 256) (&s_h[0][0])[i] = 0;
+    // 16 sub-histograms: lanes of a wave spread over 4 copies (lane % 4), so a text of
+    // few distinct bytes (DNA) has a quarter of the same-word atomics per step
+    __shared__ u32 s_h[16][256];
+    for (int i = threadIdx.x; i < 16 * 256; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const u32 w = threadIdx.x / SAGPU_WAVE;
+    const u32 w = (threadIdx.x / SAGPU_WAVE) * 4 + (threadIdx.x & 3);
     u32 nbad = 0;
     const u64 nvec = n / 16;
     const uint4 *T16 = reinterpret_cast<const uint4 *>(T);
@@ -40,7 +42,8 @@ __global__ __launch_bounds__(256) void k_histogram(const uint8_t *__restrict__ T
     }
     __syncthreads();
     for (int c = threadIdx.x; c < 256; c += 256) {
-        u32 t = s_h[0][c] + s_h[1][c] + s_h[2][c] + s_h[3][c];
+        u32 t = 0;
+        for (int q = 0; q < 16; q++) t += s_h[q][c];
         if (t) atomicAdd(&hist[c], (u64)t);
     }
     if (gsa && nbad) atomicAdd(bad, nbad);
