@@ -1644,9 +1644,11 @@ double now_s()
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Device bytes one build of an n-byte text may hold at once (buffers of sa_build,
-// outputs and staging), used for admission.
-u64 job_footprint(u64 n) { return 72ull * n + (64ull << 20); }
+// Device bytes one build of an n-byte text may hold at once, used for admission:
+// text, SA, ISA, LCP marks and PLCP (17 n), the three 8 n sort/list buffers (24 n),
+// code stream, BWT seeds and radix look-back (about 3 n), PLCP pair words (8.5 n),
+// and up to 24 n of big-group sort buffers in one round; plus staging.
+u64 job_footprint(u64 n) { return 80ull * n + (64ull << 20); }
 
 int run_tiny(sagpu_job *j)
 {
