@@ -77,23 +77,24 @@ struct WsScratch {
     u32 *bcur;        // [64] ISA bucket cursors (initial grouping)
 };
 
-// BWT seeding: the initial sort's values carry (suffix << cb) | code of the symbol
-// before the suffix (launch_pack_keys32 with cb > 0).  group_init then stores that
-// code per SA slot (codes, n bytes) and a bitmap of the slots still in groups
-// (abits, n/64 words); launch_bwt_seeded finishes the BWT after the rounds,
-// reading the text only at those slots.  inv: code -> byte (256 bytes, device).
 // The packed code stream of a byte text (KeyGen), kept for the later stages.
 struct CodeStream {
     const u32 *pw;
     int bits;
 };
+// BWT seeding: the initial sort's values carry (suffix << cb) | code of the symbol
+// before the suffix, the active lists keep that packed form, and every stage that
+// places a suffix in SA slot j writes its code to codes[j] (n bytes), so the BWT is
+// one sequential pass over codes (launch_bwt_seeded).  inv: code -> byte (256 bytes).
 struct BwtSeed {
     int cb;
     uint8_t *codes;
-    u64 *abits;
     const uint8_t *inv;
-    const u32 *pw;    // packed code stream of the text (KeyGen), or null: grouped slots
-                      // then read the code of T[SA[j]-1] there instead of the text
+};
+// What the rounds need of it: cb = 0 and codes = null when the build is not seeded.
+struct ActCode {
+    int cb;
+    uint8_t *codes;
 };
 // PV, PG: scratch of n entries each ((suffix, group head) pairs bucketed for the
 // ISA scatter).
@@ -117,14 +118,14 @@ struct RawKey1 {
 };
 // Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s, const RawKey1 *rk = nullptr);
+                 u32 *SA, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
                u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s);
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac);
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
                 u64 *comp, u32 *vals, hipStream_t s);
 int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s);
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);

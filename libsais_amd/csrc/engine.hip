@@ -426,8 +426,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     }
     if (want_seed && ((n - 1) >> (32 - bits)) == 0) {
         GET(codes, uint8_t, n);
-        GET(abits, u64, (n + 63) / 64);
-        *seed = BwtSeed{bits, codes, abits, d_inv, nullptr};
+        *seed = BwtSeed{bits, codes, d_inv};
     }
     const int cb = seed ? seed->cb : 0;
     st.initial_symbols = ksym;
@@ -486,7 +485,6 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 SAGPU_TRY(launch_pack_stream(dT, n, d_lut, bits, pw, s));
                 gen = KeyGen{pw, pack_stream_words(n, bits), bits, ksym * bits, cb};
                 use_gen = true;
-                if (cb) seed->pw = pw;
                 if (cs) *cs = CodeStream{pw, bits};
             } else {
                 SAGPU_TRY(launch_pack_keys32(dT, n, d_lut, bits, ksym, gsa, cb, k0, v0, s));
@@ -530,6 +528,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     SAGPU_CHECK(hipStreamSynchronize(s));
     a = c.pinned[0];
 
+    const ActCode ac{cb, cb ? seed->codes : nullptr};   // list entries carry BWT codes when seeded
     u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
@@ -542,10 +541,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         {
             Span sp(c, F_ROUNDS);
             SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s,
-                                   st.rounds == 1 && rk1.pw ? &rk1 : nullptr));
+                                   st.rounds == 1 && rk1.pw ? &rk1 : nullptr, ac.cb));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
-                                 out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s));
+                                 out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac));
         }
         SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
@@ -601,7 +600,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             {
                 Span sp(c, F_ROUNDS);
                 SAGPU_TRY(big_emit(ws, alt ? b1 : b0, alt ? bv1 : bv0, d_bslot, total, h, wide, dSA, dISA, act_i[nxt],
-                                   act_g[nxt], out_counter, dLCP, s));
+                                   act_g[nxt], out_counter, dLCP, s, ac));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));

@@ -35,15 +35,14 @@ __global__ __launch_bounds__(256) void k_bwt(const ST *__restrict__ T, const u32
     }
 }
 
-// BWT from the seeds left by the initial grouping: output byte p comes from SA slot
-// j = p - 1 (p <= j0) or j = p (p > j0), U[0] = T[n-1].  Slots that were alone
-// after the initial sort already hold their symbol's code (codes[j]); only the
-// slots flagged in abits read the text at SA[j] - 1.  Four output bytes per lane.
-__global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ T, const u32 *__restrict__ SA,
-                                                    const u32 *__restrict__ ISA, const uint8_t *__restrict__ codes,
-                                                    const u64 *__restrict__ abits, const uint8_t *__restrict__ inv,
-                                                    const u32 *__restrict__ pw, int bits, u64 n,
-                                                    uint8_t *__restrict__ U, int aligned)
+// BWT from the seeds: every stage that placed a suffix in SA slot j wrote the code
+// of its preceding symbol to codes[j], so output byte p is inv[codes[j]] with
+// j = p - 1 (p <= j0) or j = p (p > j0), and U[0] = T[n-1].  One sequential pass,
+// four output bytes per lane.
+__global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ T, const u32 *__restrict__ ISA,
+                                                    const uint8_t *__restrict__ codes,
+                                                    const uint8_t *__restrict__ inv, u64 n, uint8_t *__restrict__ U,
+                                                    int aligned)
 {
     __shared__ uint8_t s_inv[256];
     s_inv[threadIdx.x] = inv[threadIdx.x];
@@ -53,36 +52,13 @@ __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ 
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
         const u64 p0 = 4 * q;
-        u64 j[4];
-        bool grouped[4];
-        uint8_t code[4];
-        u32 sa[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const u64 p = p0 + k;
-            j[k] = p <= j0 ? p - 1 : p;   // p = 0 is U[0]
-            const bool ok = p >= 1 && p < n;
-            grouped[k] = ok && ((abits[j[k] >> 6] >> (j[k] & 63)) & 1);
-            code[k] = ok ? codes[j[k]] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) sa[k] = grouped[k] ? SA[j[k]] : 1u;
         u32 word = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const u64 p = p0 + k;
-            uint8_t b;
-            if (p == 0) {
-                b = T[n - 1];
-            } else if (grouped[k] && pw) {
-                // the code of T[sa - 1] from the packed stream (a smaller array to gather from)
-                const u64 bitpos = (u64)(sa[k] - 1u) * (u64)bits;
-                const u64 w = bitpos >> 5;
-                const u64 x = ((u64)pw[w] << 32) | (u64)pw[w + 1];
-                b = s_inv[(u32)((x << (bitpos & 31)) >> (64 - bits))];
-            } else {
-                b = grouped[k] ? T[sa[k] - 1u] : s_inv[code[k]];
-            }
+            uint8_t b = 0;
+            if (p == 0) b = T[n - 1];
+            else if (p < n) b = s_inv[codes[p <= j0 ? p - 1 : p]];
             word |= (u32)b << (8 * k);
         }
         if (aligned && p0 + 4 <= n) {
@@ -622,8 +598,9 @@ int launch_bwt_seeded(const uint8_t *T, const u32 *SA, const u32 *ISA, const Bwt
                       hipStream_t s)
 {
     const int aligned = ((uintptr_t)U & 3) == 0;
-    hipLaunchKernelGGL(k_bwt_seeded, dim3(grid_for((n + 3) / 4, 256 * 4, 65535)), dim3(256), 0, s, T, SA, ISA,
-                       seed.codes, seed.abits, seed.inv, seed.pw, seed.cb, n, U, aligned);
+    (void)SA;
+    hipLaunchKernelGGL(k_bwt_seeded, dim3(grid_for((n + 3) / 4, 256 * 4, 65535)), dim3(256), 0, s, T, ISA, seed.codes,
+                       seed.inv, n, U, aligned);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -185,11 +185,10 @@ struct InitF {
     u32 *PV, *PG, *bcur;
     int wlog;
     // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
-    // it.  codes[e] receives that code and bit e of abits is set when slot e is
-    // still in a group (its occupant, and so its BWT symbol, is decided later).
+    // it.  codes[e] receives that code; entries of the active list keep the packed
+    // form, and the rounds write the code of every suffix they place.
     int cb;
     uint8_t *codes;
-    u64 *abits;
 
     __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
     // Group flags of element e from its key and its neighbours' keys.
@@ -376,10 +375,6 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         const u32 is = wave_incl_sum((u32)act);
         const u32 g = umax(c.m1, wave_incl_max((st[r] >> 30) & 1u ? (u32)e : 0u));
         const u64 o = c.s + is - (u64)act;
-        if (f.abits) {
-            const u64 m = __ballot(act);   // rows are 64-aligned
-            if (lane == 0 && e < n) f.abits[e >> 6] = m;
-        }
         K kp = 0, kn;
         if (f.LCP) sk.neighbours(r, kp, kn);
         if (e < n) {
@@ -392,7 +387,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
             // overwritten when their suffixes are placed
             f.SA[e] = v;
             if (act) {
-                f.act_i[o] = v;
+                f.act_i[o] = vv[r];   // with the BWT seed the code travels in the list entry
                 f.act_g[o] = g;
             }
             if (f.LCP) f.LCP[e] = e > 0 ? f.lcp_mark((u32)sk.k[r], (u32)kp) : 0u;
@@ -429,12 +424,13 @@ __global__ __launch_bounds__(256) void k_isa_scatter(const u32 *__restrict__ PV,
 // ------------------------------------------------------- big-group emission
 struct BigF {
     const u64 *comp;     // (dense big-group index << kb) | key2, sorted
-    const u32 *vals;     // suffix index
+    const u32 *vals;     // list entry: suffix index, or (suffix << cb) | BWT code
     const u32 *gslot;    // first SA slot of each big group
     u64 n;
     u32 h;
     int kb;              // key2 width: 32, or 33 for texts longer than 2^31 - 1
     u32 *SA, *ISA, *out_i, *out_g, *LCP;
+    ActCode ac;
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
@@ -456,16 +452,18 @@ struct BigF {
     __device__ void emit(u64 e, bool act, u64 o, u32 gstart, u32 hstart) const
     {
         if (e >= n) return;
-        u32 v = vals[e];
+        const u32 pv = vals[e];
+        const u32 v = pv >> ac.cb;
         u64 c = comp[e];
         u32 g = gslot[(u32)(c >> kb)];
         u32 r = g + (hstart - gstart);
         if (hstart != gstart) ISA[v] = r;  // the first subgroup keeps the group's rank
         if (act) {
-            out_i[o] = v;
+            out_i[o] = pv;
             out_g[o] = r;
         } else {
             SA[r] = v;
+            if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
         }
         if (LCP && hstart == (u32)e && hstart != gstart) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
     }
@@ -495,7 +493,8 @@ constexpr int IG = 4;
 template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                       u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
-                                                      KT *__restrict__ key2, u32 *__restrict__ SA, RawKey1 rk)
+                                                      KT *__restrict__ key2, u32 *__restrict__ SA, RawKey1 rk,
+                                                      int cb)
 {
     const u64 chunk = 256ull * IG;
     for (u64 base = (u64)blockIdx.x * chunk; base < a; base += (u64)gridDim.x * chunk) {
@@ -504,7 +503,7 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
 #pragma unroll
         for (int q = 0; q < IG; q++) {
             const u64 k = base + (u64)q * 256 + threadIdx.x;
-            i[q] = k < a ? act_i[k] : 0u;
+            i[q] = k < a ? act_i[k] >> cb : 0u;
             g[q] = k < a ? act_g[k] : 0u;
             gn[q] = k + 1 < a ? act_g[k + 1] : ~g[q];
         }
@@ -553,7 +552,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                                                            u64 *__restrict__ out_counter, u32 *__restrict__ err,
                                                            u32 *__restrict__ big_head, u32 *__restrict__ big_g,
                                                            u32 *__restrict__ big_size, u32 *__restrict__ big_counter,
-                                                           u64 big_cap, u32 *__restrict__ LCP)
+                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac)
 {
     constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
     static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
@@ -665,8 +664,12 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     for (u32 x = tid; x < len; x += RT_THREADS) {
         u32 st = s_os[x];
         if (st == 0) continue;
-        if (!(st & 4)) ISA[s_oi[x]] = s_og[x];
-        if ((st & 3) == 1) SA[s_og[x]] = s_oi[x];
+        const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
+        if (!(st & 4)) ISA[v] = s_og[x];
+        if ((st & 3) == 1) {
+            SA[s_og[x]] = v;
+            if (ac.codes) ac.codes[s_og[x]] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
+        }
     }
 
     // 4. survivors, in sorted order, to the next active list
@@ -745,7 +748,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     int wlog = g_isa_window_log2;
     while ((((n - 1) >> wlog) + 1) > (u64)IB_MAX) wlog++;
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
-               cb, seed ? seed->codes : nullptr, seed ? seed->abits : nullptr};
+               cb, seed ? seed->codes : nullptr};
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
@@ -778,10 +781,10 @@ int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bit
 
 template <typename KT>
 int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
-                   hipStream_t s, const RawKey1 &rk)
+                   hipStream_t s, const RawKey1 &rk, int cb)
 {
     hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
-                       n, h, key2, SA, rk);
+                       n, h, key2, SA, rk, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -789,13 +792,13 @@ int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u6
 template <typename KT>
 int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
                  u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter,
-                 u64 big_cap, u32 *LCP, hipStream_t s)
+                 u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
 #define RT_LAUNCH(R)                                                                                              \
     hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, \
                        hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,  \
-                       LCP)
+                       LCP, ac)
     if (g_tile_range == 256) RT_LAUNCH(256);
     else if (g_tile_range == 1024) RT_LAUNCH(1024);
     else RT_LAUNCH(512);
@@ -815,21 +818,21 @@ int big_extract_t(const u32 *act_i, const KT *key2, const u32 *bhead, const u64 
 }
 
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s, const RawKey1 *rk)
+                 u32 *SA, hipStream_t s, const RawKey1 *rk, int cb)
 {
     const RawKey1 r = rk ? *rk : RawKey1{};
-    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s, r)
-                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s, r);
+    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s, r, cb)
+                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s, r, cb);
 }
 
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
                u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s)
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac)
 {
     return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s)
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac)
                 : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s);
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac);
 }
 
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
@@ -840,9 +843,9 @@ int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, 
 }
 
 int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s)
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP};
+    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP, ac};
     return ws_run(ws, f, total, out_counter, s);
 }

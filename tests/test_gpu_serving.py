@@ -74,7 +74,7 @@ def test_admission_rejects_job_that_never_fits():
 def test_tight_budget_serialises_but_completes():
     reqs = [serving.Request(workloads.markov(1 << 20, seed=q, order=2), q % 3) for q in range(8)]
     alone = [serving.Request(r.T, r.kind) for r in reqs]
-    budget = 72 * ((1 << 20) + 1) + (64 << 20) + 4096  # room for exactly one such job
+    budget = 80 * ((1 << 20) + 1) + (64 << 20) + 4096  # room for exactly one such job (80 n + 64 MiB each)
     with serving.Server(slots=4, hbm_budget=budget, batch_bytes=0) as sv:
         for r in reqs:
             sv.submit(r)
