@@ -181,6 +181,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    packed key that follows (same order as its rank; default); 0 = by rank
  *   "stream_compare" 1 = with key_stream, the PLCP pass compares suffixes on the packed code
  *                    stream, 64 / bits symbols per step (default); 0 = on the bytes
+ *   "bucketed_phi"   1 = Φ of a caller's suffix array (libsais_plcp and the like) through
+ *                    pairs bucketed by target, scattered bucket after bucket (default); 0 = one pass
  *   "plcp_pairs"     1 = the fused PLCP pass reads one 8-byte pair word per position, built in
  *                    SA order (default); 0 = it gathers the mark, SA[j-1] and T[k-1] separately
  *   "isa_window_log2" ISA scatter after the initial sort: (suffix, rank) pairs are bucketed by

@@ -103,6 +103,8 @@ int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits,
 // ISA scatter after the initial sort: buckets of 2^v suffixes (v in [10, 32]; at most
 // 64 buckets, larger ones for longer texts).
 void set_isa_window_log2(int v);
+// log2 of the bucket size used for an n-entry permutation scatter (at most 64 buckets).
+int isa_bucket_log2(u64 n);
 // List entries per workgroup of the tile round kernel: 256, 512 (default) or 1024.
 void set_tile_range(int v);
 // Batched GSA: keys = (string id << 32) | packed symbols.
@@ -168,6 +170,10 @@ int launch_bwt16(const uint16_t *T, const u32 *SA, const u32 *ISA, u64 n, uint16
 // sequentially and scattering bytes.
 int launch_bwt_scatter(const uint8_t *T, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);
 int launch_phi(const u32 *SA, u64 n, u32 *PHI, hipStream_t s);
+// The same Φ through (target, value) pairs bucketed by target (PV, PG: n entries
+// each; bcur: 64 cursors, cleared here) and scattered bucket after bucket, so the
+// random writes of the device stay inside one 2^wlog-entry stretch of PHI at a time.
+int launch_phi_bucketed(const u32 *SA, u64 n, u32 *PHI, u32 *PV, u32 *PG, u32 *bcur, int wlog, hipStream_t s);
 // PLCP from either PHI (ISA == nullptr) or ISA+SA (phi computed on the fly).
 // marks (SA order, with ISA+SA only): exact lcps or LCP_UNKNOWN | lower bound from
 // the construction; position i reads marks[ISA[i]] and only unknown ones compare.

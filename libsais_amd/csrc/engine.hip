@@ -133,6 +133,8 @@ int g_key_stream = 1;
 int g_raw_round1 = 1;
 // 1: the PLCP pass compares suffixes on the code stream when there is one
 int g_stream_compare = 1;
+// 1: Φ of a caller's suffix array by the bucketed two-pass scatter; 0: one pass
+int g_bucketed_phi = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
 // SA and text gathered separately
 int g_plcp_pairs = 1;
@@ -619,6 +621,18 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     return SAGPU_OK;
 }
 
+// Φ of a suffix array given by the caller: bucketed two-pass scatter (launch_phi_bucketed)
+// or, with the bucketed_phi option off, one unordered scatter.
+int phi_run(Ctx &c, Scope &sc, const u32 *dSA, u64 n, u32 *dPHI)
+{
+    if (n == 0) return SAGPU_OK;
+    if (!g_bucketed_phi) return launch_phi(dSA, n, dPHI, c.stream);
+    GET(pv, u32, n);
+    GET(pg, u32, n);
+    GET(bcur, u32, 64);
+    return launch_phi_bucketed(dSA, n, dPHI, pv, pg, bcur, isa_bucket_log2(n), c.stream);
+}
+
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
              u32 *PLCP, const u32 *marks, int sym_bytes = 1, const uint8_t *dU = nullptr,
              const CodeStream *cs = nullptr)
@@ -926,7 +940,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void
     u32 *dPHI = sc.get<u32>((u64)n);
     u32 *dP = sc.get<u32>((u64)n);
     if (!dPHI || !dP) return SAGPU_ENOMEM;
-    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(phi_run(*c, sc, dSA, (u64)n, dPHI));
     SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, 0, dPHI, nullptr, nullptr, dP, nullptr, 4));
     SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
@@ -1189,7 +1203,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp16(const uint16_t *T, const void 
     u32 *dPHI = sc.get<u32>((u64)n);
     u32 *dP = sc.get<u32>((u64)n);
     if (!dPHI || !dP) return SAGPU_ENOMEM;
-    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(phi_run(*c, sc, dSA, (u64)n, dPHI));
     SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, nullptr, 2));
     SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
@@ -1229,7 +1243,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
     u32 *dPHI = sc.get<u32>((u64)n);
     u32 *dP = sc.get<u32>((u64)n);
     if (!dPHI || !dP) return SAGPU_ENOMEM;
-    SAGPU_TRY(launch_phi(dSA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(phi_run(*c, sc, dSA, (u64)n, dPHI));
     SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, gsa, dPHI, nullptr, nullptr, dP, nullptr));
     SAGPU_TRY(download_idx(*c, sc, dP, PLCP, (u64)n, width));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
@@ -1358,6 +1372,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "bucketed_phi")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_bucketed_phi = (int)value;
         return 0;
     }
     if (!strcmp(name, "stream_compare")) {
@@ -1555,7 +1574,7 @@ extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t
     SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
     u32 *dPHI = sc.get<u32>((u64)n);
     if (!dPHI) return SAGPU_ENOMEM;
-    SAGPU_TRY(launch_phi((const u32 *)d_SA, (u64)n, dPHI, c->stream));
+    SAGPU_TRY(phi_run(*c, sc, (const u32 *)d_SA, (u64)n, dPHI));
     SAGPU_TRY(plcp_run(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, dPHI, nullptr, nullptr, (u32 *)d_PLCP, nullptr));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;

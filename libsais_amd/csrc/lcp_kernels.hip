@@ -98,6 +98,79 @@ __global__ __launch_bounds__(256) void k_phi(const u32 *__restrict__ SA, u64 n, 
     }
 }
 
+// Φ in two passes.  k_phi_bucket: per SA slot j, the pair (SA[j], SA[j-1]) goes to
+// the bucket of its target SA[j] (rank inside the workgroup's share by an LDS atomic,
+// the share reserved with one atomic per bucket).  A target outside [0, n) is
+// skipped, and a bucket that receives more pairs than it has entries (a caller's SA
+// with repeated values) drops the excess: Φ of an invalid SA is not defined, and
+// nothing is written out of bounds.
+constexpr int PB_ROWS = 16, PB_TILE = 256 * PB_ROWS, PB_MAXB = 64;
+__global__ __launch_bounds__(256) void k_phi_bucket(const u32 *__restrict__ SA, u64 n, int wlog,
+                                                    u32 *__restrict__ PV, u32 *__restrict__ PG,
+                                                    u32 *__restrict__ bcur)
+{
+    __shared__ u32 s_cnt[PB_MAXB];
+    __shared__ u64 s_base[PB_MAXB];
+    if (threadIdx.x < PB_MAXB) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * PB_TILE;
+    u32 p[PB_ROWS], val[PB_ROWS], rk[PB_ROWS];
+#pragma unroll
+    for (int r = 0; r < PB_ROWS; r++) {
+        const u64 j = base + (u64)r * 256 + threadIdx.x;
+        p[r] = ~0u;
+        if (j < n) {
+            const u32 x = SA[j];
+            if (x < n) {
+                p[r] = x;
+                val[r] = j == 0 ? x : SA[j - 1];   // the first suffix points at itself
+                rk[r] = atomicAdd(&s_cnt[(u32)((u64)x >> wlog)], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const u32 nb = (u32)(((n - 1) >> wlog) + 1);
+    if (threadIdx.x < nb && s_cnt[threadIdx.x]) s_base[threadIdx.x] = atomicAdd(&bcur[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PB_ROWS; r++) {
+        if (p[r] == ~0u) continue;
+        const u32 b = (u32)((u64)p[r] >> wlog);
+        const u64 lo = (u64)b << wlog;
+        const u64 cap = n - lo < (1ull << wlog) ? n - lo : (1ull << wlog);
+        const u64 pos = s_base[b] + rk[r];
+        if (pos < cap) {
+            PV[lo + pos] = p[r];
+            PG[lo + pos] = val[r];
+        }
+    }
+}
+
+// k_phi_scatter: PHI[PV[e]] = PG[e] over the filled part of each bucket, one sweep
+// over the bucket array in order: the workgroups in flight cover a few consecutive
+// entries of one bucket, so the device's random writes stay in one stretch of PHI.
+__global__ __launch_bounds__(256) void k_phi_scatter(const u32 *__restrict__ PV, const u32 *__restrict__ PG,
+                                                     const u32 *__restrict__ bcur, u64 n, int wlog,
+                                                     u32 *__restrict__ PHI)
+{
+    const u64 chunk = 256ull * 8;
+    for (u64 e0 = (u64)blockIdx.x * chunk; e0 < n; e0 += (u64)gridDim.x * chunk) {
+        u32 v[8], g[8];
+        bool ok[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const u64 e = e0 + (u64)k * 256 + threadIdx.x;
+            const u64 b = e >> wlog;
+            ok[k] = e < n && (e - (b << wlog)) < (u64)bcur[b];   // filled part of bucket b
+            v[k] = ok[k] ? PV[e] : 0u;
+            g[k] = ok[k] ? PG[e] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (ok[k]) PHI[v[k]] = g[k];
+    }
+}
+
 // 8 bytes starting at an arbitrary address; the text buffer carries >= 16 bytes of
 // zero padding so both aligned words are in bounds.
 __device__ __forceinline__ u64 load8u(const uint8_t *p)
@@ -615,6 +688,20 @@ int launch_bwt16(const uint16_t *T, const u32 *SA, const u32 *ISA, u64 n, uint16
 int launch_bwt_scatter(const uint8_t *T, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bwt_scatter, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, ISA, n, U);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_phi_bucketed(const u32 *SA, u64 n, u32 *PHI, u32 *PV, u32 *PG, u32 *bcur, int wlog, hipStream_t s)
+{
+    if ((((n - 1) >> wlog) + 1) > (u64)PB_MAXB) {
+        sagpu_set_error("phi: too many buckets", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    SAGPU_TRY(launch_fill_u32(bcur, PB_MAXB, 0, s));
+    hipLaunchKernelGGL(k_phi_bucket, dim3((unsigned)((n + PB_TILE - 1) / PB_TILE)), dim3(256), 0, s, SA, n, wlog, PV,
+                       PG, bcur);
+    hipLaunchKernelGGL(k_phi_scatter, dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, PV, PG, bcur, n, wlog, PHI);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -744,9 +744,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr)
 {
     const int cb = seed ? seed->cb : 0;
-    // ISA buckets of 2^wlog suffixes, at most IB_MAX of them
-    int wlog = g_isa_window_log2;
-    while ((((n - 1) >> wlog) + 1) > (u64)IB_MAX) wlog++;
+    const int wlog = isa_bucket_log2(n);
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
                cb, seed ? seed->codes : nullptr};
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
@@ -765,6 +763,14 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
 }
 
 void set_isa_window_log2(int v) { g_isa_window_log2 = v; }
+
+int isa_bucket_log2(u64 n)
+{
+    // buckets of 2^wlog entries, at most IB_MAX of them
+    int wlog = g_isa_window_log2;
+    while ((((n - 1) >> wlog) + 1) > (u64)IB_MAX) wlog++;
+    return wlog;
+}
 void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,

@@ -279,6 +279,31 @@ def test_stream_shortcuts_off(name, option):
         assert lib.sagpu_set_option(option, 1) == 0
 
 
+@pytest.mark.parametrize("bucketed", [1, 0])
+def test_plcp_api_phi_forms(bucketed):
+    """libsais_plcp computes Φ of the caller's SA with the bucketed two-pass scatter
+    (default) or one pass; both match the oracle.  An SA with repeated and
+    out-of-range entries has no defined PLCP: the call still returns 0 and touches
+    nothing out of bounds (both kernels skip or drop such entries)."""
+    lib = la.load()
+    assert lib.sagpu_set_option(b"bucketed_phi", bucketed) == 0
+    try:
+        for name in ("acgt200003", "dna_like_1m", "all_a_70000", "bytes_k256"):
+            T = dict(CASES)[name]
+            rc, SA = ol.sa(T)
+            P = np.zeros(len(T), np.int32)
+            assert la.libsais_plcp(T, SA, P, len(T)) == 0
+            assert np.array_equal(P, ol.plcp(T, SA))
+        T = dict(CASES)["acgt200003"]
+        bad = ol.sa(T)[1].copy()
+        bad[100:5000] = bad[7]            # repeated targets
+        bad[6000:6100] = len(T) + 12345   # out of range
+        P = np.zeros(len(T), np.int32)
+        assert la.libsais_plcp(T, bad, P, len(T)) == 0
+    finally:
+        assert lib.sagpu_set_option(b"bucketed_phi", 1) == 0
+
+
 @pytest.mark.parametrize("n", [2, 3, 5, 4097, 100001])
 def test_bwt_unaligned_output(n):
     """sagpu_build32 with a BWT output that is not 4-byte aligned."""
