@@ -181,6 +181,11 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    packed key that follows (same order as its rank; default); 0 = by rank
  *   "stream_compare" 1 = with key_stream, the PLCP pass compares suffixes on the packed code
  *                    stream, 64 / bits symbols per step (default); 0 = on the bytes
+ *   "host_staging"   copies between caller host memory and the device: 1 = copies of at least
+ *                    4 MiB through pinned 32 MiB slots, host side split over copy threads
+ *                    (default); 0 = direct hipMemcpy from pageable memory; 2 = every copy
+ *                    staged in 64 KiB slots
+ *   "copy_threads"   host threads for staged copies, 1-64; 0 = min(16, hardware threads) (default)
  *   "bucketed_phi"   1 = Φ of a caller's suffix array (libsais_plcp and the like) through
  *                    pairs bucketed by target, scattered bucket after bucket (default); 0 = one pass
  *   "plcp_pairs"     1 = the fused PLCP pass reads one 8-byte pair word per position, built in

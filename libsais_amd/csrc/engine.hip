@@ -10,11 +10,14 @@
 #include "../../include/sagpu.h"
 
 #include <algorithm>
+#include <condition_variable>
 #include <map>
 #include <mutex>
+#include <pthread.h>
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -99,6 +102,118 @@ struct Timer {
     std::vector<Rec> recs;
 };
 
+// ------------------------------------------------------ host copies
+// Bulk copies between caller host memory (pageable) and the device run through
+// pinned slots: the host side of each chunk is copied by a pool of threads while
+// the DMA of the neighbouring chunk runs.  On the MI355X host, pageable hipMemcpy
+// moves 22-29 GB/s to the device and 4 GB/s into pages the caller has not touched
+// yet (first-touch faults taken by one thread); the staged form 54 GB/s and
+// 27-29 GB/s with 16 copy threads.
+constexpr int XS_SLOTS = 4;
+// 1: copies of at least 4 MiB staged in 32 MiB chunks (default); 0: every copy
+// direct; 2: every copy staged in 64 KiB chunks (exercises the chunk logic on
+// small inputs)
+int g_host_staging = 1;
+int g_copy_threads = 0;   // 0: min(16, hardware threads)
+
+int copy_threads()
+{
+    if (g_copy_threads > 0) return g_copy_threads;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw == 0 ? 4 : (int)(hw < 16 ? hw : 16);
+}
+
+// memcpy split over worker threads; the calling thread takes parts too.  Workers
+// are detached and live until the process ends; a forked child starts a new pool.
+class CopyPool {
+    struct Job {
+        uint8_t *d;
+        const uint8_t *s;
+        u64 len, per;
+        int parts, next, left;
+    };
+    std::mutex m;
+    std::condition_variable work_cv, done_cv;
+    std::vector<Job *> open;   // jobs with parts not taken yet
+    int workers = 0;
+
+    static void run(Job *j, int p)
+    {
+        const u64 a = (u64)p * j->per;
+        if (a >= j->len) return;
+        const u64 b = a + j->per < j->len ? a + j->per : j->len;
+        memcpy(j->d + a, j->s + a, b - a);
+    }
+    // Takes the next part of the first open job; m is held.
+    Job *take(int *p)
+    {
+        Job *j = open.front();
+        *p = j->next++;
+        if (j->next == j->parts) open.erase(open.begin());
+        return j;
+    }
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            work_cv.wait(lk, [&] { return !open.empty(); });
+            int p;
+            Job *j = take(&p);
+            lk.unlock();
+            run(j, p);
+            lk.lock();
+            if (--j->left == 0) done_cv.notify_all();
+        }
+    }
+
+  public:
+    void copy(void *d, const void *s, u64 len, int threads)
+    {
+        if (threads <= 1 || len < (1u << 20)) {
+            memcpy(d, s, len);
+            return;
+        }
+        // parts of whole pages, at least 1 MiB
+        u64 per = (len + threads - 1) / threads;
+        per = per < (1u << 20) ? (1u << 20) : (per + 4095) & ~(u64)4095;
+        Job j{(uint8_t *)d, (const uint8_t *)s, len, per, (int)((len + per - 1) / per), 0, 0};
+        j.left = j.parts;
+        std::unique_lock<std::mutex> lk(m);
+        while (workers < threads - 1) {
+            std::thread([this] { loop(); }).detach();
+            workers++;
+        }
+        open.push_back(&j);
+        work_cv.notify_all();
+        while (j.next < j.parts) {
+            int p = j.next++;
+            if (j.next == j.parts) open.erase(std::find(open.begin(), open.end(), &j));
+            lk.unlock();
+            run(&j, p);
+            lk.lock();
+            j.left--;
+        }
+        done_cv.wait(lk, [&] { return j.left == 0; });
+    }
+    void lock() { m.lock(); }
+    void unlock() { m.unlock(); }
+};
+
+CopyPool *g_copies = nullptr;
+std::once_flag g_copies_once;
+
+CopyPool &copies()
+{
+    std::call_once(g_copies_once, [] {
+        g_copies = new CopyPool();
+        // fork: hold the pool's lock across it; the child leaves the old pool (its
+        // workers do not exist there) and makes a new one
+        pthread_atfork([] { g_copies->lock(); }, [] { g_copies->unlock(); },
+                       [] { g_copies = new CopyPool(); });
+    });
+    return *g_copies;
+}
+
 enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_COUNT };
 
 struct Ctx {
@@ -109,6 +224,12 @@ struct Ctx {
     u64 *pinned = nullptr;      // small pinned host staging (counts, histograms)
     uint8_t *stage = nullptr;   // large pinned staging for batched jobs
     size_t stage_cap = 0;
+    // pinned slots for copies from and to caller host memory (h2d / d2h)
+    uint8_t *xs = nullptr;
+    u64 xs_chunk = 0;
+    hipEvent_t xs_ev[XS_SLOTS] = {};
+    bool xs_busy[XS_SLOTS] = {};
+    int xs_next = 0;
     Timer timer;
     sagpu_stats stats{};
     bool sort_u32 = false;  // the radix passes being timed are u32-key passes
@@ -181,6 +302,9 @@ void ctx_destroy(Ctx *c)
         (void)hipEventDestroy(e.second);
     }
     if (c->stage) (void)hipHostFree(c->stage);
+    if (c->xs) (void)hipHostFree(c->xs);
+    for (auto &e : c->xs_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipHostFree(c->pinned);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -662,13 +786,115 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
                        cs && g_stream_compare ? cs->pw : nullptr, cs ? cs->bits : 0);
 }
 
+// Pinned slots of `chunk` bytes for the staged copies; false if the mode asks for
+// direct copies of this size (or pinned memory is not available).
+bool xs_ready(Ctx &c, u64 bytes)
+{
+    if (g_host_staging == 0 || (g_host_staging == 1 && bytes < (4u << 20))) return false;
+    const u64 chunk = g_host_staging == 2 ? (64u << 10) : (32u << 20);
+    if (c.xs && c.xs_chunk == chunk) return true;
+    if (c.xs) {
+        for (int q = 0; q < XS_SLOTS; q++)
+            if (c.xs_busy[q]) (void)hipEventSynchronize(c.xs_ev[q]);
+        (void)hipHostFree(c.xs);
+        c.xs = nullptr;
+    }
+    for (int q = 0; q < XS_SLOTS; q++) {
+        c.xs_busy[q] = false;
+        if (!c.xs_ev[q] && hipEventCreateWithFlags(&c.xs_ev[q], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+    }
+    if (hipHostMalloc((void **)&c.xs, XS_SLOTS * chunk, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c.xs = nullptr;
+        return false;
+    }
+    c.xs_chunk = chunk;
+    return true;
+}
+
+// Host to device on the context's stream.  Returns once every byte of src has
+// been read, so the caller may then overwrite it (outputs that alias inputs);
+// the DMA of the last chunks may still be running.
+int h2d(Ctx &c, void *dst, const void *src, u64 bytes)
+{
+    if (!bytes) return SAGPU_OK;
+    if (!xs_ready(c, bytes)) {
+        SAGPU_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+        return SAGPU_OK;
+    }
+    const int th = copy_threads();
+    for (u64 off = 0; off < bytes; off += c.xs_chunk) {
+        const int q = c.xs_next;
+        c.xs_next = (q + 1) % XS_SLOTS;
+        if (c.xs_busy[q]) {
+            SAGPU_CHECK(hipEventSynchronize(c.xs_ev[q]));
+            c.xs_busy[q] = false;
+        }
+        const u64 len = bytes - off < c.xs_chunk ? bytes - off : c.xs_chunk;
+        uint8_t *slot = c.xs + (u64)q * c.xs_chunk;
+        copies().copy(slot, (const uint8_t *)src + off, len, th);
+        SAGPU_CHECK(hipMemcpyAsync((uint8_t *)dst + off, slot, len, hipMemcpyHostToDevice, c.stream));
+        SAGPU_CHECK(hipEventRecord(c.xs_ev[q], c.stream));
+        c.xs_busy[q] = true;
+    }
+    return SAGPU_OK;
+}
+
+// Device to host after the work already queued on the context's stream.  With
+// staging it returns with every byte in dst; the direct form leaves the copy to
+// the caller's stream synchronisation.
+int d2h(Ctx &c, void *dst, const void *src, u64 bytes)
+{
+    if (!bytes) return SAGPU_OK;
+    if (!xs_ready(c, bytes)) {
+        SAGPU_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+        return SAGPU_OK;
+    }
+    const int th = copy_threads();
+    const u64 ch = c.xs_chunk, nch = (bytes + ch - 1) / ch;
+    auto issue = [&](u64 k) -> int {
+        const int q = (int)(k % XS_SLOTS);
+        if (c.xs_busy[q]) {
+            SAGPU_CHECK(hipEventSynchronize(c.xs_ev[q]));
+            c.xs_busy[q] = false;
+        }
+        const u64 len = bytes - k * ch < ch ? bytes - k * ch : ch;
+        SAGPU_CHECK(hipMemcpyAsync(c.xs + (u64)q * ch, (const uint8_t *)src + k * ch, len, hipMemcpyDeviceToHost,
+                                   c.stream));
+        SAGPU_CHECK(hipEventRecord(c.xs_ev[q], c.stream));
+        c.xs_busy[q] = true;
+        return SAGPU_OK;
+    };
+    for (u64 k = 0; k < nch && k < (u64)XS_SLOTS; k++) SAGPU_TRY(issue(k));
+    for (u64 k = 0; k < nch; k++) {
+        const int q = (int)(k % XS_SLOTS);
+        SAGPU_CHECK(hipEventSynchronize(c.xs_ev[q]));
+        c.xs_busy[q] = false;
+        const u64 len = bytes - k * ch < ch ? bytes - k * ch : ch;
+        copies().copy((uint8_t *)dst + k * ch, c.xs + (u64)q * ch, len, th);
+        if (k + XS_SLOTS < nch) SAGPU_TRY(issue(k + XS_SLOTS));
+    }
+    return SAGPU_OK;
+}
+
+// Where a text comes from: device memory, caller host memory, or pinned memory
+// of the library (copied directly).
+enum { SRC_DEVICE = 0, SRC_HOST = 1, SRC_PINNED = 2 };
+
 // Copies a text into a pool buffer with 64 zero bytes of padding.
-int padded_text(Ctx &c, Scope &sc, const uint8_t *src, u64 n, bool src_on_host, uint8_t **out)
+int padded_text(Ctx &c, Scope &sc, const uint8_t *src, u64 n, int src_kind, uint8_t **out)
 {
     GET(dT, uint8_t, n + 64);
     SAGPU_CHECK(hipMemsetAsync(dT + n, 0, 64, c.stream));
-    if (n) SAGPU_CHECK(hipMemcpyAsync(dT, src, n, src_on_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
-                                      c.stream));
+    if (src_kind == SRC_HOST) {
+        SAGPU_TRY(h2d(c, dT, src, n));
+    } else if (n) {
+        SAGPU_CHECK(hipMemcpyAsync(dT, src, n, src_kind == SRC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                   c.stream));
+    }
     *out = dT;
     return SAGPU_OK;
 }
@@ -761,11 +987,11 @@ unsigned grid_n(u64 n)
 int download_idx(Ctx &c, Scope &sc, const u32 *d, void *h, u64 n, int width)
 {
     if (width == 4) {
-        SAGPU_CHECK(hipMemcpyAsync(h, d, n * 4, hipMemcpyDeviceToHost, c.stream));
+        SAGPU_TRY(d2h(c, h, d, n * 4));
     } else {
         GET(w, int64_t, n);
         hipLaunchKernelGGL(k_widen, dim3(grid_n(n)), dim3(256), 0, c.stream, d, w, n);
-        SAGPU_CHECK(hipMemcpyAsync(h, w, n * 8, hipMemcpyDeviceToHost, c.stream));
+        SAGPU_TRY(d2h(c, h, w, n * 8));
     }
     return SAGPU_OK;
 }
@@ -774,10 +1000,10 @@ int upload_idx(Ctx &c, Scope &sc, const void *h, u64 n, int width, u32 **out)
 {
     GET(d, u32, n);
     if (width == 4) {
-        SAGPU_CHECK(hipMemcpyAsync(d, h, n * 4, hipMemcpyHostToDevice, c.stream));
+        SAGPU_TRY(h2d(c, d, h, n * 4));
     } else {
         GET(w, int64_t, n);
-        SAGPU_CHECK(hipMemcpyAsync(w, h, n * 8, hipMemcpyHostToDevice, c.stream));
+        SAGPU_TRY(h2d(c, w, h, n * 8));
         hipLaunchKernelGGL(k_narrow, dim3(grid_n(n)), dim3(256), 0, c.stream, w, d, n);
     }
     *out = d;
@@ -814,7 +1040,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_
     {
         Scope sc(*c);
         uint8_t *dT;
-        SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+        SAGPU_TRY(padded_text(*c, sc, T, (u64)n, SRC_HOST, &dT));
         u32 *dSA = sc.get<u32>((u64)n);
         if (!dSA) return SAGPU_ENOMEM;
         u64 f[256];
@@ -839,13 +1065,13 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, SRC_HOST, &dT));
     uint8_t *dU = sc.get<uint8_t>((u64)n);
     if (!dU) return SAGPU_ENOMEM;
     u64 f[256];
     int64_t primary = build_dev(*c, sc, dT, (u64)n, 0, nullptr, dU, nullptr, nullptr, f);
     if (primary < 0) return primary;
-    SAGPU_CHECK(hipMemcpyAsync(U, dU, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    SAGPU_TRY(d2h(*c, U, dU, (u64)n));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     store_freq(f, freq, width);
     return primary;
@@ -875,7 +1101,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa_int(const void *T, void *SA, int64
         uint8_t *dTin = in.get<uint8_t>(N * (u64)width);
         u64 *d_max = in.get<u64>(1);
         if (!dTin || !d_max) return SAGPU_ENOMEM;
-        SAGPU_CHECK(hipMemcpyAsync(dTin, T, N * (u64)width, hipMemcpyHostToDevice, s));
+        SAGPU_TRY(h2d(*c, dTin, T, N * (u64)width));
         SAGPU_TRY(launch_fill_u64(d_max, 1, 0, s));
         SAGPU_TRY(launch_max_sym(dTin, width, N, d_max, s));
         SAGPU_CHECK(hipMemcpyAsync(c->pinned, d_max, 8, hipMemcpyDeviceToHost, s));
@@ -934,7 +1160,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 4 * (u64)n, true, &dT));
+    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 4 * (u64)n, SRC_HOST, &dT));
     u32 *dSA;
     SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
     u32 *dPHI = sc.get<u32>((u64)n);
@@ -961,7 +1187,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U,
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, SRC_HOST, &dT));
     uint8_t *dU = sc.get<uint8_t>((u64)n);
     u32 *dISA = sc.get<u32>((u64)n);
     const u64 count = ((u64)n - 1) / (u64)r + 1;
@@ -973,7 +1199,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U,
     SAGPU_TRY(launch_aux_samples(dISA, (u64)n, (u64)r, count, dS, c->stream));
     std::vector<u32> hs(count);
     SAGPU_CHECK(hipMemcpyAsync(hs.data(), dS, count * 4, hipMemcpyDeviceToHost, c->stream));
-    SAGPU_CHECK(hipMemcpyAsync(U, dU, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    SAGPU_TRY(d2h(*c, U, dU, (u64)n));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     for (u64 k = 0; k < count; k++) {
         if (width == 4) ((int32_t *)I)[k] = (int32_t)(hs[k] + 1);
@@ -999,7 +1225,7 @@ int unbwt_host(Ctx &c, const void *T, void *U, u64 N, u64 r, const void *I, int 
     GET(dOut, uint8_t, bytes);
     GET(S, u32, N);
     GET(d_err, u32, 64);
-    SAGPU_CHECK(hipMemcpyAsync(dB, T, bytes, hipMemcpyHostToDevice, s));
+    SAGPU_TRY(h2d(c, dB, T, bytes));
     SAGPU_CHECK(hipMemsetAsync(dOut, 0, bytes, s));
     SAGPU_TRY(launch_fill_u32(d_err, 2, 0, s));
     const u64 pcap = N / 4096 + 2;
@@ -1055,7 +1281,7 @@ int unbwt_host(Ctx &c, const void *T, void *U, u64 N, u64 r, const void *I, int 
         sagpu_set_error("inverse BWT: a sample gap exceeded its step limit", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
-    SAGPU_CHECK(hipMemcpyAsync(U, dOut, bytes, hipMemcpyDeviceToHost, s));
+    SAGPU_TRY(d2h(c, U, dOut, bytes));
     SAGPU_CHECK(hipStreamSynchronize(s));
     return 0;
 }
@@ -1129,7 +1355,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa16(const uint16_t *T, void *SA, int
     const u64 N = (u64)n;
     Scope sc(*c);
     GET(dT, uint16_t, N);
-    SAGPU_CHECK(hipMemcpyAsync(dT, T, 2 * N, hipMemcpyHostToDevice, c->stream));
+    SAGPU_TRY(h2d(*c, dT, T, 2 * N));
     GET(dSA, u32, N);
     GET(dISA, u32, N);
     std::vector<u64> f(65536);
@@ -1157,7 +1383,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U,
     hipStream_t s = c->stream;
     Scope sc(*c);
     GET(dT, uint16_t, N);
-    SAGPU_CHECK(hipMemcpyAsync(dT, T, 2 * N, hipMemcpyHostToDevice, s));
+    SAGPU_TRY(h2d(*c, dT, T, 2 * N));
     GET(dSA, u32, N);
     GET(dISA, u32, N);
     GET(dU, uint16_t, N);
@@ -1178,7 +1404,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U,
             else ((int64_t *)I)[k] = (int64_t)hs[k] + 1;
         }
     }
-    SAGPU_CHECK(hipMemcpyAsync(U, dU, 2 * N, hipMemcpyDeviceToHost, s));
+    SAGPU_TRY(d2h(*c, U, dU, 2 * N));
     SAGPU_CHECK(hipStreamSynchronize(s));
     store_freq16(f.data(), freq, width);
     return r > 0 ? 0 : (int64_t)j0 + 1;
@@ -1197,7 +1423,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp16(const uint16_t *T, const void 
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 2 * (u64)n, true, &dT));
+    SAGPU_TRY(padded_text(*c, sc, (const uint8_t *)T, 2 * (u64)n, SRC_HOST, &dT));
     u32 *dSA;
     SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
     u32 *dPHI = sc.get<u32>((u64)n);
@@ -1237,7 +1463,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, true, &dT));
+    SAGPU_TRY(padded_text(*c, sc, T, (u64)n, SRC_HOST, &dT));
     u32 *dSA;
     SAGPU_TRY(upload_idx(*c, sc, SA, (u64)n, width, &dSA));
     u32 *dPHI = sc.get<u32>((u64)n);
@@ -1372,6 +1598,16 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
         if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "host_staging")) {
+        if (value < 0 || value > 2) return SAGPU_EINVAL;
+        g_host_staging = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "copy_threads")) {
+        if (value < 0 || value > 64) return SAGPU_EINVAL;
+        g_copy_threads = (int)value;
         return 0;
     }
     if (!strcmp(name, "bucketed_phi")) {
@@ -1554,7 +1790,7 @@ extern "C" int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, i
     stats_reset(*c);
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, SRC_DEVICE, &dT));
     u64 f[256];
     int64_t rc = build_dev(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, (u32 *)d_SA, d_U, (u32 *)d_PLCP,
                            (u32 *)d_LCP, f);
@@ -1571,7 +1807,7 @@ extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t
     SAGPU_TRY(ctx_get(&c));
     Scope sc(*c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, SRC_DEVICE, &dT));
     u32 *dPHI = sc.get<u32>((u64)n);
     if (!dPHI) return SAGPU_ENOMEM;
     SAGPU_TRY(phi_run(*c, sc, (const u32 *)d_SA, (u64)n, dPHI));
@@ -1605,7 +1841,7 @@ extern "C" int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_
     Scope sc(*c);
     hipStream_t s = c->stream;
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, false, &dT));
+    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, SRC_DEVICE, &dT));
     u32 *sid = sc.get<u32>((u64)n);
     int64_t *d_off = sc.get<int64_t>((u64)count + 1);
     u32 *d_bad = sc.get<u32>(64);
@@ -1689,7 +1925,7 @@ int run_single(Ctx &c, sagpu_job *j)
     stats_reset(c);
     Scope sc(c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(c, sc, j->T, n, true, &dT));
+    SAGPU_TRY(padded_text(c, sc, j->T, n, SRC_HOST, &dT));
     GET(dSA, u32, n);
     uint8_t *dU = nullptr;
     u32 *dL = nullptr;
@@ -1703,9 +1939,9 @@ int run_single(Ctx &c, sagpu_job *j)
     }
     int64_t r = build_dev(c, sc, dT, n, 0, dSA, dU, nullptr, dL, nullptr);
     if (r < 0) return (int)r;
-    SAGPU_CHECK(hipMemcpyAsync(j->SA, dSA, n * 4, hipMemcpyDeviceToHost, c.stream));
-    if (dU) SAGPU_CHECK(hipMemcpyAsync(j->U, dU, n, hipMemcpyDeviceToHost, c.stream));
-    if (dL) SAGPU_CHECK(hipMemcpyAsync(j->LCP, dL, n * 4, hipMemcpyDeviceToHost, c.stream));
+    SAGPU_TRY(d2h(c, j->SA, dSA, n * 4));
+    if (dU) SAGPU_TRY(d2h(c, j->U, dU, n));
+    if (dL) SAGPU_TRY(d2h(c, j->LCP, dL, n * 4));
     SAGPU_CHECK(hipStreamSynchronize(c.stream));
     if (dU) j->primary = r;
     return 0;
@@ -1752,7 +1988,7 @@ int run_batch(Ctx &c, sagpu_job **jobs, int count)
     Scope sc(c);
     hipStream_t s = c.stream;
     uint8_t *dT;
-    SAGPU_TRY(padded_text(c, sc, hin, N, true, &dT));
+    SAGPU_TRY(padded_text(c, sc, hin, N, SRC_PINNED, &dT));
     GET(sid, u32, N);
     GET(d_off, int64_t, (u64)count + 1);
     GET(d_bad, u32, 64);

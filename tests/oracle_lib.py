@@ -145,7 +145,9 @@ def plcp(T, SA, gsa=False):
 def lcp(P, SA):
     n = len(SA)
     L = np.zeros(max(n, 1), np.int32)
-    oracle().oracle_lcp32(_p(np.ascontiguousarray(P, np.int32)), _p(np.ascontiguousarray(SA, np.int32)), _p(L), n)
+    P = np.ascontiguousarray(P, np.int32)   # converted copies must outlive the call
+    SA = np.ascontiguousarray(SA, np.int32)
+    oracle().oracle_lcp32(_p(P), _p(SA), _p(L), n)
     return L[:n]
 
 
@@ -378,7 +380,8 @@ def bwt16(T, SA):
     T = np.ascontiguousarray(T, np.uint16)
     n = len(T)
     U = np.zeros(max(n, 1), np.uint16)
-    idx = _o16().oracle_bwt16(_p(T), _p(np.ascontiguousarray(SA, np.int64)), _p(U), n)
+    SA = np.ascontiguousarray(SA, np.int64)
+    idx = _o16().oracle_bwt16(_p(T), _p(SA), _p(U), n)
     return U[:n], idx
 
 
@@ -386,7 +389,8 @@ def plcp16(T, SA, gsa=False):
     T = np.ascontiguousarray(T, np.uint16)
     n = len(T)
     P = np.zeros(max(n, 1), np.int64)
-    _o16().oracle_plcp16(_p(T), _p(np.ascontiguousarray(SA, np.int64)), _p(P), n, int(gsa))
+    SA = np.ascontiguousarray(SA, np.int64)
+    _o16().oracle_plcp16(_p(T), _p(SA), _p(P), n, int(gsa))
     return P[:n]
 
 
@@ -420,8 +424,8 @@ def ref_plcp16(T, SA, gsa=False):
     T = np.ascontiguousarray(T, np.uint16)
     n = len(T)
     P = np.zeros(max(n, 1), np.int32)
-    (ref().libsais16_plcp_gsa if gsa else ref().libsais16_plcp)(_p(T), _p(np.ascontiguousarray(SA, np.int32)),
-                                                                _p(P), n)
+    SA = np.ascontiguousarray(SA, np.int32)
+    (ref().libsais16_plcp_gsa if gsa else ref().libsais16_plcp)(_p(T), _p(SA), _p(P), n)
     return P[:n]
 
 
