@@ -6,12 +6,15 @@ index, PLCP and LCP, all int32, outputs left in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 
-roofline: the dominant kernel is the onesweep radix-sort pass of the initial sort
-(u32 key + u32 value).  Its algorithmic traffic per launch is n x (4 B key + 4 B
-value) x 2 (read + write); the launch durations come from HIP events recorded on
-the engine stream around every pass inside the timed steps.  traffic is the HBM
-bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy
-kernel of known size), read from profiles/pmc_traffic.json.
+roofline: the kernel with the largest share of the build, the text-order PLCP
+pass over pair words (k_plcp_irr_pairs).  Its algorithmic bytes per launch are
+17 B per position (ISA 4, the preceding byte 1, one 8-byte pair word, the value
+written 4; the compare reads at irreducible positions are not counted); the
+launch duration comes from HIP events recorded on the engine stream around it
+inside the timed steps.  The onesweep radix pass of the initial sort (u32 key +
+u32 value, n x 16 B per launch) is reported beside it as radix_pass.  traffic is
+the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled
+by a copy kernel of known size), read from profiles/pmc_traffic.json.
 pipeline: B_model = 269 n + 244 sum_r a_r bytes per step (prefix-doubling yardstick
 with 8 initial symbols; a_r counts suffixes with max(LCP[j], LCP[j+1]) >= 8*2^(r-1),
 computed from the LCP this run produced), divided by the step time.
@@ -61,8 +64,8 @@ def copy_peak(lib, nbytes: int = 1 << 30, reps: int = 5):
     return 2 * nbytes / best / 1e9
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per radix-pass launch from the committed PMC summary (rocprofv3
+def measured_traffic(workload: str, key: str = "radix_pass_bytes_per_launch"):
+    """HBM bytes per launch of a kernel from the committed PMC summary (rocprofv3
     FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
@@ -72,7 +75,7 @@ def measured_traffic(workload: str):
         return None
     if d.get("workload") != workload:
         return None
-    return d.get("radix_pass_bytes_per_launch")
+    return d.get(key)
 
 
 def a_r_from_lcp(L: np.ndarray, h0: int = 8):
@@ -301,6 +304,7 @@ def main():
     lib.sagpu_set_timing(1)
     sweep_ms, sweep_launch, sweep_elems, sweep_bytes = 0.0, 0, 0, 0
     u32_ms, u32_launch, u32_bytes = 0.0, 0, 0
+    pp_ms, pp_launch = 0.0, 0
     dev_ms = 0.0
     fam = {"initial": 0.0, "rounds": 0.0, "bwt": 0.0, "plcp": 0.0, "lcp": 0.0}
     lib.sagpu_synchronize()
@@ -315,6 +319,8 @@ def main():
         u32_ms += st.ms_onesweep_u32
         u32_launch += st.onesweep_launches_u32
         u32_bytes += st.sorted_bytes_u32
+        pp_ms += st.ms_plcp_pairs_pass
+        pp_launch += st.plcp_pairs_launches
         dev_ms += st.ms_total
         fam["initial"] += st.ms_initial
         fam["rounds"] += st.ms_rounds
@@ -332,12 +338,14 @@ def main():
     rounds = st.rounds
     active = [int(st.active[r]) for r in range(min(rounds, 64))]
 
-    # dominant kernel roofline (radix pass): (key + value) bytes read and written
-    # per element moved, 16 B for u32 keys, 24 B for u64 keys
-    # dominant kernel: the u32-key radix passes of the initial sort (16 B per
-    # element read + written per pass); the u64 passes of the big-group sorts are
-    # reported beside it
+    # radix pass: the u32-key passes of the initial sort (16 B per element read +
+    # written per pass); all passes, the u64 ones of the big-group sorts included,
+    # are reported beside it
     achieved = u32_bytes / (u32_ms / 1000.0) / 1e9 if u32_ms > 0 else 0.0
+    # the PLCP text-order pass: 17 B of sequential and pair-word bytes per position
+    pp_bytes = 17 * n
+    pp_avg = pp_ms / max(1, pp_launch)
+    pp_achieved = pp_bytes / (pp_avg / 1000.0) / 1e9 if pp_ms > 0 else 0.0
     all_achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
 
     workload = "dna_like_256MiB_sa_bwt_plcp_lcp" if n == 1 << 28 else f"dna_like_{n}B_sa_bwt_plcp_lcp"
@@ -374,19 +382,30 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_radix_pass<u32 key, u32 value> (onesweep LSD pass of the initial sort)",
-            "achieved": round(achieved, 1),
+            "kernel": "k_plcp_irr_pairs (text-order PLCP pass over pair words)",
+            "achieved": round(pp_achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(workload),
+            "frac": round(pp_achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload, "plcp_pairs_pass_bytes_per_launch"),
+            "launches": pp_launch,
+            "avg_launch_ms": round(pp_avg, 4),
+            "algorithmic_bytes_per_launch": pp_bytes,
+            "random_reads_G_per_s": round(n / (pp_avg / 1000.0) / 1e9, 1) if pp_ms > 0 else None,
+            "note": "one random 8-byte pair-word read per position; each costs a whole line of HBM traffic",
             "copy_peak_GBs": round(cpeak, 1),
-            "frac_of_copy": round(achieved / cpeak, 4),
-            "launches": u32_launch,
-            "avg_launch_ms": round(u32_ms / max(1, u32_launch), 4),
-            "algorithmic_bytes_per_launch": u32_bytes // max(1, u32_launch),
-            "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes,
-                                 "achieved_GBs": round(all_achieved, 1)},
+            "radix_pass": {
+                "kernel": "k_radix_pass<u32 key, u32 value> (onesweep LSD pass of the initial sort)",
+                "achieved": round(achieved, 1),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "frac_of_copy": round(achieved / cpeak, 4),
+                "traffic": measured_traffic(workload),
+                "launches": u32_launch,
+                "avg_launch_ms": round(u32_ms / max(1, u32_launch), 4),
+                "algorithmic_bytes_per_launch": u32_bytes // max(1, u32_launch),
+                "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes,
+                                     "achieved_GBs": round(all_achieved, 1)},
+            },
         },
         "pipeline": {
             "b_model_bytes": b_model,

@@ -151,6 +151,8 @@ typedef struct sagpu_stats {
     double  ms_onesweep_u32;      /* the u32-key passes among ms_onesweep           */
     int32_t onesweep_launches_u32;
     int64_t sorted_bytes_u32;     /* algorithmic bytes of the u32-key passes        */
+    double  ms_plcp_pairs_pass;   /* the text-order PLCP pass over pair words      */
+    int32_t plcp_pairs_launches;
 } sagpu_stats;
 
 /*

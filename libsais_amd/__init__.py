@@ -75,6 +75,8 @@ class SagpuStats(ctypes.Structure):
         ("ms_onesweep_u32", ctypes.c_double),
         ("onesweep_launches_u32", ctypes.c_int32),
         ("sorted_bytes_u32", ctypes.c_int64),
+        ("ms_plcp_pairs_pass", ctypes.c_double),
+        ("plcp_pairs_launches", ctypes.c_int32),
     ]
 
 

@@ -160,6 +160,9 @@ struct LcpScratch {
     u64 queue_cap;
     void *tile_max;   // [tiles] per-tile max for the prefix-max scan (u32 or u64)
     u64 tiles_cap;
+    // optional: called with 0 / 1 around the launch of the pair-word text-order pass
+    void (*hook)(void *user, int phase);
+    void *hook_user;
 };
 
 int launch_bwt(const uint8_t *T, const u32 *SA, const u32 *ISA, u64 n, uint8_t *U, hipStream_t s);

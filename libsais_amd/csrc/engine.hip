@@ -214,7 +214,7 @@ CopyPool &copies()
     return *g_copies;
 }
 
-enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_COUNT };
+enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_PLCP_PAIRS, F_COUNT };
 
 struct Ctx {
     int device = 0;
@@ -388,6 +388,7 @@ void timer_collect(Ctx &c)
         case F_PLCP: c.stats.ms_plcp += ms; break;
         case F_LCP: c.stats.ms_lcp += ms; break;
         case F_TOTAL: c.stats.ms_total += ms; break;
+        case F_PLCP_PAIRS: c.stats.ms_plcp_pairs_pass += ms; c.stats.plcp_pairs_launches++; break;
         }
         c.timer.free_ev.push_back({r.a, r.b});
     }
@@ -437,6 +438,15 @@ void onesweep_hook(void *user, int phase)
     Ctx &c = *(Ctx *)user;
     static thread_local Span *open = nullptr;
     if (phase == 0) open = new Span(c, c.sort_u32 && !c.rs.gen_pass ? F_ONESWEEP32 : F_ONESWEEP);
+    else { delete open; open = nullptr; }
+}
+
+// Brackets the pair-word PLCP pass (timing mode only); nested inside the F_PLCP span.
+void plcp_pairs_hook(void *user, int phase)
+{
+    Ctx &c = *(Ctx *)user;
+    static thread_local Span *open = nullptr;
+    if (phase == 0) open = new Span(c, F_PLCP_PAIRS);
     else { delete open; open = nullptr; }
 }
 
@@ -772,6 +782,8 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     ls.queue_l = ql;
     ls.queue_n = qn;
     ls.tile_max = tm;
+    ls.hook = c.timer.on ? plcp_pairs_hook : nullptr;
+    ls.hook_user = &c;
     u64 *V64 = nullptr;
     if (n > g_wide_from) {  // PLCP[i] + i may reach 2^32
         V64 = sc.get<u64>(n);

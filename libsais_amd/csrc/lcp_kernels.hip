@@ -765,10 +765,13 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
     const bool fused_max = pairs && W == 1;   // the pair-word pass writes the tile maxima itself
     if (fused_max) {
-        if constexpr (W == 1)
+        if constexpr (W == 1) {
+            if (ls.hook) ls.hook(ls.hook_user, 0);
             hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3((unsigned)(tiles < 65535 ? tiles : 65535)), dim3(256), 0, s,
                                T, n, gsa, ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap,
                                gsa ? nullptr : pw, bits, tmax);
+            if (ls.hook) ls.hook(ls.hook_user, 1);
+        }
     } else {
         hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA,
                            SA, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
