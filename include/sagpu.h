@@ -41,6 +41,10 @@ SAGPU_API int32_t sagpu_memset(void *d_dst, int value, size_t bytes);
  * stores); also the copy-bandwidth yardstick of the benchmark. */
 SAGPU_API int32_t sagpu_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
 SAGPU_API int32_t sagpu_synchronize(void);
+/* Host-to-host copy split over the library's copy threads, the pool that fills and
+ * drains the pinned slots of the libsais calls (threads = 0: the copy_threads option).
+ * Touches no device; returns -1 for a negative thread count. */
+SAGPU_API int32_t sagpu_host_memcpy(void *h_dst, const void *h_src, size_t bytes, int32_t threads);
 
 /* Caps the engine's device working memory (bytes; 0 = unlimited).  Requests beyond
  * the cap fail with -2, exactly like an allocation failure in libsais. */

@@ -211,6 +211,7 @@ ABI_SAGPU = [
     ("sagpu_free", None, [_vp]),
     ("sagpu_memcpy_h2d", _i32, [_vp, _vp, ctypes.c_size_t]),
     ("sagpu_memcpy_d2h", _i32, [_vp, _vp, ctypes.c_size_t]),
+    ("sagpu_host_memcpy", _i32, [_vp, _vp, ctypes.c_size_t, _i32]),
     ("sagpu_memset", _i32, [_vp, ctypes.c_int, ctypes.c_size_t]),
     ("sagpu_memcpy_d2d", _i32, [_vp, _vp, ctypes.c_size_t]),
     ("sagpu_synchronize", _i32, []),

@@ -1553,6 +1553,13 @@ extern "C" int32_t sagpu_memcpy_d2h(void *h, const void *d, size_t bytes)
     return hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : SAGPU_EDEVICE;
 }
 
+extern "C" int32_t sagpu_host_memcpy(void *h, const void *s, size_t bytes, int32_t threads)
+{
+    if (threads < 0) return SAGPU_EINVAL;
+    if (bytes) copies().copy(h, s, bytes, threads ? threads : copy_threads());
+    return 0;
+}
+
 extern "C" int32_t sagpu_memset(void *d, int value, size_t bytes)
 {
     return hipMemset(d, value, bytes) == hipSuccess ? 0 : SAGPU_EDEVICE;
