@@ -18,6 +18,9 @@ by a copy kernel of known size), read from profiles/pmc_traffic.json.
 pipeline: B_model = 269 n + 244 sum_r a_r bytes per step (prefix-doubling yardstick
 with 8 initial symbols; a_r counts suffixes with max(LCP[j], LCP[j+1]) >= 8*2^(r-1),
 computed from the LCP this run produced), divided by the step time.
+extras: batched GSA (20,000 strings), the serving trace (600 jobs, served against
+one job at a time), and the 64 MiB high-LCP inputs (all 'A', period 17, Fibonacci)
+with the reference beside them.
 cpu_baseline: the reference libsais 2.10.4 (built from its sources by oracle/Makefile)
 with OpenMP on this host's cores, on the same input: libsais_omp + libsais_bwt_omp
 + libsais_plcp_omp + libsais_lcp_omp, median of the repetitions.
@@ -199,6 +202,62 @@ def bench_batched(reps: int = 2, plain_sample: int = 1000):
     }
 
 
+def bench_degenerate(lib, with_cpu: bool, reps: int = 3):
+    """The 64 MiB high-LCP inputs (all 'A', period 17 with one mutated byte at n/2,
+    Fibonacci word): fused SA+BWT+PLCP+LCP on a device-resident text, median of
+    `reps` builds, with the doubling rounds taken; beside it the reference libsais
+    with OpenMP on the host cores (one run of libsais_omp + _bwt_omp + _plcp_omp +
+    _lcp_omp) when it is built."""
+    import libsais_amd as la
+    from libsais_amd import workloads
+
+    n = 1 << 26
+    out = {}
+    cases = [("all_a", workloads.all_a(n)), ("period17_mutated", workloads.periodic(n, 17, 3)),
+             ("fibonacci", workloads.fibonacci(n))]
+    dT = lib.sagpu_malloc(n)
+    bufs = [lib.sagpu_malloc(4 * n), lib.sagpu_malloc(n), lib.sagpu_malloc(4 * n), lib.sagpu_malloc(4 * n)]
+    try:
+        for name, T in cases:
+            assert lib.sagpu_memcpy_h2d(dT, T.ctypes.data, n) == 0
+            f64 = np.zeros(256, np.int64)
+            times = []
+            for _ in range(reps + 1):
+                lib.sagpu_synchronize()
+                t0 = time.perf_counter()
+                r = lib.sagpu_build32(dT, n, 0, bufs[0], bufs[1], bufs[2], bufs[3], f64.ctypes.data)
+                lib.sagpu_synchronize()
+                times.append(time.perf_counter() - t0)
+                if r < 0:
+                    raise RuntimeError(f"sagpu_build32 failed ({r}): {la.last_error()}")
+            t = float(np.median(times[1:]))
+            row = {"ms": round(t * 1000, 2), "MBps": round(n / t / 1e6, 1), "rounds": la.stats().rounds}
+            if with_cpu:
+                from tests import oracle_lib as ol
+                if ol.have_ref():
+                    try:
+                        cores = len(os.sched_getaffinity(0))
+                    except AttributeError:
+                        cores = os.cpu_count() or 1
+                    cores = min(cores, 64)
+                    t0 = time.perf_counter()
+                    rc, SA, _ = ol.ref_sa(T, threads=cores)
+                    ol.ref_bwt(T, threads=cores)
+                    P = ol.ref_plcp(T, SA, threads=cores)
+                    ol.ref_lcp(P, SA, threads=cores)
+                    tc = time.perf_counter() - t0
+                    row["cpu_reference_MBps"] = round(n / tc / 1e6, 1)
+                    row["cpu_cores"] = cores
+                    del SA, P
+            out[name] = row
+            log(f"degenerate {name}: {row}")
+    finally:
+        lib.sagpu_free(dT)
+        for b in bufs:
+            lib.sagpu_free(b)
+    return {"workload": "64MiB_all_a_period17_fibonacci_sa_bwt_plcp_lcp", "n": n, **out}
+
+
 def bench_serving(jobs: int, max_bytes: int, slots: int = 4):
     """Jobs with sizes log-uniform from 4 KiB, order-2 Markov ACGT, SA / SA+BWT /
     SA+LCP = 0.4/0.4/0.2, replayed twice: Poisson arrivals at 200 req/s (latency
@@ -242,8 +301,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the batched and serving sections")
-    ap.add_argument("--mode", choices=["main", "serving-full", "batched", "copy"], default="main")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and degenerate-input sections")
+    ap.add_argument("--mode", choices=["main", "serving-full", "batched", "copy", "degenerate"], default="main")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="engine option (sagpu_set_option), repeatable")
     args = ap.parse_args()
@@ -269,6 +328,9 @@ def main():
         return
     if args.mode == "batched":
         print(json.dumps({"batched": bench_batched()}), flush=True)
+        return
+    if args.mode == "degenerate":
+        print(json.dumps({"degenerate": bench_degenerate(lib, not args.no_cpu)}), flush=True)
         return
     if args.mode == "copy":
         print(json.dumps({"copy_GBs": round(copy_peak(lib), 1), "bytes": 1 << 30}), flush=True)
@@ -448,6 +510,8 @@ def main():
         log("serving trace ...")
         result["serving"] = bench_serving(600, 64 << 20)
         log(f"serving: {result['serving']}")
+        log("degenerate inputs ...")
+        result["degenerate"] = bench_degenerate(lib, not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(T, args.cpu_reps)

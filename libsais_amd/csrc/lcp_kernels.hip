@@ -114,17 +114,21 @@ __global__ __launch_bounds__(256) void k_phi_bucket(const u32 *__restrict__ SA, 
     if (threadIdx.x < PB_MAXB) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const u64 base = (u64)blockIdx.x * PB_TILE;
+    // valid rows in a bit mask: every u32 is a suffix index when n = 2^32, so no
+    // value can stand for "skip"
     u32 p[PB_ROWS], val[PB_ROWS], rk[PB_ROWS];
+    u32 valid = 0;
 #pragma unroll
     for (int r = 0; r < PB_ROWS; r++) {
         const u64 j = base + (u64)r * 256 + threadIdx.x;
-        p[r] = ~0u;
+        p[r] = 0;
         if (j < n) {
             const u32 x = SA[j];
-            if (x < n) {
+            if ((u64)x < n) {
                 p[r] = x;
                 val[r] = j == 0 ? x : SA[j - 1];   // the first suffix points at itself
                 rk[r] = atomicAdd(&s_cnt[(u32)((u64)x >> wlog)], 1u);
+                valid |= 1u << r;
             }
         }
     }
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(256) void k_phi_bucket(const u32 *__restrict__ SA, 
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PB_ROWS; r++) {
-        if (p[r] == ~0u) continue;
+        if (!((valid >> r) & 1u)) continue;
         const u32 b = (u32)((u64)p[r] >> wlog);
         const u64 lo = (u64)b << wlog;
         const u64 cap = n - lo < (1ull << wlog) ? n - lo : (1ull << wlog);
