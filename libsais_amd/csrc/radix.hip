@@ -47,9 +47,9 @@ template <typename K, bool GEN>
 __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, KeyGen gen, u64 n, int begin_bit,
                                                     int end_bit, int npass, u64 *__restrict__ hist)
 {
-    // 32-bit keys (at most 4 passes): 4 copies per digit position, picked by lane % 4,
-    // so skewed digits (DNA) contend less on the same LDS words
-    constexpr int C = sizeof(K) == 4 ? 4 : 1;
+    // 4 copies per digit position, picked by lane % 4, so skewed digits (DNA)
+    // contend less on the same LDS words
+    constexpr int C = 4;
     constexpr int NP = sizeof(K) == 4 ? 4 : 8;
     __shared__ u32 s_hc[NP * C][256];
     for (int i = threadIdx.x; i < NP * C * 256; i += blockDim.x) (&s_hc[0][0])[i] = 0;
@@ -64,6 +64,16 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
             const int sh = begin_bit + 8 * p;
             const u32 dm = end_bit - sh >= 8 ? 255u : ((1u << (end_bit - sh)) - 1u);
             u32 d = (u32)(k >> sh) & dm;
+            if constexpr (sizeof(K) == 8) {
+                // the big-group sorts of periodic and degenerate texts see long runs
+                // of equal keys: a wave whose lanes share the digit adds once
+                const u64 act = __ballot(1);
+                const u32 d0 = __builtin_amdgcn_readfirstlane(d);
+                if (__ballot(d == d0) == act) {
+                    if (lane_id() == (u32)(__ffsll((long long)act) - 1)) atomicAdd(&s_hc[p * C][d0], (u32)__popcll(act));
+                    continue;
+                }
+            }
             atomicAdd(&s_hc[p * C + copy][d], 1u);
         }
     }
