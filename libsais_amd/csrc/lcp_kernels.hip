@@ -455,8 +455,9 @@ __global__ __launch_bounds__(256) void k_plcp_pairs(const u32 *__restrict__ SA, 
     }
 }
 
-// One workgroup per queued position: 256 lanes x 8 bytes per step, first mismatch
-// found with a workgroup min-reduction (byte offsets; W bytes per symbol).
+// One workgroup per queued position: 256 lanes x LW words of 8 bytes per step, first
+// mismatch found with a workgroup min-reduction (byte offsets; W bytes per symbol).
+constexpr int LW = 8;
 template <typename VT, int W>
 __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
@@ -482,18 +483,24 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
         const u64 m = W * (n - (i > k ? i : k));
         u64 l = W * (u64)queue_l[q];
         for (;;) {
-            u64 off = l + (u64)threadIdx.x * 8;
+            // LW words of 8 bytes per lane and step (16 KiB per workgroup), all
+            // loads in flight together; the step's answer is the smallest offset
             u64 first = ~0ull;
-            if (off < m) {
-                u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
-                u64 stop = x ^ y;
-                if (gsa) stop |= zero_lanes<W>(x);
-                if (stop) {
-                    first = off + (u64)(__builtin_ctzll(stop) >> 3);
-                    if (first > m) first = m;
+#pragma unroll
+            for (int u = 0; u < LW; u++) {
+                const u64 off = l + ((u64)u * 256 + threadIdx.x) * 8;
+                u64 f = m;
+                if (off < m) {
+                    const u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
+                    u64 stop = x ^ y;
+                    if (gsa) stop |= zero_lanes<W>(x);
+                    f = ~0ull;
+                    if (stop) {
+                        f = off + (u64)(__builtin_ctzll(stop) >> 3);
+                        if (f > m) f = m;
+                    }
                 }
-            } else {
-                first = m;
+                first = f < first ? f : first;
             }
             // workgroup min
             for (int d = 32; d >= 1; d >>= 1) {
@@ -518,7 +525,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
                 }
                 break;
             }
-            l += 256 * 8;
+            l += 256ull * 8 * LW;
         }
     }
 }
