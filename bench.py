@@ -98,9 +98,11 @@ def a_r_from_lcp(L: np.ndarray, h0: int = 8):
     return out
 
 
-def cpu_baseline(T: np.ndarray, reps: int):
+def cpu_baseline(T: np.ndarray, reps: int, degenerate: bool = False, batched: bool = False,
+                 serving_jobs: list | None = None):
     """Reference libsais with OpenMP, or the oracle restatement if the reference
-    build is absent."""
+    build is absent.  With the reference, also the reference side of the extra
+    sections that ran (degenerate inputs, batched GSA, serving trace)."""
     from tests import oracle_lib as ol
 
     n = len(T)
@@ -136,10 +138,19 @@ def cpu_baseline(T: np.ndarray, reps: int):
             ol.ref_lcp(P_s, SA_s, threads=th)
             sweep[str(th)] = round(m / (time.perf_counter() - t0) / 1e6, 2)
         log(f"cpu thread sweep on {m} bytes: {sweep}")
-        return {"value": round(n / t / 1e6, 2), "unit": "MB/s", "cores": cores, "kind": "reference",
-                "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp, "
-                          f"median of {reps}",
-                "thread_sweep_MBps": {"sample_bytes": m, **sweep}}
+        out = {"value": round(n / t / 1e6, 2), "unit": "MB/s", "cores": cores, "kind": "reference",
+               "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp, "
+                         f"median of {reps}",
+               "cpu_model": _cpu_model(),
+               "thread_sweep_MBps": {"sample_bytes": m, **sweep}}
+        ol.ref()   # loaded once, before any worker thread calls it
+        if degenerate:
+            out["degenerate"] = cpu_degenerate(ol, cores)
+        if batched:
+            out["batched"] = cpu_batched(ol, cores)
+        if serving_jobs:
+            out["serving"] = cpu_serving(ol, cores, serving_jobs)
+        return out
     # oracle port, single thread, on a bounded prefix
     m = min(n, 1 << 24)
     S = np.ascontiguousarray(T[:m])
@@ -151,6 +162,96 @@ def cpu_baseline(T: np.ndarray, reps: int):
     t = time.perf_counter() - t0
     return {"value": round(m / t / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port",
             "sample": f"first {m} bytes, SA+BWT+PLCP+LCP with the oracle restatement"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_degenerate(ol, cores: int):
+    """Reference with OpenMP on each 64 MiB high-LCP input, once."""
+    out = {}
+    for name, T in degenerate_inputs(1 << 26):
+        t0 = time.perf_counter()
+        rc, SA, _ = ol.ref_sa(T, threads=cores)
+        ol.ref_bwt(T, threads=cores)
+        P = ol.ref_plcp(T, SA, threads=cores)
+        ol.ref_lcp(P, SA, threads=cores)
+        out[name] = round(len(T) / (time.perf_counter() - t0) / 1e6, 1)
+    log(f"cpu reference, degenerate inputs: {out}")
+    return {"MBps": out, "sample": "each 64 MiB input once: libsais_omp + _bwt_omp + _plcp_omp + _lcp_omp"}
+
+
+def cpu_batched(ol, cores: int):
+    """The batched GSA set through the reference: every string on its own
+    (libsais_gsa + libsais_plcp_gsa + libsais_lcp, single-threaded, on `cores`
+    worker threads), and a 64 MiB prefix of whole strings as one GSA with OpenMP."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from libsais_amd import workloads
+
+    T, offs = workloads.gsa_batch(count=20000, seed=5)
+    count = len(offs) - 1
+
+    def one(k):
+        S = T[int(offs[k]):int(offs[k + 1])]   # ends with its 0 separator
+        rc, SA, _ = ol.ref_sa(S, gsa=True)
+        P = ol.ref_plcp(S, SA, gsa=True)
+        ol.ref_lcp(P, SA)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(one, range(count)))
+    t_per = time.perf_counter() - t0
+    k = int(np.searchsorted(offs, 64 << 20, side="right")) - 1
+    C = np.ascontiguousarray(T[:int(offs[k])])
+    t0 = time.perf_counter()
+    rc, SA, _ = ol.ref_sa(C, gsa=True, threads=cores)
+    P = ol.ref_plcp(C, SA, gsa=True, threads=cores)
+    ol.ref_lcp(P, SA, threads=cores)
+    t_cat = time.perf_counter() - t0
+    out = {"per_string_MBps": round(int(offs[-1]) / t_per / 1e6, 1),
+           "per_string_sample": f"all {count} strings ({int(offs[-1])} bytes), one reference call chain per "
+                                f"string on {cores} worker threads",
+           "concatenation_MBps": round(len(C) / t_cat / 1e6, 1),
+           "concatenation_sample": f"first {k} strings ({len(C)} bytes) as one GSA: libsais_gsa_omp + "
+                                   f"_plcp_gsa_omp + _lcp_omp"}
+    log(f"cpu reference, batched: {out}")
+    return out
+
+
+def cpu_serving(ol, cores: int, jobs: list):
+    """The serving trace replayed on CPU workers: all jobs at once, in arrival
+    order, one job per worker thread with the single-threaded reference (SA:
+    libsais; SA+BWT: libsais + libsais_bwt; SA+LCP: libsais + libsais_plcp +
+    libsais_lcp)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(job):
+        T, kind = job
+        rc, SA, _ = ol.ref_sa(T)
+        if kind == 1:
+            ol.ref_bwt(T)
+        elif kind == 2:
+            P = ol.ref_plcp(T, SA)
+            ol.ref_lcp(P, SA)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(one, jobs))
+    t = time.perf_counter() - t0
+    total = sum(len(T) for T, _ in jobs)
+    out = {"aggregate_MBps": round(total / t / 1e6, 2), "makespan_s": round(t, 3),
+           "sample": f"the {len(jobs)}-job trace all at once on {cores} worker threads"}
+    log(f"cpu reference, serving: {out}")
+    return out
 
 
 def bench_batched(reps: int = 2, plain_sample: int = 1000):
@@ -202,23 +303,19 @@ def bench_batched(reps: int = 2, plain_sample: int = 1000):
     }
 
 
-def bench_degenerate(lib, with_cpu: bool, reps: int = 3):
+def bench_degenerate(lib, reps: int = 3):
     """The 64 MiB high-LCP inputs (all 'A', period 17 with one mutated byte at n/2,
     Fibonacci word): fused SA+BWT+PLCP+LCP on a device-resident text, median of
-    `reps` builds, with the doubling rounds taken; beside it the reference libsais
-    with OpenMP on the host cores (one run of libsais_omp + _bwt_omp + _plcp_omp +
-    _lcp_omp) when it is built."""
+    `reps` builds, with the doubling rounds taken."""
     import libsais_amd as la
     from libsais_amd import workloads
 
     n = 1 << 26
     out = {}
-    cases = [("all_a", workloads.all_a(n)), ("period17_mutated", workloads.periodic(n, 17, 3)),
-             ("fibonacci", workloads.fibonacci(n))]
     dT = lib.sagpu_malloc(n)
     bufs = [lib.sagpu_malloc(4 * n), lib.sagpu_malloc(n), lib.sagpu_malloc(4 * n), lib.sagpu_malloc(4 * n)]
     try:
-        for name, T in cases:
+        for name, T in degenerate_inputs(n):
             assert lib.sagpu_memcpy_h2d(dT, T.ctypes.data, n) == 0
             f64 = np.zeros(256, np.int64)
             times = []
@@ -231,26 +328,8 @@ def bench_degenerate(lib, with_cpu: bool, reps: int = 3):
                 if r < 0:
                     raise RuntimeError(f"sagpu_build32 failed ({r}): {la.last_error()}")
             t = float(np.median(times[1:]))
-            row = {"ms": round(t * 1000, 2), "MBps": round(n / t / 1e6, 1), "rounds": la.stats().rounds}
-            if with_cpu:
-                from tests import oracle_lib as ol
-                if ol.have_ref():
-                    try:
-                        cores = len(os.sched_getaffinity(0))
-                    except AttributeError:
-                        cores = os.cpu_count() or 1
-                    cores = min(cores, 64)
-                    t0 = time.perf_counter()
-                    rc, SA, _ = ol.ref_sa(T, threads=cores)
-                    ol.ref_bwt(T, threads=cores)
-                    P = ol.ref_plcp(T, SA, threads=cores)
-                    ol.ref_lcp(P, SA, threads=cores)
-                    tc = time.perf_counter() - t0
-                    row["cpu_reference_MBps"] = round(n / tc / 1e6, 1)
-                    row["cpu_cores"] = cores
-                    del SA, P
-            out[name] = row
-            log(f"degenerate {name}: {row}")
+            out[name] = {"ms": round(t * 1000, 2), "MBps": round(n / t / 1e6, 1), "rounds": la.stats().rounds}
+            log(f"degenerate {name}: {out[name]}")
     finally:
         lib.sagpu_free(dT)
         for b in bufs:
@@ -258,7 +337,14 @@ def bench_degenerate(lib, with_cpu: bool, reps: int = 3):
     return {"workload": "64MiB_all_a_period17_fibonacci_sa_bwt_plcp_lcp", "n": n, **out}
 
 
-def bench_serving(jobs: int, max_bytes: int, slots: int = 4):
+def degenerate_inputs(n: int):
+    from libsais_amd import workloads
+
+    return [("all_a", workloads.all_a(n)), ("period17_mutated", workloads.periodic(n, 17, 3)),
+            ("fibonacci", workloads.fibonacci(n))]
+
+
+def bench_serving(jobs: int, max_bytes: int, slots: int = 4, keep: list | None = None):
     """Jobs with sizes log-uniform from 4 KiB, order-2 Markov ACGT, SA / SA+BWT /
     SA+LCP = 0.4/0.4/0.2, replayed twice: Poisson arrivals at 200 req/s (latency
     under the specified load) and all at once (throughput when the GPU is the
@@ -289,6 +375,8 @@ def bench_serving(jobs: int, max_bytes: int, slots: int = 4):
     out["poisson200_plain"] = serving.summarize(reqs, t1)
     out["saturated_ratio_MBps"] = round(out["saturated_served"]["aggregate_MBps"] /
                                         out["saturated_plain"]["aggregate_MBps"], 2)
+    if keep is not None:   # texts and kinds for the reference replay of the same trace
+        keep.extend((r.T, r.kind) for r in reqs)
     return out
 
 
@@ -330,7 +418,7 @@ def main():
         print(json.dumps({"batched": bench_batched()}), flush=True)
         return
     if args.mode == "degenerate":
-        print(json.dumps({"degenerate": bench_degenerate(lib, not args.no_cpu)}), flush=True)
+        print(json.dumps({"degenerate": bench_degenerate(lib)}), flush=True)
         return
     if args.mode == "copy":
         print(json.dumps({"copy_GBs": round(copy_peak(lib), 1), "bytes": 1 << 30}), flush=True)
@@ -503,18 +591,22 @@ def main():
                           "seconds": round(t_api, 3), "MBps": round(n / t_api / 1e6, 1)}
     del SAh, Uh, Ph, Lh
 
+    jobs = None
     if not args.no_extras:
         log("batched GSA ...")
         result["batched"] = bench_batched()
         log(f"batched: {result['batched']}")
         log("serving trace ...")
-        result["serving"] = bench_serving(600, 64 << 20)
+        jobs = [] if not args.no_cpu else None
+        result["serving"] = bench_serving(600, 64 << 20, keep=jobs)
         log(f"serving: {result['serving']}")
         log("degenerate inputs ...")
-        result["degenerate"] = bench_degenerate(lib, not args.no_cpu)
+        result["degenerate"] = bench_degenerate(lib)
     if not args.no_cpu:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(T, args.cpu_reps)
+        extras = not args.no_extras
+        result["cpu_baseline"] = cpu_baseline(T, args.cpu_reps, degenerate=extras, batched=extras,
+                                              serving_jobs=jobs if extras else None)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
