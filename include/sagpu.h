@@ -185,6 +185,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    first pass from a packed code stream (default); 0 = written first
  *   "raw_round1"     1 = with key_stream, the first doubling round orders each group by the
  *                    packed key that follows (same order as its rank; default); 0 = by rank
+ *   "exact_round1"   1 = with raw_round1, the first round records the exact lcp across each
+ *                    split, from the two raw keys (default); 0 = a lower bound, left to the PLCP pass
  *   "stream_compare" 1 = with key_stream, the PLCP pass compares suffixes on the packed code
  *                    stream, 64 / bits symbols per step (default); 0 = on the bytes
  *   "host_staging"   copies between caller host memory and the device: 1 = copies of at least

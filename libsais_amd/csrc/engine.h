@@ -118,16 +118,40 @@ struct RawKey1 {
     int bits, kbits, nspecial;
     u32 special[32];
 };
+// Round 1 on raw keys: key2 is the packed `kbits`-bit code of the ksym symbols at
+// i + h, so where the round splits a group, the lcp of the two adjacent subgroups
+// is h + the leading symbols their keys share, exactly.  Groups in `skip` keep the
+// LCP_UNKNOWN | h mark: those holding one of the last 2 ksym suffixes, whose keys
+// are rank-form or padded past the text end (padding would count as shared symbols).
+struct ExactLcp {
+    int on = 0;
+    int bits = 0, kbits = 0, nskip = 0;
+    u32 skip[64] = {};
+    __device__ bool skipped(u32 g) const
+    {
+        for (int t = 0; t < nskip; t++)
+            if (skip[t] == g) return true;
+        return false;
+    }
+    // symbols two different raw keys share from the top
+    __device__ u32 shared(u64 a, u64 b) const
+    {
+        const u32 x = (u32)(a ^ b);
+        return (u32)(__clz(x) - (32 - kbits)) / (u32)bits;
+    }
+};
 // Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
                  u32 *SA, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
                u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac);
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac,
+               const ExactLcp *ex = nullptr);
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
                 u64 *comp, u32 *vals, hipStream_t s);
 int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac);
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
+             const ExactLcp *ex = nullptr);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);

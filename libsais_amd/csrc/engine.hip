@@ -254,6 +254,8 @@ int g_key_stream = 1;
 int g_raw_round1 = 1;
 // 1: the PLCP pass compares suffixes on the code stream when there is one
 int g_stream_compare = 1;
+// 1: round 1 on raw keys records exact lcps across its splits; 0: lower bounds
+int g_exact_round1 = 1;
 // 1: Φ of a caller's suffix array by the bucketed two-pass scatter; 0: one pass
 int g_bucketed_phi = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
@@ -606,6 +608,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
 
     int alt = 0;
     RawKey1 rk1{};   // first round from the code stream (when there is one)
+    ExactLcp ex1{};  // and its exact lcps across the splits
     if (!sid) {
         u32 *k0 = kA, *k1 = kA + n, *v0 = vA, *v1 = vA + n;
         KeyGen gen{};
@@ -635,16 +638,25 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
                              alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr));
         if (use_gen && g_raw_round1) {
-            // group heads of the suffixes shorter than ksym: their groups keep ranks
+            // group heads of the last 2 ksym suffixes: those of the last ksym (shorter
+            // than ksym) keep ranks in round 1; none of them gets exact round-1 lcps
+            const u64 cnt2 = n < 2 * (u64)ksym ? n : 2 * (u64)ksym;
             const u64 cnt = n < (u64)ksym ? n : (u64)ksym;
             u32 *hs = (u32 *)(c.pinned + 6144);   // past the histogram and LUT staging
-            SAGPU_CHECK(hipMemcpyAsync(hs, dISA + (n - cnt), cnt * sizeof(u32), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(hs, dISA + (n - cnt2), cnt2 * sizeof(u32), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
             rk1.pw = gen.pw;
             rk1.bits = bits;
             rk1.kbits = ksym * bits;
             rk1.nspecial = (int)cnt;
-            for (u64 t = 0; t < cnt; t++) rk1.special[t] = hs[t];
+            for (u64 t = 0; t < cnt; t++) rk1.special[t] = hs[cnt2 - cnt + t];
+            if (g_exact_round1) {
+                ex1.on = 1;
+                ex1.bits = bits;
+                ex1.kbits = ksym * bits;
+                ex1.nskip = (int)cnt2;
+                for (u64 t = 0; t < cnt2; t++) ex1.skip[t] = hs[t];
+            }
         }
     } else {
         // batched GSA: (string id, symbols) keys, so every group stays inside one string
@@ -680,7 +692,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                    st.rounds == 1 && rk1.pw ? &rk1 : nullptr, ac.cb));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
-                                 out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac));
+                                 out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
+                                 st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
         }
         SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
@@ -736,7 +749,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             {
                 Span sp(c, F_ROUNDS);
                 SAGPU_TRY(big_emit(ws, alt ? b1 : b0, alt ? bv1 : bv0, d_bslot, total, h, wide, dSA, dISA, act_i[nxt],
-                                   act_g[nxt], out_counter, dLCP, s, ac));
+                                   act_g[nxt], out_counter, dLCP, s, ac, st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
@@ -1632,6 +1645,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "bucketed_phi")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_bucketed_phi = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "exact_round1")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_exact_round1 = (int)value;
         return 0;
     }
     if (!strcmp(name, "stream_compare")) {

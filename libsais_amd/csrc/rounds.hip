@@ -431,6 +431,7 @@ struct BigF {
     int kb;              // key2 width: 32, or 33 for texts longer than 2^31 - 1
     u32 *SA, *ISA, *out_i, *out_g, *LCP;
     ActCode ac;
+    ExactLcp ex;
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
@@ -465,7 +466,15 @@ struct BigF {
             SA[r] = v;
             if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
         }
-        if (LCP && hstart == (u32)e && hstart != gstart) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+        if (LCP && hstart == (u32)e && hstart != gstart) {
+            // e - 1 is the last entry of the previous subgroup of the same group
+            u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+            if (ex.on && !ex.skipped(g)) {
+                const u64 km = (1ull << kb) - 1;
+                mark = h + ex.shared(c & km, comp[e - 1] & km);
+            }
+            LCP[r] = mark;
+        }
     }
 };
 
@@ -552,7 +561,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                                                            u64 *__restrict__ out_counter, u32 *__restrict__ err,
                                                            u32 *__restrict__ big_head, u32 *__restrict__ big_g,
                                                            u32 *__restrict__ big_size, u32 *__restrict__ big_counter,
-                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac)
+                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac,
+                                                           ExactLcp ex)
 {
     constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
     static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
@@ -646,18 +656,24 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         const u32 gsz = (u32)gsz64;
         const KT key = s_k[x];
         u32 lt = 0, eq = 0, eqb = 0;
+        KT pk = 0;   // the largest key below this one: the previous subgroup's
         for (u32 y = x0; y < x0 + gsz; y++) {
             KT ky = s_k[y];
             lt += ky < key;
             eq += ky == key;
             eqb += (ky == key) & (y < x);
+            pk = (ky < key && ky > pk) ? ky : pk;
         }
         const u32 pos = x0 + lt + eqb;
         const u32 r = g + lt;
         s_oi[pos] = s_i[x];
         s_og[pos] = r;
         s_os[pos] = (eq > 1 ? 2 : 1) | (lt == 0 ? 4 : 0);  // 4: rank unchanged, ISA already holds it
-        if (LCP && lt > 0 && eqb == 0) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+        if (LCP && lt > 0 && eqb == 0) {
+            u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+            if (ex.on && !ex.skipped(g)) mark = h + ex.shared((u64)key, (u64)pk);
+            LCP[r] = mark;
+        }
     }
     __syncthreads();
     // ISA and SA writes happen after every thread has read SA[g] (group end marker).
@@ -798,13 +814,13 @@ int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u6
 template <typename KT>
 int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
                  u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter,
-                 u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac)
+                 u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp &ex)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
 #define RT_LAUNCH(R)                                                                                              \
     hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, \
                        hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,  \
-                       LCP, ac)
+                       LCP, ac, ex)
     if (g_tile_range == 256) RT_LAUNCH(256);
     else if (g_tile_range == 1024) RT_LAUNCH(1024);
     else RT_LAUNCH(512);
@@ -833,12 +849,13 @@ int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 
 
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
                u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac)
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex)
 {
+    const ExactLcp e = ex ? *ex : ExactLcp{};
     return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac)
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e)
                 : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac);
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e);
 }
 
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
@@ -849,9 +866,10 @@ int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, 
 }
 
 int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac)
+             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
+             const ExactLcp *ex)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP, ac};
+    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP, ac, ex ? *ex : ExactLcp{}};
     return ws_run(ws, f, total, out_counter, s);
 }

@@ -261,11 +261,13 @@ def test_keys_written_before_the_sort(name):
 
 
 @pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana", "acgt17"])
-@pytest.mark.parametrize("option", [b"raw_round1", b"stream_compare"])
+@pytest.mark.parametrize("option", [b"raw_round1", b"stream_compare", b"exact_round1"])
 def test_stream_shortcuts_off(name, option):
     """raw_round1 = 0: the first doubling round orders groups by rank (the default
     orders them by the packed key that follows, read from the code stream);
-    stream_compare = 0: the PLCP pass compares bytes instead of stream symbols."""
+    stream_compare = 0: the PLCP pass compares bytes instead of stream symbols;
+    exact_round1 = 0: round 1 marks its splits with lower bounds instead of the
+    exact lcps its raw keys give."""
     T = dict(CASES)[name]
     lib = la.load()
     assert lib.sagpu_set_option(option, 0) == 0
@@ -337,3 +339,51 @@ def test_isa_window_option(wlog):
         assert np.array_equal(la.suffix_array(T), SA_ref)
     finally:
         assert lib.sagpu_set_option(b"isa_window_log2", 25) == 0
+
+
+def _round1_cases():
+    rng = np.random.default_rng(4242)
+
+    def sym(alpha, k):
+        return np.frombuffer(alpha, np.uint8)[rng.integers(0, len(alpha), k)]
+
+    cases = []
+    # a 16-symbol prefix repeated 3000 times with random continuations: one initial
+    # group far above the LDS tile size, split in round 1 by the global sort
+    base = sym(b"ACGT", 16)
+    cases.append(("big_group", np.concatenate([np.concatenate([base, sym(b"ACGT", 24)]) for _ in range(3000)])))
+    # the text ends inside a repeat of earlier text: the last suffixes share their
+    # initial groups with suffixes whose round-1 keys are not padded
+    R = sym(b"ACGT", 48)
+    for cut in (48, 40, 33, 31, 20, 17, 16, 9):
+        cases.append((f"tail_repeat_{cut}", np.concatenate([sym(b"ACGT", 5000), R, sym(b"ACGT", 3000), R[:cut]])))
+    # alphabet widths (bits 1, 2, 3, 5, 8) with repeats so that round 1 splits groups
+    for alpha in (b"ab", b"ACGT", b"ACGTN", b"abcdefghijklmnopqrstu", bytes(range(256))):
+        unit = sym(alpha, 64)
+        parts = []
+        for _ in range(400):
+            parts.append(unit[: int(rng.integers(20, 64))])
+            parts.append(sym(alpha, int(rng.integers(1, 30))))
+        cases.append((f"alpha{len(alpha)}", np.concatenate(parts)))
+    return cases
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("name,T", _round1_cases(), ids=[c[0] for c in _round1_cases()])
+def test_round1_exact_lcps(name, T, exact):
+    """Round-1 splits on raw keys: exact lcps (default) and lower bounds both give
+    the oracle's SA, BWT, PLCP and LCP, also for groups split by the global sort
+    and for texts whose last suffixes repeat earlier text."""
+    lib = la.load()
+    assert lib.sagpu_set_option(b"exact_round1", exact) == 0
+    try:
+        rc, SA_ref = ol.sa(T)
+        P_ref = ol.plcp(T, SA_ref)
+        U_ref, idx_ref = ol.bwt(T, SA_ref)
+        fSA, fU, fidx, fP, fL, _ = fused(T)
+        assert np.array_equal(fSA, SA_ref)
+        assert np.array_equal(fU, U_ref) and fidx == idx_ref
+        assert np.array_equal(fP, P_ref)
+        assert np.array_equal(fL, ol.lcp(P_ref, SA_ref))
+    finally:
+        assert lib.sagpu_set_option(b"exact_round1", 1) == 0
