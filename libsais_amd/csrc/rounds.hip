@@ -200,13 +200,6 @@ struct InitF {
         act = e < n && !(head && tail);
         v1 = e < n && head ? (u32)e : 0u;
     }
-    // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
-    __device__ u32 pad_tail(u32 k) const
-    {
-        u32 t = 0;
-        while (t < (u32)ksym && ((k >> (bits * t)) & sepmask) == 0) t++;
-        return t;
-    }
     // Leading symbols two packed keys share, stopped at a separator in GSA mode.
     __device__ u32 key_lcp(u32 k, u32 kp) const
     {
@@ -223,18 +216,34 @@ struct InitF {
         }
         return l;
     }
-    // LCP mark of slot e >= 1: the lcp with slot e-1 when the keys decide it, else
-    // LCP_UNKNOWN | lower bound.  A pair from two different strings (batched GSA)
+    // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
+    __device__ u32 pad_tail(u32 k) const
+    {
+        u32 t = 0;
+        while (t < (u32)ksym && ((k >> (bits * t)) & sepmask) == 0) t++;
+        return t;
+    }
+    // LCP mark of slot e >= 1: the lcp when the keys decide it, else LCP_UNKNOWN |
+    // lower bound.  one / onep: slot e / e-1 is alone in its group, so it keeps its
+    // suffix, of length len / lenp.  A pair from two different strings (batched GSA)
     // always has a separator suffix second, whose lcp is 0 by the separator rule.
-    __device__ u32 lcp_mark(u32 k, u32 kp) const
+    __device__ u32 lcp_mark(u32 k, u32 kp, bool one, u64 len, bool onep, u64 lenp) const
     {
         const u32 l = key_lcp(k, kp);
         if (l >= (u32)ksym && !sep(k)) return LCP_UNKNOWN | (u32)ksym;
-        // One of the two groups may hold a suffix that ends within the first l
-        // symbols (its key is padded with code 0 there); its true lcp is then
-        // shorter, so leave it to the LCP pass with bound l.
-        if (!gsa && (pad_tail(k) + l >= (u32)ksym || pad_tail(kp) + l >= (u32)ksym)) return LCP_UNKNOWN | l;
-        return l;
+        if (gsa) return l;
+        // A suffix shorter than ksym has code-0 padding in its key past its end; where
+        // the keys agree that far, it is a prefix of the other and the lcp is its
+        // length.  A slot alone in its group has its suffix's length at hand.  A slot
+        // of a larger group may end up holding another member, possibly a shorter
+        // one: trailing code-0 symbols of the key reaching back to l leave the pair to
+        // the LCP pass with bound l.
+        u64 m = l;
+        if (one) m = len < m ? len : m;
+        else if (pad_tail(k) + l >= (u32)ksym) return LCP_UNKNOWN | l;
+        if (onep) m = lenp < m ? lenp : m;
+        else if (pad_tail(kp) + l >= (u32)ksym) return LCP_UNKNOWN | l;
+        return (u32)m;
     }
 };
 
@@ -377,6 +386,14 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         const u64 o = c.s + is - (u64)act;
         K kp = 0, kn;
         if (f.LCP) sk.neighbours(r, kp, kn);
+        // the slot before: its value (suffix) and whether it is alone in its group
+        // (unknown for the first slot of the span: taken as grouped)
+        const u32 vup = __shfl_up(vv[r], 1, SAGPU_WAVE);
+        const u32 vlast = r > 0 ? __shfl(vv[r > 0 ? r - 1 : 0], SAGPU_WAVE - 1, SAGPU_WAVE) : 0u;
+        const u32 vp = lane ? vup : vlast;
+        const u32 gup = __shfl_up(st[r] >> 31, 1, SAGPU_WAVE);
+        const u32 glast = r > 0 ? __shfl(st[r > 0 ? r - 1 : 0] >> 31, SAGPU_WAVE - 1, SAGPU_WAVE) : 1u;
+        const bool onep = !(lane ? gup : glast);
         if (e < n) {
             const u32 v = vv[r] >> f.cb;
             if (f.codes) f.codes[e] = (uint8_t)(vv[r] & cmask);
@@ -390,7 +407,10 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
                 f.act_i[o] = vv[r];   // with the BWT seed the code travels in the list entry
                 f.act_g[o] = g;
             }
-            if (f.LCP) f.LCP[e] = e > 0 ? f.lcp_mark((u32)sk.k[r], (u32)kp) : 0u;
+            if (f.LCP)
+                f.LCP[e] = e > 0 ? f.lcp_mark((u32)sk.k[r], (u32)kp, !act, n - (u64)(vv[r] >> f.cb), onep,
+                                              n - (u64)(vp >> f.cb))
+                                 : 0u;
         }
         c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m1 = __shfl(g, SAGPU_WAVE - 1, SAGPU_WAVE);
