@@ -63,6 +63,12 @@ int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, 
                    int end_bit, hipStream_t s, int *result_in_alt, int *passes_run);
 
 // ------------------------------------------------------- rounds (rounds.hip)
+// ISA words of texts of at most 2^28 suffixes may carry a PLCP hint in bits 28-31
+// (the fused build with LCP: 1 + the lcp of a suffix alone in its initial group,
+// 0 = none).  Every ISA reader takes the rank through isa_rank; for longer texts
+// nothing is packed and the word is the rank.
+__host__ __device__ __forceinline__ u32 isa_rank(u32 x, u64 n) { return n <= (1ull << 28) ? (x & 0x0FFFFFFFu) : x; }
+__host__ __device__ __forceinline__ u32 isa_hint(u32 x, u64 n) { return n <= (1ull << 28) ? (x >> 28) : 0u; }
 // LCP marks written during construction: LCP_UNKNOWN | lower bound for pairs that
 // were separated by a doubling round (their lcp is resolved afterwards).
 #define LCP_UNKNOWN 0x80000000u
@@ -99,7 +105,8 @@ struct ActCode {
 // PV, PG: scratch of n entries each ((suffix, group head) pairs bucketed for the
 // ISA scatter).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr);
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr,
+               int hint = 0);
 // ISA scatter after the initial sort: buckets of 2^v suffixes (v in [10, 32]; at most
 // 64 buckets, larger ones for longer texts).
 void set_isa_window_log2(int v);

@@ -256,6 +256,10 @@ int g_raw_round1 = 1;
 int g_stream_compare = 1;
 // 1: round 1 on raw keys records exact lcps across its splits; 0: lower bounds
 int g_exact_round1 = 1;
+// 1: the fused build with LCP packs the exact lcp of each suffix alone in its
+// initial group into its ISA word (texts up to 2^28), so the PLCP pass skips its
+// pair-word read; 0: every position reads its pair word
+int g_plcp_hint = 1;
 // 1: Φ of a caller's suffix array by the bucketed two-pass scatter; 0: one pass
 int g_bucketed_phi = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
@@ -547,7 +551,7 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
 // launch_bwt_seeded (seed->cb > 0 on return); otherwise seed->cb stays 0.
 int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
              const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0,
-             BwtSeed *seed = nullptr, CodeStream *cs = nullptr)
+             BwtSeed *seed = nullptr, CodeStream *cs = nullptr, int hint = 0)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
@@ -636,7 +640,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         Span sp(c, F_INITIAL);
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
-                             alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr));
+                             alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr, hint && !gsa && !dSym));
         if (use_gen && g_raw_round1) {
             // group heads of the last 2 ksym suffixes: those of the last ksym (shorter
             // than ksym) keep ranks in round 1; none of them gets exact round-1 lcps
@@ -649,6 +653,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             rk1.bits = bits;
             rk1.kbits = ksym * bits;
             rk1.nspecial = (int)cnt;
+            for (u64 t = 0; t < cnt2; t++) hs[t] = isa_rank(hs[t], n);
             for (u64 t = 0; t < cnt; t++) rk1.special[t] = hs[cnt2 - cnt + t];
             if (g_exact_round1) {
                 ex1.on = 1;
@@ -938,6 +943,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         dSA = sc.get<u32>(n);
         if (!dSA) return SAGPU_ENOMEM;
     }
+    const bool own_isa = !dISA;
     if (!dISA) {
         dISA = sc.get<u32>(n);
         if (!dISA) return SAGPU_ENOMEM;
@@ -958,8 +964,9 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         Scope work(c);  // construction buffers go back to the pool before the caller's downloads
         BwtSeed seed{};
         CodeStream cs{};
+        // the PLCP hints ride in the ISA words only when the ISA stays inside this call
         rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr, nullptr, 0, nullptr, 0,
-                      dU && g_bwt_variant == 0 ? &seed : nullptr, &cs);
+                      dU && g_bwt_variant == 0 ? &seed : nullptr, &cs, want_lcp && own_isa && g_plcp_hint);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
@@ -983,7 +990,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
     if (dU) {
         u32 j0 = 0;
         SAGPU_TRY(read_u32(c, dISA, &j0));
-        ret = (int64_t)j0 + 1;
+        ret = (int64_t)isa_rank(j0, n) + 1;
     }
     SAGPU_CHECK(hipStreamSynchronize(c.stream));
     timer_collect(c);
@@ -1417,6 +1424,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U,
     SAGPU_TRY(launch_bwt16(dT, dSA, dISA, N, dU, s));
     u32 j0 = 0;
     SAGPU_TRY(read_u32(*c, dISA, &j0));
+    j0 = isa_rank(j0, N);
     if (r > 0) {
         const u64 count = (N - 1) / (u64)r + 1;
         GET(dS, u32, count);
@@ -1645,6 +1653,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "bucketed_phi")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_bucketed_phi = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "plcp_hint")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_plcp_hint = (int)value;
         return 0;
     }
     if (!strcmp(name, "exact_round1")) {

@@ -26,7 +26,7 @@ template <typename ST>
 __global__ __launch_bounds__(256) void k_bwt(const ST *__restrict__ T, const u32 *__restrict__ SA,
                                              const u32 *__restrict__ ISA, u64 n, ST *__restrict__ U)
 {
-    const u64 j0 = ISA[0];
+    const u64 j0 = isa_rank(ISA[0], n);
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         u32 s = SA[j];
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ 
     __shared__ uint8_t s_inv[256];
     s_inv[threadIdx.x] = inv[threadIdx.x];
     __syncthreads();
-    const u64 j0 = ISA[0];
+    const u64 j0 = isa_rank(ISA[0], n);
     const u64 quads = (n + 3) / 4;
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
@@ -74,14 +74,14 @@ __global__ __launch_bounds__(256) void k_bwt_seeded(const uint8_t *__restrict__ 
 __global__ __launch_bounds__(256) void k_bwt_scatter(const uint8_t *__restrict__ T, const u32 *__restrict__ ISA,
                                                      u64 n, uint8_t *__restrict__ U)
 {
-    const u64 j0 = ISA[0];
+    const u64 j0 = isa_rank(ISA[0], n);
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         if (i == 0) {
             U[0] = T[n - 1];
             continue;
         }
-        const u64 j = ISA[i];
+        const u64 j = isa_rank(ISA[i], n);
         U[j + (j < j0)] = T[i - 1];
     }
 }
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr(const uint8_t *__restrict__ T,
             k = PHI[i];
             if (k == i) k = n;  // first suffix in SA order
         } else {
-            u32 r = ISA[i];
+            u32 r = isa_rank(ISA[i], n);
             if (marks) {
                 // construction already knows most lcps: read it where i sits in SA
                 u32 p = r < n ? marks[r] : 0u;
@@ -355,23 +355,27 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
     for (u64 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         VT mx = 0;
         for (u64 base = tile * MS_TILE; base < (tile + 1) * MS_TILE && base < n; base += STEP) {
-            u32 r[PI];
+            u32 r[PI], hint[PI];
             u64 w[PI];
             uint8_t a[PI];
 #pragma unroll
             for (int q = 0; q < PI; q++) {
                 const u64 i = base + (u64)q * 256 + threadIdx.x;
-                r[q] = i < n ? ISA[i] : 0u;
+                const u32 x = i < n ? ISA[i] : 0u;
+                r[q] = isa_rank(x, n);
+                hint[q] = isa_hint(x, n);   // 1 + PLCP[i] known from the initial grouping
                 a[q] = (i > 0 && i < n) ? T[i - 1] : 0;
             }
 #pragma unroll
-            for (int q = 0; q < PI; q++) w[q] = r[q] ? pairs[r[q]] : 0ull;
+            for (int q = 0; q < PI; q++) w[q] = (r[q] && !hint[q]) ? pairs[r[q]] : 0ull;
 #pragma unroll
             for (int q = 0; q < PI; q++) {
                 const u64 i = base + (u64)q * 256 + threadIdx.x;
                 if (i >= n) continue;
                 VT v;
-                if (r[q] == 0) {   // first suffix in SA order: lcp 0
+                if (hint[q]) {
+                    v = (VT)(hint[q] - 1 + i);
+                } else if (r[q] == 0) {   // first suffix in SA order: lcp 0
                     v = (VT)i;
                 } else if (!(w[q] >> 63)) {
                     v = (VT)((u64)(u32)w[q] + i);
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(256) void k_plcp_pairs(const u32 *__restrict__ SA, 
                                                     const uint8_t *__restrict__ U, const uint8_t *__restrict__ T,
                                                     const u32 *__restrict__ ISA, u64 n, u64 *__restrict__ pairs)
 {
-    const u64 j0 = ISA[0];
+    const u64 j0 = isa_rank(ISA[0], n);
     const u64 chunk = 256ull * PP;
     for (u64 base = (u64)blockIdx.x * chunk; base < n; base += (u64)gridDim.x * chunk) {
         u32 m[PP], k[PP];
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
             k = PHI[i];
             if (k == i) k = n;
         } else {
-            u32 r = ISA[i];
+            u32 r = isa_rank(ISA[i], n);
             k = r == 0 ? n : SA[r - 1];
         }
         if (k > n) k = n;

@@ -184,6 +184,9 @@ struct InitF {
     // range); bcur[b] counts the entries taken so far
     u32 *PV, *PG, *bcur;
     int wlog;
+    // 1: the head of a slot alone in its group carries 1 + its exact lcp mark (< 15)
+    // in bits 28-31, the PLCP hint of its suffix (texts of at most 2^28 suffixes)
+    int hint;
     // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
     // it.  codes[e] receives that code; entries of the active list keep the packed
     // form, and the rounds write the code of every suffix they place.
@@ -397,9 +400,13 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
         if (e < n) {
             const u32 v = vv[r] >> f.cb;
             if (f.codes) f.codes[e] = (uint8_t)(vv[r] & cmask);
+            u32 mark = 0;
+            if (f.LCP && e > 0)
+                mark = f.lcp_mark((u32)sk.k[r], (u32)kp, !act, n - (u64)(vv[r] >> f.cb), onep, n - (u64)(vp >> f.cb));
+            const u32 hint = (f.hint && f.LCP && !act && mark < 15u) ? (mark + 1u) << 28 : 0u;   // unknown: >= 2^31
             const u64 slot = s_base[(u32)((u64)v >> f.wlog)] + (st[r] & 0x3fffffffu);
             f.PV[slot] = v;
-            f.PG[slot] = g;
+            f.PG[slot] = g | hint;
             // every slot is written (full lines); slots of grouped elements are
             // overwritten when their suffixes are placed
             f.SA[e] = v;
@@ -407,10 +414,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
                 f.act_i[o] = vv[r];   // with the BWT seed the code travels in the list entry
                 f.act_g[o] = g;
             }
-            if (f.LCP)
-                f.LCP[e] = e > 0 ? f.lcp_mark((u32)sk.k[r], (u32)kp, !act, n - (u64)(vv[r] >> f.cb), onep,
-                                              n - (u64)(vp >> f.cb))
-                                 : 0u;
+            if (f.LCP) f.LCP[e] = mark;
         }
         c.s += __shfl(is, SAGPU_WAVE - 1, SAGPU_WAVE);
         c.m1 = __shfl(g, SAGPU_WAVE - 1, SAGPU_WAVE);
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
                 const u64 x = ((u64)rk.pw[w] << 32) | (u64)rk.pw[w + 1];
                 r[q] = (u32)((x << (bitpos & 31)) >> (64 - rk.kbits));
             } else {
-                r[q] = (k < a && ih < n) ? ISA[ih] : 0u;
+                r[q] = (k < a && ih < n) ? isa_rank(ISA[ih], n) : 0u;
             }
         }
 #pragma unroll
@@ -777,12 +781,13 @@ int g_tile_range = 512;
 
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr)
+                 u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr,
+                 int hint = 0)
 {
     const int cb = seed ? seed->cb : 0;
     const int wlog = isa_bucket_log2(n);
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
-               cb, seed ? seed->codes : nullptr};
+               hint && n <= (1ull << 28) ? 1 : 0, cb, seed ? seed->codes : nullptr};
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
@@ -810,9 +815,10 @@ int isa_bucket_log2(u64 n)
 void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed)
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed, int hint)
 {
-    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, PV, PG, s, seed);
+    return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, PV, PG, s, seed,
+                             hint);
 }
 
 int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bits, int ksym, u32 *SA, u32 *ISA,

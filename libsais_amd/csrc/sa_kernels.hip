@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_batch_extract(const u32 *__restrict__ g
         if (outLCP) outLCP[po + jj] = gLCP[j];
         if (outU) {
             const u64 nk = (u64)off[k + 1] - o - 1;
-            const u64 j0 = (u64)ISA[o] - o - 1;
+            const u64 j0 = (u64)isa_rank(ISA[o], N) - o - 1;
             if (v == 0) outU[po] = T[o + nk - 1];
             else outU[po + jj + (jj < j0)] = T[o + v - 1];
         }

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_aux_samples(const u32 *__restrict__ ISA
                                                      u32 *__restrict__ out)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) out[k] = ISA[k * r];
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) out[k] = isa_rank(ISA[k * r], n);
 }
 
 unsigned grid_of(u64 work, unsigned per_block, unsigned cap)

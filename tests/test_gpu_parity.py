@@ -261,13 +261,14 @@ def test_keys_written_before_the_sort(name):
 
 
 @pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana", "acgt17"])
-@pytest.mark.parametrize("option", [b"raw_round1", b"stream_compare", b"exact_round1"])
+@pytest.mark.parametrize("option", [b"raw_round1", b"stream_compare", b"exact_round1", b"plcp_hint"])
 def test_stream_shortcuts_off(name, option):
     """raw_round1 = 0: the first doubling round orders groups by rank (the default
     orders them by the packed key that follows, read from the code stream);
     stream_compare = 0: the PLCP pass compares bytes instead of stream symbols;
     exact_round1 = 0: round 1 marks its splits with lower bounds instead of the
-    exact lcps its raw keys give."""
+    exact lcps its raw keys give; plcp_hint = 0: no PLCP hints in the ISA words,
+    every position of the PLCP pass reads its pair word."""
     T = dict(CASES)[name]
     lib = la.load()
     assert lib.sagpu_set_option(option, 0) == 0
