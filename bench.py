@@ -6,15 +6,16 @@ index, PLCP and LCP, all int32, outputs left in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 
-roofline: the kernel with the largest share of the build, the text-order PLCP
-pass over pair words (k_plcp_irr_pairs).  Its algorithmic bytes per launch are
-17 B per position (ISA 4, the preceding byte 1, one 8-byte pair word, the value
-written 4; the compare reads at irreducible positions are not counted); the
-launch duration comes from HIP events recorded on the engine stream around it
-inside the timed steps.  The onesweep radix pass of the initial sort (u32 key +
-u32 value, n x 16 B per launch) is reported beside it as radix_pass.  traffic is
-the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled
-by a copy kernel of known size), read from profiles/pmc_traffic.json.
+roofline: the kernel function with the largest share of the build, the onesweep
+radix pass (k_radix_pass: the generating first pass and three array-to-array
+passes of the initial sort).  Reported for its array-to-array launches (u32 key +
+u32 value): n x (4 + 4) B read and written per launch, launch durations from HIP
+events recorded on the engine stream around every pass inside the timed steps.
+The text-order PLCP pass (k_plcp_irr_pairs, 17 B per position: ISA 4, the
+preceding byte 1, one 8-byte pair word, the value written 4) is reported beside
+it as plcp_pass.  traffic is the HBM bytes per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
+profiles/pmc_traffic.json.
 pipeline: B_model = 269 n + 244 sum_r a_r bytes per step (prefix-doubling yardstick
 with 8 initial symbols; a_r counts suffixes with max(LCP[j], LCP[j+1]) >= 8*2^(r-1),
 computed from the LCP this run produced), divided by the step time.
@@ -532,29 +533,29 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_plcp_irr_pairs (text-order PLCP pass over pair words)",
-            "achieved": round(pp_achieved, 1),
+            "kernel": "k_radix_pass<u32 key, u32 value> (onesweep LSD pass of the initial sort)",
+            "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(pp_achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(workload, "plcp_pairs_pass_bytes_per_launch"),
-            "launches": pp_launch,
-            "avg_launch_ms": round(pp_avg, 4),
-            "algorithmic_bytes_per_launch": pp_bytes,
-            "random_reads_G_per_s": round(n / (pp_avg / 1000.0) / 1e9, 1) if pp_ms > 0 else None,
-            "note": "one random 8-byte pair-word read per position; each costs a whole line of HBM traffic",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload),
             "copy_peak_GBs": round(cpeak, 1),
-            "radix_pass": {
-                "kernel": "k_radix_pass<u32 key, u32 value> (onesweep LSD pass of the initial sort)",
-                "achieved": round(achieved, 1),
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "frac_of_copy": round(achieved / cpeak, 4),
-                "traffic": measured_traffic(workload),
-                "launches": u32_launch,
-                "avg_launch_ms": round(u32_ms / max(1, u32_launch), 4),
-                "algorithmic_bytes_per_launch": u32_bytes // max(1, u32_launch),
-                "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes,
-                                     "achieved_GBs": round(all_achieved, 1)},
+            "frac_of_copy": round(achieved / cpeak, 4),
+            "launches": u32_launch,
+            "avg_launch_ms": round(u32_ms / max(1, u32_launch), 4),
+            "algorithmic_bytes_per_launch": u32_bytes // max(1, u32_launch),
+            "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes,
+                                 "achieved_GBs": round(all_achieved, 1)},
+            "plcp_pass": {
+                "kernel": "k_plcp_irr_pairs (text-order PLCP pass)",
+                "achieved": round(pp_achieved, 1),
+                "frac": round(pp_achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(workload, "plcp_pairs_pass_bytes_per_launch"),
+                "launches": pp_launch,
+                "avg_launch_ms": round(pp_avg, 4),
+                "algorithmic_bytes_per_launch": pp_bytes,
+                "note": "random reads: one 8-byte pair word per position without a PLCP hint in its ISA word, "
+                        "one code-stream window per irreducible position; each costs a whole line",
             },
         },
         "pipeline": {
