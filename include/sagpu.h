@@ -187,9 +187,10 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    packed key that follows (same order as its rank; default); 0 = by rank
  *   "exact_round1"   1 = with raw_round1, the first round records the exact lcp across each
  *                    split, from the two raw keys (default); 0 = a lower bound, left to the PLCP pass
- *   "plcp_hint"      1 = the fused build with LCP (texts up to 2^28) packs 1 + the exact lcp of
- *                    each suffix alone in its initial group into bits 28-31 of its ISA word, and
- *                    the PLCP pass takes it instead of reading a pair word (default); 0 = no hints
+ *   "plcp_hint"      1 = the fused build with LCP (texts up to 2^28) packs the exact lcp of each
+ *                    suffix alone in its initial group into bits 28-31 of its ISA word (15 values
+ *                    on the largest marks the initial key allows), and the PLCP pass takes it
+ *                    instead of reading a pair word (default); 0 = no hints
  *   "stream_compare" 1 = with key_stream, the PLCP pass compares suffixes on the packed code
  *                    stream, 64 / bits symbols per step (default); 0 = on the bytes
  *   "host_staging"   copies between caller host memory and the device: 1 = copies of at least

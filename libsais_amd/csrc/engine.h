@@ -69,6 +69,13 @@ int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, 
 // nothing is packed and the word is the rank.
 __host__ __device__ __forceinline__ u32 isa_rank(u32 x, u64 n) { return n <= (1ull << 28) ? (x & 0x0FFFFFFFu) : x; }
 __host__ __device__ __forceinline__ u32 isa_hint(u32 x, u64 n) { return n <= (1ull << 28) ? (x >> 28) : 0u; }
+// A hint h (1..15) stands for the lcp off + h - 1: off = max(0, ksym - 15) puts the
+// 15 values on the largest marks an initial key allows (marks of DNA with 16-symbol
+// keys: 1..15).  on (in) asks for the hints; on (out) tells whether they were packed.
+struct PlcpHint {
+    int on = 0;
+    int off = 0;
+};
 // LCP marks written during construction: LCP_UNKNOWN | lower bound for pairs that
 // were separated by a doubling round (their lcp is resolved afterwards).
 #define LCP_UNKNOWN 0x80000000u
@@ -106,7 +113,7 @@ struct ActCode {
 // ISA scatter).
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr,
-               int hint = 0);
+               const PlcpHint *hint = nullptr);
 // ISA scatter after the initial sort: buckets of 2^v suffixes (v in [10, 32]; at most
 // 64 buckets, larger ones for longer texts).
 void set_isa_window_log2(int v);
@@ -219,7 +226,8 @@ int launch_phi_bucketed(const u32 *SA, u64 n, u32 *PHI, u32 *PV, u32 *PG, u32 *b
 // compares instead of the bytes.
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA,
                 const u32 *SA, u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes = 1,
-                u64 *pairs = nullptr, const uint8_t *U = nullptr, const u32 *pw = nullptr, int bits = 0);
+                u64 *pairs = nullptr, const uint8_t *U = nullptr, const u32 *pw = nullptr, int bits = 0,
+                int hint_off = 0);
 // LCP[j] = PLCP[SA[j]] where LCP[j] is still an unknown mark.
 int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // BWT auxiliary samples and inverse BWT (unbwt.hip).

@@ -551,7 +551,7 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
 // launch_bwt_seeded (seed->cb > 0 on return); otherwise seed->cb stays 0.
 int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32 *dISA, u64 *freq_out, u32 *dLCP,
              const u32 *sid = nullptr, u32 nstrings = 0, const u32 *dSym = nullptr, int sym_bits = 0,
-             BwtSeed *seed = nullptr, CodeStream *cs = nullptr, int hint = 0)
+             BwtSeed *seed = nullptr, CodeStream *cs = nullptr, PlcpHint *hint = nullptr)
 {
     hipStream_t s = c.stream;
     sagpu_stats &st = c.stats;
@@ -638,9 +638,13 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         c.rs.gen = nullptr;
         SAGPU_TRY(src);
         Span sp(c, F_INITIAL);
+        if (hint) {   // byte texts without separators, up to 2^28 suffixes
+            hint->on = hint->on && !gsa && !dSym && n <= (1ull << 28);
+            hint->off = ksym > 15 ? ksym - 15 : 0;
+        }
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
-                             alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr, hint && !gsa && !dSym));
+                             alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr, hint));
         if (use_gen && g_raw_round1) {
             // group heads of the last 2 ksym suffixes: those of the last ksym (shorter
             // than ksym) keep ranks in round 1; none of them gets exact round-1 lcps
@@ -787,7 +791,7 @@ int phi_run(Ctx &c, Scope &sc, const u32 *dSA, u64 n, u32 *dPHI)
 
 int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
              u32 *PLCP, const u32 *marks, int sym_bytes = 1, const uint8_t *dU = nullptr,
-             const CodeStream *cs = nullptr)
+             const CodeStream *cs = nullptr, int hint_off = 0)
 {
     LcpScratch ls{};
     ls.queue_cap = n / 16 + 4096;
@@ -813,7 +817,7 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
         if (!pairs) return SAGPU_ENOMEM;
     }
     return launch_plcp(ls, dT, n, gsa, PHI, ISA, SA, PLCP, V64, marks, c.stream, sym_bytes, pairs, dU,
-                       cs && g_stream_compare ? cs->pw : nullptr, cs ? cs->bits : 0);
+                       cs && g_stream_compare ? cs->pw : nullptr, cs ? cs->bits : 0, hint_off);
 }
 
 // Pinned slots of `chunk` bytes for the staged copies; false if the mode asks for
@@ -965,8 +969,10 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         BwtSeed seed{};
         CodeStream cs{};
         // the PLCP hints ride in the ISA words only when the ISA stays inside this call
+        PlcpHint hint;
+        hint.on = want_lcp && own_isa && g_plcp_hint;
         rc = sa_build(c, work, dTpad, n, gsa, dSA, dISA, freq, want_lcp ? marks : nullptr, nullptr, 0, nullptr, 0,
-                      dU && g_bwt_variant == 0 ? &seed : nullptr, &cs, want_lcp && own_isa && g_plcp_hint);
+                      dU && g_bwt_variant == 0 ? &seed : nullptr, &cs, &hint);
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
@@ -978,7 +984,7 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
             {
                 Span sp(c, F_PLCP);
                 SAGPU_TRY(plcp_run(c, work, dTpad, n, gsa, nullptr, dISA, dSA, dPLCP, marks, 1, dU,
-                                   cs.pw ? &cs : nullptr));
+                                   cs.pw ? &cs : nullptr, hint.on ? hint.off : 0));
             }
             if (dLCP) {
                 Span sp(c, F_LCP);

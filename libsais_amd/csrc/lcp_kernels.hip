@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
                                                         VT *__restrict__ V, u32 *__restrict__ queue,
                                                         u32 *__restrict__ queue_l, u32 *__restrict__ queue_n,
                                                         u64 queue_cap, const u32 *__restrict__ pw, int bits,
-                                                        VT *__restrict__ tmax)
+                                                        VT *__restrict__ tmax, int hint_off)
 {
     constexpr u64 STEP = 256ull * PI;
     static_assert(MS_TILE % STEP == 0, "tile must be whole steps");
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
                 if (i >= n) continue;
                 VT v;
                 if (hint[q]) {
-                    v = (VT)(hint[q] - 1 + i);
+                    v = (VT)((u64)hint_off + hint[q] - 1 + i);
                 } else if (r[q] == 0) {   // first suffix in SA order: lcp 0
                     v = (VT)i;
                 } else if (!(w[q] >> 63)) {
@@ -769,7 +769,7 @@ int launch_lcp_fix(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t 
 template <typename VT, int W>
 int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
                   u32 *PLCP, VT *V, const u32 *marks, hipStream_t s, const u64 *pairs = nullptr,
-                  const u32 *pw = nullptr, int bits = 0)
+                  const u32 *pw = nullptr, int bits = 0, int hint_off = 0)
 {
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
     if (tiles > ls.tiles_cap) {
@@ -784,7 +784,7 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
             if (ls.hook) ls.hook(ls.hook_user, 0);
             hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3((unsigned)(tiles < 65535 ? tiles : 65535)), dim3(256), 0, s,
                                T, n, gsa, ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap,
-                               gsa ? nullptr : pw, bits, tmax);
+                               gsa ? nullptr : pw, bits, tmax, hint_off);
             if (ls.hook) ls.hook(ls.hook_user, 1);
         }
     } else {
@@ -802,7 +802,7 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
 
 int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI, const u32 *ISA, const u32 *SA,
                 u32 *PLCP, u64 *V64, const u32 *marks, hipStream_t s, int sym_bytes, u64 *pairs, const uint8_t *U,
-                const u32 *pw, int bits)
+                const u32 *pw, int bits, int hint_off)
 {
     if (sym_bytes == 2) {
         if (marks || !PHI) {
@@ -826,8 +826,8 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
     } else {
         pairs = nullptr;
     }
-    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s, pairs, pw, bits);
-    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s, pairs, pw, bits);
+    if (V64) return launch_plcp_t<u64, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, V64, marks, s, pairs, pw, bits, hint_off);
+    return launch_plcp_t<u32, 1>(ls, T, n, gsa, PHI, ISA, SA, PLCP, PLCP, marks, s, pairs, pw, bits, hint_off);
 }
 
 int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s)

@@ -184,9 +184,10 @@ struct InitF {
     // range); bcur[b] counts the entries taken so far
     u32 *PV, *PG, *bcur;
     int wlog;
-    // 1: the head of a slot alone in its group carries 1 + its exact lcp mark (< 15)
-    // in bits 28-31, the PLCP hint of its suffix (texts of at most 2^28 suffixes)
-    int hint;
+    // 1: the head of a slot alone in its group carries its exact lcp mark as a PLCP
+    // hint in bits 28-31 (mark - hint_off + 1, for marks in [hint_off, hint_off + 14];
+    // texts of at most 2^28 suffixes)
+    int hint, hint_off;
     // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
     // it.  codes[e] receives that code; entries of the active list keep the packed
     // form, and the rounds write the code of every suffix they place.
@@ -403,7 +404,10 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
             u32 mark = 0;
             if (f.LCP && e > 0)
                 mark = f.lcp_mark((u32)sk.k[r], (u32)kp, !act, n - (u64)(vv[r] >> f.cb), onep, n - (u64)(vp >> f.cb));
-            const u32 hint = (f.hint && f.LCP && !act && mark < 15u) ? (mark + 1u) << 28 : 0u;   // unknown: >= 2^31
+            // (an unknown mark is >= 2^31 and gets none)
+            const u32 hint = (f.hint && f.LCP && !act && mark >= (u32)f.hint_off && mark - (u32)f.hint_off < 15u)
+                                 ? (mark - (u32)f.hint_off + 1u) << 28
+                                 : 0u;
             const u64 slot = s_base[(u32)((u64)v >> f.wlog)] + (st[r] & 0x3fffffffu);
             f.PV[slot] = v;
             f.PG[slot] = g | hint;
@@ -782,12 +786,12 @@ int g_tile_range = 512;
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
                  u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed = nullptr,
-                 int hint = 0)
+                 const PlcpHint *hint = nullptr)
 {
     const int cb = seed ? seed->cb : 0;
     const int wlog = isa_bucket_log2(n);
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
-               hint && n <= (1ull << 28) ? 1 : 0, cb, seed ? seed->codes : nullptr};
+               hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, cb, seed ? seed->codes : nullptr};
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
@@ -815,7 +819,8 @@ int isa_bucket_log2(u64 n)
 void set_tile_range(int v) { g_tile_range = v; }
 
 int group_init(WsScratch &ws, const u32 *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
-               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed, int hint)
+               u32 *act_i, u32 *act_g, u32 *LCP, u32 *PV, u32 *PG, hipStream_t s, const BwtSeed *seed,
+               const PlcpHint *hint)
 {
     return group_init_t<u32>(ws, keys, vals, n, bits, ksym, gsa, SA, ISA, act_i, act_g, LCP, PV, PG, s, seed,
                              hint);
