@@ -64,9 +64,9 @@ int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, 
 
 // ------------------------------------------------------- rounds (rounds.hip)
 // ISA words of texts of at most 2^28 suffixes may carry a PLCP hint in bits 28-31
-// (the fused build with LCP: 1 + the lcp of a suffix alone in its initial group,
-// 0 = none).  Every ISA reader takes the rank through isa_rank; for longer texts
-// nothing is packed and the word is the rank.
+// (the fused build with LCP: the lcp of a suffix alone in its initial group, coded
+// as in PlcpHint; 0 = none).  Every ISA reader takes the rank through isa_rank; for
+// longer texts nothing is packed and the word is the rank.
 __host__ __device__ __forceinline__ u32 isa_rank(u32 x, u64 n) { return n <= (1ull << 28) ? (x & 0x0FFFFFFFu) : x; }
 __host__ __device__ __forceinline__ u32 isa_hint(u32 x, u64 n) { return n <= (1ull << 28) ? (x >> 28) : 0u; }
 // A hint h (1..15) stands for the lcp off + h - 1: off = max(0, ksym - 15) puts the
