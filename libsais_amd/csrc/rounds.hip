@@ -223,9 +223,9 @@ struct InitF {
     // Number of trailing code-0 symbols of a key (where end-of-text padding would be).
     __device__ u32 pad_tail(u32 k) const
     {
-        u32 t = 0;
-        while (t < (u32)ksym && ((k >> (bits * t)) & sepmask) == 0) t++;
-        return t;
+        const u32 kb = (u32)(ksym * bits);
+        const u32 kk = kb >= 32 ? k : (k & ((1u << kb) - 1u));
+        return kk ? (u32)__builtin_ctz(kk) / (u32)bits : (u32)ksym;
     }
     // LCP mark of slot e >= 1: the lcp when the keys decide it, else LCP_UNKNOWN |
     // lower bound.  one / onep: slot e / e-1 is alone in its group, so it keeps its
