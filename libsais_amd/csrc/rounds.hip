@@ -188,6 +188,7 @@ struct InitF {
     // hint in bits 28-31 (mark - hint_off + 1, for marks in [hint_off, hint_off + 14];
     // texts of at most 2^28 suffixes)
     int hint, hint_off;
+    u32 lowbits;   // the lowest bit of each of the ksym symbol slots of a key
     // BWT seeding (cb > 0): vals carry (suffix << cb) | code of the symbol before
     // it.  codes[e] receives that code; entries of the active list keep the packed
     // form, and the rounds write the code of every suffix they place.
@@ -211,11 +212,15 @@ struct InitF {
         u32 x = k ^ kp;
         if (x) l = (u32)(__clz(x) - (32 - ksym * bits)) / (u32)bits;
         if (gsa) {
-            for (int t = 0; t < ksym; t++) {
-                if (((k >> (bits * (ksym - 1 - t))) & sepmask) == 0) {
-                    l = (u32)t < l ? (u32)t : l;
-                    break;
-                }
+            // the first separator (code 0) from the top: fold each symbol's bits of ~k
+            // into its lowest bit, keep those bits, take the highest
+            const u32 z = ~k;
+            u32 m = z;
+            for (int b = 1; b < bits; b++) m &= z >> b;
+            m &= lowbits;
+            if (m) {
+                const u32 t = (u32)(ksym - 1) - (u32)(31 - __clz(m)) / (u32)bits;
+                l = t < l ? t : l;
             }
         }
         return l;
@@ -791,7 +796,9 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     const int cb = seed ? seed->cb : 0;
     const int wlog = isa_bucket_log2(n);
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
-               hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, cb, seed ? seed->codes : nullptr};
+               hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, 0u, cb,
+               seed ? seed->codes : nullptr};
+    for (int i = 0; i < ksym; i++) f.lowbits |= 1u << (bits * i);
     const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
