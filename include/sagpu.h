@@ -46,8 +46,8 @@ SAGPU_API int32_t sagpu_synchronize(void);
  * Touches no device; returns -1 for a negative thread count. */
 SAGPU_API int32_t sagpu_host_memcpy(void *h_dst, const void *h_src, size_t bytes, int32_t threads);
 
-/* Caps the engine's device working memory (bytes; 0 = unlimited).  Requests beyond
- * the cap fail with -2, exactly like an allocation failure in libsais. */
+/* Caps the device working memory of each engine context (bytes; 0 = unlimited).
+ * Requests beyond the cap fail with -2, exactly like an allocation failure in libsais. */
 SAGPU_API void    sagpu_set_memory_limit(uint64_t bytes);
 /* Releases cached (pooled) device buffers. */
 SAGPU_API void    sagpu_trim(void);
@@ -130,7 +130,7 @@ SAGPU_API int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t 
 SAGPU_API int32_t sagpu_lcp32(const int32_t *d_PLCP, const int32_t *d_SA, int32_t *d_LCP, int64_t n);
 
 /*
- * Kernel timing and round statistics of the most recent sagpu_build32 call, for
+ * Kernel timing and round statistics of the most recent build (of any thread), for
  * benchmarks: per-kernel-family device time (ms, from events on the engine stream)
  * and the number of suffixes still unsorted entering each doubling round.
  */
@@ -157,6 +157,8 @@ typedef struct sagpu_stats {
     int64_t sorted_bytes_u32;     /* algorithmic bytes of the u32-key passes        */
     double  ms_plcp_pairs_pass;   /* the text-order PLCP pass over pair words      */
     int32_t plcp_pairs_launches;
+    int64_t device_allocs;        /* device / pinned allocations the engine has made for
+                                     its working memory in this process so far       */
 } sagpu_stats;
 
 /*
@@ -207,6 +209,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    scattered bucket after bucket
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)
+ *   "api_slots"      contexts that calls made without a libsais context share, 1-16
+ *                    (default 8): that many such calls run at once, each on its own stream
  * Returns 0, or -1 for an unknown name or value.
  */
 SAGPU_API int32_t sagpu_set_option(const char *name, int64_t value);

@@ -77,6 +77,7 @@ class SagpuStats(ctypes.Structure):
         ("sorted_bytes_u32", ctypes.c_int64),
         ("ms_plcp_pairs_pass", ctypes.c_double),
         ("plcp_pairs_launches", ctypes.c_int32),
+        ("device_allocs", ctypes.c_int64),
     ]
 
 

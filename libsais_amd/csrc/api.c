@@ -10,6 +10,7 @@
 #define LIBSAIS_OPENMP 1 /* declare the _omp forms, which this library defines */
 #include "../../include/libsais.h"
 #include "../../include/libsais64.h"
+#include "api_ctx.h"
 
 #include <stddef.h>
 #include <stdlib.h>
@@ -292,28 +293,15 @@ int32_t libsais_plcp_int(const int32_t *T, const int32_t *SA, int32_t *PLCP, int
 }
 
 /* ------------------------------------------------------------- contexts -- */
-/* The reference's contexts hold reusable host buffers (src/libsais.c:7008-7016,
- * 8020-8028).  Here the device buffers are pooled inside the engine, so a context
- * is only a token the caller must pass back; the calls behave as the context-free
- * ones after the same checks. */
+/* The reference's contexts hold reusable buffers, one context per thread
+ * (src/libsais.c:7008-7016, 8020-8028; include/libsais.h:54).  Here a context owns
+ * an engine context of its own: a device stream and a pool of device buffers,
+ * created on the context's first call.  Calls through two contexts run concurrently
+ * on the device; calls without a context share the engine's slot contexts. */
 
-typedef struct { uint32_t magic; } sagpu_ctx_token;
-#define SAGPU_CTX_MAGIC 0x5341474Bu
+static void *ctx_new(void) { return sagpu_host_token_new(); }
 
-static void *ctx_new(void)
-{
-    sagpu_ctx_token *t = (sagpu_ctx_token *)malloc(sizeof *t);
-    if (t) t->magic = SAGPU_CTX_MAGIC;
-    return t;
-}
-
-static void ctx_free(void *ctx)
-{
-    if (ctx) {
-        ((sagpu_ctx_token *)ctx)->magic = 0;
-        free(ctx);
-    }
-}
+static void ctx_free(void *ctx) { sagpu_host_token_free(ctx); }
 
 /* reference src/libsais.c:7008-7011 */
 void *libsais_create_ctx(void) { return ctx_new(); }
@@ -337,14 +325,14 @@ void libsais_unbwt_free_ctx(void *ctx) { ctx_free(ctx); }
 int32_t libsais_ctx(const void *ctx, const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais(T, SA, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais(T, SA, n, fs, freq));
 }
 
 /* reference src/libsais.c:7081-7095 */
 int32_t libsais_gsa_ctx(const void *ctx, const uint8_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais_gsa(T, SA, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais_gsa(T, SA, n, fs, freq));
 }
 
 /* reference src/libsais.c:7148-7172 */
@@ -352,7 +340,7 @@ int32_t libsais_bwt_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32_t *
                         int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais_bwt(T, U, A, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais_bwt(T, U, A, n, fs, freq));
 }
 
 /* reference src/libsais.c:7174-7197 */
@@ -360,7 +348,7 @@ int32_t libsais_bwt_aux_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int32
                             int32_t *freq, int32_t r, int32_t *I)
 {
     if (ctx == NULL) return -1;
-    return libsais_bwt_aux(T, U, A, n, fs, freq, r, I);
+    WITH_CTX(ctx, int32_t, libsais_bwt_aux(T, U, A, n, fs, freq, r, I));
 }
 
 /* reference src/libsais.c:8035-8038 */
@@ -375,7 +363,7 @@ int32_t libsais_unbwt_aux_ctx(const void *ctx, const uint8_t *T, uint8_t *U, int
                               const int32_t *freq, int32_t r, const int32_t *I)
 {
     if (ctx == NULL) return -1;
-    return libsais_unbwt_aux(T, U, A, n, freq, r, I);
+    WITH_CTX(ctx, int32_t, libsais_unbwt_aux(T, U, A, n, freq, r, I));
 }
 
 /* ------------------------------------------------------- OpenMP entries -- */

@@ -4,13 +4,15 @@
  * Argument checks and n <= 1 cases follow the reference line for line; the work
  * goes to the GPU engine through its host-buffer layer (sagpu_host_*16, engine.hip).
  * Return codes: 0 / primary index on success, -1 invalid arguments, -2 device
- * failure.  Contexts are the tokens of api.c (libsais_create_ctx).
+ * failure.  Contexts are those of api.c (libsais_create_ctx): each owns an engine
+ * context, bound to the call by WITH_CTX.
  */
 #define LIBSAIS_OPENMP 1 /* declare the _omp forms, which this library defines */
 #include "../../include/libsais.h"
 #include "../../include/libsais64.h"
 #include "../../include/libsais16.h"
 #include "../../include/libsais16x64.h"
+#include "api_ctx.h"
 
 #include <stddef.h>
 #include <string.h>
@@ -168,14 +170,14 @@ void libsais16_unbwt_free_ctx(void *ctx) { libsais_unbwt_free_ctx(ctx); }
 int32_t libsais16_ctx(const void *ctx, const uint16_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais16(T, SA, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais16(T, SA, n, fs, freq));
 }
 
 /* reference src/libsais16.c:7058-7072 */
 int32_t libsais16_gsa_ctx(const void *ctx, const uint16_t *T, int32_t *SA, int32_t n, int32_t fs, int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais16_gsa(T, SA, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais16_gsa(T, SA, n, fs, freq));
 }
 
 /* reference src/libsais16.c:7125-7149 */
@@ -183,7 +185,7 @@ int32_t libsais16_bwt_ctx(const void *ctx, const uint16_t *T, uint16_t *U, int32
                           int32_t *freq)
 {
     if (ctx == NULL) return -1;
-    return libsais16_bwt(T, U, A, n, fs, freq);
+    WITH_CTX(ctx, int32_t, libsais16_bwt(T, U, A, n, fs, freq));
 }
 
 /* reference src/libsais16.c:7151-7174 */
@@ -191,7 +193,7 @@ int32_t libsais16_bwt_aux_ctx(const void *ctx, const uint16_t *T, uint16_t *U, i
                               int32_t *freq, int32_t r, int32_t *I)
 {
     if (ctx == NULL) return -1;
-    return libsais16_bwt_aux(T, U, A, n, fs, freq, r, I);
+    WITH_CTX(ctx, int32_t, libsais16_bwt_aux(T, U, A, n, fs, freq, r, I));
 }
 
 /* reference src/libsais16.c:7807-7823 */
@@ -199,7 +201,7 @@ int32_t libsais16_unbwt_aux_ctx(const void *ctx, const uint16_t *T, uint16_t *U,
                                 const int32_t *freq, int32_t r, const int32_t *I)
 {
     if (ctx == NULL) return -1;
-    return libsais16_unbwt_aux(T, U, A, n, freq, r, I);
+    WITH_CTX(ctx, int32_t, libsais16_unbwt_aux(T, U, A, n, freq, r, I));
 }
 
 /* reference src/libsais16.c:7784-7787 */

@@ -186,8 +186,9 @@ int launch_check_offsets(const uint8_t *T, const u32 *sid, const int64_t *offset
 // of job k start at off[k] - k.  outLCP / outU may be null.
 int launch_batch_extract(const u32 *gSA, const u32 *gLCP, const u32 *sid, const int64_t *off, const u32 *ISA,
                          const uint8_t *T, u64 N, u32 *outSA, u32 *outLCP, uint8_t *outU, hipStream_t s);
-// prim[k] = ISA[off[k]] - off[k]  (1-based primary index of job k's BWT)
-int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u32 *prim, hipStream_t s);
+// prim[k] = rank of ISA[off[k]] - off[k]  (1-based primary index of job k's BWT);
+// N = length of the batched text (selects the ISA word form, isa_rank)
+int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u64 N, u32 *prim, hipStream_t s);
 // SA_local[j] = SA[j] - offsets[sid[j]] (slot j lies in the same string block).
 int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n, u32 *out, hipStream_t s);
 // ------------------------------------------------------------ BWT / PLCP / LCP
@@ -249,6 +250,7 @@ int launch_lcp(const u32 *PLCP, const u32 *SA, u64 n, u32 *LCP, hipStream_t s);
 // Tiny fill kernel (graph-capture safe replacement for hipMemsetAsync).
 int launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
 int launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);
+int launch_fill_u8(uint8_t *p, u64 count, uint8_t value, hipStream_t s);
 int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s);
 // Integer alphabet (libsais_int / libsais64_long): T is int32 (width 4) or int64.
 int launch_pack_int(const u32 *S, u64 n, int bits, int k, int gsa, u32 *keys, u32 *vals, hipStream_t s);

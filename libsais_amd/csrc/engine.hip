@@ -10,9 +10,11 @@
 #include "../../include/sagpu.h"
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <map>
 #include <mutex>
+#include <new>
 #include <pthread.h>
 #include <stdio.h>
 #include <string.h>
@@ -36,10 +38,22 @@ void sagpu_set_error(const char *msg, const char *file, int line)
 // ------------------------------------------------------------------ pool
 namespace {
 
+// Cap on the live bytes of each context's pool (sagpu_set_memory_limit; 0 = none).
+std::atomic<u64> g_mem_limit{0};
+// Device and pinned-host allocations the engine has made for its working memory
+// (pools, staging); reported in sagpu_stats.device_allocs.
+std::atomic<long long> g_device_allocs{0};
+
+void trim_idle_slots();
+
+// Device memory of one context.  Buffers are cached by size and reused by later
+// calls (stream-ordered: a buffer returned while kernels on the context's stream
+// still use it is only handed out again to work queued behind them on the same
+// stream).
 struct Pool {
     std::multimap<size_t, void *> cache;
     std::unordered_map<void *, size_t> live;
-    size_t limit = 0, live_bytes = 0;
+    size_t live_bytes = 0;
 
     static size_t round(size_t b)
     {
@@ -50,6 +64,7 @@ struct Pool {
     void *get(size_t bytes)
     {
         bytes = round(bytes ? bytes : 1);
+        const u64 limit = g_mem_limit.load();
         if (limit && live_bytes + bytes > limit) {
             sagpu_set_error("device memory limit reached", __FILE__, __LINE__);
             return nullptr;
@@ -64,9 +79,11 @@ struct Pool {
             return p;
         }
         void *p = nullptr;
+        g_device_allocs++;
         if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             trim();
+            trim_idle_slots();
             if (hipMalloc(&p, bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 sagpu_set_error("hipMalloc failed", __FILE__, __LINE__);
@@ -216,11 +233,25 @@ CopyPool &copies()
 
 enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_PLCP_PAIRS, F_COUNT };
 
+// Host copies of one round's big-group list (sa_build), kept with the context so
+// that their capacity is reused.
+struct BigHost {
+    std::vector<u32> head, slot, size, order, shead, sslot;
+    std::vector<u64> off;
+    void reserve(size_t nb)
+    {
+        head.reserve(nb); slot.reserve(nb); size.reserve(nb); order.reserve(nb);
+        shead.reserve(nb); sslot.reserve(nb); off.reserve(nb + 1);
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool;
     RadixScratch rs{};
+    BigHost bigh;
+    bool stats_fresh = false;   // stats were reset by the current call (copied out on release)
     u64 *pinned = nullptr;      // small pinned host staging (counts, histograms)
     uint8_t *stage = nullptr;   // large pinned staging for batched jobs
     size_t stage_cap = 0;
@@ -235,8 +266,7 @@ struct Ctx {
     bool sort_u32 = false;  // the radix passes being timed are u32-key passes
 };
 
-std::mutex g_mu;
-Ctx *g_ctx = nullptr;
+std::mutex g_opt_mu;   // serialises option writes
 bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
@@ -268,7 +298,16 @@ int g_plcp_pairs = 1;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
-int ctx_create(Ctx **out)
+void apply_options(Ctx &c)
+{
+    c.rs.variant = g_radix_variant;
+    c.rs.items = g_radix_items;
+    c.rs.threads = g_radix_threads;
+}
+
+void ctx_destroy(Ctx *c);
+
+int ctx_create(Ctx **out, int device = 0)
 {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -277,13 +316,26 @@ int ctx_create(Ctx **out)
         return SAGPU_EDEVICE;
     }
     Ctx *c = new Ctx();
-    SAGPU_CHECK(hipSetDevice(c->device));
-    SAGPU_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    SAGPU_CHECK(hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault));
+    c->device = device;
+    int rc = SAGPU_OK;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        sagpu_set_error("stream creation failed", __FILE__, __LINE__);
+        rc = SAGPU_EDEVICE;
+    } else {
+        g_device_allocs++;
+        if (hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault) != hipSuccess) {
+            sagpu_set_error("pinned host allocation failed", __FILE__, __LINE__);
+            c->pinned = nullptr;
+            rc = SAGPU_ENOMEM;
+        }
+    }
+    if (rc != SAGPU_OK) {
+        (void)hipGetLastError();
+        ctx_destroy(c);
+        return rc;
+    }
     c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
-    c->rs.variant = g_radix_variant;
-    c->rs.items = g_radix_items;
-    c->rs.threads = g_radix_threads;
+    apply_options(*c);
     *out = c;
     return SAGPU_OK;
 }
@@ -291,7 +343,7 @@ int ctx_create(Ctx **out)
 void ctx_destroy(Ctx *c)
 {
     if (!c) return;
-    (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->rs.hist) c->pool.put(c->rs.hist);
     if (c->rs.tile_ctr) c->pool.put(c->rs.tile_ctr);
     if (c->rs.lookback) c->pool.put(c->rs.lookback);
@@ -311,20 +363,113 @@ void ctx_destroy(Ctx *c)
     if (c->xs) (void)hipHostFree(c->xs);
     for (auto &e : c->xs_ev)
         if (e) (void)hipEventDestroy(e);
-    (void)hipHostFree(c->pinned);
-    (void)hipStreamDestroy(c->stream);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
-int ctx_get(Ctx **out)
+// ------------------------------------------------------------------ leases
+// Every host entry point runs on one context for the whole call (a "lease").
+//  - Calls made through a libsais context (libsais_ctx and the like) use that
+//    context's own engine context: a stream and a pool of its own, created on the
+//    context's first call.  Two threads with two contexts run concurrently on the
+//    device, as two threads with two contexts do on the CPU in the reference
+//    (include/libsais.h:54, src/libsais.c:7008-7016).
+//  - Calls without a context take a free slot context (created on demand, up to
+//    g_api_slots of them) and wait only when all are busy.
+std::mutex g_slot_mu;
+std::condition_variable g_slot_cv;
+std::vector<Ctx *> g_slots;
+std::vector<char> g_slot_busy;
+int g_api_slots = 8;
+
+struct Token {
+    u32 magic = 0x5341474Bu;
+    std::mutex mu;
+    Ctx *c = nullptr;
+};
+thread_local Token *t_bound = nullptr;
+
+std::mutex g_stats_mu;
+sagpu_stats g_last_stats{};
+
+// Releases the pooled buffers of slot contexts no call holds (their streams are
+// idle: every call synchronises its stream before it ends).
+void trim_idle_slots()
 {
-    if (g_ctx) { *out = g_ctx; return SAGPU_OK; }
-    Ctx *c;
-    SAGPU_TRY(ctx_create(&c));
-    g_ctx = c;
-    *out = c;
-    return SAGPU_OK;
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    for (size_t q = 0; q < g_slots.size(); q++)
+        if (!g_slot_busy[q]) g_slots[q]->pool.trim();
 }
+
+struct Lease {
+    Ctx *ctx = nullptr;
+    Token *tok = nullptr;
+    int slot = -1;
+
+    int acquire()
+    {
+        if (t_bound) {
+            tok = t_bound;
+            tok->mu.lock();
+            if (!tok->c) {
+                const int rc = ctx_create(&tok->c);
+                if (rc != SAGPU_OK) {
+                    tok->c = nullptr;
+                    tok->mu.unlock();
+                    tok = nullptr;
+                    return rc;
+                }
+            }
+            ctx = tok->c;
+        } else {
+            std::unique_lock<std::mutex> lk(g_slot_mu);
+            for (;;) {
+                for (size_t q = 0; q < g_slots.size() && slot < 0; q++)
+                    if (!g_slot_busy[q]) slot = (int)q;
+                if (slot >= 0) break;
+                if ((int)g_slots.size() < g_api_slots) {
+                    Ctx *c = nullptr;
+                    SAGPU_TRY(ctx_create(&c));
+                    g_slots.push_back(c);
+                    g_slot_busy.push_back(0);
+                    slot = (int)g_slots.size() - 1;
+                    break;
+                }
+                g_slot_cv.wait(lk);
+            }
+            g_slot_busy[slot] = 1;
+            ctx = g_slots[slot];
+        }
+        (void)hipSetDevice(ctx->device);
+        apply_options(*ctx);
+        ctx->stats_fresh = false;
+        return SAGPU_OK;
+    }
+
+    ~Lease()
+    {
+        if (!ctx) return;
+        if (ctx->stats_fresh) {
+            std::lock_guard<std::mutex> lk(g_stats_mu);
+            g_last_stats = ctx->stats;
+        }
+        if (tok) {
+            tok->mu.unlock();
+        } else {
+            {
+                std::lock_guard<std::mutex> lk(g_slot_mu);
+                g_slot_busy[slot] = 0;
+            }
+            g_slot_cv.notify_one();
+        }
+    }
+};
+
+#define LEASE(c)                   \
+    Lease lease_;                  \
+    SAGPU_TRY(lease_.acquire());   \
+    Ctx *c = lease_.ctx
 
 // Per-call scope: every buffer taken from the pool returns to it on exit.
 struct Scope {
@@ -541,7 +686,8 @@ int byte_alphabet(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u64 *fre
     return SAGPU_OK;
 }
 
-// Suffix array of the padded device text dT[0..n) (n >= 2, n <= 2^31 - 1).  On
+// Suffix array of the padded device text dT[0..n) (2 <= n <= 2^32; above
+// g_wide_from, INT32_MAX by default, the rounds use 64-bit rank keys).  On
 // return dSA holds the suffix array and dISA its inverse.  freq_out (host, 256)
 // optional.  dLCP (optional) receives, per SA slot j >= 1, the exact lcp with the
 // previous suffix when the initial keys separate them, else LCP_UNKNOWN | bound.
@@ -689,8 +835,11 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
-    std::vector<u32> bh, bg, bs, order, hhead, hslot;
-    std::vector<u64> hoff;
+    // the host copies of each round's big-group list keep their capacity in the
+    // context
+    std::vector<u32> &bh = c.bigh.head, &bg = c.bigh.slot, &bs = c.bigh.size, &order = c.bigh.order,
+                     &hhead = c.bigh.shead, &hslot = c.bigh.sslot;
+    std::vector<u64> &hoff = c.bigh.off;
     while (a > 0) {
         if (st.rounds < 64) st.active[st.rounds] = (int64_t)a;
         st.rounds++;
@@ -922,7 +1071,7 @@ enum { SRC_DEVICE = 0, SRC_HOST = 1, SRC_PINNED = 2 };
 int padded_text(Ctx &c, Scope &sc, const uint8_t *src, u64 n, int src_kind, uint8_t **out)
 {
     GET(dT, uint8_t, n + 64);
-    SAGPU_CHECK(hipMemsetAsync(dT + n, 0, 64, c.stream));
+    SAGPU_TRY(launch_fill_u8(dT + n, 64, 0, c.stream));
     if (src_kind == SRC_HOST) {
         SAGPU_TRY(h2d(c, dT, src, n));
     } else if (n) {
@@ -937,6 +1086,7 @@ void stats_reset(Ctx &c)
 {
     c.stats = sagpu_stats{};
     c.timer.on = g_timing;
+    c.stats_fresh = true;
 }
 
 // Fused device pipeline.  Returns 0 or the primary index when U is requested.
@@ -1059,6 +1209,30 @@ void store_freq(const u64 *f, void *freq, int width)
 
 } // namespace
 
+// ======================================================== libsais contexts
+
+extern "C" SAGPU_HIDDEN void *sagpu_host_token_new(void) { return new (std::nothrow) Token(); }
+
+extern "C" SAGPU_HIDDEN void sagpu_host_token_free(void *token)
+{
+    if (!token) return;
+    Token *t = (Token *)token;
+    {
+        std::lock_guard<std::mutex> lk(t->mu);   // waits for a call still running on it
+        ctx_destroy(t->c);
+        t->c = nullptr;
+        t->magic = 0;
+    }
+    delete t;
+}
+
+extern "C" SAGPU_HIDDEN void *sagpu_host_bind(const void *token)
+{
+    Token *prev = t_bound;
+    t_bound = (Token *)token;
+    return prev;
+}
+
 // ======================================================== host-buffer layer
 // Used by the libsais-compatible wrappers (api.c).  Arguments are already
 // validated there; n >= 2.  width = 4 (int32 arrays) or 8 (int64 arrays).
@@ -1066,9 +1240,7 @@ void store_freq(const u64 *f, void *freq, int width)
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_t n, int32_t gsa, void *freq,
                                               int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1093,9 +1265,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa(const uint8_t *T, void *SA, int64_
 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1121,9 +1291,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt(const uint8_t *T, uint8_t *U, int
 // the initial keys still hold as many symbols as the alphabet allows.
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa_int(const void *T, void *SA, int64_t n, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1188,9 +1356,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa_int(const void *T, void *SA, int64
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void *SA, void *PLCP, int64_t n,
                                                     int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1215,9 +1381,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp_int(const int32_t *T, const void
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt_aux(const uint8_t *T, uint8_t *U, int64_t n, void *freq, int64_t r,
                                                    void *I, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1327,9 +1491,7 @@ int unbwt_host(Ctx &c, const void *T, void *U, u64 N, u64 r, const void *I, int 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt(const uint8_t *T, uint8_t *U, int64_t n, int64_t r, const void *I,
                                                  int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n >= (1ull << 32)) {
         sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1382,9 +1544,7 @@ void store_freq16(const u64 *f, void *freq, int width)
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa16(const uint16_t *T, void *SA, int64_t n, int32_t gsa, void *freq,
                                                 int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1409,9 +1569,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_sa16(const uint16_t *T, void *SA, int
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U, int64_t n, void *freq, int64_t r,
                                                  void *I, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1452,9 +1610,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_bwt16(const uint16_t *T, uint16_t *U,
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp16(const uint16_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,
                                                   int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1478,9 +1634,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp16(const uint16_t *T, const void 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt16(const uint16_t *T, uint16_t *U, int64_t n, int64_t r,
                                                    const void *I, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n >= (1ull << 32)) {
         sagpu_set_error("inverse BWT: n >= 2^32 is not supported", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1492,9 +1646,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_unbwt16(const uint16_t *T, uint16_t *
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA, void *PLCP, int64_t n, int32_t gsa,
                                                 int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1517,9 +1669,7 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_plcp(const uint8_t *T, const void *SA
 
 extern "C" SAGPU_HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA, void *LCP, int64_t n, int32_t width)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     if ((u64)n > (1ull << 32)) {
         sagpu_set_error("n > 2^32 is not supported (indices are kept as 32-bit words)", __FILE__, __LINE__);
         return SAGPU_ENOMEM;
@@ -1538,19 +1688,29 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA,
 
 // ======================================================== public device ABI
 
+// Creates the first slot context (otherwise created on first use).
 extern "C" int32_t sagpu_init(void)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    return ctx_get(&c);
+    LEASE(c);
+    (void)c;
+    return SAGPU_OK;
 }
 
+// Destroys the slot contexts (after the calls that hold them end).  Contexts of
+// live libsais context handles and servers stay.
 extern "C" void sagpu_shutdown(void)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!g_ctx) return;
-    ctx_destroy(g_ctx);
-    g_ctx = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(g_slot_mu);
+        g_slot_cv.wait(lk, [] {
+            for (char b : g_slot_busy)
+                if (b) return false;
+            return true;
+        });
+        for (Ctx *c : g_slots) ctx_destroy(c);
+        g_slots.clear();
+        g_slot_busy.clear();
+    }
 }
 
 extern "C" const char *sagpu_last_error(void) { return g_err.c_str(); }
@@ -1596,9 +1756,7 @@ extern "C" int32_t sagpu_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes
 {
     if ((!d_dst || !d_src) && bytes) return SAGPU_EINVAL;
     if (!bytes) return 0;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     SAGPU_TRY(launch_copy(d_dst, d_src, bytes, c->stream));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1609,41 +1767,34 @@ extern "C" int32_t sagpu_synchronize(void)
     return hipDeviceSynchronize() == hipSuccess ? 0 : SAGPU_EDEVICE;
 }
 
-extern "C" void sagpu_set_memory_limit(uint64_t bytes)
-{
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    if (ctx_get(&c) == SAGPU_OK) c->pool.limit = (size_t)bytes;
-}
+extern "C" void sagpu_set_memory_limit(uint64_t bytes) { g_mem_limit.store(bytes); }
 
-extern "C" void sagpu_trim(void)
-{
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (g_ctx) g_ctx->pool.trim();
-}
+extern "C" void sagpu_trim(void) { trim_idle_slots(); }
 
 extern "C" void sagpu_set_timing(int32_t enabled) { g_timing = enabled != 0; }
 
 extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
 {
     if (!name) return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::mutex> lk(g_opt_mu);
     if (!strcmp(name, "radix_items")) {
         if (value != 8 && value != 16 && value != 32) return SAGPU_EINVAL;
         g_radix_items = (int)value;
-        if (g_ctx) g_ctx->rs.items = g_radix_items;
         return 0;
     }
     if (!strcmp(name, "radix_threads")) {
         if (value != 256 && value != 512) return SAGPU_EINVAL;
         g_radix_threads = (int)value;
-        if (g_ctx) g_ctx->rs.threads = g_radix_threads;
         return 0;
     }
     if (!strcmp(name, "radix_variant")) {
         if (value != RADIX_ONESWEEP && value != RADIX_REDUCE_THEN_SCAN) return SAGPU_EINVAL;
         g_radix_variant = (int)value;
-        if (g_ctx) g_ctx->rs.variant = g_radix_variant;
+        return 0;
+    }
+    if (!strcmp(name, "api_slots")) {
+        if (value < 1 || value > 16) return SAGPU_EINVAL;
+        g_api_slots = (int)value;
         return 0;
     }
     if (!strcmp(name, "host_staging")) {
@@ -1722,9 +1873,7 @@ int sort_pairs_dev(K *d_keys, u32 *d_vals, int64_t n, int begin_bit, int end_bit
     if (!d_keys || !d_vals || n < 0 || begin_bit < 0 || end_bit > (int)(8 * sizeof(K)) || begin_bit > end_bit)
         return SAGPU_EINVAL;
     if (n <= 1) return 0;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     Scope sc(*c);
     K *k1 = sc.get<K>((u64)n);
     u32 *v1 = sc.get<u32>((u64)n);
@@ -1777,9 +1926,7 @@ extern "C" int32_t sagpu_sort_bench(int64_t n, int32_t key_bytes, int32_t key_bi
 {
     if (n < 2 || (key_bytes != 4 && key_bytes != 8) || key_bits < 1 || key_bits > 8 * key_bytes || reps < 1)
         return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     Scope sc(*c);
     u64 *k64a = nullptr, *k64b = nullptr;
     u32 *k32a = nullptr, *k32b = nullptr;
@@ -1830,37 +1977,43 @@ extern "C" int32_t sagpu_sort_bench(int64_t n, int32_t key_bytes, int32_t key_bi
 
 extern "C" int32_t sagpu_get_stats(sagpu_stats *out)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!g_ctx || !out) return SAGPU_EINVAL;
-    *out = g_ctx->stats;
+    if (!out) return SAGPU_EINVAL;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    *out = g_last_stats;
+    out->device_allocs = g_device_allocs.load();
     return 0;
 }
 
-extern "C" int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA, uint8_t *d_U,
-                                 int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq)
+namespace {
+// sagpu_build32 on one context.
+int64_t build32_on(Ctx &c, const uint8_t *d_T, u64 n, int32_t flags, int32_t *d_SA, uint8_t *d_U, int32_t *d_PLCP,
+                   int32_t *d_LCP, int64_t *h_freq)
 {
-    if (!d_T || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
-    stats_reset(*c);
-    Scope sc(*c);
+    stats_reset(c);
+    Scope sc(c);
     uint8_t *dT;
-    SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, SRC_DEVICE, &dT));
+    SAGPU_TRY(padded_text(c, sc, d_T, n, SRC_DEVICE, &dT));
     u64 f[256];
-    int64_t rc = build_dev(*c, sc, dT, (u64)n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, (u32 *)d_SA, d_U, (u32 *)d_PLCP,
+    int64_t rc = build_dev(c, sc, dT, n, (flags & SAGPU_FLAG_GSA) ? 1 : 0, (u32 *)d_SA, d_U, (u32 *)d_PLCP,
                            (u32 *)d_LCP, f);
     if (rc >= 0 && h_freq)
         for (int ch = 0; ch < 256; ch++) h_freq[ch] = (int64_t)f[ch];
     return rc;
 }
+} // namespace
+
+extern "C" int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA, uint8_t *d_U,
+                                 int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq)
+{
+    if (!d_T || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
+    LEASE(c);
+    return build32_on(*c, d_T, (u64)n, flags, d_SA, d_U, d_PLCP, d_LCP, h_freq);
+}
 
 extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags)
 {
     if (!d_T || !d_SA || !d_PLCP || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     Scope sc(*c);
     uint8_t *dT;
     SAGPU_TRY(padded_text(*c, sc, d_T, (u64)n, SRC_DEVICE, &dT));
@@ -1875,9 +2028,7 @@ extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t
 extern "C" int32_t sagpu_lcp32(const int32_t *d_PLCP, const int32_t *d_SA, int32_t *d_LCP, int64_t n)
 {
     if (!d_PLCP || !d_SA || !d_LCP || n < 1 || n > INT32_MAX) return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     SAGPU_TRY(launch_lcp((const u32 *)d_PLCP, (const u32 *)d_SA, (u64)n, (u32 *)d_LCP, c->stream));
     SAGPU_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1890,9 +2041,7 @@ extern "C" int32_t sagpu_gsa_batch32(const uint8_t *d_T, int64_t n, const int64_
     if (h_offsets[0] != 0 || h_offsets[count] != n) return SAGPU_EINVAL;
     for (int32_t q = 0; q < count; q++)
         if (h_offsets[q + 1] <= h_offsets[q] + 1) return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     stats_reset(*c);
     Scope sc(*c);
     hipStream_t s = c->stream;
@@ -2085,7 +2234,7 @@ int run_batch(Ctx &c, sagpu_job **jobs, int count)
     }
     GET(prim, u32, (u64)count);
     SAGPU_TRY(launch_batch_extract(gSA, gL, sid, d_off, ISA, dT, N, oSA, oL, oU, s));
-    SAGPU_TRY(launch_batch_primary(ISA, d_off, (u32)count, prim, s));
+    SAGPU_TRY(launch_batch_primary(ISA, d_off, (u32)count, N, prim, s));
     uint8_t *hSA = c.stage + ((N + 63) & ~(u64)63);
     uint8_t *hL = hSA + 4 * M;
     uint8_t *hU = c.stage;  // input text no longer needed on the host side
@@ -2185,6 +2334,7 @@ void worker(Server *svp, int slot)
         }
         double t0 = now_s();
         for (sagpu_job *j : take) j->t_start = t0;
+        apply_options(c);
         int rc;
         if (take.size() == 1) {
             rc = run_single(c, take[0]);
@@ -2215,9 +2365,7 @@ extern "C" int32_t sagpu_run_job(sagpu_job *job)
     if (!job || !job->T || !job->SA || job->n < 0 || job->kind < 0 || job->kind > 2 ||
         (job->kind == SAGPU_JOB_BWT && !job->U) || (job->kind == SAGPU_JOB_LCP && !job->LCP))
         return SAGPU_EINVAL;
-    std::lock_guard<std::mutex> lk(g_mu);
-    Ctx *c;
-    SAGPU_TRY(ctx_get(&c));
+    LEASE(c);
     job->batched = 0;
     job->t_start = now_s();
     int rc = run_single(*c, job);

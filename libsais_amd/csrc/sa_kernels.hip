@@ -177,11 +177,11 @@ __global__ __launch_bounds__(256) void k_batch_extract(const u32 *__restrict__ g
     }
 }
 
-__global__ void k_batch_primary(const u32 *__restrict__ ISA, const int64_t *__restrict__ off, u32 count,
+__global__ void k_batch_primary(const u32 *__restrict__ ISA, const int64_t *__restrict__ off, u32 count, u64 N,
                                 u32 *__restrict__ prim)
 {
     for (u32 q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
-        prim[q] = ISA[off[q]] - (u32)off[q];
+        prim[q] = isa_rank(ISA[off[q]], N) - (u32)off[q];
 }
 
 // Plain device copy, 16 bytes per lane per step (the copy-bandwidth yardstick).
@@ -197,6 +197,12 @@ __global__ void k_copy_tail(const uint8_t *__restrict__ src, uint8_t *__restrict
 }
 
 __global__ void k_fill_u32(u32 *p, u64 count, u32 v)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, u64 count, uint8_t v)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
@@ -476,9 +482,9 @@ int launch_batch_extract(const u32 *gSA, const u32 *gLCP, const u32 *sid, const 
     return SAGPU_OK;
 }
 
-int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u32 *prim, hipStream_t s)
+int launch_batch_primary(const u32 *ISA, const int64_t *off, u32 count, u64 N, u32 *prim, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_primary, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, ISA, off, count, prim);
+    hipLaunchKernelGGL(k_batch_primary, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, ISA, off, count, N, prim);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -501,6 +507,14 @@ int launch_copy(void *dst, const void *src, u64 bytes, hipStream_t s)
     if (bytes > vec * 16)
         hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(256), 0, s, (const uint8_t *)src + vec * 16,
                            (uint8_t *)dst + vec * 16, bytes - vec * 16);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_fill_u8(uint8_t *p, u64 count, uint8_t value, hipStream_t s)
+{
+    if (!count) return SAGPU_OK;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(count, 256 * 16, 16384)), dim3(256), 0, s, p, count, value);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

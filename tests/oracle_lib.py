@@ -161,6 +161,14 @@ def gsa_valid(T):
     return bool(oracle().oracle_gsa_valid(_p(T), len(T)))
 
 
+def gsa_valid16(T):
+    o = oracle()
+    o.oracle_gsa_valid16.argtypes = [_vp, _i64]
+    o.oracle_gsa_valid16.restype = ctypes.c_int
+    T = np.ascontiguousarray(T, np.uint16)
+    return bool(o.oracle_gsa_valid16(_p(T), len(T)))
+
+
 # ---------------------------------------------------------------- reference
 def ref_sa(T, gsa=False, threads=0):
     n = len(T)

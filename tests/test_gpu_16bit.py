@@ -130,6 +130,49 @@ def test_gsa_requires_trailing_zero():
     assert la.libsais16_plcp_gsa(T, SA, SA, 3) == -1
 
 
+def _invalid16(rng):
+    """16-bit GSA inputs with an empty string (a 0 at position 0, or two adjacent 0s),
+    which the engine detects on the device while it counts symbols."""
+    out = []
+    for k, count, at in ((3, 40, "start"), (1000, 200, "middle"), (65536, 50, "end"), (5, 3000, "many")):
+        G = ol.gsa_set16(rng, count, 60, k)
+        if at == "start":
+            G = np.concatenate([np.zeros(1, np.uint16), G])
+        elif at == "middle":
+            z = np.flatnonzero(G == 0)[count // 2]
+            G = np.concatenate([G[:z + 1], np.zeros(1, np.uint16), G[z + 1:]])
+        elif at == "end":
+            G = np.concatenate([G, np.zeros(1, np.uint16)])
+        else:
+            z = np.flatnonzero(G == 0)
+            G = np.insert(G, z[::7] + 1, 0)
+        out.append((at, G))
+    return out
+
+
+def test_gsa16_rejects_empty_strings_like_the_oracle():
+    """libsais16_gsa and libsais16x64_gsa return -1 for a set with an empty string
+    (src/libsais16.c:6863), and 0 for the same set without it."""
+    rng = np.random.default_rng(166)
+    for at, G in _invalid16(rng):
+        n = len(G)
+        print(f"16-bit gsa invalid ({at}) n={n}", flush=True)
+        assert not ol.gsa_valid16(G)
+        SA = np.zeros(n, np.int32)
+        f = np.zeros(65536, np.int32)
+        assert la.libsais16_gsa(G, SA, n, 0, f) == -1, at
+        S64 = np.zeros(n, np.int64)
+        assert la.libsais16x64_gsa(G, S64, n, 0, None) == -1, at
+        if ol.have_ref():
+            rc, _, _ = ol.ref_sa16(G, gsa=True)
+            assert rc == -1, at
+    G = ol.gsa_set16(rng, 500, 60, 300)
+    assert ol.gsa_valid16(G)
+    SA = np.zeros(len(G), np.int32)
+    assert la.libsais16_gsa(G, SA, len(G), 0, None) == 0
+    assert np.array_equal(SA, ol.sa16(G, gsa=True)[1].astype(np.int32))
+
+
 @pytest.mark.skipif(not ol.have_ref(), reason="reference build oracle/_ref absent")
 @pytest.mark.parametrize("name,T", CASES[2::4], ids=IDS[2::4])
 def test_against_reference(name, T):

@@ -1,6 +1,7 @@
-"""Integer alphabets on the GPU (libsais_int, libsais64_long, libsais_plcp_int) and
-the context / OpenMP forms, through the C ABI, against the oracle (and the compiled
-reference where it is built)."""
+"""Integer alphabets on the GPU (libsais_int, libsais64_long, libsais_plcp_int)
+through the C ABI, against the oracle (and the compiled reference where it is
+built).  The context and OpenMP forms are compared with the reference's in
+test_gpu_forms.py."""
 import numpy as np
 import pytest
 
@@ -71,34 +72,3 @@ def test_int_against_reference():
         P = np.zeros(len(T), np.int32)
         assert la.libsais_plcp_int(T, SA, P, len(T)) == 0
         assert np.array_equal(P, P_ref)
-
-
-def test_context_and_omp_forms_match_plain():
-    T = workloads.dna_like(300000, seed=12)
-    n = len(T)
-    SA = np.zeros(n, np.int32)
-    assert la.libsais(T, SA, n, 0, None) == 0
-    ctx = la.libsais_create_ctx()
-    try:
-        S2 = np.zeros(n, np.int32)
-        assert la.libsais_ctx(ctx, T, S2, n, 0, None) == 0 and np.array_equal(S2, SA)
-        U1, U2 = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
-        A = np.zeros(n, np.int32)
-        p1 = la.libsais_bwt(T, U1, A, n, 0, None)
-        p2 = la.libsais_bwt_ctx(ctx, T, U2, A, n, 0, None)
-        assert p1 == p2 and np.array_equal(U1, U2)
-    finally:
-        la.libsais_free_ctx(ctx)
-    S3 = np.zeros(n, np.int32)
-    assert la.libsais_omp(T, S3, n, 0, None, 4) == 0 and np.array_equal(S3, SA)
-    P1, P2 = np.zeros(n, np.int32), np.zeros(n, np.int32)
-    assert la.libsais_plcp(T, SA, P1, n) == 0
-    assert la.libsais_plcp_omp(T, SA, P2, n, 0) == 0 and np.array_equal(P1, P2)
-    uctx = la.libsais_unbwt_create_ctx()
-    try:
-        out = np.zeros(n, np.uint8)
-        assert la.libsais_unbwt_ctx(uctx, U1, out, A, n, None, p1) == 0 and np.array_equal(out, T)
-    finally:
-        la.libsais_unbwt_free_ctx(uctx)
-    S64 = np.zeros(n, np.int64)
-    assert la.libsais64_omp(T, S64, n, 0, None, 2) == 0 and np.array_equal(S64, SA.astype(np.int64))

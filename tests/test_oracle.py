@@ -325,6 +325,24 @@ def test_int_oracle_against_reference(k):
 
 
 # ------------------------------------------------------------ 16-bit alphabet
+@needs_ref
+def test_gsa16_validity_matches_reference():
+    """oracle_gsa_valid16 accepts exactly the 16-bit GSA inputs libsais16_gsa accepts:
+    every string over {0, 1, 40000} of length 1..8 that ends in 0 (a 0 at the start or
+    two adjacent 0s make an empty string, src/libsais16.c:6863)."""
+    seen_invalid = 0
+    for n in range(1, 9):
+        for t in itertools.product((0, 1, 40000), repeat=n):
+            if t[-1] != 0:
+                continue
+            T = np.array(t, np.uint16)
+            rc, _, _ = ol.ref_sa16(T, gsa=True)
+            assert rc in (0, -1)
+            assert (rc == 0) == ol.gsa_valid16(T), t
+            seen_invalid += rc == -1
+    assert seen_invalid > 100
+
+
 def test_16bit_oracle_brute_force():
     rng = np.random.default_rng(61)
     for n in range(1, 30):
