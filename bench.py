@@ -6,16 +6,15 @@ index, PLCP and LCP, all int32, outputs left in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 
-roofline: the kernel function with the largest share of the build, the onesweep
-radix pass (k_radix_pass: the generating first pass and three array-to-array
-passes of the initial sort).  Reported for its array-to-array launches (u32 key +
-u32 value): n x (4 + 4) B read and written per launch, launch durations from HIP
-events recorded on the engine stream around every pass inside the timed steps.
-The text-order PLCP pass (k_plcp_irr_pairs, 17 B per position: ISA 4, the
-preceding byte 1, one 8-byte pair word, the value written 4) is reported beside
-it as plcp_pass.  traffic is the HBM bytes per launch measured with rocprofv3
-FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
-profiles/pmc_traffic.json.
+roofline: the kernel with the largest device time per build, among the kernels
+the engine times (HIP events recorded on the engine stream around every launch
+inside the timed steps; sagpu_stats.kernel_ms).  achieved = its algorithmic
+bytes per launch (reads + writes at element size, counted by the engine per
+launch: e.g. k_round_tile 28 B per active list entry, k_isa_scatter 12 B per
+position) / its average launch duration.  roofline.kernels lists the five kernels
+with the largest time per build the same way.  traffic is the HBM bytes per
+launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of
+known size), read from profiles/pmc_traffic.json.
 pipeline: B_model = 269 n + 244 sum_r a_r bytes per step (prefix-doubling yardstick
 with 8 initial symbols; a_r counts suffixes with max(LCP[j], LCP[j+1]) >= 8*2^(r-1),
 computed from the LCP this run produced), divided by the step time.
@@ -24,7 +23,8 @@ one job at a time), and the 64 MiB high-LCP inputs (all 'A', period 17, Fibonacc
 with the reference beside them.
 cpu_baseline: the reference libsais 2.10.4 (built from its sources by oracle/Makefile)
 with OpenMP on this host's cores, on the same input: libsais_omp + libsais_bwt_omp
-+ libsais_plcp_omp + libsais_lcp_omp, median of the repetitions.
++ libsais_plcp_omp + libsais_lcp_omp, each timed, summed; median and p99 of 15
+runs on all cores, and one run each at 1 and 8 threads on the full input.
 """
 from __future__ import annotations
 
@@ -41,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# sagpu_stats.kernel_ms index -> kernel (include/sagpu.h)
+KERNELS = ["k_radix_pass<u32 key, u32 value>", "k_radix_pass (generating first pass)",
+           "k_radix_pass<u64 key, u32 value>", "k_init_emit", "k_isa_scatter", "k_round_gather", "k_round_tile",
+           "k_plcp_pairs", "k_plcp_irr_pairs", "k_lcp_fix", "k_bwt_seeded"]
 METRIC = "SA+BWT input MB/s @256 MiB DNA; batched vs plain aggregate MB/s; p99 latency"
 
 
@@ -68,7 +72,7 @@ def copy_peak(lib, nbytes: int = 1 << 30, reps: int = 5):
     return 2 * nbytes / best / 1e9
 
 
-def measured_traffic(workload: str, key: str = "radix_pass_bytes_per_launch"):
+def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of a kernel from the committed PMC summary (rocprofv3
     FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -79,7 +83,7 @@ def measured_traffic(workload: str, key: str = "radix_pass_bytes_per_launch"):
         return None
     if d.get("workload") != workload:
         return None
-    return d.get(key)
+    return d.get("bytes_per_launch", {}).get(kernel)
 
 
 def a_r_from_lcp(L: np.ndarray, h0: int = 8):
@@ -113,37 +117,42 @@ def cpu_baseline(T: np.ndarray, reps: int, degenerate: bool = False, batched: bo
         cores = os.cpu_count() or 1
     cores = min(cores, 64)
     if ol.have_ref():
-        times = []
-        for _ in range(reps):
+        def run(th):
             t0 = time.perf_counter()
-            rc, SA, _ = ol.ref_sa(T, threads=cores)
+            rc, SA, _ = ol.ref_sa(T, threads=th)
             t1 = time.perf_counter()
-            U, idx = ol.ref_bwt(T, threads=cores)
+            ol.ref_bwt(T, threads=th)
             t2 = time.perf_counter()
-            P = ol.ref_plcp(T, SA, threads=cores)
-            L = ol.ref_lcp(P, SA, threads=cores)
+            P = ol.ref_plcp(T, SA, threads=th)
             t3 = time.perf_counter()
-            times.append(t3 - t0)
-            log(f"cpu reference: sa {t1 - t0:.2f}s bwt {t2 - t1:.2f}s plcp+lcp {t3 - t2:.2f}s")
+            ol.ref_lcp(P, SA, threads=th)
+            t4 = time.perf_counter()
+            return t1 - t0, t2 - t1, t3 - t2, t4 - t3
+
+        times, parts = [], []
+        for r in range(reps):
+            p = run(cores)
+            parts.append(p)
+            times.append(sum(p))
+            log(f"cpu reference run {r + 1}/{reps}: sa {p[0]:.2f}s bwt {p[1]:.2f}s plcp {p[2]:.2f}s lcp {p[3]:.2f}s")
         t = float(np.median(times))
-        # thread sweep on a 32 MiB prefix (the reference saturates near the memory
-        # channel count), single-threaded included
-        m = min(n, 1 << 25)
-        S = np.ascontiguousarray(T[:m])
-        sweep = {}
-        for th in sorted({1, 8, cores}):
-            t0 = time.perf_counter()
-            rc, SA_s, _ = ol.ref_sa(S, threads=th)
-            ol.ref_bwt(S, threads=th)
-            P_s = ol.ref_plcp(S, SA_s, threads=th)
-            ol.ref_lcp(P_s, SA_s, threads=th)
-            sweep[str(th)] = round(m / (time.perf_counter() - t0) / 1e6, 2)
-        log(f"cpu thread sweep on {m} bytes: {sweep}")
+        t99 = float(np.percentile(times, 99))
+        # thread sweep on the full input, one run per count (the reference saturates
+        # near the memory-channel count), single-threaded included
+        sweep = {str(cores): round(n / t / 1e6, 2)}
+        for th in sorted({1, 8} - {cores}):
+            p = run(th)
+            sweep[str(th)] = round(n / sum(p) / 1e6, 2)
+            log(f"cpu reference, {th} threads: {sum(p):.2f}s")
+        med = np.median(np.array(parts), axis=0)
         out = {"value": round(n / t / 1e6, 2), "unit": "MB/s", "cores": cores, "kind": "reference",
-               "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp, "
-                         f"median of {reps}",
+               "sample": f"full {n}-byte input, SA+BWT+PLCP+LCP via libsais_omp/_bwt_omp/_plcp_omp/_lcp_omp "
+                         f"(each timed, summed), median of {reps} runs on {cores} threads",
+               "p99_MBps": round(n / t99 / 1e6, 2),
+               "median_s": {"sa": round(float(med[0]), 3), "bwt": round(float(med[1]), 3),
+                            "plcp": round(float(med[2]), 3), "lcp": round(float(med[3]), 3)},
                "cpu_model": _cpu_model(),
-               "thread_sweep_MBps": {"sample_bytes": m, **sweep}}
+               "thread_sweep_MBps": {"sample_bytes": n, **{k: sweep[k] for k in sorted(sweep, key=int)}}}
         ol.ref()   # loaded once, before any worker thread calls it
         if degenerate:
             out["degenerate"] = cpu_degenerate(ol, cores)
@@ -388,7 +397,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and degenerate-input sections")
     ap.add_argument("--mode", choices=["main", "serving-full", "batched", "copy", "degenerate"], default="main")
@@ -454,8 +463,9 @@ def main():
 
     lib.sagpu_set_timing(1)
     sweep_ms, sweep_launch, sweep_elems, sweep_bytes = 0.0, 0, 0, 0
-    u32_ms, u32_launch, u32_bytes = 0.0, 0, 0
-    pp_ms, pp_launch = 0.0, 0
+    k_ms = [0.0] * len(KERNELS)
+    k_launch = [0] * len(KERNELS)
+    k_bytes = [0] * len(KERNELS)
     dev_ms = 0.0
     fam = {"initial": 0.0, "rounds": 0.0, "bwt": 0.0, "plcp": 0.0, "lcp": 0.0}
     lib.sagpu_synchronize()
@@ -467,11 +477,10 @@ def main():
         sweep_launch += st.onesweep_launches
         sweep_elems += st.sorted_elems
         sweep_bytes += st.sorted_bytes
-        u32_ms += st.ms_onesweep_u32
-        u32_launch += st.onesweep_launches_u32
-        u32_bytes += st.sorted_bytes_u32
-        pp_ms += st.ms_plcp_pairs_pass
-        pp_launch += st.plcp_pairs_launches
+        for q in range(len(KERNELS)):
+            k_ms[q] += st.kernel_ms[q]
+            k_launch[q] += st.kernel_launches[q]
+            k_bytes[q] += st.kernel_bytes[q]
         dev_ms += st.ms_total
         fam["initial"] += st.ms_initial
         fam["rounds"] += st.ms_rounds
@@ -489,18 +498,27 @@ def main():
     rounds = st.rounds
     active = [int(st.active[r]) for r in range(min(rounds, 64))]
 
-    # radix pass: the u32-key passes of the initial sort (16 B per element read +
-    # written per pass); all passes, the u64 ones of the big-group sorts included,
-    # are reported beside it
-    achieved = u32_bytes / (u32_ms / 1000.0) / 1e9 if u32_ms > 0 else 0.0
-    # the PLCP text-order pass: 17 B of sequential and pair-word bytes per position
-    pp_bytes = 17 * n
-    pp_avg = pp_ms / max(1, pp_launch)
-    pp_achieved = pp_bytes / (pp_avg / 1000.0) / 1e9 if pp_ms > 0 else 0.0
-    all_achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
-
     workload = "dna_like_256MiB_sa_bwt_plcp_lcp" if n == 1 << 28 else f"dna_like_{n}B_sa_bwt_plcp_lcp"
     cpeak = copy_peak(lib)
+
+    # per-kernel rooflines: algorithmic bytes per launch over the average launch time,
+    # kernels ordered by device time per build
+    kernels = []
+    for q, name in enumerate(KERNELS):
+        if k_launch[q] == 0 or k_ms[q] <= 0:
+            continue
+        avg_ms = k_ms[q] / k_launch[q]
+        bpl = k_bytes[q] / k_launch[q]
+        ach = bpl / (avg_ms / 1000.0) / 1e9
+        tr = measured_traffic(workload, name)
+        kernels.append({"kernel": name, "ms_per_build": round(k_ms[q] / args.steps, 4),
+                        "launches_per_build": round(k_launch[q] / args.steps, 2),
+                        "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bpl),
+                        "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": tr, "traffic_over_algorithmic": round(tr / bpl, 2) if tr else None})
+    kernels.sort(key=lambda k: -k["ms_per_build"])
+    top = kernels[0] if kernels else None
+    timed_ms = sum(k["ms_per_build"] for k in kernels)
 
     # pipeline yardstick from this run's LCP
     L = np.empty(n, np.int32)
@@ -533,30 +551,21 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_radix_pass<u32 key, u32 value> (onesweep LSD pass of the initial sort)",
-            "achieved": round(achieved, 1),
+            "kernel": top["kernel"] if top else None,
+            "achieved": top["achieved"] if top else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(workload),
+            "frac": top["frac"] if top else None,
+            "traffic": top["traffic"] if top else None,
             "copy_peak_GBs": round(cpeak, 1),
-            "frac_of_copy": round(achieved / cpeak, 4),
-            "launches": u32_launch,
-            "avg_launch_ms": round(u32_ms / max(1, u32_launch), 4),
-            "algorithmic_bytes_per_launch": u32_bytes // max(1, u32_launch),
-            "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes,
-                                 "achieved_GBs": round(all_achieved, 1)},
-            "plcp_pass": {
-                "kernel": "k_plcp_irr_pairs (text-order PLCP pass)",
-                "achieved": round(pp_achieved, 1),
-                "frac": round(pp_achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(workload, "plcp_pairs_pass_bytes_per_launch"),
-                "launches": pp_launch,
-                "avg_launch_ms": round(pp_avg, 4),
-                "algorithmic_bytes_per_launch": pp_bytes,
-                "note": "random reads: one 8-byte pair word per position without a PLCP hint in its ISA word, "
-                        "one code-stream window per irreducible position; each costs a whole line",
-            },
+            "frac_of_copy": round(top["achieved"] / cpeak, 4) if top else None,
+            "launches": int(top["launches_per_build"] * args.steps) if top else 0,
+            "avg_launch_ms": top["avg_launch_ms"] if top else None,
+            "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_launch"] if top else None,
+            "share_of_build": round(top["ms_per_build"] / (ms_per_step), 4) if top else None,
+            "kernels": kernels[:5],
+            "timed_kernels_ms_per_build": round(timed_ms, 3),
+            "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes},
         },
         "pipeline": {
             "b_model_bytes": b_model,

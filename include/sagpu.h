@@ -159,6 +159,17 @@ typedef struct sagpu_stats {
     int32_t plcp_pairs_launches;
     int64_t device_allocs;        /* device / pinned allocations the engine has made for
                                      its working memory in this process so far       */
+    int64_t peak_bytes;           /* largest device working memory the call held at once
+                                     (pool buffers, or the allocation-free scratch)   */
+    /* Per-kernel device time (ms, events around each launch), launches and
+     * algorithmic bytes (reads + writes at element size) of the build, timing mode
+     * only.  Index: 0 radix pass u32 key + u32 value, 1 generating first radix pass,
+     * 2 radix pass u64 key + u32 value, 3 k_init_emit, 4 k_isa_scatter,
+     * 5 k_round_gather, 6 k_round_tile, 7 k_plcp_pairs, 8 k_plcp_irr_pairs,
+     * 9 k_lcp_fix, 10 k_bwt_seeded; 11-15 unused. */
+    double  kernel_ms[16];
+    int32_t kernel_launches[16];
+    int64_t kernel_bytes[16];
 } sagpu_stats;
 
 /*

@@ -27,7 +27,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libsais.so")
+# LIBSAIS_AMD_LIB selects another build of the same library (the host-sanitizer
+# builds under lib/san and lib/tsan, used by tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("LIBSAIS_AMD_LIB") or os.path.join(LIB_DIR, "libsais.so")
 GEN_PATH = os.path.join(LIB_DIR, "libsagen.so")
 
 SAGPU_FLAG_GSA = 1
@@ -78,6 +80,10 @@ class SagpuStats(ctypes.Structure):
         ("ms_plcp_pairs_pass", ctypes.c_double),
         ("plcp_pairs_launches", ctypes.c_int32),
         ("device_allocs", ctypes.c_int64),
+        ("peak_bytes", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double * 16),
+        ("kernel_launches", ctypes.c_int32 * 16),
+        ("kernel_bytes", ctypes.c_int64 * 16),
     ]
 
 

@@ -29,6 +29,27 @@ int launch_pack_stream(const uint8_t *T, u64 n, const uint8_t *lut, int bits, u3
 
 // Onesweep LSD radix sort, 8-bit digits, stable.  Keys are u64 or u32, values u32.
 // The result ends in (k0,v0) or in (k1,v1); *result_in_alt says which.
+// Kernel timing for benchmarks: when timing is on for the calling thread's engine
+// context, ktime_begin / ktime_end bracket a launch with events on its stream and
+// record the launch's algorithmic bytes (reads + writes at element size).  The ids
+// index sagpu_stats.kernel_ms / kernel_launches / kernel_bytes.
+enum KernelId {
+    KI_RADIX32 = 0,   // onesweep pass, u32 key + u32 value
+    KI_RADIX_GEN,     // the initial sort's first pass, keys cut from the code stream
+    KI_RADIX64,       // onesweep pass, u64 key + u32 value
+    KI_INIT_EMIT,     // k_init_emit
+    KI_ISA_SCATTER,   // k_isa_scatter
+    KI_ROUND_GATHER,  // k_round_gather
+    KI_ROUND_TILE,    // k_round_tile (+ k_big_sizes)
+    KI_PLCP_WORDS,    // k_plcp_pairs
+    KI_PLCP_PASS,     // k_plcp_irr_pairs
+    KI_LCP_FIX,       // k_lcp_fix
+    KI_BWT,           // k_bwt_seeded
+    KI_COUNT
+};
+void ktime_begin(int id);
+void ktime_end(int id, u64 bytes);
+
 struct RadixScratch {
     u64 *hist;        // [8][256] global digit histograms
     u64 *lookback;    // [lb_tiles][256] look-back words (epoch-tagged)

@@ -54,12 +54,15 @@ struct Pool {
     std::multimap<size_t, void *> cache;
     std::unordered_map<void *, size_t> live;
     size_t live_bytes = 0;
+    size_t peak_bytes = 0;   // largest live_bytes since the last reset_peak()
 
     static size_t round(size_t b)
     {
         if (b < (1u << 20)) return (b + 255) & ~(size_t)255;
         return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
     }
+
+    void reset_peak() { peak_bytes = live_bytes; }
 
     void *get(size_t bytes)
     {
@@ -76,6 +79,7 @@ struct Pool {
             cache.erase(it);
             live[p] = sz;
             live_bytes += sz;
+            if (live_bytes > peak_bytes) peak_bytes = live_bytes;
             return p;
         }
         void *p = nullptr;
@@ -92,6 +96,7 @@ struct Pool {
         }
         live[p] = bytes;
         live_bytes += bytes;
+        if (live_bytes > peak_bytes) peak_bytes = live_bytes;
         return p;
     }
 
@@ -115,8 +120,10 @@ struct Pool {
 struct Timer {
     bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> free_ev;
-    struct Rec { int family; hipEvent_t a, b; };
+    struct Rec { int family; hipEvent_t a, b; u64 bytes; };
     std::vector<Rec> recs;
+    std::pair<hipEvent_t, hipEvent_t> open[KI_COUNT] = {};   // kernel spans begun, not ended
+    u64 hook_n = 0;   // elements of the sort whose passes the radix hook brackets
 };
 
 // ------------------------------------------------------ host copies
@@ -232,6 +239,7 @@ CopyPool &copies()
 }
 
 enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_PLCP_PAIRS, F_COUNT };
+constexpr int F_KERNEL0 = 64;   // + KernelId: one kernel's launches
 
 // Host copies of one round's big-group list (sa_build), kept with the context so
 // that their capacity is reused.
@@ -265,6 +273,9 @@ struct Ctx {
     sagpu_stats stats{};
     bool sort_u32 = false;  // the radix passes being timed are u32-key passes
 };
+
+// The engine context of the calling thread's current call (for ktime_*).
+thread_local Ctx *t_ctx = nullptr;
 
 std::mutex g_opt_mu;   // serialises option writes
 bool g_timing = false;
@@ -393,19 +404,56 @@ thread_local Token *t_bound = nullptr;
 std::mutex g_stats_mu;
 sagpu_stats g_last_stats{};
 
+// Engine contexts of freed libsais context handles, kept for the next handle's
+// first call (a stream, pinned staging and a warm pool each), so that callers that
+// create and free a context per call do not pay for stream and pool set-up.
+std::vector<Ctx *> g_parked;
+constexpr size_t PARKED_MAX = 16;
+
 // Releases the pooled buffers of slot contexts no call holds (their streams are
-// idle: every call synchronises its stream before it ends).
+// idle: every call synchronises its stream before it ends) and of parked contexts.
 void trim_idle_slots()
 {
     std::lock_guard<std::mutex> lk(g_slot_mu);
     for (size_t q = 0; q < g_slots.size(); q++)
         if (!g_slot_busy[q]) g_slots[q]->pool.trim();
+    for (Ctx *c : g_parked) c->pool.trim();
+}
+
+// A context for a libsais context handle: a parked one, or a new one.
+int ctx_for_token(Ctx **out)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_slot_mu);
+        if (!g_parked.empty()) {
+            *out = g_parked.back();
+            g_parked.pop_back();
+            return SAGPU_OK;
+        }
+    }
+    return ctx_create(out);
+}
+
+// Parks the context of a freed handle, or destroys it when enough are parked.
+void ctx_park(Ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    {
+        std::lock_guard<std::mutex> lk(g_slot_mu);
+        if (g_parked.size() < PARKED_MAX) {
+            g_parked.push_back(c);
+            return;
+        }
+    }
+    ctx_destroy(c);
 }
 
 struct Lease {
     Ctx *ctx = nullptr;
     Token *tok = nullptr;
     int slot = -1;
+    Ctx *prev_ctx = nullptr;
 
     int acquire()
     {
@@ -413,7 +461,7 @@ struct Lease {
             tok = t_bound;
             tok->mu.lock();
             if (!tok->c) {
-                const int rc = ctx_create(&tok->c);
+                const int rc = ctx_for_token(&tok->c);
                 if (rc != SAGPU_OK) {
                     tok->c = nullptr;
                     tok->mu.unlock();
@@ -444,13 +492,18 @@ struct Lease {
         (void)hipSetDevice(ctx->device);
         apply_options(*ctx);
         ctx->stats_fresh = false;
+        ctx->pool.reset_peak();
+        prev_ctx = t_ctx;
+        t_ctx = ctx;
         return SAGPU_OK;
     }
 
     ~Lease()
     {
         if (!ctx) return;
+        t_ctx = prev_ctx;
         if (ctx->stats_fresh) {
+            ctx->stats.peak_bytes = (int64_t)ctx->pool.peak_bytes;
             std::lock_guard<std::mutex> lk(g_stats_mu);
             g_last_stats = ctx->stats;
         }
@@ -514,7 +567,7 @@ struct Span {
     {
         if (!a) return;
         (void)hipEventRecord(b, c.stream);
-        c.timer.recs.push_back({fam, a, b});
+        c.timer.recs.push_back({fam, a, b, 0});
     }
 };
 
@@ -540,11 +593,49 @@ void timer_collect(Ctx &c)
         case F_LCP: c.stats.ms_lcp += ms; break;
         case F_TOTAL: c.stats.ms_total += ms; break;
         case F_PLCP_PAIRS: c.stats.ms_plcp_pairs_pass += ms; c.stats.plcp_pairs_launches++; break;
+        default:
+            if (r.family >= F_KERNEL0 && r.family < F_KERNEL0 + KI_COUNT) {
+                const int id = r.family - F_KERNEL0;
+                c.stats.kernel_ms[id] += ms;
+                c.stats.kernel_launches[id]++;
+                c.stats.kernel_bytes[id] += (int64_t)r.bytes;
+            }
         }
         c.timer.free_ev.push_back({r.a, r.b});
     }
     c.timer.recs.clear();
 }
+
+} // namespace
+
+void ktime_begin(int id)
+{
+    Ctx *c = t_ctx;
+    if (!c || !c->timer.on || id < 0 || id >= KI_COUNT) return;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (!c->timer.free_ev.empty()) {
+        a = c->timer.free_ev.back().first;
+        b = c->timer.free_ev.back().second;
+        c->timer.free_ev.pop_back();
+    } else if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    (void)hipEventRecord(a, c->stream);
+    c->timer.open[id] = {a, b};
+}
+
+void ktime_end(int id, u64 bytes)
+{
+    Ctx *c = t_ctx;
+    if (!c || !c->timer.on || id < 0 || id >= KI_COUNT || !c->timer.open[id].first) return;
+    auto ev = c->timer.open[id];
+    c->timer.open[id] = {};
+    (void)hipEventRecord(ev.second, c->stream);
+    c->timer.recs.push_back({F_KERNEL0 + id, ev.first, ev.second, bytes});
+}
+
+namespace {
 
 // ------------------------------------------------------------------ pieces
 int bits_for(u64 v)
@@ -588,8 +679,22 @@ void onesweep_hook(void *user, int phase)
 {
     Ctx &c = *(Ctx *)user;
     static thread_local Span *open = nullptr;
-    if (phase == 0) open = new Span(c, c.sort_u32 && !c.rs.gen_pass ? F_ONESWEEP32 : F_ONESWEEP);
-    else { delete open; open = nullptr; }
+    static thread_local int kid = 0;
+    if (phase == 0) {
+        open = new Span(c, c.sort_u32 && !c.rs.gen_pass ? F_ONESWEEP32 : F_ONESWEEP);
+        kid = !c.sort_u32 ? KI_RADIX64 : c.rs.gen_pass ? KI_RADIX_GEN : KI_RADIX32;
+        ktime_begin(kid);
+    } else {
+        const u64 n = c.timer.hook_n;
+        // u32 pass: key + value read and written; generating pass: the code stream
+        // read, key + value written; u64 pass: 8-byte key + value read and written
+        const u64 bytes = kid == KI_RADIX32 ? 16 * n
+                          : kid == KI_RADIX_GEN ? 8 * n + n * (u64)(c.rs.gen ? c.rs.gen->bits : 0) / 8
+                                                : 24 * n;
+        ktime_end(kid, bytes);
+        delete open;
+        open = nullptr;
+    }
 }
 
 // Brackets the pair-word PLCP pass (timing mode only); nested inside the F_PLCP span.
@@ -597,8 +702,15 @@ void plcp_pairs_hook(void *user, int phase)
 {
     Ctx &c = *(Ctx *)user;
     static thread_local Span *open = nullptr;
-    if (phase == 0) open = new Span(c, F_PLCP_PAIRS);
-    else { delete open; open = nullptr; }
+    if (phase == 0) {
+        open = new Span(c, F_PLCP_PAIRS);
+        ktime_begin(KI_PLCP_PASS);
+    } else {
+        // ISA 4, the preceding byte 1, one pair word 8, the value written 4
+        ktime_end(KI_PLCP_PASS, 17 * c.timer.hook_n);
+        delete open;
+        open = nullptr;
+    }
 }
 
 int sort_u64(Ctx &c, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, int end_bit, int *alt)
@@ -607,6 +719,7 @@ int sort_u64(Ctx &c, u64 *k0, u32 *v0, u64 *k1, u32 *v1, u64 n, int begin_bit, i
     c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
     c.rs.hook_user = &c;
     c.sort_u32 = false;
+    c.timer.hook_n = n;
     int rc = radix_sort_u64(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
     c.stats.sort_passes += passes;
     c.stats.sorted_elems += (int64_t)passes * (int64_t)n;
@@ -620,6 +733,7 @@ int sort_u32(Ctx &c, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, i
     c.rs.hook = c.timer.on ? onesweep_hook : nullptr;
     c.rs.hook_user = &c;
     c.sort_u32 = true;
+    c.timer.hook_n = n;
     int rc = radix_sort_u32(c.rs, k0, v0, k1, v1, n, begin_bit, end_bit, c.stream, alt, &passes);
     c.sort_u32 = false;
     c.stats.sort_passes += passes;
@@ -846,12 +960,21 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         const int nxt = cur ^ 1;
         {
             Span sp(c, F_ROUNDS);
-            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s,
-                                   st.rounds == 1 && rk1.pw ? &rk1 : nullptr, ac.cb));
+            const bool raw = st.rounds == 1 && rk1.pw;
+            ktime_begin(KI_ROUND_GATHER);
+            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s, raw ? &rk1 : nullptr,
+                                   ac.cb));
+            // suffix read, one random 4-byte read (rank, or code-stream window), key written
+            // (+ the group read for the first round's form)
+            ktime_end(KI_ROUND_GATHER, a * ((wide ? 16 : 12) + (raw ? 4 : 0)));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
+            ktime_begin(KI_ROUND_TILE);
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
-                                 st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
+                                 raw ? &ex1 : nullptr));
+            // list entry and key read (12 B); about 16 B written per entry: ranks,
+            // placed suffixes, marks and survivors
+            ktime_end(KI_ROUND_TILE, a * (wide ? 32 : 28));
         }
         SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
@@ -955,6 +1078,7 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     ls.tile_max = tm;
     ls.hook = c.timer.on ? plcp_pairs_hook : nullptr;
     ls.hook_user = &c;
+    c.timer.hook_n = n;
     u64 *V64 = nullptr;
     if (n > g_wide_from) {  // PLCP[i] + i may reach 2^32
         V64 = sc.get<u64>(n);
@@ -1126,7 +1250,12 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
         if (rc != SAGPU_OK) return rc;
         if (dU) {
             Span sp(c, F_BWT);
-            if (seed.cb) SAGPU_TRY(launch_bwt_seeded(dTpad, dSA, dISA, seed, n, dU, c.stream));
+            if (seed.cb) {
+                ktime_begin(KI_BWT);
+                SAGPU_TRY(launch_bwt_seeded(dTpad, dSA, dISA, seed, n, dU, c.stream));
+                // seed byte and bitmap bit read, BWT byte written
+                ktime_end(KI_BWT, 2 * n + n / 8);
+            }
             else if (g_bwt_variant == 1) SAGPU_TRY(launch_bwt_scatter(dTpad, dISA, n, dU, c.stream));
             else SAGPU_TRY(launch_bwt(dTpad, dSA, dISA, n, dU, c.stream));
         }
@@ -1138,7 +1267,9 @@ int64_t build_dev(Ctx &c, Scope &sc, const uint8_t *dTpad, u64 n, int gsa, u32 *
             }
             if (dLCP) {
                 Span sp(c, F_LCP);
+                ktime_begin(KI_LCP_FIX);
                 SAGPU_TRY(launch_lcp_fix(dPLCP, dSA, n, dLCP, c.stream));
+                ktime_end(KI_LCP_FIX, 12 * n);   // mark and SA read, LCP written
             }
         }
     }
@@ -1219,7 +1350,7 @@ extern "C" SAGPU_HIDDEN void sagpu_host_token_free(void *token)
     Token *t = (Token *)token;
     {
         std::lock_guard<std::mutex> lk(t->mu);   // waits for a call still running on it
-        ctx_destroy(t->c);
+        ctx_park(t->c);
         t->c = nullptr;
         t->magic = 0;
     }
@@ -1710,6 +1841,8 @@ extern "C" void sagpu_shutdown(void)
         for (Ctx *c : g_slots) ctx_destroy(c);
         g_slots.clear();
         g_slot_busy.clear();
+        for (Ctx *c : g_parked) ctx_destroy(c);
+        g_parked.clear();
     }
 }
 

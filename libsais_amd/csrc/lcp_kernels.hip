@@ -821,8 +821,10 @@ int launch_plcp(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *PHI
         return launch_plcp_t<u32, 4>(ls, T, n, 0, PHI, ISA, SA, PLCP, PLCP, nullptr, s);
     }
     if (marks && pairs) {
+        ktime_begin(KI_PLCP_WORDS);
         hipLaunchKernelGGL(k_plcp_pairs, dim3(grid_for(n, 256 * PP, 65535)), dim3(256), 0, s, SA, marks, U, T, ISA, n,
                            pairs);
+        ktime_end(KI_PLCP_WORDS, 16 * n);   // mark and SA read, one pair word written
     } else {
         pairs = nullptr;
     }

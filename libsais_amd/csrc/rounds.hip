@@ -7,13 +7,13 @@
 // the final position once it is alone).
 //
 // Round with offset h (all group members share their first h symbols):
-//   k_round_gather   key2[k] = rank of i+h (or a length code past the end), and
-//                    SA[g] = list index of the group's last member (SA slots of
-//                    unsorted groups are free until their suffixes are placed;
-//                    the last index, not the end, so 2^32 members still fit)
+//   k_round_gather   key2[k] = rank of i+h (or a length code past the end)
 //   k_round_tile     groups of <= RT_SMALL members whose first element lies in a
 //                    512-wide stretch of the list (tunable): ranked by key2 inside LDS,
-//                    regrouped, ISA/SA written, survivors emitted
+//                    regrouped, ISA/SA written, survivors emitted.  Group extents
+//                    come from a window of act_g; a big group's last list index is
+//                    written to SA[g] (its SA slots are free until its suffixes are
+//                    placed; the last index, not the end, so 2^32 members still fit)
 //   big path         larger groups: extracted, sorted by (group, key2) with the
 //                    radix sort, regrouped with a segmented scan (BigF)
 // A pair of neighbours separated in a round has lcp in [h, 2h); the slot of the
@@ -535,33 +535,31 @@ constexpr int IG = 4;
 template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                       u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
-                                                      KT *__restrict__ key2, u32 *__restrict__ SA, RawKey1 rk,
-                                                      int cb)
+                                                      KT *__restrict__ key2, RawKey1 rk, int cb)
 {
     const u64 chunk = 256ull * IG;
     for (u64 base = (u64)blockIdx.x * chunk; base < a; base += (u64)gridDim.x * chunk) {
-        u32 i[IG], g[IG], gn[IG], r[IG];
+        u32 i[IG], r[IG];
         bool raw[IG];
 #pragma unroll
         for (int q = 0; q < IG; q++) {
             const u64 k = base + (u64)q * 256 + threadIdx.x;
             i[q] = k < a ? act_i[k] >> cb : 0u;
-            g[q] = k < a ? act_g[k] : 0u;
-            gn[q] = k + 1 < a ? act_g[k + 1] : ~g[q];
-        }
-#pragma unroll
-        for (int q = 0; q < IG; q++) {
-            const u64 k = base + (u64)q * 256 + threadIdx.x;
-            const u64 ih = (u64)i[q] + h;
             // round 1 with a code stream: the packed key of i + h orders the group as
             // its rank does (equal keys, equal ranks), read from the 2-bits-per-symbol
             // stream instead of the 4-byte ISA; groups holding a suffix shorter than
             // h keep the rank form
             raw[q] = false;
-            if (rk.pw) {
+            if (rk.pw && k < a) {
+                const u32 g = act_g[k];
                 raw[q] = true;
-                for (int t = 0; t < rk.nspecial; t++) raw[q] = raw[q] && g[q] != rk.special[t];
+                for (int t = 0; t < rk.nspecial; t++) raw[q] = raw[q] && g != rk.special[t];
             }
+        }
+#pragma unroll
+        for (int q = 0; q < IG; q++) {
+            const u64 k = base + (u64)q * 256 + threadIdx.x;
+            const u64 ih = (u64)i[q] + h;
             if (k < a && raw[q]) {
                 const u64 bitpos = ih * (u64)rk.bits;
                 const u64 w = bitpos >> 5;
@@ -577,7 +575,6 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
             if (k >= a) continue;
             const u64 ih = (u64)i[q] + h;
             key2[k] = raw[q] ? (KT)r[q] : ih < n ? (KT)(h + r[q]) : (KT)(n - 1 - i[q]);
-            if (gn[q] != g[q]) SA[g[q]] = (u32)k;
         }
     }
 }
@@ -586,6 +583,14 @@ constexpr int RT_THREADS = 256;
 
 // RANGE = list entries whose group heads one workgroup takes (256, 512 or 1024; the
 // "tile_range" option): smaller ranges use less LDS and fit more workgroups per CU.
+//
+// Group extents come from the workgroup's own window of act_g, [lo, lo + RANGE +
+// RT_SMALL + 1): a group with its head in [lo, lo + RANGE) and at most RT_SMALL
+// members ends inside it.  Groups of more members are big: their heads go to the
+// big list, and the workgroup holding a big group's last entry writes that entry's
+// list index to SA[g] (the group's SA slots are free until its suffixes are
+// placed, and no workgroup places suffixes of a big group here), from which
+// k_big_sizes takes the size after this kernel.
 template <typename KT, int RANGE>
 __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                            const KT *__restrict__ key2, u64 a, u32 h,
@@ -593,105 +598,133 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                                                            u32 *__restrict__ out_i, u32 *__restrict__ out_g,
                                                            u64 *__restrict__ out_counter, u32 *__restrict__ err,
                                                            u32 *__restrict__ big_head, u32 *__restrict__ big_g,
-                                                           u32 *__restrict__ big_size, u32 *__restrict__ big_counter,
-                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac,
-                                                           ExactLcp ex)
+                                                           u32 *__restrict__ big_counter, u64 big_cap,
+                                                           u32 *__restrict__ LCP, ActCode ac, ExactLcp ex)
 {
     constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
+    constexpr int WIN = RANGE + RT_SMALL + 1;                  // act_g window [lo, lo + WIN)
+    constexpr int W_PER = (WIN + RT_THREADS - 1) / RT_THREADS; // window entries per thread
     static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
-    __shared__ u32 s_i[RT_CAP], s_g[RT_CAP], s_x0[RT_CAP];
+    constexpr u32 NONE = 0xFFFFFFFFu;
+    __shared__ u32 s_w[WIN];      // act_g over the window
+    __shared__ u32 s_hx[WIN];     // local index of the position's group head (NONE: head before lo)
+    __shared__ u32 s_gend[RANGE]; // last local index of the group headed at local x (NONE: past the window)
+    __shared__ u32 s_i[RT_CAP];
     __shared__ KT s_k[RT_CAP];
     __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
     __shared__ uint8_t s_os[RT_CAP];
-    __shared__ u32 s_first, s_end, s_wsum[4];
+    __shared__ u32 s_first, s_end, s_wsum[4], s_prev;
     __shared__ u64 s_base;
 
     const u32 tid = threadIdx.x;
     const u64 lo = (u64)blockIdx.x * RANGE;
+    if (lo >= a) return;
     const u64 hi = lo + RANGE < a ? lo + RANGE : a;
+    const u32 wlen = (u32)((a - lo) < (u64)WIN ? (a - lo) : (u64)WIN);
+    const u32 rlen = (u32)(hi - lo);
+
+    // 1. the window, the entry before it, and the group head of every position
+    //    (a max-scan over head flags: local index + 1, 0 = head before lo)
+    for (u32 x = tid; x < wlen; x += RT_THREADS) s_w[x] = act_g[lo + x];
     if (tid == 0) {
-        s_first = 0xFFFFFFFFu;
+        s_first = NONE;
         s_end = 0;
+        s_prev = lo > 0 ? act_g[lo - 1] : NONE;
+    }
+    if (tid < RANGE)
+        for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE;
+    __syncthreads();
+    {
+        u32 loc[W_PER];
+        u32 m = 0;
+#pragma unroll
+        for (int j = 0; j < W_PER; j++) {
+            const u32 x = tid * W_PER + j;
+            const bool head = x < wlen && (x == 0 ? (lo == 0 || s_prev != s_w[0]) : s_w[x - 1] != s_w[x]);
+            if (head) m = x + 1;
+            loc[j] = m;
+        }
+        const u32 inc = wave_incl_max(m);
+        if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
+        __syncthreads();
+        u32 c = 0;
+        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) c = umax(c, s_wsum[q]);
+        const u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
+        if (lane_id() > 0) c = umax(c, prev);
+#pragma unroll
+        for (int j = 0; j < W_PER; j++) {
+            const u32 x = tid * W_PER + j;
+            if (x < wlen) s_hx[x] = umax(c, loc[j]) - 1u;   // 0 - 1 = NONE
+        }
     }
     __syncthreads();
-
-    // 1. group heads in [lo, hi): small ones define the tile's element range, big
-    //    ones are handed to the big path.
-    for (u64 k = lo + tid; k < hi; k += RT_THREADS) {
-        u32 g = act_g[k];
-        if (k == 0 || act_g[k - 1] != g) {
-            const u64 end = (u64)SA[g] + 1;
-            const u64 size = end - k;
-            if (size <= RT_SMALL) {
-                atomicMin(&s_first, (u32)(k - lo));
-                atomicMax(&s_end, (u32)(end - lo));
-            } else {
-                u32 slot = atomicAdd(big_counter, 1u);
-                if (slot < big_cap) {
-                    big_head[slot] = (u32)k;
-                    big_g[slot] = g;
-                    big_size[slot] = (u32)(size - 1);  // size - 1: a group may hold 2^32 members
-                }
+    // 2. group ends: a position is its group's last entry when the next entry
+    //    differs (or it is the list's last).  Ends of groups headed in [lo, hi) are
+    //    recorded per head; ends in [lo, hi) of big groups get their SA marker.
+    for (u32 x = tid; x < wlen; x += RT_THREADS) {
+        const bool is_end = (x + 1 < wlen) ? s_w[x + 1] != s_w[x] : (lo + x + 1 == a);
+        if (!is_end) continue;
+        const u32 hx = s_hx[x];
+        if (hx != NONE && hx < rlen) s_gend[hx] = x;
+        if (x < rlen) {
+            bool big;
+            if (hx != NONE) big = x - hx + 1 > (u32)RT_SMALL;
+            else {
+                // the group holding position lo has its head before lo
+                const u64 e = lo + x;
+                big = e >= (u64)RT_SMALL && act_g[e - RT_SMALL] == s_w[x];
+            }
+            if (big) SA[s_w[x]] = (u32)(lo + x);
+        }
+    }
+    __syncthreads();
+    // 3. heads in [lo, hi): small groups define the tile's element range, big ones
+    //    go to the big list
+    for (u32 x = tid; x < rlen; x += RT_THREADS) {
+        if (s_hx[x] != x) continue;
+        const u32 ge = s_gend[x];
+        if (ge != NONE && ge - x + 1 <= (u32)RT_SMALL) {
+            atomicMin(&s_first, x);
+            atomicMax(&s_end, ge + 1);
+        } else {
+            const u32 slot = atomicAdd(big_counter, 1u);
+            if (slot < big_cap) {
+                big_head[slot] = (u32)(lo + x);
+                big_g[slot] = s_w[x];
             }
         }
     }
     __syncthreads();
-    if (s_first == 0xFFFFFFFFu) return;
+    if (s_first == NONE) return;
     const u32 first = s_first, len = s_end - s_first;
-    if (s_end < s_first || len > RT_CAP) {
+    if (s_end < s_first || len > (u32)RT_CAP || s_end > wlen) {
         // inconsistent group bookkeeping: report instead of overrunning LDS
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
     const u64 gfirst = lo + first;
 
-    // 2. load the range; local group start by a max-scan over head positions
+    // 4. load the range's suffixes and keys
     for (u32 x = tid; x < len; x += RT_THREADS) {
-        u64 k = gfirst + x;
+        const u64 k = gfirst + x;
         s_i[x] = act_i[k];
         s_k[x] = key2[k];
-        s_g[x] = act_g[k];
-    }
-    __syncthreads();
-    {
-        u32 loc[RT_PER];
-        u32 m = 0;
-#pragma unroll
-        for (int j = 0; j < RT_PER; j++) {
-            u32 x = tid * RT_PER + j;
-            bool head = x < len && (x == 0 || s_g[x - 1] != s_g[x]);
-            if (head) m = x;
-            loc[j] = m;
-        }
-        u32 inc = wave_incl_max(m);
-        if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
-        __syncthreads();
-        u32 c = 0;
-        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) c = umax(c, s_wsum[q]);
-        u32 prev = __shfl_up(inc, 1, SAGPU_WAVE);
-        if (lane_id() > 0) c = umax(c, prev);
-#pragma unroll
-        for (int j = 0; j < RT_PER; j++) {
-            u32 x = tid * RT_PER + j;
-            if (x < len) s_x0[x] = umax(c, loc[j]);
-        }
+        s_os[x] = 0;
     }
     __syncthreads();
 
-    // 3. rank inside the group by key2 (count smaller, equal, equal-before)
-    for (u32 x = tid; x < len; x += RT_THREADS) s_os[x] = 0;
-    __syncthreads();
+    // 5. rank inside the group by key2 (count smaller, equal, equal-before)
     for (u32 x = tid; x < len; x += RT_THREADS) {
-        const u32 x0 = s_x0[x];
-        const u32 g = s_g[x];
-        const u64 gsz64 = (u64)SA[g] + 1 - (gfirst + x0);  // last index was stored at the first slot
-        if (gsz64 > RT_SMALL) continue;                    // big group: not ours
-        const u32 gsz = (u32)gsz64;
+        const u32 hx = s_hx[first + x];
+        const u32 ge = s_gend[hx];
+        if (ge == NONE || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
+        const u32 x0 = hx - first, gsz = ge - hx + 1;
+        const u32 g = s_w[hx];
         const KT key = s_k[x];
         u32 lt = 0, eq = 0, eqb = 0;
         KT pk = 0;   // the largest key below this one: the previous subgroup's
         for (u32 y = x0; y < x0 + gsz; y++) {
-            KT ky = s_k[y];
+            const KT ky = s_k[y];
             lt += ky < key;
             eq += ky == key;
             eqb += (ky == key) & (y < x);
@@ -709,9 +742,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
     }
     __syncthreads();
-    // ISA and SA writes happen after every thread has read SA[g] (group end marker).
     for (u32 x = tid; x < len; x += RT_THREADS) {
-        u32 st = s_os[x];
+        const u32 st = s_os[x];
         if (st == 0) continue;
         const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
         if (!(st & 4)) ISA[v] = s_og[x];
@@ -721,22 +753,22 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
     }
 
-    // 4. survivors, in sorted order, to the next active list
+    // 6. survivors, in sorted order, to the next active list
     {
         u32 cnt[RT_PER];
         u32 tot = 0;
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
-            u32 x = tid * RT_PER + j;
+            const u32 x = tid * RT_PER + j;
             cnt[j] = (x < len && (s_os[x] & 3) == 2) ? 1u : 0u;
             tot += cnt[j];
         }
-        u32 inc = wave_incl_sum(tot);
+        const u32 inc = wave_incl_sum(tot);
         __syncthreads();
         if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
         __syncthreads();
         if (tid == 0) {
-            u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+            const u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
             s_base = all ? atomicAdd(out_counter, (u64)all) : 0ull;
         }
         __syncthreads();
@@ -744,7 +776,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
-            u32 x = tid * RT_PER + j;
+            const u32 x = tid * RT_PER + j;
             if (cnt[j]) {
                 out_i[o] = s_oi[x];
                 out_g[o] = s_og[x];
@@ -752,6 +784,18 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             }
         }
     }
+}
+
+// Sizes of the big groups the tile kernel listed: the entry holding a big group's
+// last list index wrote it to SA[g].  Stored minus one (a group may hold 2^32
+// members).
+__global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ big_head, const u32 *__restrict__ big_g,
+                                                   const u32 *__restrict__ big_counter, u64 big_cap,
+                                                   const u32 *__restrict__ SA, u32 *__restrict__ big_size)
+{
+    const u64 nb = *big_counter < big_cap ? *big_counter : big_cap;
+    for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (u64)gridDim.x * blockDim.x)
+        big_size[q] = SA[big_g[q]] - big_head[q];
 }
 
 // Copies the members of the big groups (sorted by list position) into the sort
@@ -808,8 +852,14 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     SAGPU_TRY(launch_fill_u32(ws.bcur, IB_MAX, 0, s));
     hipLaunchKernelGGL(k_init_reduce<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
+    ktime_begin(KI_INIT_EMIT);
     hipLaunchKernelGGL(k_init_emit<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
+    // key and value read; SA, the (suffix, rank) pair, LCP mark and BWT code written
+    // per position (the active-list entries are not counted)
+    ktime_end(KI_INIT_EMIT, n * (sizeof(K) + 4 + 4 + 8 + (LCP ? 4 : 0) + (f.codes ? 1 : 0)));
+    ktime_begin(KI_ISA_SCATTER);
     hipLaunchKernelGGL(k_isa_scatter, dim3(grid_for(n, 256 * IW, 65535)), dim3(256), 0, s, PV, PG, n, ISA);
+    ktime_end(KI_ISA_SCATTER, 12 * n);   // pair read, rank written
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -843,8 +893,9 @@ template <typename KT>
 int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
                    hipStream_t s, const RawKey1 &rk, int cb)
 {
+    (void)SA;
     hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
-                       n, h, key2, SA, rk, cb);
+                       n, h, key2, rk, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -857,12 +908,14 @@ int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
 #define RT_LAUNCH(R)                                                                                              \
     hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, \
-                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_size, big_counter, big_cap,  \
-                       LCP, ac, ex)
+                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_counter, big_cap, LCP, ac,  \
+                       ex)
     if (g_tile_range == 256) RT_LAUNCH(256);
     else if (g_tile_range == 1024) RT_LAUNCH(1024);
     else RT_LAUNCH(512);
 #undef RT_LAUNCH
+    hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, big_head, big_g,
+                       big_counter, big_cap, SA, big_size);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

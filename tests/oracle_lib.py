@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_PATH = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+ORACLE_PATH = os.environ.get("LIBSAIS_ORACLE_LIB") or os.path.join(ROOT, "oracle", "lib", "liboracle.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libsais_ref.so")
 
 _o = None
