@@ -124,6 +124,25 @@ SAGPU_API int32_t sagpu_server_drain(void *server);
 SAGPU_API void    sagpu_server_destroy(void *server);
 SAGPU_API double  sagpu_clock(void);
 
+/*
+ * Allocation-free form of sagpu_build32.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and return codes are those of sagpu_build32; every working
+ * buffer is carved from one static device array of SAGPU_NOALLOC_SCRATCH_BYTES
+ * per GPU (held by each GPU the library runs on until the process ends), so the
+ * call makes no device or pinned allocation.  n is limited to SAGPU_NOALLOC_MAX_N
+ * (-1 beyond, with a message in sagpu_last_error()).  Set-up (the scratch address,
+ * a stream, 64 KiB of pinned staging) happens in sagpu_init() for the current
+ * device, or on the first call on a device.  All calls on one GPU share its array,
+ * which keeps the last call's contents: the library serialises them (a second
+ * thread waits until the first call returns), since overlapping calls would race
+ * on it.
+ */
+#define SAGPU_NOALLOC_MAX_N 4194304
+#define SAGPU_NOALLOC_SCRATCH_BYTES ((size_t)288 << 20)
+SAGPU_API int64_t sagpu_build32_noalloc(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA, uint8_t *d_U,
+                                        int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq);
 /* PLCP from a device-resident text and suffix array (libsais_plcp semantics). */
 SAGPU_API int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags);
 /* LCP[i] = PLCP[SA[i]] on device buffers (libsais_lcp semantics). */

@@ -225,6 +225,7 @@ ABI_SAGPU = [
     ("sagpu_set_memory_limit", None, [ctypes.c_uint64]),
     ("sagpu_trim", None, []),
     ("sagpu_build32", _i64, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    ("sagpu_build32_noalloc", _i64, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("sagpu_gsa_batch32", _i32, [_vp, _i64, _vp, _i32, _vp, _vp]),
     ("sagpu_run_job", _i32, [_vp]),
     ("sagpu_server_create", _vp, [_i32, ctypes.c_uint64, _i64]),

@@ -56,6 +56,15 @@ struct Pool {
     size_t live_bytes = 0;
     size_t peak_bytes = 0;   // largest live_bytes since the last reset_peak()
 
+    // Arena mode (the allocation-free build): buffers are carved from one fixed
+    // device region, bottom up, and never come from hipMalloc.  A freed block is
+    // reclaimed once every block above it is freed too (the engine frees its
+    // buffers in scopes, so the region works as a stack).
+    uint8_t *arena = nullptr;
+    size_t arena_cap = 0, arena_top = 0;
+    struct Block { size_t off, end; bool freed; };
+    std::vector<Block> arena_blocks;
+
     static size_t round(size_t b)
     {
         if (b < (1u << 20)) return (b + 255) & ~(size_t)255;
@@ -64,8 +73,37 @@ struct Pool {
 
     void reset_peak() { peak_bytes = live_bytes; }
 
+    void *arena_get(size_t bytes)
+    {
+        const size_t off = (arena_top + 255) & ~(size_t)255;
+        if (off > arena_cap || bytes > arena_cap - off) {
+            sagpu_set_error("allocation-free build: the static scratch is too small for this call",
+                            __FILE__, __LINE__);
+            return nullptr;
+        }
+        arena_blocks.push_back({off, off + bytes, false});
+        arena_top = off + bytes;
+        live_bytes = arena_top;
+        if (live_bytes > peak_bytes) peak_bytes = live_bytes;
+        return arena + off;
+    }
+
+    void arena_put(void *p)
+    {
+        const size_t off = (size_t)((uint8_t *)p - arena);
+        for (auto it = arena_blocks.rbegin(); it != arena_blocks.rend(); ++it)
+            if (it->off == off && !it->freed) {
+                it->freed = true;
+                break;
+            }
+        while (!arena_blocks.empty() && arena_blocks.back().freed) arena_blocks.pop_back();
+        arena_top = arena_blocks.empty() ? 0 : arena_blocks.back().end;
+        live_bytes = arena_top;
+    }
+
     void *get(size_t bytes)
     {
+        if (arena) return arena_get(bytes ? bytes : 1);
         bytes = round(bytes ? bytes : 1);
         const u64 limit = g_mem_limit.load();
         if (limit && live_bytes + bytes > limit) {
@@ -103,6 +141,10 @@ struct Pool {
     void put(void *p)
     {
         if (!p) return;
+        if (arena) {
+            arena_put(p);
+            return;
+        }
         auto it = live.find(p);
         if (it == live.end()) return;
         cache.emplace(it->second, p);
@@ -1819,12 +1861,19 @@ extern "C" SAGPU_HIDDEN int64_t sagpu_host_lcp(const void *PLCP, const void *SA,
 
 // ======================================================== public device ABI
 
-// Creates the first slot context (otherwise created on first use).
+namespace {
+int noalloc_init_current();
+} // namespace
+
+// Creates the first slot context (otherwise created on first use) and sets up the
+// allocation-free build on the current device.
 extern "C" int32_t sagpu_init(void)
 {
-    LEASE(c);
-    (void)c;
-    return SAGPU_OK;
+    {
+        LEASE(c);
+        (void)c;
+    }
+    return noalloc_init_current();
 }
 
 // Destroys the slot contexts (after the calls that hold them end).  Contexts of
@@ -2141,6 +2190,109 @@ extern "C" int64_t sagpu_build32(const uint8_t *d_T, int64_t n, int32_t flags, i
     if (!d_T || n < 2 || n > INT32_MAX) return SAGPU_EINVAL;
     LEASE(c);
     return build32_on(*c, d_T, (u64)n, flags, d_SA, d_U, d_PLCP, d_LCP, h_freq);
+}
+
+// ------------------------------------------------------ allocation-free build
+// The working buffers of sagpu_build32_noalloc: one static array per GPU (each
+// device holds its own copy, at its own address, from module load until the
+// process ends).  Host code never passes this name to a kernel or a copy: the
+// device address comes from hipGetSymbolAddress at set-up (NoAllocDev::ptr).
+static __device__ __align__(256) uint8_t scratch[SAGPU_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+
+// Per-device state of the allocation-free build: the scratch address and an engine
+// context whose pool carves every buffer from it (arena mode).  The radix sort's
+// persistent scratch is carved once, at the bottom, for SAGPU_NOALLOC_MAX_N.
+struct NoAllocDev {
+    std::mutex mu;        // one call at a time per device: calls share the array
+    Ctx *c = nullptr;
+    uint8_t *ptr = nullptr;
+};
+NoAllocDev g_noalloc[NOALLOC_DEVICES];
+
+// Set-up for `dev` (the current device); caller holds g_noalloc[dev].mu.
+int noalloc_setup(int dev)
+{
+    NoAllocDev &d = g_noalloc[dev];
+    if (d.c) return SAGPU_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        sagpu_set_error("hipGetSymbolAddress of the allocation-free scratch failed", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    Ctx *c = nullptr;
+    SAGPU_TRY(ctx_create(&c, dev));
+    c->pool.arena = (uint8_t *)ptr;
+    // the top 64 KiB stay unused, so a vector load that runs a few bytes past the
+    // last buffer still reads inside the array
+    c->pool.arena_cap = SAGPU_NOALLOC_SCRATCH_BYTES - (64u << 10);
+    Scope keep(*c);   // nothing is put back: the radix scratch stays at the bottom
+    int rc = ensure_radix_scratch(*c, keep, SAGPU_NOALLOC_MAX_N);
+    keep.held.clear();
+    if (rc == SAGPU_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = SAGPU_EDEVICE;
+    if (rc != SAGPU_OK) {
+        (void)hipGetLastError();
+        ctx_destroy(c);
+        return rc;
+    }
+    c->bigh.reserve(SAGPU_NOALLOC_MAX_N / (RT_SMALL + 1) + 16);
+    d.ptr = (uint8_t *)ptr;
+    d.c = c;
+    return SAGPU_OK;
+}
+
+int current_device(int *dev)
+{
+    if (hipGetDevice(dev) != hipSuccess || *dev < 0 || *dev >= NOALLOC_DEVICES) {
+        (void)hipGetLastError();
+        sagpu_set_error("no usable HIP device", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
+    return SAGPU_OK;
+}
+
+int noalloc_init_current()
+{
+    int dev = 0;
+    SAGPU_TRY(current_device(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc[dev].mu);
+    return noalloc_setup(dev);
+}
+
+} // namespace
+
+extern "C" int64_t sagpu_build32_noalloc(const uint8_t *d_T, int64_t n, int32_t flags, int32_t *d_SA,
+                                         uint8_t *d_U, int32_t *d_PLCP, int32_t *d_LCP, int64_t *h_freq)
+{
+    if (!d_T || n < 2) return SAGPU_EINVAL;
+    if (n > SAGPU_NOALLOC_MAX_N) {
+        sagpu_set_error("sagpu_build32_noalloc: n exceeds SAGPU_NOALLOC_MAX_N (the static scratch holds "
+                        "the working buffers of at most that many positions)", __FILE__, __LINE__);
+        return SAGPU_EINVAL;
+    }
+    int dev = 0;
+    SAGPU_TRY(current_device(&dev));
+    NoAllocDev &d = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(d.mu);
+    SAGPU_TRY(noalloc_setup(dev));
+    Ctx &c = *d.c;
+    apply_options(c);
+    c.pool.reset_peak();
+    Ctx *prev = t_ctx;
+    t_ctx = &c;
+    const int64_t rc = build32_on(c, d_T, (u64)n, flags, d_SA, d_U, d_PLCP, d_LCP, h_freq);
+    t_ctx = prev;
+    c.stats.peak_bytes = (int64_t)c.pool.peak_bytes;
+    {
+        std::lock_guard<std::mutex> sl(g_stats_mu);
+        g_last_stats = c.stats;
+    }
+    return rc;
 }
 
 extern "C" int32_t sagpu_plcp32(const uint8_t *d_T, const int32_t *d_SA, int32_t *d_PLCP, int64_t n, int32_t flags)
