@@ -7,8 +7,9 @@ index, PLCP and LCP, all int32, outputs left in HBM.  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 
 roofline: the kernel with the largest device time per build, among the kernels
-the engine times (HIP events recorded on the engine stream around every launch
-inside the timed steps; sagpu_stats.kernel_ms).  achieved = its algorithmic
+the engine times (HIP events recorded on the engine stream around every launch,
+sagpu_stats.kernel_ms, in up to 5 extra steps after the timed ones: the timed
+steps run without events, with the doubling rounds replaying captured graphs).  achieved = its algorithmic
 bytes per launch (reads + writes at element size, counted by the engine per
 launch: e.g. k_round_tile 28 B per active list entry, k_isa_scatter 12 B per
 position) / its average launch duration.  roofline.kernels lists the five kernels
@@ -43,7 +44,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # sagpu_stats.kernel_ms index -> kernel (include/sagpu.h)
 KERNELS = ["k_radix_pass<u32 key, u32 value>", "k_radix_pass (generating first pass)",
-           "k_radix_pass<u64 key, u32 value>", "k_init_emit", "k_isa_scatter", "k_round_gather", "k_round_tile",
+           "k_radix_pass<u64 key, u32 value>", "k_init_emit", "k_scatter_windows (initial ranks)", "k_round_gather",
+           "k_round_tile",
            "k_plcp_pairs", "k_plcp_irr_pairs", "k_lcp_fix", "k_bwt_seeded"]
 METRIC = "SA+BWT input MB/s @256 MiB DNA; batched vs plain aggregate MB/s; p99 latency"
 
@@ -461,6 +463,22 @@ def main():
         step()
         log(f"warmup {i} done")
 
+    # the timed steps run as in production (no timing events; the rounds replay their
+    # captured graphs) ...
+    lib.sagpu_synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        primary = step()
+    lib.sagpu_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    value = n / (ms_per_step / 1000.0) / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} MB/s")
+    graphs = la.stats().round_graph_launches
+
+    # ... then the same steps again with HIP events around every kernel launch (kernels
+    # one by one), for the per-kernel rooflines and the family times
+    psteps = max(2, min(args.steps, 5))
     lib.sagpu_set_timing(1)
     sweep_ms, sweep_launch, sweep_elems, sweep_bytes = 0.0, 0, 0, 0
     k_ms = [0.0] * len(KERNELS)
@@ -468,10 +486,9 @@ def main():
     k_bytes = [0] * len(KERNELS)
     dev_ms = 0.0
     fam = {"initial": 0.0, "rounds": 0.0, "bwt": 0.0, "plcp": 0.0, "lcp": 0.0}
-    lib.sagpu_synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        primary = step()
+    for i in range(psteps):
+        step()
         st = la.stats()
         sweep_ms += st.ms_onesweep
         sweep_launch += st.onesweep_launches
@@ -488,11 +505,8 @@ def main():
         fam["plcp"] += st.ms_plcp
         fam["lcp"] += st.ms_lcp
     lib.sagpu_synchronize()
-    wall = time.perf_counter() - t0
+    timed_wall_ms = (time.perf_counter() - t0) * 1000.0 / psteps
     lib.sagpu_set_timing(0)
-    ms_per_step = wall * 1000.0 / args.steps
-    value = n / (ms_per_step / 1000.0) / 1e6
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} MB/s")
 
     st = la.stats()
     rounds = st.rounds
@@ -511,8 +525,8 @@ def main():
         bpl = k_bytes[q] / k_launch[q]
         ach = bpl / (avg_ms / 1000.0) / 1e9
         tr = measured_traffic(workload, name)
-        kernels.append({"kernel": name, "ms_per_build": round(k_ms[q] / args.steps, 4),
-                        "launches_per_build": round(k_launch[q] / args.steps, 2),
+        kernels.append({"kernel": name, "ms_per_build": round(k_ms[q] / psteps, 4),
+                        "launches_per_build": round(k_launch[q] / psteps, 2),
                         "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bpl),
                         "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": tr, "traffic_over_algorithmic": round(tr / bpl, 2) if tr else None})
@@ -559,11 +573,13 @@ def main():
             "traffic": top["traffic"] if top else None,
             "copy_peak_GBs": round(cpeak, 1),
             "frac_of_copy": round(top["achieved"] / cpeak, 4) if top else None,
-            "launches": int(top["launches_per_build"] * args.steps) if top else 0,
+            "launches": int(top["launches_per_build"] * psteps) if top else 0,
             "avg_launch_ms": top["avg_launch_ms"] if top else None,
             "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_launch"] if top else None,
-            "share_of_build": round(top["ms_per_build"] / (ms_per_step), 4) if top else None,
-            "kernels": kernels[:5],
+            "share_of_build": round(top["ms_per_build"] / timed_wall_ms, 4) if top else None,
+            "timing_steps": psteps,
+            "ms_per_step_with_events": round(timed_wall_ms, 3),
+            "kernels": kernels,
             "timed_kernels_ms_per_build": round(timed_ms, 3),
             "all_radix_passes": {"launches": sweep_launch, "ms": round(sweep_ms, 3), "bytes": sweep_bytes},
         },
@@ -572,8 +588,9 @@ def main():
             "a_r": a_r,
             "achieved_GBs": round(b_model / (ms_per_step / 1000.0) / 1e9, 1),
             "frac": round(b_model / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
-            "device_ms_per_step": round(dev_ms / args.steps, 3),
-            "family_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam.items()},
+            "device_ms_per_step": round(dev_ms / psteps, 3),
+            "family_ms_per_step": {k: round(v / psteps, 3) for k, v in fam.items()},
+            "round_graph_replays_per_build": graphs,
             "engine_rounds": rounds,
             "engine_active": active,
             "initial_symbols": st.initial_symbols,

@@ -183,12 +183,15 @@ typedef struct sagpu_stats {
     /* Per-kernel device time (ms, events around each launch), launches and
      * algorithmic bytes (reads + writes at element size) of the build, timing mode
      * only.  Index: 0 radix pass u32 key + u32 value, 1 generating first radix pass,
-     * 2 radix pass u64 key + u32 value, 3 k_init_emit, 4 k_isa_scatter,
+     * 2 radix pass u64 key + u32 value, 3 k_init_emit, 4 the initial ranks' scatter,
      * 5 k_round_gather, 6 k_round_tile, 7 k_plcp_pairs, 8 k_plcp_irr_pairs,
      * 9 k_lcp_fix, 10 k_bwt_seeded; 11-15 unused. */
     double  kernel_ms[16];
     int32_t kernel_launches[16];
     int64_t kernel_bytes[16];
+    int32_t round_graphs;         /* doubling-round graphs the build captured (0 when the
+                                     previous build's graphs served unchanged)        */
+    int32_t round_graph_launches; /* doubling rounds the build ran as graph replays   */
 } sagpu_stats;
 
 /*
@@ -207,6 +210,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    tiles; default 32)
  *   "radix_threads"  onesweep workgroup size with 32 keys per thread: 256 (8192-key tiles) or
  *                    512 (16384-key tiles, default; 32-bit keys only)
+ *   "radix_bits"     digit width of the onesweep radix sort: 8 (default), 4 or 11 (4 and 11 use
+ *                    256-thread tiles and the onesweep variant; the allocation-free build keeps 8)
  *   "tile_range"     list entries per workgroup of the doubling-round tile kernel: 256, 512
  *                    (default) or 1024
  *   "bwt_variant"    0 = seeded (default): the initial sort carries each suffix's BWT symbol
@@ -239,6 +244,13 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    scattered bucket after bucket
  *   "wide_from"      texts longer than this use 64-bit rank keys (default 2^31 - 1, the
  *                    largest the 32-bit keys hold; lower values exercise the wide mode)
+ *   "round_graphs"   1 = doubling rounds from the second on replay a captured HIP graph per
+ *                    buffer parity (parameter copy, gather, tile kernel, counter copy), updated
+ *                    in place across builds (default; not in timing mode or the allocation-free
+ *                    build); 0 = kernels launched one by one
+ *   "windowed_scatter" 1 = texts over 2^22 write the initial ranks through a radix pass into
+ *                    windows of the ISA and a scatter in which each XCD writes only its own windows
+ *                    (default); 0 = one direct scatter
  *   "api_slots"      contexts that calls made without a libsais context share, 1-16
  *                    (default 8): that many such calls run at once, each on its own stream
  * Returns 0, or -1 for an unknown name or value.

@@ -84,6 +84,8 @@ class SagpuStats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double * 16),
         ("kernel_launches", ctypes.c_int32 * 16),
         ("kernel_bytes", ctypes.c_int64 * 16),
+        ("round_graphs", ctypes.c_int32),
+        ("round_graph_launches", ctypes.c_int32),
     ]
 
 

@@ -2,8 +2,11 @@
 
 The knobs are process-wide options of the engine (sagpu_set_option):
 
+  radix_bits     digit width of the radix sort: 4, 8 or 11 bits (16, 256 or 2048
+                 buckets per pass; 4 and 11 run the onesweep variant on 256-thread
+                 tiles)
   radix_items    keys per thread of a radix tile: 8, 16, 32 (tiles of 2048, 4096,
-                 8192 keys)
+                 8192 keys; 16384 with 512 threads and 8-bit digits)
   radix_variant  0 = onesweep (decoupled look-back), 1 = reduce-then-scan
   radix_threads  onesweep workgroup size with 32 keys per thread: 256 or 512
   tile_range     list entries per workgroup of the doubling-round tile kernel:
@@ -11,12 +14,11 @@ The knobs are process-wide options of the engine (sagpu_set_option):
 
 The tuner runs a workload once per point of the grid on device-resident data,
 checks that every point produces output identical to the default configuration's
-(the knobs change speed only), and keeps the fastest.  Radix digits stay 8 bits:
-4-bit digits double the passes of a bandwidth-bound sort, and 11-bit digits need
-2048 look-back counters per tile and leave digit runs of ~4 keys per tile, which
-breaks write coalescing.  Active lists are compacted every round at no extra pass
-(the tile kernel emits survivors as it goes), so there is no compaction threshold
-to tune.
+(the knobs change speed only), and keeps the fastest.  The default grid sweeps
+digit width x radix tile x tile range.  Active lists are compacted every round at
+no extra pass (the tile kernel writes the survivors of its groups as it ranks them,
+in one contiguous run per workgroup), so there is no compaction threshold to tune:
+leaving retired entries in place would save no traffic.
 
   python -m libsais_amd.autotune --workload dna --n 67108864 --out tune.json
   python -m libsais_amd.autotune --workload batched --out tune.json
@@ -34,12 +36,11 @@ import numpy as np
 from . import SAGPU_FLAG_GSA, last_error, load as _load_lib
 
 GRID = {
+    "radix_bits": (4, 8, 11),
     "radix_items": (8, 16, 32),
-    "radix_variant": (0, 1),
-    "radix_threads": (256, 512),
     "tile_range": (256, 512, 1024),
 }
-DEFAULT = {"radix_items": 32, "radix_variant": 0, "radix_threads": 512, "tile_range": 512}
+DEFAULT = {"radix_bits": 8, "radix_items": 32, "radix_variant": 0, "radix_threads": 512, "tile_range": 512}
 
 
 def apply(cfg: dict) -> None:

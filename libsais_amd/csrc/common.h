@@ -8,6 +8,7 @@
 
 typedef unsigned long long u64;
 typedef uint32_t u32;
+typedef uint16_t u16;
 
 // Status codes used inside the engine; they map 1:1 onto the libsais return codes.
 enum { SAGPU_OK = 0, SAGPU_EINVAL = -1, SAGPU_ENOMEM = -2, SAGPU_EDEVICE = -3 };

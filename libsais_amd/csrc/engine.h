@@ -50,13 +50,18 @@ enum KernelId {
 void ktime_begin(int id);
 void ktime_end(int id, u64 bytes);
 
+// Digit histograms of a sort: counts [pass][bins] in the first RADIX_HIST_WORDS
+// words, bucket offsets in the next RADIX_HIST_WORDS (passes x bins <= 6 x 2048).
+constexpr u64 RADIX_HIST_WORDS = 12288;
+constexpr int RADIX_MAX_PASSES = 16;   // 64-bit keys with 4-bit digits
 struct RadixScratch {
-    u64 *hist;        // [8][256] global digit histograms
-    u64 *lookback;    // [lb_tiles][256] look-back words (epoch-tagged)
-    u32 *tile_ctr;    // [8] tile tickets, one per pass
-    u64 lb_tiles;     // capacity of lookback in tiles
+    u64 *hist;        // [2 * RADIX_HIST_WORDS] global digit histograms, then offsets
+    u64 *lookback;    // [tiles][bins] look-back words (epoch-tagged)
+    u32 *tile_ctr;    // [RADIX_MAX_PASSES] tile tickets, one per pass
+    u64 lb_words;     // capacity of lookback in words
     u32 epoch;        // advanced once per pass; lookback words of older passes are ignored
-    u64 *hist_host;   // pinned host copy of hist
+    u64 *hist_host;   // pinned host copy of the counts (RADIX_HIST_WORDS)
+    int bits;         // digit width: 8 (default), 4 or 11
     // Optional hook called right before (phase 0) and after (phase 1) each pass
     // launch; the engine uses it to bracket passes with timing events.
     void (*hook)(void *user, int phase);
@@ -108,7 +113,9 @@ struct WsScratch {
     u64 tiles_cap;    // tiles of 4096 elements
     u64 *total;       // [1]
     u64 *base;        // [1] reserved output base
-    u32 *bcur;        // [64] ISA bucket cursors (initial grouping)
+    u32 *bcnt;        // [bcnt_cap] ISA bucket counts per tile, then their scan (initial grouping)
+    u64 bcnt_cap;
+    ScanScratch scan; // for bcnt
 };
 
 // The packed code stream of a byte text (KeyGen), kept for the later stages.
@@ -176,16 +183,34 @@ struct ExactLcp {
     }
 };
 // Rounds.  wide = 0: key2 is u32 (n < 2^31); wide = 1: key2 is u64 (n up to 2^32).
-int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
-int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
-               u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac,
-               const ExactLcp *ex = nullptr);
+// The round's list length a and offset h are read from rp[0], rp[1] in device memory
+// (the launch geometry is sized on the host for a list of a_grid entries, which
+// must be at least rp[0]), so one captured launch serves every later round.
+extern int g_tile_range;   // list entries per workgroup of the tile kernel (the "tile_range" option)
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a_grid, const u64 *rp, const u32 *ISA, u64 n, void *key2,
+                 int wide, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
+// The tile kernel's survivors go to 8 per-XCD segments of seg_i / seg_g (capacity
+// 8 * round_seg_cap(a_grid) each) and are concatenated into out_i / out_g, whose
+// length goes to *out_counter; seg_ctr: 128 u64.
+u64 round_seg_cap(u64 a);
+int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a_grid, const u64 *rp, u32 *SA,
+               u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr);
+// ISA[PV[e]] = PG[e] for e < m, in the pairs' order.
+int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_t s);
+// The same for pairs grouped into windows of the target by a radix pass on bits
+// [s1, s1 + 8) of PV over input bucketed by PV >> s2 (window of v: ((v >> s1) & 255)
+// * nc + (v >> s2), nc buckets, 256 * nc <= 2048 windows): each XCD writes only its
+// own windows.  bounds: 256 * nc + 1 u64; ctr: 256 u32.
+int launch_scatter_windows(const u32 *PV, const u32 *PG, u64 m, int s1, u32 nc, int s2, u32 *ISA, u64 *bounds,
+                           u32 *ctr, hipStream_t s);
+// Big groups: comp = (dense group index << kb) | key2, u32 when comp32 (index and
+// key2 fit 32 bits together), else u64.
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
-                u64 *comp, u32 *vals, hipStream_t s);
-int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
+                int kb, int comp32, void *comp, u32 *vals, hipStream_t s);
+int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
+             u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
              const ExactLcp *ex = nullptr);
 
 // ------------------------------------------------------------ SA kernels

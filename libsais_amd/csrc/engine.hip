@@ -285,6 +285,8 @@ constexpr int F_KERNEL0 = 64;   // + KernelId: one kernel's launches
 
 // Host copies of one round's big-group list (sa_build), kept with the context so
 // that their capacity is reused.
+constexpr int ROUND_SIG = 26;   // words of a round's launch signature (Ctx::round_sig)
+
 struct BigHost {
     std::vector<u32> head, slot, size, order, shead, sslot;
     std::vector<u64> off;
@@ -314,6 +316,12 @@ struct Ctx {
     Timer timer;
     sagpu_stats stats{};
     bool sort_u32 = false;  // the radix passes being timed are u32-key passes
+    // Executable graphs of a doubling round, one per buffer parity (rounds 2, 4, ...
+    // and 3, 5, ...), kept across builds: reused as they are while every argument and
+    // launch size of the round is the same (round_sig), else captured again and
+    // updated in place.
+    hipGraphExec_t round_exec[2] = {nullptr, nullptr};
+    u64 round_sig[2][ROUND_SIG] = {};
 };
 
 // The engine context of the calling thread's current call (for ktime_*).
@@ -324,6 +332,7 @@ bool g_timing = false;
 int g_radix_variant = RADIX_ONESWEEP;
 int g_radix_items = 32;
 int g_radix_threads = 512;
+int g_radix_bits = 8;
 // Texts longer than this use 64-bit rank keys and PLCP scan values.
 u64 g_wide_from = 0x7FFFFFFFull;
 // 0: seeded from the initial sort where the symbol code fits beside the suffix
@@ -348,6 +357,13 @@ int g_bucketed_phi = 1;
 // 1: the fused PLCP pass reads one pair word per position (k_plcp_pairs); 0: marks,
 // SA and text gathered separately
 int g_plcp_pairs = 1;
+// 1: doubling rounds from the second on replay a captured graph (round parameters
+// in device memory); 0: kernels launched one by one.  Timing mode launches them one
+// by one (its events bracket single kernels).
+int g_round_graphs = 1;
+// 1: texts over 2^22 write the initial ranks through a radix pass into windows and
+// a scatter in which each XCD writes only its own windows; 0: one direct scatter
+int g_windowed_scatter = 1;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -356,6 +372,8 @@ void apply_options(Ctx &c)
     c.rs.variant = g_radix_variant;
     c.rs.items = g_radix_items;
     c.rs.threads = g_radix_threads;
+    // the allocation-free build's look-back scratch is sized for 8-bit digits
+    c.rs.bits = c.pool.arena ? 8 : g_radix_bits;
 }
 
 void ctx_destroy(Ctx *c);
@@ -376,7 +394,7 @@ int ctx_create(Ctx **out, int device = 0)
         rc = SAGPU_EDEVICE;
     } else {
         g_device_allocs++;
-        if (hipHostMalloc((void **)&c->pinned, 64 * 1024, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void **)&c->pinned, 256 * 1024, hipHostMallocDefault) != hipSuccess) {
             sagpu_set_error("pinned host allocation failed", __FILE__, __LINE__);
             c->pinned = nullptr;
             rc = SAGPU_ENOMEM;
@@ -387,7 +405,7 @@ int ctx_create(Ctx **out, int device = 0)
         ctx_destroy(c);
         return rc;
     }
-    c->rs.hist_host = c->pinned + 1024;   // 8*256 u64 = 16 KiB
+    c->rs.hist_host = c->pinned + 16384;   // RADIX_HIST_WORDS u64 = 96 KiB
     apply_options(*c);
     *out = c;
     return SAGPU_OK;
@@ -412,6 +430,8 @@ void ctx_destroy(Ctx *c)
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    for (auto &e : c->round_exec)
+        if (e) (void)hipGraphExecDestroy(e);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->xs) (void)hipHostFree(c->xs);
     for (auto &e : c->xs_ev)
@@ -691,7 +711,7 @@ int ensure_radix_scratch(Ctx &c, Scope &sc, u64 n)
 {
     u64 tiles = (n + RS_TILE - 1) / RS_TILE + 1;
     if (!c.rs.hist) {
-        c.rs.hist = (u64 *)c.pool.get(16 * 256 * sizeof(u64));
+        c.rs.hist = (u64 *)c.pool.get(2 * RADIX_HIST_WORDS * sizeof(u64));
         c.rs.tile_ctr = (u32 *)c.pool.get(64 * sizeof(u32));
         if (!c.rs.hist || !c.rs.tile_ctr) return SAGPU_ENOMEM;
     }
@@ -704,12 +724,13 @@ int ensure_radix_scratch(Ctx &c, Scope &sc, u64 n)
         if (!c.rs.counts || !c.rs.scan.partials) { c.rs.counts_cap = 0; return SAGPU_ENOMEM; }
         c.rs.counts_cap = tiles * 256;
     }
-    if (c.rs.lb_tiles < tiles) {
+    const u64 words = tiles * ((u64)1 << (c.rs.bits > 8 ? c.rs.bits : 8));   // look-back words per tile: one per digit
+    if (c.rs.lb_words < words) {
         if (c.rs.lookback) c.pool.put(c.rs.lookback);
-        c.rs.lookback = (u64 *)c.pool.get(tiles * 256 * sizeof(u64));
-        if (!c.rs.lookback) { c.rs.lb_tiles = 0; return SAGPU_ENOMEM; }
-        c.rs.lb_tiles = tiles;
-        SAGPU_TRY(launch_fill_u64(c.rs.lookback, tiles * 256, 0, c.stream));
+        c.rs.lookback = (u64 *)c.pool.get(words * sizeof(u64));
+        if (!c.rs.lookback) { c.rs.lb_words = 0; return SAGPU_ENOMEM; }
+        c.rs.lb_words = words;
+        SAGPU_TRY(launch_fill_u64(c.rs.lookback, words, 0, c.stream));
         c.rs.epoch = 0;
     }
     (void)sc;
@@ -788,6 +809,32 @@ int sort_u32(Ctx &c, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, int begin_bit, i
     if (gen) c.stats.sorted_bytes += (int64_t)n * 8 + (int64_t)(n * (u64)c.rs.gen->bits / 8);
     c.stats.sorted_bytes_u32 += (int64_t)plain * (int64_t)n * 16;
     return rc;
+}
+
+// ISA from the (suffix, rank) pairs of the initial grouping, bucketed by suffix >> cw
+// (bucket-major).  Texts over 2^22 with at most 8 buckets: one radix pass on bits
+// [cw - 8, cw) groups the pairs into windows of 2^(cw - 8) suffixes, and each XCD
+// writes only its own windows (launch_scatter_windows); otherwise one pass over the
+// pairs in bucket order.  K1 / V1: free buffers of n words for the radix pass.
+int isa_from_pairs(Ctx &c, Scope &sc, const u32 *PV, const u32 *PG, u64 n, u32 *ISA, u32 *K1, u32 *V1)
+{
+    const int cw = isa_bucket_log2(n);
+    const u64 nc = ((n - 1) >> cw) + 1;
+    if (g_windowed_scatter && n > (1ull << 22) && nc <= 8 && cw >= 16) {
+        int alt = 0;
+        SAGPU_TRY(sort_u32(c, (u32 *)PV, (u32 *)PG, K1, V1, n, cw - 8, cw, &alt));
+        GET(bounds, u64, 256 * nc + 1);
+        GET(ctr, u32, 256);
+        ktime_begin(KI_ISA_SCATTER);
+        SAGPU_TRY(launch_scatter_windows(alt ? K1 : PV, alt ? V1 : PG, n, cw - 8, (u32)nc, cw, ISA, bounds, ctr,
+                                         c.stream));
+        ktime_end(KI_ISA_SCATTER, 12 * n);   // pair read, rank written
+        return SAGPU_OK;
+    }
+    ktime_begin(KI_ISA_SCATTER);
+    SAGPU_TRY(launch_isa_scatter(PV, PG, n, ISA, c.stream));
+    ktime_end(KI_ISA_SCATTER, 12 * n);
+    return SAGPU_OK;
 }
 
 int read_u32(Ctx &c, const u32 *d, u32 *out)
@@ -907,10 +954,15 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         ws.tile_agg = tagg;
         ws.total = d_cnt + 2;
         ws.base = d_cnt + 3;
-        GET(bcur, u32, 64);
-        ws.bcur = bcur;
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
+    {
+        // bucket counts of the initial grouping: at most 64 buckets per 4096-entry tile
+        ws.bcnt_cap = 64 * ws.tiles_cap;
+        GET(bcnt, u32, ws.bcnt_cap);
+        ws.bcnt = bcnt;
+        ws.scan = c.rs.scan;
+    }
 
     int alt = 0;
     RawKey1 rk1{};   // first round from the code stream (when there is one)
@@ -947,6 +999,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
                              alt ? k0 : k1, alt ? v0 : v1, s, cb ? seed : nullptr, hint));
+        // the sorted keys and values are dead now: they take the pairs' radix pass
+        SAGPU_TRY(isa_from_pairs(c, sc, alt ? k0 : k1, alt ? v0 : v1, n, dISA, alt ? k1 : k0, alt ? v1 : v0));
         if (use_gen && g_raw_round1) {
             // group heads of the last 2 ksym suffixes: those of the last ksym (shorter
             // than ksym) keep ranks in round 1; none of them gets exact round-1 lcps
@@ -981,6 +1035,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         Span sp(c, F_INITIAL);
         SAGPU_TRY(group_init64(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, dSA, dISA, act_i[0], act_g[0], dLCP,
                                (u32 *)(alt ? k0 : k1), (u32 *)(alt ? k0 : k1) + n, s));
+        SAGPU_TRY(isa_from_pairs(c, sc, (u32 *)(alt ? k0 : k1), (u32 *)(alt ? k0 : k1) + n, n, dISA,
+                                 (u32 *)(alt ? k1 : k0), (u32 *)(alt ? k1 : k0) + n));
     }
     u64 a = 0;
     SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));
@@ -996,29 +1052,105 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     std::vector<u32> &bh = c.bigh.head, &bg = c.bigh.slot, &bs = c.bigh.size, &order = c.bigh.order,
                      &hhead = c.bigh.shead, &hslot = c.bigh.sslot;
     std::vector<u64> &hoff = c.bigh.off;
+    // The round's list length and offset reach the kernels through device memory
+    // (d_rp), copied from pinned memory as the round's first operation, so rounds
+    // from the second on replay one captured graph per buffer parity: parameter copy,
+    // gather, counter reset, tile kernel, big-group sizes, counter copy back.  The
+    // graph's launch geometry is sized for the round it was captured in; later
+    // rounds of the same parity have shorter lists.
+    GET(d_rp, u64, 2);
+    u64 *h_rp = c.pinned + 7168;   // past the histogram, LUT and tail-rank staging
+    // per-XCD survivor segments of the tile kernel, sized for the first (longest) list
+    const u64 seg_words = 8 * round_seg_cap(a);
+    GET(seg_i, u32, seg_words);
+    GET(seg_g, u32, seg_words);
+    GET(seg_ctr, u64, 128);
+    // (not in the allocation-free build: instantiating a graph allocates)
+    const bool graphs = g_round_graphs && !c.timer.on && !c.pool.arena;
+    bool captured[2] = {false, false};
     while (a > 0) {
         if (st.rounds < 64) st.active[st.rounds] = (int64_t)a;
         st.rounds++;
         const int nxt = cur ^ 1;
-        {
-            Span sp(c, F_ROUNDS);
-            const bool raw = st.rounds == 1 && rk1.pw;
-            ktime_begin(KI_ROUND_GATHER);
-            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a, dISA, n, h, key2, wide, dSA, s, raw ? &rk1 : nullptr,
+        h_rp[0] = a;
+        h_rp[1] = h;
+        const bool raw = st.rounds == 1 && rk1.pw;
+        // one round's work on the stream (launch geometry for a_grid entries)
+        auto enqueue = [&](u64 a_grid, bool timed) -> int {
+            SAGPU_CHECK(hipMemcpyAsync(d_rp, h_rp, 2 * sizeof(u64), hipMemcpyHostToDevice, s));
+            if (timed) ktime_begin(KI_ROUND_GATHER);
+            SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a_grid, d_rp, dISA, n, key2, wide, s, raw ? &rk1 : nullptr,
                                    ac.cb));
-            // suffix read, one random 4-byte read (rank, or code-stream window), key written
-            // (+ the group read for the first round's form)
-            ktime_end(KI_ROUND_GATHER, a * ((wide ? 16 : 12) + (raw ? 4 : 0)));
+            // suffix read, one random 4-byte read (rank, or code-stream window), key
+            // written (+ the group read for the first round's form)
+            if (timed) ktime_end(KI_ROUND_GATHER, a * ((wide ? 16 : 12) + (raw ? 4 : 0)));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
-            ktime_begin(KI_ROUND_TILE);
-            SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a, h, dSA, dISA, act_i[nxt], act_g[nxt],
+            if (timed) ktime_begin(KI_ROUND_TILE);
+            SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a_grid, d_rp, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
-                                 raw ? &ex1 : nullptr));
+                                 raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr));
             // list entry and key read (12 B); about 16 B written per entry: ranks,
             // placed suffixes, marks and survivors
-            ktime_end(KI_ROUND_TILE, a * (wide ? 32 : 28));
+            if (timed) ktime_end(KI_ROUND_TILE, a * (wide ? 32 : 28));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
+            return SAGPU_OK;
+        };
+        {
+            Span sp(c, F_ROUNDS);
+            if (graphs && !raw && st.rounds >= 2) {
+                hipGraphExec_t &exec = c.round_exec[cur];
+                // every argument and launch size the round's operations take
+                const u64 sig[ROUND_SIG] = {(u64)act_i[cur], (u64)act_g[cur], (u64)act_i[nxt], (u64)act_g[nxt],
+                                            (u64)key2, (u64)dISA, (u64)dSA, (u64)dLCP, (u64)d_cnt, (u64)big_head,
+                                            (u64)big_g, (u64)big_size, (u64)ac.codes, (u64)d_rp, (u64)c.pinned,
+                                            (u64)seg_i, (u64)seg_g, (u64)seg_ctr, (u64)ac.cb, a, n, (u64)wide,
+                                            (u64)g_tile_range, big_cap, (u64)(dLCP != nullptr), (u64)exec};
+                if (!captured[cur] && exec && !memcmp(sig, c.round_sig[cur], sizeof sig)) captured[cur] = true;
+                if (!captured[cur]) {
+                    // capture this parity's round once per build; the executable graph of
+                    // the previous build is updated when its topology still matches
+                    SAGPU_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const int rc = enqueue(a, false);
+                    hipGraph_t graph = nullptr;
+                    const hipError_t ec = hipStreamEndCapture(s, &graph);
+                    if (rc != SAGPU_OK || ec != hipSuccess) {
+                        if (graph) (void)hipGraphDestroy(graph);
+                        (void)hipGetLastError();
+                        if (rc != SAGPU_OK) return rc;
+                        sagpu_set_error("round graph capture failed", __FILE__, __LINE__);
+                        return SAGPU_EDEVICE;
+                    }
+                    bool ok = false;
+                    if (exec) {
+                        hipGraphNode_t bad = nullptr;
+                        hipGraphExecUpdateResult ur;
+                        ok = hipGraphExecUpdate(exec, graph, &bad, &ur) == hipSuccess &&
+                             ur == hipGraphExecUpdateSuccess;
+                        if (!ok) {
+                            (void)hipGetLastError();
+                            (void)hipGraphExecDestroy(exec);
+                            exec = nullptr;
+                        }
+                    }
+                    if (!ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                        (void)hipGetLastError();
+                        exec = nullptr;
+                        (void)hipGraphDestroy(graph);
+                        sagpu_set_error("round graph instantiation failed", __FILE__, __LINE__);
+                        return SAGPU_EDEVICE;
+                    }
+                    (void)hipGraphDestroy(graph);
+                    captured[cur] = true;
+                    st.round_graphs++;
+                    memcpy(c.round_sig[cur], sig, sizeof sig);
+                    c.round_sig[cur][ROUND_SIG - 1] = (u64)exec;   // the executable graph now serving them
+                }
+                SAGPU_CHECK(hipGraphLaunch(exec, s));
+                st.round_graph_launches++;
+            } else {
+                SAGPU_TRY(enqueue(a, true));
+            }
         }
-        SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
         u64 out_n = c.pinned[0];
         const u32 nb = ((u32 *)(c.pinned + 4))[0];
@@ -1061,18 +1193,27 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             SAGPU_CHECK(hipMemcpyAsync(d_bhead, hhead.data(), nb * 4, hipMemcpyHostToDevice, s));
             SAGPU_CHECK(hipMemcpyAsync(d_bslot, hslot.data(), nb * 4, hipMemcpyHostToDevice, s));
             Scope rsc(c);  // this round's big-group sort buffers
+            // key2 < h + n in rank form; the first round's raw keys have rk1.kbits bits
+            int kb = bits_for(h + n - 1);
+            if (st.rounds == 1 && rk1.pw && rk1.kbits > kb) kb = rk1.kbits;
+            const int cbits = kb + bits_for(nb - 1);
+            // index and key2 in 32 bits (the few big groups of degenerate texts): u32
+            // sort keys, 16 B per element and pass instead of 24
+            const bool comp32 = cbits <= 32;
             u64 *b0 = rsc.get<u64>(total), *b1 = rsc.get<u64>(total);
             u32 *bv0 = rsc.get<u32>(total), *bv1 = rsc.get<u32>(total);
             if (!b0 || !b1 || !bv0 || !bv1) return SAGPU_ENOMEM;
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, b0, bv0, s));
+                SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32, b0, bv0, s));
             }
-            SAGPU_TRY(sort_u64(c, b0, bv0, b1, bv1, total, 0, (wide ? 33 : 32) + bits_for(nb - 1), &alt));
+            if (comp32) SAGPU_TRY(sort_u32(c, (u32 *)b0, bv0, (u32 *)b1, bv1, total, 0, cbits, &alt));
+            else SAGPU_TRY(sort_u64(c, b0, bv0, b1, bv1, total, 0, cbits, &alt));
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_emit(ws, alt ? b1 : b0, alt ? bv1 : bv0, d_bslot, total, h, wide, dSA, dISA, act_i[nxt],
-                                   act_g[nxt], out_counter, dLCP, s, ac, st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
+                SAGPU_TRY(big_emit(ws, alt ? b1 : b0, comp32, kb, alt ? bv1 : bv0, d_bslot, total, h, dSA, dISA,
+                                   act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac,
+                                   st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
@@ -2017,6 +2158,21 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "key_stream")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_key_stream = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "radix_bits")) {
+        if (value != 4 && value != 8 && value != 11) return SAGPU_EINVAL;
+        g_radix_bits = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "round_graphs")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_round_graphs = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "windowed_scatter")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_windowed_scatter = (int)value;
         return 0;
     }
     if (!strcmp(name, "plcp_pairs")) {

@@ -1,10 +1,11 @@
-// radix.hip -- onesweep LSD radix sort for gfx950 (8-bit digits, wave64).
+// radix.hip -- onesweep LSD radix sort for gfx950 (8-bit digits by default; 4 and 11
+// bits selectable, the "radix_bits" option), wave64.
 //
-// One upfront kernel builds the 256-bin histograms of every digit position at once
+// One upfront kernel builds the digit histograms of every digit position at once
 // (LDS-privatised, one read of the keys).  Each pass is then a single kernel:
 //   * a workgroup takes a tile ticket (atomic counter), so every tile it waits on
 //     belongs to a workgroup that is already running;
-//   * keys are ranked inside the wave with 8 ballots per item (peer mask of equal
+//   * keys are ranked inside the wave with one ballot per digit bit (peer mask of equal
 //     digits, popcount below the lane) and per-wave LDS digit counters -- stable by
 //     construction, no atomics on the ranking path;
 //   * the per-digit tile count is published to a look-back word, and the tile's
@@ -43,16 +44,17 @@ __device__ __forceinline__ u32 gen_val(const KeyGen &g, u64 p)
     return ((u32)p << g.cb) | prev;
 }
 
-template <typename K, bool GEN>
+template <typename K, bool GEN, int DB = 8>
 __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, KeyGen gen, u64 n, int begin_bit,
                                                     int end_bit, int npass, u64 *__restrict__ hist)
 {
-    // 4 copies per digit position, picked by lane % 4, so skewed digits (DNA)
-    // contend less on the same LDS words
-    constexpr int C = 4;
-    constexpr int NP = sizeof(K) == 4 ? 4 : 8;
-    __shared__ u32 s_hc[NP * C][256];
-    for (int i = threadIdx.x; i < NP * C * 256; i += blockDim.x) (&s_hc[0][0])[i] = 0;
+    // 4 copies per digit position (one with 11-bit digits), picked by lane % 4, so
+    // skewed digits (DNA) contend less on the same LDS words
+    constexpr int BINS = 1 << DB;
+    constexpr int C = DB > 8 ? 1 : 4;
+    constexpr int NP = (8 * (int)sizeof(K) + DB - 1) / DB;
+    __shared__ u32 s_hc[NP * C][BINS];
+    for (int i = threadIdx.x; i < NP * C * BINS; i += blockDim.x) (&s_hc[0][0])[i] = 0;
     __syncthreads();
     const u32 copy = threadIdx.x & (C - 1);
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -61,8 +63,8 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
         if constexpr (GEN) k = (K)gen_key(gen, i);
         else k = keys[i];
         for (int p = 0; p < npass; p++) {
-            const int sh = begin_bit + 8 * p;
-            const u32 dm = end_bit - sh >= 8 ? 255u : ((1u << (end_bit - sh)) - 1u);
+            const int sh = begin_bit + DB * p;
+            const u32 dm = end_bit - sh >= DB ? (u32)(BINS - 1) : ((1u << (end_bit - sh)) - 1u);
             u32 d = (u32)(k >> sh) & dm;
             if constexpr (sizeof(K) == 8) {
                 // the big-group sorts of periodic and degenerate texts see long runs
@@ -78,26 +80,41 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < npass * 256; i += blockDim.x) {
-        const int p = i / 256, d = i % 256;
+    for (int i = threadIdx.x; i < npass * BINS; i += blockDim.x) {
+        const int p = i / BINS, d = i % BINS;
         u32 v = 0;
         for (int c = 0; c < C; c++) v += s_hc[p * C + c][d];
         if (v) atomicAdd(&hist[i], (u64)v);
     }
 }
 
-// Exclusive scan of each pass's 256-bin histogram into bucket offsets (in place).
-__global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
+// Exclusive scan of each pass's histogram (bins per pass) into bucket offsets at
+// hist + RADIX_HIST_WORDS.  Thread t takes bins [t * per, (t + 1) * per).
+__global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass, int bins)
 {
     __shared__ u64 s_w[RS_WAVES];
+    const int per = bins > 256 ? bins / 256 : 1;   // at most 8 (2048 bins)
     for (int p = 0; p < npass; p++) {
-        u64 v = hist[p * 256 + threadIdx.x];
+        const int d0 = (int)threadIdx.x * per;
+        u64 loc[8];
+        u64 v = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            loc[q] = (q < per && d0 + q < bins) ? hist[p * bins + d0 + q] : 0ull;
+            v += loc[q];
+        }
         u64 inc = wave_incl_sum64(v);
         if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
         __syncthreads();
         u64 carry = 0;
         for (u32 w = 0; w < threadIdx.x / SAGPU_WAVE; w++) carry += s_w[w];
-        hist[8 * 256 + p * 256 + threadIdx.x] = carry + inc - v;
+        u64 run = carry + inc - v;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < per && d0 + q < bins) {
+                hist[RADIX_HIST_WORDS + p * bins + d0 + q] = run;
+                run += loc[q];
+            }
         __syncthreads();
     }
 }
@@ -106,7 +123,7 @@ __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass)
 // ticket, aggregate publish, decoupled look-back).  LOOKBACK = false: the tile's
 // global digit offsets were produced by an upsweep + scan (reduce-then-scan).
 // GEN: the input keys and values are generated (KeyGen) instead of read.
-template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS, bool GEN = false, int NT = RS_THREADS>
+template <typename K, bool HAS_VALS, bool LOOKBACK, int ITEMS, bool GEN = false, int NT = RS_THREADS, int DB = 8>
 __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, const u32 *__restrict__ vin,
                                                            K *__restrict__ kout, u32 *__restrict__ vout, u64 n,
                                                            int shift, u32 dmask, const u64 *__restrict__ offsets,
@@ -114,23 +131,27 @@ __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, co
                                                            u32 epoch, const u32 *__restrict__ scanned, u64 tiles,
                                                            KeyGen gen)
 {
-    // NT threads (256 or 512) per workgroup; the 256 digits are handled by the
-    // first 256 threads.
+    // NT threads (256 or 512) per workgroup; BINS digits, DPT per thread in the digit
+    // phase (thread t takes digits [t * DPT, (t + 1) * DPT)).
     constexpr int NW = NT / SAGPU_WAVE;
     constexpr int TILE_ = NT * ITEMS;
     constexpr int WAVE_ITEMS_ = SAGPU_WAVE * ITEMS;
+    constexpr int BINS = 1 << DB;
+    constexpr int DPT = BINS > NT ? BINS / NT : 1;
+    // digits kept per item for the value phase: 8 bits each, 16 with 11-bit digits
+    constexpr int DPB = DB > 8 ? 16 : 8, DPW = 32 / DPB;
     __shared__ u32 s_tile;
-    __shared__ u32 s_wcnt[NW][256];
-    __shared__ u32 s_dstart[256];
-    __shared__ u64 s_gbase[256];
-    __shared__ u32 s_wsum[256 / SAGPU_WAVE];
+    __shared__ u32 s_wcnt[NW][BINS];
+    __shared__ u32 s_dstart[BINS];
+    __shared__ u64 s_gbase[BINS];
+    __shared__ u32 s_wsum[NW];
     __shared__ K s_keys[TILE_];
 
     const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE, lane = lane_id();
     if (LOOKBACK) {
         if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
     }
-    for (u32 i = tid; i < (u32)NW * 256; i += NT) (&s_wcnt[0][0])[i] = 0;
+    for (u32 i = tid; i < (u32)NW * BINS; i += NT) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
 
     const u64 tile = LOOKBACK ? (u64)s_tile : (u64)blockIdx.x;
@@ -197,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, co
         u32 d = (u32)(key[j] >> shift) & dmask;
         u64 m = ~0ull;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < DB; b++) {
             bool bit = (d >> b) & 1u;
             u64 bb = __ballot(bit);
             m &= bit ? bb : ~bb;
@@ -215,62 +236,87 @@ __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, co
     }
     __syncthreads();
 
-    // Per digit (threads 0..255): wave offsets, tile count, global offset.
-    u32 run = 0, inc = 0;
-    if (tid < 256) {
-        const u32 d = tid;
+    // Per digit: wave offsets, tile count, global offset.
+    u32 run[DPT];
+    u32 tsum = 0;
+    const bool dthread = tid * DPT < (u32)BINS;
 #pragma unroll
-        for (int q = 0; q < NW; q++) {
-            u32 c = s_wcnt[q][d];
-            s_wcnt[q][d] = run;
-            run += c;
-        }
-        inc = wave_incl_sum(run);
-        if (lane == SAGPU_WAVE - 1) s_wsum[w] = inc;
-
-        u64 gbase;
-        if (LOOKBACK) {
-            u64 *my = lookback + tile * 256 + d;
-            if (tile == 0) st_release_u64(my, lb_pack(epoch, LB_PRE, run));
-            else st_release_u64(my, lb_pack(epoch, LB_AGG, run));
-            u64 excl = 0;
-            if (tile > 0) {
-                // Windowed look-back: the loads for up to LB_WIN predecessors are
-                // issued together so their latencies overlap; entries are consumed
-                // from the nearest one until an inclusive prefix is found.
-                long long t = (long long)tile - 1;
-                bool found = false;
-                while (!found) {
-                    const int cnt = t + 1 < LB_WIN ? (int)(t + 1) : LB_WIN;
-                    u64 v[LB_WIN];
+    for (int qd = 0; qd < DPT; qd++) {
+        run[qd] = 0;
+        if (dthread) {
+            const u32 d = tid * DPT + qd;
+            u32 r = 0;
 #pragma unroll
-                    for (int q = 0; q < LB_WIN; q++)
-                        if (q < cnt) v[q] = ld_acquire_u64(lookback + (u64)(t - q) * 256 + d);
-#pragma unroll
-                    for (int q = 0; q < LB_WIN; q++) {
-                        if (q >= cnt || found) continue;
-                        while ((u32)(v[q] >> 42) != epoch || ((v[q] >> LB_VAL_BITS) & 3ull) == 0) {
-                            __builtin_amdgcn_s_sleep(1);
-                            v[q] = ld_acquire_u64(lookback + (u64)(t - q) * 256 + d);
-                        }
-                        excl += v[q] & LB_VAL_MASK;
-                        if (((v[q] >> LB_VAL_BITS) & 3ull) == LB_PRE) found = true;
-                    }
-                    t -= cnt;
-                }
-                st_release_u64(my, lb_pack(epoch, LB_PRE, excl + run));
+            for (int q = 0; q < NW; q++) {
+                const u32 c = s_wcnt[q][d];
+                s_wcnt[q][d] = r;
+                r += c;
             }
-            gbase = offsets[d] + excl;
-        } else {
-            gbase = scanned[(u64)d * tiles + tile];
+            run[qd] = r;
+            tsum += r;
         }
-        s_gbase[d] = gbase;
+    }
+    const u32 inc = wave_incl_sum(tsum);
+    if (lane == SAGPU_WAVE - 1) s_wsum[w] = inc;
+    if (dthread) {
+        // every digit's aggregate is published before any look-back starts, so the
+        // tiles after this one never wait on a digit this thread has not reached
+        if (LOOKBACK) {
+#pragma unroll
+            for (int qd = 0; qd < DPT; qd++)
+                st_release_u64(lookback + tile * BINS + tid * DPT + qd,
+                               lb_pack(epoch, tile == 0 ? LB_PRE : LB_AGG, run[qd]));
+        }
+#pragma unroll
+        for (int qd = 0; qd < DPT; qd++) {
+            const u32 d = tid * DPT + qd;
+            u64 gbase;
+            if (LOOKBACK) {
+                u64 *my = lookback + tile * BINS + d;
+                u64 excl = 0;
+                if (tile > 0) {
+                    // Windowed look-back: the loads for up to LB_WIN predecessors are
+                    // issued together so their latencies overlap; entries are consumed
+                    // from the nearest one until an inclusive prefix is found.
+                    long long t = (long long)tile - 1;
+                    bool found = false;
+                    while (!found) {
+                        const int cnt = t + 1 < LB_WIN ? (int)(t + 1) : LB_WIN;
+                        u64 v[LB_WIN];
+#pragma unroll
+                        for (int q = 0; q < LB_WIN; q++)
+                            if (q < cnt) v[q] = ld_acquire_u64(lookback + (u64)(t - q) * BINS + d);
+#pragma unroll
+                        for (int q = 0; q < LB_WIN; q++) {
+                            if (q >= cnt || found) continue;
+                            while ((u32)(v[q] >> 42) != epoch || ((v[q] >> LB_VAL_BITS) & 3ull) == 0) {
+                                __builtin_amdgcn_s_sleep(1);
+                                v[q] = ld_acquire_u64(lookback + (u64)(t - q) * BINS + d);
+                            }
+                            excl += v[q] & LB_VAL_MASK;
+                            if (((v[q] >> LB_VAL_BITS) & 3ull) == LB_PRE) found = true;
+                        }
+                        t -= cnt;
+                    }
+                    st_release_u64(my, lb_pack(epoch, LB_PRE, excl + run[qd]));
+                }
+                gbase = offsets[d] + excl;
+            } else {
+                gbase = scanned[(u64)d * tiles + tile];
+            }
+            s_gbase[d] = gbase;
+        }
     }
     __syncthreads();
-    if (tid < 256) {
+    if (dthread) {
         u32 carry = 0;
         for (u32 q = 0; q < w; q++) carry += s_wsum[q];
-        s_dstart[tid] = carry + inc - run;
+        u32 st = carry + inc - tsum;
+#pragma unroll
+        for (int qd = 0; qd < DPT; qd++) {
+            s_dstart[tid * DPT + qd] = st;
+            st += run[qd];
+        }
     }
     __syncthreads();
 
@@ -287,16 +333,16 @@ __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, co
     const u32 tile_n = (u32)((n - tbase) < (u64)TILE_ ? (n - tbase) : (u64)TILE_);
     // Digits of the staged keys are recomputed and kept packed in registers for the
     // value phase, so no LDS digit array is needed (one more workgroup per CU fits).
-    u32 dpack[ITEMS / 4];
+    u32 dpack[(ITEMS + DPW - 1) / DPW];
 #pragma unroll
-    for (int q = 0; q < ITEMS / 4; q++) dpack[q] = 0;
+    for (int q = 0; q < (ITEMS + DPW - 1) / DPW; q++) dpack[q] = 0;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const u32 idx = tid + (u32)j * NT;
         if (idx < tile_n) {
             const K kk = s_keys[idx];
             const u32 d = (u32)(kk >> shift) & dmask;
-            dpack[j / 4] |= d << (8 * (j % 4));
+            dpack[j / DPW] |= d << (DPB * (j % DPW));
             kout[s_gbase[d] + (idx - s_dstart[d])] = kk;
         }
         // bound the LDS loads the scheduler hoists (each item holds four registers)
@@ -329,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_radix_pass(const K *__restrict__ kin, co
         for (int j = 0; j < ITEMS; j++) {
             const u32 idx = tid + (u32)j * NT;
             if (idx < tile_n) {
-                const u32 d = (dpack[j / 4] >> (8 * (j % 4))) & 255u;
+                const u32 d = (dpack[j / DPW] >> (DPB * (j % DPW))) & ((1u << DPB) - 1u);
                 vout[s_gbase[d] + (idx - s_dstart[d])] = s_vals[idx];
             }
             if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);
@@ -416,25 +462,26 @@ __global__ __launch_bounds__(256) void k_pack_stream(const uint8_t *__restrict__
     }
 }
 
-template <typename K>
-int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
-                    hipStream_t s, int *result_in_alt, int *passes_run)
+template <typename K, int DB>
+int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
+                  hipStream_t s, int *result_in_alt, int *passes_run)
 {
-    *result_in_alt = 0;
-    if (passes_run) *passes_run = 0;
-    if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
-    const int npass = (end_bit - begin_bit + 7) / 8;
+    constexpr int BINS = 1 << DB;
+    const int npass = (end_bit - begin_bit + DB - 1) / DB;
+    // 4- and 11-bit digits: onesweep only, 256-thread tiles
+    const int variant = DB == 8 ? rs.variant : RADIX_ONESWEEP;
     const int items = rs.items == 32 ? 32 : rs.items == 8 ? 8 : 16;
     const u64 tile_keys = (u64)RS_THREADS * items;
     const u64 tiles = (n + tile_keys - 1) / tile_keys;
-    if (tiles > rs.lb_tiles || npass > 8 || (rs.variant == RADIX_REDUCE_THEN_SCAN && tiles * 256 > rs.counts_cap)) {
+    if (tiles * BINS > rs.lb_words || npass > RADIX_MAX_PASSES || (u64)npass * BINS > RADIX_HIST_WORDS ||
+        (variant == RADIX_REDUCE_THEN_SCAN && tiles * 256 > rs.counts_cap)) {
         sagpu_set_error("radix scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
 
     const KeyGen *gen = sizeof(K) == 4 ? rs.gen : nullptr;
     const KeyGen g0 = gen ? *gen : KeyGen{};
-    if (gen && rs.variant == RADIX_REDUCE_THEN_SCAN) {
+    if (gen && variant == RADIX_REDUCE_THEN_SCAN) {
         // the upsweep reads its keys: write the generated input out first
         if constexpr (sizeof(K) == 4) {
             hipLaunchKernelGGL(k_gen_write, dim3((unsigned)((n + 2047) / 2048 < 65535 ? (n + 2047) / 2048 : 65535)),
@@ -443,74 +490,77 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
         }
         gen = nullptr;
     }
-    SAGPU_TRY(launch_fill_u64(rs.hist, 16 * 256, 0, s));
+    SAGPU_TRY(launch_fill_u64(rs.hist, (u64)npass * BINS, 0, s));
     {
         int blocks = (int)((n + 256 * 64 - 1) / (256 * 64));
         if (blocks > 4096) blocks = 4096;
         if (gen)
-            hipLaunchKernelGGL((k_radix_hist<K, true>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit, end_bit,
-                               npass, rs.hist);
+            hipLaunchKernelGGL((k_radix_hist<K, true, DB>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit,
+                               end_bit, npass, rs.hist);
         else
-            hipLaunchKernelGGL((k_radix_hist<K, false>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit, end_bit,
-                               npass, rs.hist);
-        hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass);
+            hipLaunchKernelGGL((k_radix_hist<K, false, DB>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit,
+                               end_bit, npass, rs.hist);
+        hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass, BINS);
         SAGPU_CHECK(hipGetLastError());
     }
     // Histograms decide which passes can be skipped (all keys share the digit).
-    SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, 8 * 256 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, (u64)npass * BINS * sizeof(u64), hipMemcpyDeviceToHost, s));
     SAGPU_CHECK(hipStreamSynchronize(s));
 
-    SAGPU_TRY(launch_fill_u32(rs.tile_ctr, 8, 0, s));
+    SAGPU_TRY(launch_fill_u32(rs.tile_ctr, RADIX_MAX_PASSES, 0, s));
     K *ki = k0, *ko = k1;
     u32 *vi = v0, *vo = v1;
     int alt = 0, ran = 0;
     for (int p = 0; p < npass; p++) {
         bool trivial = false;
-        for (int d = 0; d < 256; d++)
-            if (rs.hist_host[p * 256 + d] == n) trivial = true;
+        for (int d = 0; d < BINS; d++)
+            if (rs.hist_host[p * BINS + d] == n) trivial = true;
         if (trivial) continue;
         rs.epoch = (rs.epoch + 1) & ((1u << 22) - 1);
-        if (rs.epoch == 0 && rs.variant != RADIX_REDUCE_THEN_SCAN) {
-            SAGPU_TRY(launch_fill_u64(rs.lookback, rs.lb_tiles * 256, 0, s));
+        if (rs.epoch == 0 && variant != RADIX_REDUCE_THEN_SCAN) {
+            SAGPU_TRY(launch_fill_u64(rs.lookback, rs.lb_words, 0, s));
             rs.epoch = 1;
         }
-        const int shift = begin_bit + 8 * p;
-        const u32 dmask = end_bit - shift >= 8 ? 255u : ((1u << (end_bit - shift)) - 1u);
-        if (rs.variant == RADIX_REDUCE_THEN_SCAN) {
-            if (items == 8)
-                hipLaunchKernelGGL((k_rts_upsweep<K, 8>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
-                                   shift, dmask, rs.counts, tiles);
-            else if (items == 32)
-                hipLaunchKernelGGL((k_rts_upsweep<K, 32>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
-                                   shift, dmask, rs.counts, tiles);
-            else
-                hipLaunchKernelGGL((k_rts_upsweep<K, 16>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
-                                   shift, dmask, rs.counts, tiles);
-            SAGPU_TRY(scan_exclusive_u32(rs.scan, rs.counts, tiles * 256, s));
+        const int shift = begin_bit + DB * p;
+        const u32 dmask = end_bit - shift >= DB ? (u32)(BINS - 1) : ((1u << (end_bit - shift)) - 1u);
+        if constexpr (DB == 8) {
+            if (variant == RADIX_REDUCE_THEN_SCAN) {
+                if (items == 8)
+                    hipLaunchKernelGGL((k_rts_upsweep<K, 8>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                       shift, dmask, rs.counts, tiles);
+                else if (items == 32)
+                    hipLaunchKernelGGL((k_rts_upsweep<K, 32>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                       shift, dmask, rs.counts, tiles);
+                else
+                    hipLaunchKernelGGL((k_rts_upsweep<K, 16>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, n,
+                                       shift, dmask, rs.counts, tiles);
+                SAGPU_TRY(scan_exclusive_u32(rs.scan, rs.counts, tiles * 256, s));
+            }
         }
         const bool gen_now = gen && ran == 0;
         rs.gen_pass = gen_now;
         if (rs.hook) rs.hook(rs.hook_user, 0);
-        const bool lb = rs.variant != RADIX_REDUCE_THEN_SCAN;
-        const u64 *offs = lb ? rs.hist + 8 * 256 + p * 256 : nullptr;
+        const bool lb = variant != RADIX_REDUCE_THEN_SCAN;
+        const u64 *offs = lb ? rs.hist + RADIX_HIST_WORDS + (u64)p * BINS : nullptr;
         u64 *lbk = lb ? rs.lookback : nullptr;
         u32 *tctr = lb ? rs.tile_ctr + p : nullptr;
         const u32 *cnts = lb ? nullptr : rs.counts;
         // onesweep passes may use 512-thread workgroups (16384-key tiles with 32
-        // items: twice as long digit runs per tile, same waves per CU)
-        const int nt = (sizeof(K) == 4 && lb && rs.threads == 512 && items == 32) ? 512 : RS_THREADS;
+        // items: twice as long digit runs per tile, same waves per CU); 8-bit digits
+        const int nt = (DB == 8 && sizeof(K) == 4 && lb && rs.threads == 512 && items == 32) ? 512 : RS_THREADS;
         const u64 ptiles = (n + (u64)nt * items - 1) / ((u64)nt * items);
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
-    hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, \
-                       n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
+    hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT, false, RS_THREADS, DB>), dim3((unsigned)tiles), dim3(RS_THREADS), \
+                       0, s, ki, vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, tiles, g0)
 #define RS_LAUNCH512(HV, GN)                                                                                      \
     hipLaunchKernelGGL((k_radix_pass<K, HV, true, 32, GN, 512>), dim3((unsigned)ptiles), dim3(512), 0, s, ki, vi, ko, \
                        vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, ptiles, g0)
 #define RS_LAUNCH_GEN(IT)                                                                                          \
-    hipLaunchKernelGGL((k_radix_pass<K, true, true, IT, true>), dim3((unsigned)gtiles), dim3(RS_THREADS), 0, s, ki,  \
-                       vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, gtiles, g0)
+    hipLaunchKernelGGL((k_radix_pass<K, true, true, IT, true, RS_THREADS, DB>), dim3((unsigned)gtiles),              \
+                       dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, shift, dmask, offs, lbk, tctr, rs.epoch, cnts, gtiles, \
+                       g0)
         if (nt == 512) {
-            if constexpr (sizeof(K) == 4) {   // 16384 u32 keys: 64 KB of LDS staging
+            if constexpr (sizeof(K) == 4 && DB == 8) {   // 16384 u32 keys: 64 KB of LDS staging
                 if (gen_now) RS_LAUNCH512(true, true);
                 else if (v0) RS_LAUNCH512(true, false);
                 else RS_LAUNCH512(false, false);
@@ -521,12 +571,14 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
                 const u64 gtiles = (n + (u64)RS_THREADS * git - 1) / ((u64)RS_THREADS * git);
                 if (git == 8) RS_LAUNCH_GEN(8); else if (git == 16) RS_LAUNCH_GEN(16); else RS_LAUNCH_GEN(32);
             }
-        } else if (v0) {
-            if (lb) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
-            else { if (items == 32) RS_LAUNCH(true, false, 32); else if (items == 8) RS_LAUNCH(true, false, 8); else RS_LAUNCH(true, false, 16); }
+        } else if (lb) {
+            if (v0) { if (items == 32) RS_LAUNCH(true, true, 32); else if (items == 8) RS_LAUNCH(true, true, 8); else RS_LAUNCH(true, true, 16); }
+            else { if (items == 32) RS_LAUNCH(false, true, 32); else if (items == 8) RS_LAUNCH(false, true, 8); else RS_LAUNCH(false, true, 16); }
         } else {
-            if (lb) { if (items == 32) RS_LAUNCH(false, true, 32); else if (items == 8) RS_LAUNCH(false, true, 8); else RS_LAUNCH(false, true, 16); }
-            else { if (items == 32) RS_LAUNCH(false, false, 32); else if (items == 8) RS_LAUNCH(false, false, 8); else RS_LAUNCH(false, false, 16); }
+            if constexpr (DB == 8) {
+                if (v0) { if (items == 32) RS_LAUNCH(true, false, 32); else if (items == 8) RS_LAUNCH(true, false, 8); else RS_LAUNCH(true, false, 16); }
+                else { if (items == 32) RS_LAUNCH(false, false, 32); else if (items == 8) RS_LAUNCH(false, false, 8); else RS_LAUNCH(false, false, 16); }
+            }
         }
 #undef RS_LAUNCH
 #undef RS_LAUNCH512
@@ -549,6 +601,19 @@ int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int
     *result_in_alt = alt;
     if (passes_run) *passes_run = ran;
     return SAGPU_OK;
+}
+
+template <typename K>
+int radix_sort_impl(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int begin_bit, int end_bit,
+                    hipStream_t s, int *result_in_alt, int *passes_run)
+{
+    *result_in_alt = 0;
+    if (passes_run) *passes_run = 0;
+    if (n <= 1 || end_bit <= begin_bit) return SAGPU_OK;
+    if (rs.bits == 4) return radix_sort_db<K, 4>(rs, k0, v0, k1, v1, n, begin_bit, end_bit, s, result_in_alt, passes_run);
+    if (rs.bits == 11)
+        return radix_sort_db<K, 11>(rs, k0, v0, k1, v1, n, begin_bit, end_bit, s, result_in_alt, passes_run);
+    return radix_sort_db<K, 8>(rs, k0, v0, k1, v1, n, begin_bit, end_bit, s, result_in_alt, passes_run);
 }
 
 } // namespace

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
 }
 
 // ------------------------------------------------------- initial grouping
+constexpr int IB_MAX = 64;   // ISA buckets
 // Keys are the packed first `ksym` symbols in the low 32 bits (low ksym*bits
 // bits), sorted.  With K = u64 the high half holds the string id of batched GSA,
 // so groups never span two strings.
@@ -181,8 +182,10 @@ struct InitF {
     u32 *SA, *act_i, *act_g, *LCP;
     // (suffix, group head) pairs for the ISA scatter, bucketed by suffix >> wlog:
     // bucket b fills PV/PG[b << wlog, ...) (it holds exactly the suffixes of that
-    // range); bcur[b] counts the entries taken so far
-    u32 *PV, *PG, *bcur;
+    // range).  bcnt[b * tiles + t] = suffixes of bucket b in tile t (k_init_reduce),
+    // then its exclusive scan: every bucket before b is full, so the scan gives
+    // tile t's first slot in bucket b directly, with no atomics.
+    u32 *PV, *PG, *bcnt;
     int wlog;
     // 1: the head of a slot alone in its group carries its exact lcp mark as a PLCP
     // hint in bits 28-31 (mark - hint_off + 1, for marks in [hint_off, hint_off + 14];
@@ -297,19 +300,31 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__r
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
     const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    __shared__ u32 s_cnt[IB_MAX];
+    if (threadIdx.x < IB_MAX) s_cnt[threadIdx.x] = 0;
     SpanKeys<K> sk;
     sk.load(f.keys, f.n, wbase);
+    u32 vv[WS_ROWS];
+#pragma unroll
+    for (int r = 0; r < WS_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        vv[r] = e < f.n ? f.vals[e] : 0u;
+    }
+    __syncthreads();
     u64 s = 0;
     u32 m1 = 0;
 #pragma unroll
     for (int r = 0; r < WS_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         K kp, kn;
         sk.neighbours(r, kp, kn);
         bool act;
         u32 v1;
-        f.flags(wbase + (u64)r * SAGPU_WAVE + lane, sk.k[r], kp, kn, act, v1);
+        f.flags(e, sk.k[r], kp, kn, act, v1);
         s += act;
         m1 = umax(m1, v1);
+        // u64 shifts: wlog may be 32 (one bucket)
+        if (e < f.n) atomicAdd(&s_cnt[(u32)((u64)(vv[r] >> f.cb) >> f.wlog)], 1u);
     }
     for (int d = 32; d >= 1; d >>= 1) {
         s += __shfl_xor(s, d, SAGPU_WAVE);
@@ -325,12 +340,13 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__r
         }
         tagg[blockIdx.x] = t;
     }
+    const u32 nb = (u32)(((f.n - 1) >> f.wlog) + 1);
+    if (threadIdx.x < nb) f.bcnt[(u64)threadIdx.x * gridDim.x + blockIdx.x] = s_cnt[threadIdx.x];
 }
 
 // Initial grouping, pass 2 (after k_ws_scan_tiles): singletons into SA, grouped
 // elements into the active list, LCP marks, BWT seeds, and the (suffix, group head)
 // pairs into their ISA buckets.
-constexpr int IB_MAX = 64;   // ISA buckets
 template <typename K>
 __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg *__restrict__ texcl)
 {
@@ -377,8 +393,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
     }
     if (lane == 0) s_w[w] = Agg{s, m1, 0};
     __syncthreads();
-    if (threadIdx.x < nb && s_cnt[threadIdx.x])
-        s_base[threadIdx.x] = ((u64)threadIdx.x << f.wlog) + atomicAdd(&f.bcur[threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x < nb && s_cnt[threadIdx.x]) s_base[threadIdx.x] = f.bcnt[(u64)threadIdx.x * gridDim.x + blockIdx.x];
     Agg c = texcl[blockIdx.x];
     for (u32 q = 0; q < w; q++) {
         c.s += s_w[q].s;
@@ -455,13 +470,16 @@ __global__ __launch_bounds__(256) void k_isa_scatter(const u32 *__restrict__ PV,
 }
 
 // ------------------------------------------------------- big-group emission
+// CT = u32 when the group index and key2 fit 32 bits together (the big groups of
+// degenerate texts: 16 B per element and radix pass instead of 24), else u64.
+template <typename CT>
 struct BigF {
-    const u64 *comp;     // (dense big-group index << kb) | key2, sorted
+    const CT *comp;      // (dense big-group index << kb) | key2, sorted
     const u32 *vals;     // list entry: suffix index, or (suffix << cb) | BWT code
     const u32 *gslot;    // first SA slot of each big group
     u64 n;
     u32 h;
-    int kb;              // key2 width: 32, or 33 for texts longer than 2^31 - 1
+    int kb;              // key2 width
     u32 *SA, *ISA, *out_i, *out_g, *LCP;
     ActCode ac;
     ExactLcp ex;
@@ -473,9 +491,9 @@ struct BigF {
             v1 = v2 = 0;
             return;
         }
-        u64 c = comp[e];
-        u64 cp = e > 0 ? comp[e - 1] : ~0ull;
-        u64 cn = e + 1 < n ? comp[e + 1] : ~0ull;
+        const u64 c = comp[e];
+        const u64 cp = e > 0 ? (u64)comp[e - 1] : ~0ull;
+        const u64 cn = e + 1 < n ? (u64)comp[e + 1] : ~0ull;
         bool head = (e == 0) || c != cp;
         bool tail = (e + 1 == n) || c != cn;
         bool gch = (e == 0) || (c >> kb) != (cp >> kb);
@@ -488,7 +506,7 @@ struct BigF {
         if (e >= n) return;
         const u32 pv = vals[e];
         const u32 v = pv >> ac.cb;
-        u64 c = comp[e];
+        const u64 c = comp[e];
         u32 g = gslot[(u32)(c >> kb)];
         u32 r = g + (hstart - gstart);
         if (hstart != gstart) ISA[v] = r;  // the first subgroup keeps the group's rank
@@ -504,7 +522,7 @@ struct BigF {
             u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
             if (ex.on && !ex.skipped(g)) {
                 const u64 km = (1ull << kb) - 1;
-                mark = h + ex.shared(c & km, comp[e - 1] & km);
+                mark = h + ex.shared(c & km, (u64)comp[e - 1] & km);
             }
             LCP[r] = mark;
         }
@@ -534,9 +552,12 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
 constexpr int IG = 4;
 template <typename KT>
 __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
-                                                      u64 a, const u32 *__restrict__ ISA, u64 n, u64 h,
+                                                      const u64 *__restrict__ rp, const u32 *__restrict__ ISA, u64 n,
                                                       KT *__restrict__ key2, RawKey1 rk, int cb)
 {
+    // the round's list length and offset come from device memory (RoundParams), so
+    // a captured launch serves every round
+    const u64 a = rp[0], h = rp[1];
     const u64 chunk = 256ull * IG;
     for (u64 base = (u64)blockIdx.x * chunk; base < a; base += (u64)gridDim.x * chunk) {
         u32 i[IG], r[IG];
@@ -593,22 +614,25 @@ constexpr int RT_THREADS = 256;
 // k_big_sizes takes the size after this kernel.
 template <typename KT, int RANGE>
 __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
-                                                           const KT *__restrict__ key2, u64 a, u32 h,
+                                                           const KT *__restrict__ key2, const u64 *__restrict__ rp,
                                                            u32 *__restrict__ SA, u32 *__restrict__ ISA,
-                                                           u32 *__restrict__ out_i, u32 *__restrict__ out_g,
-                                                           u64 *__restrict__ out_counter, u32 *__restrict__ err,
-                                                           u32 *__restrict__ big_head, u32 *__restrict__ big_g,
-                                                           u32 *__restrict__ big_counter, u64 big_cap,
-                                                           u32 *__restrict__ LCP, ActCode ac, ExactLcp ex)
+                                                           u32 *__restrict__ seg_i, u32 *__restrict__ seg_g,
+                                                           u64 *__restrict__ seg_ctr, u64 seg_cap,
+                                                           u32 *__restrict__ err, u32 *__restrict__ big_head,
+                                                           u32 *__restrict__ big_g, u32 *__restrict__ big_counter,
+                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac, ExactLcp ex)
 {
     constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
     constexpr int WIN = RANGE + RT_SMALL + 1;                  // act_g window [lo, lo + WIN)
     constexpr int W_PER = (WIN + RT_THREADS - 1) / RT_THREADS; // window entries per thread
     static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
     constexpr u32 NONE = 0xFFFFFFFFu;
-    __shared__ u32 s_w[WIN];      // act_g over the window
-    __shared__ u32 s_hx[WIN];     // local index of the position's group head (NONE: head before lo)
-    __shared__ u32 s_gend[RANGE]; // last local index of the group headed at local x (NONE: past the window)
+    constexpr u16 NONE16 = 0xFFFFu;   // local indexes are below 2048: 16 bits each
+    __shared__ u32 s_w[WIN];        // act_g over the window
+    __shared__ u16 s_hx[WIN];       // local index of the position's group head (NONE16: head before lo)
+    __shared__ u16 s_gend[RANGE];   // last local index of the group headed at local x (NONE16: past the window)
+    // suffixes and keys of the tile's range, by local position (every small group
+    // of this workgroup lies in [lo, lo + RT_CAP)); outputs by local position
     __shared__ u32 s_i[RT_CAP];
     __shared__ KT s_k[RT_CAP];
     __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
@@ -616,23 +640,26 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     __shared__ u32 s_first, s_end, s_wsum[4], s_prev;
     __shared__ u64 s_base;
 
+    const u64 a = rp[0];
+    const u32 h = rp[1] < LCP_BOUND_CAP ? (u32)rp[1] : LCP_BOUND_CAP;   // marks carry h capped
     const u32 tid = threadIdx.x;
     const u64 lo = (u64)blockIdx.x * RANGE;
     if (lo >= a) return;
     const u64 hi = lo + RANGE < a ? lo + RANGE : a;
     const u32 wlen = (u32)((a - lo) < (u64)WIN ? (a - lo) : (u64)WIN);
     const u32 rlen = (u32)(hi - lo);
+    const u32 clen = wlen < (u32)RT_CAP ? wlen : (u32)RT_CAP;
 
-    // 1. the window, the entry before it, and the group head of every position
-    //    (a max-scan over head flags: local index + 1, 0 = head before lo)
+    // 1. the window, the entry before it, and the group head of every position (a
+    //    max-scan over head flags: local index + 1, 0 = head before lo)
     for (u32 x = tid; x < wlen; x += RT_THREADS) s_w[x] = act_g[lo + x];
+    for (u32 x = tid; x < (u32)RT_CAP; x += RT_THREADS) s_os[x] = 0;
     if (tid == 0) {
         s_first = NONE;
         s_end = 0;
         s_prev = lo > 0 ? act_g[lo - 1] : NONE;
     }
-    if (tid < RANGE)
-        for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE;
+    for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE16;
     __syncthreads();
     {
         u32 loc[W_PER];
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
 #pragma unroll
         for (int j = 0; j < W_PER; j++) {
             const u32 x = tid * W_PER + j;
-            if (x < wlen) s_hx[x] = umax(c, loc[j]) - 1u;   // 0 - 1 = NONE
+            if (x < wlen) s_hx[x] = (u16)(umax(c, loc[j]) - 1u);   // 0 - 1 = NONE16
         }
     }
     __syncthreads();
@@ -665,10 +692,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         const bool is_end = (x + 1 < wlen) ? s_w[x + 1] != s_w[x] : (lo + x + 1 == a);
         if (!is_end) continue;
         const u32 hx = s_hx[x];
-        if (hx != NONE && hx < rlen) s_gend[hx] = x;
+        if (hx != NONE16 && hx < rlen) s_gend[hx] = (u16)x;
         if (x < rlen) {
             bool big;
-            if (hx != NONE) big = x - hx + 1 > (u32)RT_SMALL;
+            if (hx != NONE16) big = x - hx + 1 > (u32)RT_SMALL;
             else {
                 // the group holding position lo has its head before lo
                 const u64 e = lo + x;
@@ -678,59 +705,68 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
     }
     __syncthreads();
-    // 3. heads in [lo, hi): small groups define the tile's element range, big ones
-    //    go to the big list
-    for (u32 x = tid; x < rlen; x += RT_THREADS) {
-        if (s_hx[x] != x) continue;
-        const u32 ge = s_gend[x];
-        if (ge != NONE && ge - x + 1 <= (u32)RT_SMALL) {
-            atomicMin(&s_first, x);
-            atomicMax(&s_end, ge + 1);
-        } else {
-            const u32 slot = atomicAdd(big_counter, 1u);
-            if (slot < big_cap) {
-                big_head[slot] = (u32)(lo + x);
-                big_g[slot] = s_w[x];
+    // 3. heads in [lo, hi): small groups define the tile's element range (a min and
+    //    a max per wave, then one LDS atomic each), big ones go to the big list
+    {
+        u32 f = NONE, e = 0;
+        for (u32 x = tid; x < rlen; x += RT_THREADS) {
+            if (s_hx[x] != x) continue;
+            const u32 ge = s_gend[x];
+            if (ge != NONE16 && ge - x + 1 <= (u32)RT_SMALL) {
+                f = x < f ? x : f;
+                e = ge + 1 > e ? ge + 1 : e;
+            } else {
+                const u32 slot = atomicAdd(big_counter, 1u);
+                if (slot < big_cap) {
+                    big_head[slot] = (u32)(lo + x);
+                    big_g[slot] = s_w[x];
+                }
             }
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+            const u32 of = __shfl_xor(f, d, SAGPU_WAVE), oe = __shfl_xor(e, d, SAGPU_WAVE);
+            f = of < f ? of : f;
+            e = oe > e ? oe : e;
+        }
+        if (lane_id() == 0 && f != NONE) {
+            atomicMin(&s_first, f);
+            atomicMax(&s_end, e);
         }
     }
     __syncthreads();
     if (s_first == NONE) return;
-    const u32 first = s_first, len = s_end - s_first;
-    if (s_end < s_first || len > (u32)RT_CAP || s_end > wlen) {
+    const u32 first = s_first, end = s_end;
+    if (end < first || end > clen) {
         // inconsistent group bookkeeping: report instead of overrunning LDS
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
-    const u64 gfirst = lo + first;
-
-    // 4. load the range's suffixes and keys
-    for (u32 x = tid; x < len; x += RT_THREADS) {
-        const u64 k = gfirst + x;
-        s_i[x] = act_i[k];
-        s_k[x] = key2[k];
-        s_os[x] = 0;
+    // the range's suffixes and keys
+    for (u32 x = first + tid; x < end; x += RT_THREADS) {
+        s_i[x] = act_i[lo + x];
+        s_k[x] = key2[lo + x];
     }
     __syncthreads();
 
-    // 5. rank inside the group by key2 (count smaller, equal, equal-before)
-    for (u32 x = tid; x < len; x += RT_THREADS) {
-        const u32 hx = s_hx[first + x];
+    // 4. rank inside the group by key2 (count smaller, equal, equal-before); local
+    //    positions throughout
+    for (u32 x = first + tid; x < end; x += RT_THREADS) {
+        const u32 hx = s_hx[x];
         const u32 ge = s_gend[hx];
-        if (ge == NONE || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
-        const u32 x0 = hx - first, gsz = ge - hx + 1;
+        if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
+        const u32 gsz = ge - hx + 1;
         const u32 g = s_w[hx];
         const KT key = s_k[x];
         u32 lt = 0, eq = 0, eqb = 0;
         KT pk = 0;   // the largest key below this one: the previous subgroup's
-        for (u32 y = x0; y < x0 + gsz; y++) {
+        for (u32 y = hx; y < hx + gsz; y++) {
             const KT ky = s_k[y];
             lt += ky < key;
             eq += ky == key;
             eqb += (ky == key) & (y < x);
             pk = (ky < key && ky > pk) ? ky : pk;
         }
-        const u32 pos = x0 + lt + eqb;
+        const u32 pos = hx + lt + eqb;
         const u32 r = g + lt;
         s_oi[pos] = s_i[x];
         s_og[pos] = r;
@@ -742,7 +778,29 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
     }
     __syncthreads();
-    for (u32 x = tid; x < len; x += RT_THREADS) {
+
+    // 5. survivors: counted first, and their range reserved in this XCD's segment
+    //    (one atomic on the XCD's counter, in flight while this workgroup writes
+    //    ranks and placed suffixes).  A counter shared by all workgroups serialises
+    //    them: 300 K returning atomics on one address take 3.6 ms, on 8 addresses
+    //    0.57 ms; workgroups go to the XCDs round-robin (blockIdx % 8).
+    u32 cnt[RT_PER];
+    u32 tot = 0;
+#pragma unroll
+    for (int j = 0; j < RT_PER; j++) {
+        const u32 x = first + tid * RT_PER + j;
+        cnt[j] = (x < end && (s_os[x] & 3) == 2) ? 1u : 0u;
+        tot += cnt[j];
+    }
+    const u32 inc = wave_incl_sum(tot);
+    if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        const u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        const u32 x = blockIdx.x & 7u;
+        s_base = (u64)x * seg_cap + (all ? atomicAdd(&seg_ctr[x * 16], (u64)all) : 0ull);
+    }
+    for (u32 x = first + tid; x < end; x += RT_THREADS) {
         const u32 st = s_os[x];
         if (st == 0) continue;
         const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
@@ -752,37 +810,42 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             if (ac.codes) ac.codes[s_og[x]] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
         }
     }
+    __syncthreads();
+    u64 o = s_base + inc - tot;
+    for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
+#pragma unroll
+    for (int j = 0; j < RT_PER; j++) {
+        const u32 x = first + tid * RT_PER + j;
+        if (cnt[j]) {
+            seg_i[o] = s_oi[x];
+            seg_g[o] = s_og[x];
+            o++;
+        }
+    }
+}
 
-    // 6. survivors, in sorted order, to the next active list
-    {
-        u32 cnt[RT_PER];
-        u32 tot = 0;
+// The next active list from the 8 XCD segments of the tile kernel's output
+// (segment x: [x * seg_cap, x * seg_cap + seg_ctr[16 x])), concatenated in segment
+// order; *out_counter = its length (the big-group emission appends behind it).
+__global__ __launch_bounds__(256) void k_seg_gather(const u32 *__restrict__ seg_i, const u32 *__restrict__ seg_g,
+                                                    const u64 *__restrict__ seg_ctr, u64 seg_cap,
+                                                    u32 *__restrict__ out_i, u32 *__restrict__ out_g,
+                                                    u64 *__restrict__ out_counter)
+{
+    u64 pre[9];
+    pre[0] = 0;
 #pragma unroll
-        for (int j = 0; j < RT_PER; j++) {
-            const u32 x = tid * RT_PER + j;
-            cnt[j] = (x < len && (s_os[x] & 3) == 2) ? 1u : 0u;
-            tot += cnt[j];
-        }
-        const u32 inc = wave_incl_sum(tot);
-        __syncthreads();
-        if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
-        __syncthreads();
-        if (tid == 0) {
-            const u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-            s_base = all ? atomicAdd(out_counter, (u64)all) : 0ull;
-        }
-        __syncthreads();
-        u64 o = s_base + inc - tot;
-        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
+    for (int x = 0; x < 8; x++) pre[x + 1] = pre[x] + seg_ctr[x * 16];
+    const u64 total = pre[8];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_counter = total;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+        int x = 0;
 #pragma unroll
-        for (int j = 0; j < RT_PER; j++) {
-            const u32 x = tid * RT_PER + j;
-            if (cnt[j]) {
-                out_i[o] = s_oi[x];
-                out_g[o] = s_og[x];
-                o++;
-            }
-        }
+        for (int q = 1; q < 8; q++) x += k >= pre[q];
+        const u64 src = (u64)x * seg_cap + (k - pre[x]);
+        out_i[k] = seg_i[src];
+        out_g[k] = seg_g[src];
     }
 }
 
@@ -800,10 +863,11 @@ __global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ big_h
 
 // Copies the members of the big groups (sorted by list position) into the sort
 // buffers: comp = (dense group index << 32) | key2, val = suffix index.
-template <typename KT>
+template <typename KT, typename CT>
 __global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
                                                      const u32 *__restrict__ bhead, const u64 *__restrict__ boff,
-                                                     u32 nb, u64 total, u64 *__restrict__ comp, u32 *__restrict__ vals)
+                                                     u32 nb, u64 total, int kb, CT *__restrict__ comp,
+                                                     u32 *__restrict__ vals)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
@@ -814,7 +878,7 @@ __global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act
             else hi = mid - 1;
         }
         u64 k = (u64)bhead[lo] + (e - boff[lo]);
-        comp[e] = ((u64)lo << (sizeof(KT) == 4 ? 32 : 33)) | (u64)key2[k];
+        comp[e] = (CT)(((u64)lo << kb) | (u64)key2[k]);
         vals[e] = act_i[k];
     }
 }
@@ -839,7 +903,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
 {
     const int cb = seed ? seed->cb : 0;
     const int wlog = isa_bucket_log2(n);
-    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcur, wlog,
+    InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcnt, wlog,
                hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, 0u, cb,
                seed ? seed->codes : nullptr};
     for (int i = 0; i < ksym; i++) f.lowbits |= 1u << (bits * i);
@@ -848,18 +912,29 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
+    const u64 nb = ((n - 1) >> wlog) + 1;
+    if (nb * tiles > ws.bcnt_cap) {
+        sagpu_set_error("bucket count scratch too small", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
-    SAGPU_TRY(launch_fill_u32(ws.bcur, IB_MAX, 0, s));
     hipLaunchKernelGGL(k_init_reduce<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
+    SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, nb * tiles, s));
     ktime_begin(KI_INIT_EMIT);
     hipLaunchKernelGGL(k_init_emit<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
     // key and value read; SA, the (suffix, rank) pair, LCP mark and BWT code written
     // per position (the active-list entries are not counted)
     ktime_end(KI_INIT_EMIT, n * (sizeof(K) + 4 + 4 + 8 + (LCP ? 4 : 0) + (f.codes ? 1 : 0)));
-    ktime_begin(KI_ISA_SCATTER);
-    hipLaunchKernelGGL(k_isa_scatter, dim3(grid_for(n, 256 * IW, 65535)), dim3(256), 0, s, PV, PG, n, ISA);
-    ktime_end(KI_ISA_SCATTER, 12 * n);   // pair read, rank written
+    (void)ISA;   // the caller scatters the pairs (launch_isa_scatter or the windowed form)
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_t s)
+{
+    if (m == 0) return SAGPU_OK;
+    hipLaunchKernelGGL(k_isa_scatter, dim3(grid_for(m, 256 * IW, 65535)), dim3(256), 0, s, PV, PG, m, ISA);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -890,77 +965,94 @@ int group_init64(WsScratch &ws, const u64 *keys, const u32 *vals, u64 n, int bit
 }
 
 template <typename KT>
-int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, KT *key2, u32 *SA,
+int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u64 *rp, const u32 *ISA, u64 n, KT *key2,
                    hipStream_t s, const RawKey1 &rk, int cb)
 {
-    (void)SA;
-    hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, a, ISA,
-                       n, h, key2, rk, cb);
+    hipLaunchKernelGGL(k_round_gather<KT>, dim3(grid_for(a, 256 * IG, 65535)), dim3(256), 0, s, act_i, act_g, rp, ISA,
+                       n, key2, rk, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
+// Segment capacity of each XCD for a list of a entries: the workgroups of XCD x are
+// the tiles x, x + 8, ..., each emitting at most RANGE + RT_SMALL survivors.
+u64 round_seg_cap(u64 a) { return (ceil_div_u(a, (u64)g_tile_range) / 8 + 1) * (u64)(g_tile_range + RT_SMALL); }
+
 template <typename KT>
-int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, u64 h, u32 *SA, u32 *ISA, u32 *out_i,
-                 u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size, u32 *big_counter,
-                 u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp &ex)
+int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, const u64 *rp, u32 *SA, u32 *ISA,
+                 u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
+                 u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp &ex,
+                 u32 *seg_i, u32 *seg_g, u64 *seg_ctr)
 {
-    u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
+    const u64 seg_cap = round_seg_cap(a);
+    SAGPU_TRY(launch_fill_u64(seg_ctr, 8 * 16, 0, s));
 #define RT_LAUNCH(R)                                                                                              \
-    hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, a, \
-                       hh, SA, ISA, out_i, out_g, out_counter, err, big_head, big_g, big_counter, big_cap, LCP, ac,  \
-                       ex)
+    hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, rp, \
+                       SA, ISA, seg_i, seg_g, seg_ctr, seg_cap, err, big_head, big_g, big_counter, big_cap, LCP, ac, ex)
     if (g_tile_range == 256) RT_LAUNCH(256);
     else if (g_tile_range == 1024) RT_LAUNCH(1024);
     else RT_LAUNCH(512);
 #undef RT_LAUNCH
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, big_head, big_g,
                        big_counter, big_cap, SA, big_size);
+    hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, seg_i, seg_g,
+                       (const u64 *)seg_ctr, seg_cap, out_i, out_g, out_counter);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-template <typename KT>
-int big_extract_t(const u32 *act_i, const KT *key2, const u32 *bhead, const u64 *boff, u32 nb, u64 total, u64 *comp,
-                  u32 *vals, hipStream_t s)
+template <typename KT, typename CT>
+int big_extract_t(const u32 *act_i, const KT *key2, const u32 *bhead, const u64 *boff, u32 nb, u64 total, int kb,
+                  CT *comp, u32 *vals, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_big_extract<KT>, dim3(grid_for(total, 256 * 4, 65535)), dim3(256), 0, s, act_i, key2, bhead,
-                       boff, nb, total, comp, vals);
+    hipLaunchKernelGGL((k_big_extract<KT, CT>), dim3(grid_for(total, 256 * 4, 65535)), dim3(256), 0, s, act_i, key2,
+                       bhead, boff, nb, total, kb, comp, vals);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u32 *ISA, u64 n, u64 h, void *key2, int wide,
-                 u32 *SA, hipStream_t s, const RawKey1 *rk, int cb)
+int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u64 *rp, const u32 *ISA, u64 n, void *key2,
+                 int wide, hipStream_t s, const RawKey1 *rk, int cb)
 {
     const RawKey1 r = rk ? *rk : RawKey1{};
-    return wide ? round_gather_t<u64>(act_i, act_g, a, ISA, n, h, (u64 *)key2, SA, s, r, cb)
-                : round_gather_t<u32>(act_i, act_g, a, ISA, n, h, (u32 *)key2, SA, s, r, cb);
+    return wide ? round_gather_t<u64>(act_i, act_g, a, rp, ISA, n, (u64 *)key2, s, r, cb)
+                : round_gather_t<u32>(act_i, act_g, a, rp, ISA, n, (u32 *)key2, s, r, cb);
 }
 
-int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, u64 h, u32 *SA, u32 *ISA,
-               u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
-               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex)
+int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, const u64 *rp, u32 *SA,
+               u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
+               u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr)
 {
     const ExactLcp e = ex ? *ex : ExactLcp{};
-    return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e)
-                : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, h, SA, ISA, out_i, out_g, out_counter, err,
-                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e);
+    return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
+                                    seg_ctr)
+                : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
+                                    big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
+                                    seg_ctr);
 }
 
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
-                u64 *comp, u32 *vals, hipStream_t s)
+                int kb, int comp32, void *comp, u32 *vals, hipStream_t s)
 {
-    return wide ? big_extract_t<u64>(act_i, (const u64 *)key2, bhead, boff, nb, total, comp, vals, s)
-                : big_extract_t<u32>(act_i, (const u32 *)key2, bhead, boff, nb, total, comp, vals, s);
+    if (comp32)
+        return wide ? big_extract_t<u64, u32>(act_i, (const u64 *)key2, bhead, boff, nb, total, kb, (u32 *)comp, vals, s)
+                    : big_extract_t<u32, u32>(act_i, (const u32 *)key2, bhead, boff, nb, total, kb, (u32 *)comp, vals, s);
+    return wide ? big_extract_t<u64, u64>(act_i, (const u64 *)key2, bhead, boff, nb, total, kb, (u64 *)comp, vals, s)
+                : big_extract_t<u32, u64>(act_i, (const u32 *)key2, bhead, boff, nb, total, kb, (u64 *)comp, vals, s);
 }
 
-int big_emit(WsScratch &ws, const u64 *comp, const u32 *vals, const u32 *gslot, u64 total, u64 h, int wide, u32 *SA,
-             u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
+int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
+             u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
              const ExactLcp *ex)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    BigF f{comp, vals, gslot, total, hh, wide ? 33 : 32, SA, ISA, out_i, out_g, LCP, ac, ex ? *ex : ExactLcp{}};
+    const ExactLcp e = ex ? *ex : ExactLcp{};
+    if (comp32) {
+        BigF<u32> f{(const u32 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e};
+        return ws_run(ws, f, total, out_counter, s);
+    }
+    BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e};
     return ws_run(ws, f, total, out_counter, s);
 }

@@ -164,6 +164,9 @@ def replay(reqs, server: Server | None):
 def summarize(reqs, t0: float):
     lat = np.array([r.job.t_done - (t0 + r.arrival) for r in reqs])
     total = sum(len(r.T) for r in reqs)
+    # bytes over the host link: the text in; SA (4 B per position) out, plus the
+    # BWT (1 B) or the LCP (4 B) by kind
+    link = sum(len(r.T) * (1 + 4 + (1 if r.kind == JOB_BWT else 4 if r.kind == JOB_LCP else 0)) for r in reqs)
     end = max(r.job.t_done for r in reqs)
     span = end - t0
     return {
@@ -171,6 +174,8 @@ def summarize(reqs, t0: float):
         "bytes": int(total),
         "makespan_s": round(span, 4),
         "aggregate_MBps": round(total / span / 1e6, 2),
+        "link_bytes": int(link),
+        "link_GBps": round(link / span / 1e9, 2),
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
         "max_ms": round(float(lat.max()) * 1e3, 3),

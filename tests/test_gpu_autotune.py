@@ -13,10 +13,17 @@ pytestmark = pytest.mark.gpu
 def test_tune_text_grid_identical_outputs():
     T = workloads.dna_like(1 << 20, seed=21)
     res = autotune.tune_text(T, reps=1)
-    assert len(res["results"]) == 36 and all(r["identical"] for r in res["results"])
+    assert len(res["results"]) == 27 and all(r["identical"] for r in res["results"])
+    assert {r["config"]["radix_bits"] for r in res["results"]} == {4, 8, 11}
     assert res["best"] in [{**autotune.DEFAULT, **r["config"]} for r in res["results"]]
-    # one non-default point against the oracle
+    # non-default points against the oracle
     autotune.apply({"radix_items": 8, "radix_variant": 1, "tile_range": 256})
+    try:
+        rc, SA_ref = ol.sa(T)
+        assert np.array_equal(la.suffix_array(T), SA_ref)
+    finally:
+        autotune.apply(autotune.DEFAULT)
+    autotune.apply({"radix_bits": 11, "radix_items": 16, "tile_range": 1024})
     try:
         rc, SA_ref = ol.sa(T)
         assert np.array_equal(la.suffix_array(T), SA_ref)
@@ -44,5 +51,6 @@ def test_tune_batched_small():
 
 def test_knob_validation():
     lib = la.load()
-    for name, bad in ((b"radix_items", 12), (b"tile_range", 300), (b"radix_variant", 2), (b"radix_threads", 384)):
+    for name, bad in ((b"radix_items", 12), (b"tile_range", 300), (b"radix_variant", 2), (b"radix_threads", 384),
+                      (b"radix_bits", 6), (b"round_graphs", 2), (b"windowed_scatter", -1)):
         assert lib.sagpu_set_option(name, bad) == -1

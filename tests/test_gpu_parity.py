@@ -342,6 +342,107 @@ def test_isa_window_option(wlog):
         assert lib.sagpu_set_option(b"isa_window_log2", 25) == 0
 
 
+@pytest.mark.parametrize("name", ["dna_like_5M", "periodic_6M", "gsa_5M"])
+def test_windowed_scatter_option(name):
+    """Texts over 2^22 write ISA through windowed scatters (initial ranks and the
+    rounds' rank updates); the fused build must equal the oracle with the option
+    on and off."""
+    if name == "dna_like_5M":
+        T = workloads.dna_like(5 << 20, seed=19)
+    elif name == "periodic_6M":
+        T = workloads.periodic(6 << 20, 37, 5)
+    else:
+        rng = np.random.default_rng(20)
+        parts = []
+        while sum(len(p) for p in parts) < (5 << 20):
+            parts.append(rng.integers(1, 5, int(rng.integers(1, 20000))).astype(np.uint8))
+            parts.append(np.zeros(1, np.uint8))
+        T = np.concatenate(parts)
+    gsa = name.startswith("gsa")
+    rc, SA_ref = ol.sa(T, gsa=gsa) if gsa else ol.sa(T)
+    assert rc == 0
+    P_ref = ol.plcp(T, SA_ref, gsa=gsa) if gsa else ol.plcp(T, SA_ref)
+    L_ref = ol.lcp(P_ref, SA_ref)
+    lib = la.load()
+    try:
+        for on in (1, 0):
+            print(f"[windowed] {name} n={len(T)} windowed_scatter={on}", flush=True)
+            assert lib.sagpu_set_option(b"windowed_scatter", on) == 0
+            SA, U, r, P, L, f = fused(T, gsa=gsa)
+            assert np.array_equal(SA, SA_ref), (name, on)
+            assert np.array_equal(P, P_ref) and np.array_equal(L, L_ref), (name, on)
+            if not gsa:
+                U_ref, idx_ref = ol.bwt(T, SA_ref)
+                assert r == idx_ref and np.array_equal(U, U_ref), (name, on)
+    finally:
+        assert lib.sagpu_set_option(b"windowed_scatter", 1) == 0
+
+
+@pytest.mark.parametrize("bits", [4, 11])
+def test_radix_bits_option(bits):
+    """4- and 11-bit radix digits (the initial sort, its generating first pass, the
+    big-group sorts and the windowed ISA scatter's partition) give the same build."""
+    lib = la.load()
+    rng = np.random.default_rng(bits)
+    cases = [("dna_like_5M", workloads.dna_like(5 << 20, seed=27), False),
+             ("periodic_300k", workloads.periodic(300000, 19, 4), False),
+             ("bytes_k200", rng.integers(0, 200, 100000).astype(np.uint8), False)]
+    parts = []
+    for _ in range(400):
+        parts.append(rng.integers(1, 5, int(rng.integers(1, 3000))).astype(np.uint8))
+        parts.append(np.zeros(1, np.uint8))
+    cases.append(("gsa_set", np.concatenate(parts), True))
+    try:
+        assert lib.sagpu_set_option(b"radix_bits", bits) == 0
+        for name, T, gsa in cases:
+            print(f"[radix_bits] {bits}: {name} n={len(T)}", flush=True)
+            rc, SA_r = ol.sa(T, gsa=gsa) if gsa else ol.sa(T)
+            assert rc == 0
+            P_r = ol.plcp(T, SA_r, gsa=gsa) if gsa else ol.plcp(T, SA_r)
+            SA, U, r, P, L, f = fused(T, gsa=gsa)
+            assert np.array_equal(SA, SA_r), name
+            assert np.array_equal(P, P_r) and np.array_equal(L, ol.lcp(P_r, SA_r)), name
+            if not gsa:
+                U_r, idx_r = ol.bwt(T, SA_r)
+                assert r == idx_r and np.array_equal(U, U_r), name
+    finally:
+        assert lib.sagpu_set_option(b"radix_bits", 8) == 0
+
+
+def test_round_graphs_replay_across_builds():
+    """Rounds from the second on replay captured graphs, updated in place from one
+    build to the next (other pointers and list lengths) and between buffer parities.
+    Consecutive builds of different texts, with graphs on and off, all equal the
+    oracle; the big-group path runs between replays on the repeat-rich texts."""
+    lib = la.load()
+    texts = [("dna_like_1M", workloads.dna_like(1 << 20, seed=23)),
+             ("acgt_300k", workloads.iid(300001, seed=24, alphabet=b"ACGT")),
+             ("copies", np.tile(workloads.iid(7000, seed=25, alphabet=b"ACGT"), 40)),
+             ("periodic_200k", workloads.periodic(200000, 23, 7)),
+             ("dna_like_1M_again", workloads.dna_like(1 << 20, seed=23))]
+    ref = {}
+    for name, T in texts:
+        rc, SA = ol.sa(T)
+        assert rc == 0
+        P = ol.plcp(T, SA)
+        ref[name] = (SA, P, ol.lcp(P, SA), ol.bwt(T, SA))
+    try:
+        for on in (1, 0, 1):
+            assert lib.sagpu_set_option(b"round_graphs", on) == 0
+            for name, T in texts:
+                SA, U, r, P, L, f = fused(T)
+                SA_r, P_r, L_r, (U_r, idx_r) = ref[name]
+                st = la.stats()
+                print(f"[graphs] {name} round_graphs={on}: rounds {st.rounds}, graphs captured {st.round_graphs}, "
+                      f"replays {st.round_graph_launches}", flush=True)
+                assert st.round_graph_launches == (max(0, st.rounds - 1) if on else 0), (name, st.rounds)
+                assert np.array_equal(SA, SA_r), (name, on)
+                assert r == idx_r and np.array_equal(U, U_r), (name, on)
+                assert np.array_equal(P, P_r) and np.array_equal(L, L_r), (name, on)
+    finally:
+        assert lib.sagpu_set_option(b"round_graphs", 1) == 0
+
+
 def _round1_cases():
     rng = np.random.default_rng(4242)
 

@@ -9,15 +9,18 @@ import libsais_amd as la
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[(0, 256), (0, 512), (1, 256)], ids=["onesweep", "onesweep_512", "reduce_then_scan"])
+@pytest.fixture(params=[(0, 256, 8), (0, 512, 8), (1, 256, 8), (0, 256, 4), (0, 256, 11)],
+                ids=["onesweep", "onesweep_512", "reduce_then_scan", "digits4", "digits11"])
 def variant(request):
     lib = la.load()
-    v, threads = request.param
+    v, threads, bits = request.param
     assert lib.sagpu_set_option(b"radix_variant", v) == 0
     assert lib.sagpu_set_option(b"radix_threads", threads) == 0
+    assert lib.sagpu_set_option(b"radix_bits", bits) == 0
     yield request.param
     lib.sagpu_set_option(b"radix_variant", 0)
     lib.sagpu_set_option(b"radix_threads", 512)
+    lib.sagpu_set_option(b"radix_bits", 8)
 
 
 def _dev(arr):
