@@ -1085,13 +1085,11 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             // written (+ the group read for the first round's form)
             if (timed) ktime_end(KI_ROUND_GATHER, a * ((wide ? 16 : 12) + (raw ? 4 : 0)));
             SAGPU_TRY(launch_fill_u64(d_cnt, 8, 0, s));
-            if (timed) ktime_begin(KI_ROUND_TILE);
+            // (the tile kernel is timed inside round_tile; the timer is off while a
+            // round is captured)
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a_grid, d_rp, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
                                  raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr));
-            // list entry and key read (12 B); about 16 B written per entry: ranks,
-            // placed suffixes, marks and survivors
-            if (timed) ktime_end(KI_ROUND_TILE, a * (wide ? 32 : 28));
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             return SAGPU_OK;
         };

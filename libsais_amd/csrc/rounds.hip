@@ -989,10 +989,14 @@ int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, cons
 #define RT_LAUNCH(R)                                                                                              \
     hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, rp, \
                        SA, ISA, seg_i, seg_g, seg_ctr, seg_cap, err, big_head, big_g, big_counter, big_cap, LCP, ac, ex)
+    // the kernel alone is timed: list entry and key read (8 + sizeof(KT) B), about
+    // 16 B written per entry (ranks, placed suffixes, marks and survivors)
+    ktime_begin(KI_ROUND_TILE);
     if (g_tile_range == 256) RT_LAUNCH(256);
     else if (g_tile_range == 1024) RT_LAUNCH(1024);
     else RT_LAUNCH(512);
 #undef RT_LAUNCH
+    ktime_end(KI_ROUND_TILE, a * (24 + sizeof(KT)));
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, big_head, big_g,
                        big_counter, big_cap, SA, big_size);
     hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, seg_i, seg_g,
