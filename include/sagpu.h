@@ -192,6 +192,9 @@ typedef struct sagpu_stats {
     int32_t round_graphs;         /* doubling-round graphs the build captured (0 when the
                                      previous build's graphs served unchanged)        */
     int32_t round_graph_launches; /* doubling rounds the build ran as graph replays   */
+    int32_t big_pivot_rounds;     /* rounds whose big groups took frequent keys out of
+                                     the sort (option big_pivot_min)                 */
+    int64_t big_pivot_elems;      /* big-group members those rounds placed unsorted  */
 } sagpu_stats;
 
 /*
@@ -251,6 +254,10 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *   "windowed_scatter" 1 = texts over 2^22 write the initial ranks through a radix pass into
  *                    windows of the ISA and a scatter in which each XCD writes only its own windows
  *                    (default); 0 = one direct scatter
+ *   "big_pivot_min"  rounds after the first whose big groups hold at least this many members in
+ *                    all take up to 4 keys frequent in a 64-member sample of each group out of the
+ *                    sort: those members become subgroups by a stable partition (default 2^20;
+ *                    0 = never)
  *   "api_slots"      contexts that calls made without a libsais context share, 1-16
  *                    (default 8): that many such calls run at once, each on its own stream
  * Returns 0, or -1 for an unknown name or value.

@@ -86,6 +86,8 @@ class SagpuStats(ctypes.Structure):
         ("kernel_bytes", ctypes.c_int64 * 16),
         ("round_graphs", ctypes.c_int32),
         ("round_graph_launches", ctypes.c_int32),
+        ("big_pivot_rounds", ctypes.c_int32),
+        ("big_pivot_elems", ctypes.c_int64),
     ]
 
 

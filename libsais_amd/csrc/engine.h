@@ -209,9 +209,33 @@ int launch_scatter_windows(const u32 *PV, const u32 *PG, u64 m, int s1, u32 nc, 
 // key2 fit 32 bits together), else u64.
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
                 int kb, int comp32, void *comp, u32 *vals, hipStream_t s);
+// Few-key partition of the big groups (pivots.hip): up to BP key2 values frequent
+// in a sample of each group become subgroups without sorting; the rest is sorted.
+enum { BP = 4 };
+struct PivRegions {
+    u64 start[BP + 1];   // pivot class b: [start[b], start[b + 1]); start[BP] = pivot members
+};
+struct BigPivots {
+    const u64 *piv = nullptr;    // [nb * BP] each group's pivot keys, ascending
+    const u32 *npiv = nullptr;   // [nb]
+    const u32 *eq = nullptr;     // [nb * BP] members equal to each pivot
+};
 int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
              u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
-             const ExactLcp *ex = nullptr);
+             const ExactLcp *ex = nullptr, const BigPivots *pv = nullptr);
+u64 big_part_tiles(u64 total);
+int big_pivots(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 *piv, u32 *npiv,
+               hipStream_t s);
+int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total, const u64 *piv,
+                   const u32 *npiv, u32 *ccnt, hipStream_t s);
+int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb,
+                     u64 total, int kb, int comp32, const u64 *piv, const u32 *npiv, const u32 *cscan, u64 npivot,
+                     void *comp, u32 *vals, u32 *pq, u32 *ppv, hipStream_t s);
+int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
+                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 nb, u32 *eq, u32 *rk, u64 *out_counter,
+                  u64 *pbase, u32 *err, hipStream_t s);
+int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *gslot,
+                 const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);

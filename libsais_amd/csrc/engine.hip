@@ -364,6 +364,9 @@ int g_round_graphs = 1;
 // 1: texts over 2^22 write the initial ranks through a radix pass into windows and
 // a scatter in which each XCD writes only its own windows; 0: one direct scatter
 int g_windowed_scatter = 1;
+// Big groups of rounds after the first take up to BP frequent keys per group out of
+// the sort when they hold at least this many members in all (0: never)
+int g_big_pivot_min = 1 << 20;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -1198,24 +1201,80 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             // index and key2 in 32 bits (the few big groups of degenerate texts): u32
             // sort keys, 16 B per element and pass instead of 24
             const bool comp32 = cbits <= 32;
-            u64 *b0 = rsc.get<u64>(total), *b1 = rsc.get<u64>(total);
-            u32 *bv0 = rsc.get<u32>(total), *bv1 = rsc.get<u32>(total);
+            // Few-key partition (rounds in rank form with many big-group members):
+            // members equal to a frequent key of their group skip the sort.
+            const bool pivots = !raw && g_big_pivot_min > 0 && total >= (u64)g_big_pivot_min &&
+                                total < (1ull << 31) && (BP + 1) * big_part_tiles(total) <= ws.bcnt_cap;
+            BigPivots bp{};
+            PivRegions reg{};
+            u64 nrest = total;
+            u32 *pq = nullptr, *ppv = nullptr, *d_rk = nullptr;
+            u64 *d_pbase = nullptr;
+            if (pivots) {
+                u64 *d_piv = rsc.get<u64>((u64)nb * BP);
+                u32 *d_np = rsc.get<u32>(nb), *d_eq = rsc.get<u32>((u64)nb * BP);
+                d_rk = rsc.get<u32>((u64)nb * BP);
+                d_pbase = rsc.get<u64>(1);
+                if (!d_piv || !d_np || !d_eq || !d_rk || !d_pbase) return SAGPU_ENOMEM;
+                const u64 ptiles = big_part_tiles(total);
+                {
+                    Span sp(c, F_ROUNDS);
+                    SAGPU_TRY(big_pivots(key2, wide, d_bhead, d_boff, nb, d_piv, d_np, s));
+                    SAGPU_TRY(big_part_count(key2, wide, d_bhead, d_boff, nb, total, d_piv, d_np, ws.bcnt, s));
+                    SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, (BP + 1) * ptiles, s));
+                }
+                u32 *preg = (u32 *)(c.pinned + 7200);   // past the round parameters
+                for (int b = 0; b <= BP; b++)
+                    SAGPU_CHECK(hipMemcpyAsync(preg + b, ws.bcnt + (u64)b * ptiles, 4, hipMemcpyDeviceToHost, s));
+                SAGPU_CHECK(hipStreamSynchronize(s));
+                for (int b = 0; b <= BP; b++) reg.start[b] = preg[b];
+                for (int b = 0; b < BP; b++)
+                    if (reg.start[b] > reg.start[b + 1] || reg.start[BP] > total) {
+                        sagpu_set_error("pivot partition inconsistent", __FILE__, __LINE__);
+                        return SAGPU_EDEVICE;
+                    }
+                nrest = total - reg.start[BP];
+                pq = rsc.get<u32>(reg.start[BP] + 1);
+                ppv = rsc.get<u32>(reg.start[BP] + 1);
+                if (!pq || !ppv) return SAGPU_ENOMEM;
+                bp = BigPivots{d_piv, d_np, d_eq};
+                st.big_pivot_rounds++;
+                st.big_pivot_elems += (int64_t)reg.start[BP];
+            }
+            u64 *b0 = rsc.get<u64>(nrest + 1), *b1 = rsc.get<u64>(nrest + 1);
+            u32 *bv0 = rsc.get<u32>(nrest + 1), *bv1 = rsc.get<u32>(nrest + 1);
             if (!b0 || !b1 || !bv0 || !bv1) return SAGPU_ENOMEM;
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32, b0, bv0, s));
+                if (pivots)
+                    SAGPU_TRY(big_part_scatter(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32,
+                                               bp.piv, bp.npiv, ws.bcnt, reg.start[BP], b0, bv0, pq, ppv, s));
+                else
+                    SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32, b0, bv0, s));
             }
-            if (comp32) SAGPU_TRY(sort_u32(c, (u32 *)b0, bv0, (u32 *)b1, bv1, total, 0, cbits, &alt));
-            else SAGPU_TRY(sort_u64(c, b0, bv0, b1, bv1, total, 0, cbits, &alt));
+            alt = 0;
+            if (comp32) SAGPU_TRY(sort_u32(c, (u32 *)b0, bv0, (u32 *)b1, bv1, nrest, 0, cbits, &alt));
+            else SAGPU_TRY(sort_u64(c, b0, bv0, b1, bv1, nrest, 0, cbits, &alt));
             {
                 Span sp(c, F_ROUNDS);
-                SAGPU_TRY(big_emit(ws, alt ? b1 : b0, comp32, kb, alt ? bv1 : bv0, d_bslot, total, h, dSA, dISA,
-                                   act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac,
-                                   st.rounds == 1 && rk1.pw ? &ex1 : nullptr));
+                if (pivots)
+                    SAGPU_TRY(big_piv_ranks(alt ? b1 : b0, comp32, kb, nrest, pq, reg, bp.piv, bp.npiv, d_bslot, nb,
+                                            (u32 *)bp.eq, d_rk, out_counter, d_pbase, err, s));
+                if (nrest > 0)
+                    SAGPU_TRY(big_emit(ws, alt ? b1 : b0, comp32, kb, alt ? bv1 : bv0, d_bslot, nrest, h, dSA, dISA,
+                                       act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac,
+                                       st.rounds == 1 && rk1.pw ? &ex1 : nullptr, pivots ? &bp : nullptr));
+                if (pivots)
+                    SAGPU_TRY(big_piv_emit(pq, ppv, reg, d_rk, d_bslot, d_pbase, h, dISA, act_i[nxt], act_g[nxt],
+                                           dLCP, ac.cb, s));
             }
-            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, sizeof(u64), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
             out_n = c.pinned[0];
+            if (((u32 *)(c.pinned + 4))[1] != 0 || out_n > a) {
+                sagpu_set_error("big-group bookkeeping inconsistent", __FILE__, __LINE__);
+                return SAGPU_EDEVICE;
+            }
             st.big_rounds++;
             st.big_elems += (int64_t)total;
         }
@@ -2166,6 +2225,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "round_graphs")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_round_graphs = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "big_pivot_min")) {
+        if (value < 0 || value > 0x7FFFFFFF) return SAGPU_EINVAL;
+        g_big_pivot_min = (int)value;
         return 0;
     }
     if (!strcmp(name, "windowed_scatter")) {

@@ -1,0 +1,374 @@
+// pivots.hip -- few-key partition of the big groups of a doubling round.
+//
+// On texts of long repeats (periodic texts, runs of one symbol, Fibonacci words) a
+// round's big groups hold millions of members whose key2 (the rank at i + h) takes
+// only one to a few values: a group of suffixes in one residue class of a period
+// maps as a whole to the group of the next class.  Sorting such members by
+// (group, key2) with the radix sort moves every member through every pass, round
+// after round.  Here each big group samples 64 of its members' keys (one wave per
+// group) and takes up to BP values that occur at least twice in the sample as
+// pivots; a stable partition (count, scan, scatter over tiles of 4096 members)
+// puts the members equal to pivot b of their group into class b and the others
+// into the rest, and only the rest goes through the radix sort.  Every member of a
+// pivot class of one group forms one new subgroup (they share the key), placed
+// after the group's rest members with smaller keys and the members of the smaller
+// pivots: rank = first slot of the group + #rest below + #pivot members below.
+// The rest is ranked by the usual segmented scan over the sorted keys, with each
+// rank raised by the members of the pivots below its key.
+//
+// Tie order: members of one pivot class keep their list order; members of one
+// subgroup are tied, and any order of tied members is a valid list order.
+#include "engine.h"
+
+namespace {
+
+constexpr int PT_THREADS = 256, PT_ROWS = 16, PT_SPAN = 1024, PT_TILE = 4096;
+constexpr u32 NOQ = 0xFFFFFFFFu;
+
+__device__ __forceinline__ u32 group_of(const u64 *__restrict__ boff, u32 nb, u64 e)
+{
+    u32 lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const u32 mid = (lo + hi + 1) >> 1;
+        if (boff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A thread's walk over increasing member indexes e of the concatenated big groups
+// (group q holds [boff[q], boff[q + 1]), list entries from bhead[q]).
+struct PivCursor {
+    u32 q = NOQ;
+    u64 beg = 0, end = 0, head = 0;
+    u32 np = 0;
+    u64 p[BP];
+
+    __device__ void to(const u64 *__restrict__ boff, const u32 *__restrict__ bhead, const u64 *__restrict__ piv,
+                       const u32 *__restrict__ npiv, u32 nb, u64 e)
+    {
+        if (q != NOQ && e >= beg && e < end) return;
+        // big groups hold more than 64 members, so a step of one wave row moves
+        // at most into the next group
+        if (q != NOQ && e >= end && q + 1 < nb && e < boff[q + 2]) q = q + 1;
+        else q = group_of(boff, nb, e);
+        beg = boff[q];
+        end = boff[q + 1];
+        head = bhead[q];
+        np = npiv[q];
+#pragma unroll
+        for (int b = 0; b < BP; b++) p[b] = (u32)b < np ? piv[(u64)q * BP + b] : 0ull;
+    }
+    // pivot index the key equals, BP for the rest
+    __device__ u32 cls(u64 k) const
+    {
+        u32 c = BP;
+#pragma unroll
+        for (int b = 0; b < BP; b++)
+            if ((u32)b < np && p[b] == k) c = (u32)b;
+        return c;
+    }
+};
+
+// One wave per big group: 64 keys sampled at even steps over the group's members;
+// the values sampled at least twice, most frequent first, at most BP, ascending.
+template <typename KT>
+__global__ __launch_bounds__(256) void k_big_pivots(const KT *__restrict__ key2, const u32 *__restrict__ bhead,
+                                                    const u64 *__restrict__ boff, u32 nb, u64 *__restrict__ piv,
+                                                    u32 *__restrict__ npiv)
+{
+    const u32 q = blockIdx.x * (256 / SAGPU_WAVE) + threadIdx.x / SAGPU_WAVE;
+    if (q >= nb) return;   // whole waves leave together
+    const u32 lane = lane_id();
+    const u64 size = boff[q + 1] - boff[q];
+    // size > RT_SMALL >= 64: the 64 sample positions are distinct members
+    const u64 k = (u64)key2[(u64)bhead[q] + (u64)lane * size / SAGPU_WAVE];
+    u32 cnt = 0;
+    for (int l = 0; l < SAGPU_WAVE; l++) cnt += __shfl(k, l, SAGPU_WAVE) == k;
+    u64 val[BP];
+    u32 np = 0;
+    for (int b = 0; b < BP; b++) {
+        // the most frequent remaining value; ties to the lowest lane
+        u32 score = cnt >= 2 ? (cnt << 6) | (SAGPU_WAVE - 1 - lane) : 0u;
+        for (int d = 32; d >= 1; d >>= 1) {
+            const u32 o = __shfl_xor(score, d, SAGPU_WAVE);
+            score = o > score ? o : score;
+        }
+        if (score == 0) break;
+        const u64 kv = __shfl(k, (int)(SAGPU_WAVE - 1 - (score & 63u)), SAGPU_WAVE);
+        val[np++] = kv;
+        if (k == kv) cnt = 0;
+    }
+    if (lane == 0) {
+        for (u32 i = 1; i < np; i++)   // ascending
+            for (u32 j = i; j > 0 && val[j - 1] > val[j]; j--) {
+                const u64 t = val[j];
+                val[j] = val[j - 1];
+                val[j - 1] = t;
+            }
+        for (u32 b = 0; b < np; b++) piv[(u64)q * BP + b] = val[b];
+        npiv[q] = np;
+    }
+}
+
+// Partition pass 1: members of each class per tile, class-major: ccnt[c * tiles + t].
+template <typename KT>
+__global__ __launch_bounds__(PT_THREADS) void k_part_count(const KT *__restrict__ key2, const u32 *__restrict__ bhead,
+                                                           const u64 *__restrict__ boff, u32 nb, u64 total,
+                                                           const u64 *__restrict__ piv, const u32 *__restrict__ npiv,
+                                                           u32 *__restrict__ ccnt, u64 tiles)
+{
+    __shared__ u32 s_c[BP + 1];
+    if (threadIdx.x <= BP) s_c[threadIdx.x] = 0;
+    __syncthreads();
+    const u32 lane = lane_id();
+    const u64 wbase = (u64)blockIdx.x * PT_TILE + (u64)(threadIdx.x / SAGPU_WAVE) * PT_SPAN;
+    PivCursor cur;
+    u32 cnt[BP + 1];
+#pragma unroll
+    for (int b = 0; b <= BP; b++) cnt[b] = 0;
+    for (int r = 0; r < PT_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        u32 c = BP + 1;
+        if (e < total) {
+            cur.to(boff, bhead, piv, npiv, nb, e);
+            c = cur.cls((u64)key2[cur.head + (e - cur.beg)]);
+        }
+#pragma unroll
+        for (int b = 0; b <= BP; b++) cnt[b] += (u32)__popcll(__ballot(c == (u32)b));
+    }
+    if (lane == 0)
+        for (int b = 0; b <= BP; b++)
+            if (cnt[b]) atomicAdd(&s_c[b], cnt[b]);
+    __syncthreads();
+    if (threadIdx.x <= BP) ccnt[(u64)threadIdx.x * tiles + blockIdx.x] = s_c[threadIdx.x];
+}
+
+// Partition pass 2 (after the exclusive scan of ccnt): members in tile order into
+// their class.  Pivot classes go to (pq, ppv) = (dense group index, list entry) at
+// the scanned position; the rest to the sort buffers at position - npivot, keyed
+// (dense group index << kb) | key2 as the plain extraction keys them.
+template <typename KT, typename CT>
+__global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
+                                                             const u32 *__restrict__ bhead,
+                                                             const u64 *__restrict__ boff, u32 nb, u64 total, int kb,
+                                                             const u64 *__restrict__ piv,
+                                                             const u32 *__restrict__ npiv,
+                                                             const u32 *__restrict__ cscan, u64 tiles, u64 npivot,
+                                                             CT *__restrict__ comp, u32 *__restrict__ vals,
+                                                             u32 *__restrict__ pq, u32 *__restrict__ ppv)
+{
+    __shared__ u32 s_wc[PT_THREADS / SAGPU_WAVE][BP + 1];
+    const u32 lane = lane_id(), w = threadIdx.x / SAGPU_WAVE;
+    const u64 wbase = (u64)blockIdx.x * PT_TILE + (u64)w * PT_SPAN;
+    const u64 lt = lanemask_lt();
+    PivCursor cur;
+    u64 packed = 0;   // class of each row, 3 bits per row
+    u32 cnt[BP + 1];
+#pragma unroll
+    for (int b = 0; b <= BP; b++) cnt[b] = 0;
+    for (int r = 0; r < PT_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        u32 c = BP + 1;
+        if (e < total) {
+            cur.to(boff, bhead, piv, npiv, nb, e);
+            c = cur.cls((u64)key2[cur.head + (e - cur.beg)]);
+        }
+        packed |= (u64)c << (3 * r);
+#pragma unroll
+        for (int b = 0; b <= BP; b++) cnt[b] += (u32)__popcll(__ballot(c == (u32)b));
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int b = 0; b <= BP; b++) s_wc[w][b] = cnt[b];
+    __syncthreads();
+    u64 pos[BP + 1];   // next output position of each class for this wave
+#pragma unroll
+    for (int b = 0; b <= BP; b++) {
+        u64 o = cscan[(u64)b * tiles + blockIdx.x];
+        for (u32 x = 0; x < w; x++) o += s_wc[x][b];
+        pos[b] = o;
+    }
+    for (int r = 0; r < PT_ROWS; r++) {
+        const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
+        const u32 c = (u32)(packed >> (3 * r)) & 7u;
+        u64 mine = 0;
+#pragma unroll
+        for (int b = 0; b <= BP; b++) {
+            const u64 m = __ballot(c == (u32)b);
+            if (c == (u32)b) mine = pos[b] + (u64)__popcll(m & lt);
+            pos[b] += (u64)__popcll(m);
+        }
+        if (c > BP) continue;
+        cur.to(boff, bhead, piv, npiv, nb, e);
+        const u64 k = cur.head + (e - cur.beg);
+        if (c < BP) {
+            pq[mine] = cur.q;
+            ppv[mine] = act_i[k];
+        } else {
+            const u64 o = mine - npivot;
+            comp[o] = (CT)(((u64)cur.q << kb) | (u64)key2[k]);
+            vals[o] = act_i[k];
+        }
+    }
+}
+
+// First index in sorted a[0, n) with a[i] >= v.
+template <typename T>
+__device__ u64 lower_bound_dev(const T *__restrict__ a, u64 n, u64 v)
+{
+    u64 lo = 0, hi = n;
+    while (lo < hi) {
+        const u64 mid = (lo + hi) >> 1;
+        if ((u64)a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One thread per big group: members equal to each pivot (from the pivot classes,
+// grouped by dense group index) and the new subgroup's rank.  The first thread
+// reserves the pivot members' entries of the next active list.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, u64 nrest, int kb,
+                                                   const u32 *__restrict__ pq, PivRegions reg,
+                                                   const u64 *__restrict__ piv, const u32 *__restrict__ npiv,
+                                                   const u32 *__restrict__ gslot, u32 nb, u32 *__restrict__ eq,
+                                                   u32 *__restrict__ rk, u64 *__restrict__ out_counter,
+                                                   u64 *__restrict__ pbase, u32 *__restrict__ err)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pbase = atomicAdd(out_counter, reg.start[BP]);
+    const u32 q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    const u32 np = npiv[q];
+    const u64 g0 = (u64)q << kb;
+    const u64 rest0 = lower_bound_dev(rest, nrest, g0);
+    u64 below = 0;   // pivot members of smaller keys
+    for (u32 b = 0; b < np; b++) {
+        const u64 lt = lower_bound_dev(rest, nrest, g0 | piv[(u64)q * BP + b]) - rest0;
+        const u64 r0 = reg.start[b], len = reg.start[b + 1] - r0;
+        const u64 c = lower_bound_dev(pq + r0, len, (u64)q + 1) - lower_bound_dev(pq + r0, len, (u64)q);
+        if (c < 2) atomicOr(err, 4u);   // a pivot is sampled twice, so it has two members
+        eq[(u64)q * BP + b] = (u32)c;
+        rk[(u64)q * BP + b] = gslot[q] + (u32)(lt + below);
+        below += c;
+    }
+}
+
+// The pivot classes' members: one subgroup per (group, pivot), entered in the next
+// active list (the reserved range, in class order), ranks written where they
+// change, and the split's LCP mark at the subgroup's first slot.
+__global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, const u32 *__restrict__ ppv,
+                                                  PivRegions reg, const u32 *__restrict__ rk,
+                                                  const u32 *__restrict__ gslot, const u64 *__restrict__ pbase, u32 h,
+                                                  u32 *__restrict__ ISA, u32 *__restrict__ out_i,
+                                                  u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb)
+{
+    const u64 npv = reg.start[BP];
+    const u64 base = *pbase;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 m = (u64)blockIdx.x * blockDim.x + threadIdx.x; m < npv; m += stride) {
+        u32 b = 0;
+#pragma unroll
+        for (int t = 1; t < BP; t++) b += m >= reg.start[t];
+        const u32 q = pq[m];
+        const u32 r = rk[(u64)q * BP + b], g = gslot[q];
+        const u32 pv = ppv[m];
+        if (r != g) ISA[pv >> cb] = r;
+        out_i[base + m] = pv;
+        out_g[base + m] = r;
+        if (LCP && r != g && (m == reg.start[b] || pq[m - 1] != q))
+            LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+    }
+}
+
+unsigned grid_cap(u64 work, unsigned per_block, unsigned cap)
+{
+    u64 b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+} // namespace
+
+u64 big_part_tiles(u64 total) { return (total + PT_TILE - 1) / PT_TILE; }
+
+int big_pivots(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 *piv, u32 *npiv,
+               hipStream_t s)
+{
+    if (nb == 0) return SAGPU_OK;
+    const unsigned grid = (nb + 3) / 4;
+    if (wide)
+        hipLaunchKernelGGL(k_big_pivots<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)key2, bhead, boff, nb, piv,
+                           npiv);
+    else
+        hipLaunchKernelGGL(k_big_pivots<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)key2, bhead, boff, nb, piv,
+                           npiv);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total, const u64 *piv,
+                   const u32 *npiv, u32 *ccnt, hipStream_t s)
+{
+    const u64 tiles = big_part_tiles(total);
+    if (total == 0 || nb == 0) return SAGPU_OK;
+    if (wide)
+        hipLaunchKernelGGL(k_part_count<u64>, dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, (const u64 *)key2, bhead,
+                           boff, nb, total, piv, npiv, ccnt, tiles);
+    else
+        hipLaunchKernelGGL(k_part_count<u32>, dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, (const u32 *)key2, bhead,
+                           boff, nb, total, piv, npiv, ccnt, tiles);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb,
+                     u64 total, int kb, int comp32, const u64 *piv, const u32 *npiv, const u32 *cscan, u64 npivot,
+                     void *comp, u32 *vals, u32 *pq, u32 *ppv, hipStream_t s)
+{
+    const u64 tiles = big_part_tiles(total);
+    if (total == 0 || nb == 0) return SAGPU_OK;
+#define PT_SCATTER(KT, CT)                                                                                          \
+    hipLaunchKernelGGL((k_part_scatter<KT, CT>), dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, act_i,            \
+                       (const KT *)key2, bhead, boff, nb, total, kb, piv, npiv, cscan, tiles, npivot, (CT *)comp, \
+                       vals, pq, ppv)
+    if (wide) {
+        if (comp32) PT_SCATTER(u64, u32);
+        else PT_SCATTER(u64, u64);
+    } else {
+        if (comp32) PT_SCATTER(u32, u32);
+        else PT_SCATTER(u32, u64);
+    }
+#undef PT_SCATTER
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
+                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 nb, u32 *eq, u32 *rk, u64 *out_counter,
+                  u64 *pbase, u32 *err, hipStream_t s)
+{
+    const unsigned grid = (nb + 255) / 256 > 0 ? (nb + 255) / 256 : 1;
+    if (comp32)
+        hipLaunchKernelGGL(k_piv_ranks<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)rest, nrest, kb, pq, reg, piv,
+                           npiv, gslot, nb, eq, rk, out_counter, pbase, err);
+    else
+        hipLaunchKernelGGL(k_piv_ranks<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)rest, nrest, kb, pq, reg, piv,
+                           npiv, gslot, nb, eq, rk, out_counter, pbase, err);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
+int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *gslot,
+                 const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s)
+{
+    const u64 npv = reg.start[BP];
+    if (npv == 0) return SAGPU_OK;
+    const u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
+    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, ppv, reg, rk, gslot,
+                       pbase, hh, ISA, out_i, out_g, LCP, cb);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}

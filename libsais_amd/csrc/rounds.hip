@@ -483,6 +483,7 @@ struct BigF {
     u32 *SA, *ISA, *out_i, *out_g, *LCP;
     ActCode ac;
     ExactLcp ex;
+    BigPivots bp;        // the pivot classes taken out of the sort (bp.eq null: none)
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
@@ -507,9 +508,17 @@ struct BigF {
         const u32 pv = vals[e];
         const u32 v = pv >> ac.cb;
         const u64 c = comp[e];
-        u32 g = gslot[(u32)(c >> kb)];
+        const u32 q = (u32)(c >> kb);
+        const u32 g = gslot[q];
         u32 r = g + (hstart - gstart);
-        if (hstart != gstart) ISA[v] = r;  // the first subgroup keeps the group's rank
+        if (bp.eq) {
+            // members of the group's pivot classes with smaller keys rank below
+            const u64 key = c & ((1ull << kb) - 1);
+            const u32 np = bp.npiv[q];
+            for (u32 b = 0; b < np; b++)
+                if (bp.piv[(u64)q * BP + b] < key) r += bp.eq[(u64)q * BP + b];
+        }
+        if (r != g) ISA[v] = r;  // the first subgroup keeps the group's rank
         if (act) {
             out_i[o] = pv;
             out_g[o] = r;
@@ -517,10 +526,11 @@ struct BigF {
             SA[r] = v;
             if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
         }
-        if (LCP && hstart == (u32)e && hstart != gstart) {
-            // e - 1 is the last entry of the previous subgroup of the same group
+        if (LCP && hstart == (u32)e && r != g) {
+            // a subgroup after the group's first: e - 1 is the last entry of the
+            // previous subgroup (exact marks come only without pivot classes)
             u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
-            if (ex.on && !ex.skipped(g)) {
+            if (ex.on && !ex.skipped(g) && hstart != gstart) {
                 const u64 km = (1ull << kb) - 1;
                 mark = h + ex.shared(c & km, (u64)comp[e - 1] & km);
             }
@@ -1049,14 +1059,15 @@ int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, 
 
 int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
              u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
-             const ExactLcp *ex)
+             const ExactLcp *ex, const BigPivots *pv)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
     const ExactLcp e = ex ? *ex : ExactLcp{};
+    const BigPivots p = pv ? *pv : BigPivots{};
     if (comp32) {
-        BigF<u32> f{(const u32 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e};
+        BigF<u32> f{(const u32 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p};
         return ws_run(ws, f, total, out_counter, s);
     }
-    BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e};
+    BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p};
     return ws_run(ws, f, total, out_counter, s);
 }

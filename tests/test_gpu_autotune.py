@@ -52,5 +52,6 @@ def test_tune_batched_small():
 def test_knob_validation():
     lib = la.load()
     for name, bad in ((b"radix_items", 12), (b"tile_range", 300), (b"radix_variant", 2), (b"radix_threads", 384),
-                      (b"radix_bits", 6), (b"round_graphs", 2), (b"windowed_scatter", -1)):
+                      (b"radix_bits", 6), (b"round_graphs", 2), (b"windowed_scatter", -1),
+                      (b"big_pivot_min", -1)):
         assert lib.sagpu_set_option(name, bad) == -1

@@ -443,6 +443,78 @@ def test_round_graphs_replay_across_builds():
         assert lib.sagpu_set_option(b"round_graphs", 1) == 0
 
 
+def _pivot_cases():
+    rng = np.random.default_rng(31)
+    return [
+        ("all_a_300k", np.full(300000, ord("A"), np.uint8), True),
+        ("periodic17_1M", workloads.periodic(1 << 20, 17, 3), True),
+        ("periodic3_500k", workloads.periodic(500000, 3, 9), True),
+        ("fibonacci_1M", workloads.fibonacci(1 << 20), True),
+        ("copies_280k", np.tile(workloads.iid(7000, seed=25, alphabet=b"ACGT"), 40), False),
+        ("ab_runs_400k", np.repeat(rng.integers(65, 67, 4000).astype(np.uint8), 100), False),
+        ("dna_like_1M", workloads.dna_like(1 << 20, seed=23), False),
+        ("binary_iid_600k", workloads.iid(600000, seed=32, alphabet=b"01"), False),
+    ]
+
+
+@pytest.mark.parametrize("name,T,degenerate", _pivot_cases(), ids=[c[0] for c in _pivot_cases()])
+@pytest.mark.parametrize("wide", [0, 1])
+def test_big_pivots(name, T, degenerate, wide):
+    """Big groups with few distinct keys: up to 4 keys per group frequent in a sample
+    skip the sort (the pivot classes) and the rest is sorted and ranked around them.
+    Forced on at every size (big_pivot_min = 1), in the 32-bit and the wide (u64
+    key2) engine, the fused build equals the oracle; on repeat-rich texts the path
+    runs."""
+    lib = la.load()
+    rc, SA_ref = ol.sa(T)
+    assert rc == 0
+    P_ref = ol.plcp(T, SA_ref)
+    L_ref = ol.lcp(P_ref, SA_ref)
+    U_ref, idx_ref = ol.bwt(T, SA_ref)
+    try:
+        assert lib.sagpu_set_option(b"big_pivot_min", 1) == 0
+        if wide:
+            assert lib.sagpu_set_option(b"wide_from", 0) == 0
+        print(f"[pivots] {name} n={len(T)} wide={wide}", flush=True)
+        SA, U, r, P, L, f = fused(T)
+        st = la.stats()
+        print(f"[pivots] {name}: rounds {st.rounds}, big rounds {st.big_rounds}, pivot rounds "
+              f"{st.big_pivot_rounds}, pivot members {st.big_pivot_elems}", flush=True)
+        assert np.array_equal(SA, SA_ref), name
+        assert r == idx_ref and np.array_equal(U, U_ref), name
+        assert np.array_equal(P, P_ref) and np.array_equal(L, L_ref), name
+        if degenerate:
+            assert st.big_pivot_rounds > 0 and st.big_pivot_elems > 0, name
+    finally:
+        assert lib.sagpu_set_option(b"big_pivot_min", 1 << 20) == 0
+        assert lib.sagpu_set_option(b"wide_from", 0x7FFFFFFF) == 0
+
+
+def test_big_pivots_gsa():
+    """The pivot path on a string set (separators end the repeats): equal to the
+    oracle's GSA outputs."""
+    lib = la.load()
+    unit = workloads.iid(3000, seed=33, alphabet=b"ACGT")
+    parts = []
+    for k in range(60):
+        parts.append(np.tile(unit[: 1000 + 17 * k], 3))
+        parts.append(np.zeros(1, np.uint8))
+    T = np.concatenate(parts)
+    rc, SA_ref = ol.sa(T, gsa=True)
+    assert rc == 0
+    P_ref = ol.plcp(T, SA_ref, gsa=True)
+    L_ref = ol.lcp(P_ref, SA_ref)
+    try:
+        assert lib.sagpu_set_option(b"big_pivot_min", 1) == 0
+        SA, U, r, P, L, f = fused(T, gsa=True)
+        st = la.stats()
+        print(f"[pivots] gsa n={len(T)}: big rounds {st.big_rounds}, pivot rounds {st.big_pivot_rounds}", flush=True)
+        assert np.array_equal(SA, SA_ref)
+        assert np.array_equal(P, P_ref) and np.array_equal(L, L_ref)
+    finally:
+        assert lib.sagpu_set_option(b"big_pivot_min", 1 << 20) == 0
+
+
 def _round1_cases():
     rng = np.random.default_rng(4242)
 
