@@ -340,7 +340,10 @@ def bench_degenerate(lib, reps: int = 3):
                 if r < 0:
                     raise RuntimeError(f"sagpu_build32 failed ({r}): {la.last_error()}")
             t = float(np.median(times[1:]))
-            out[name] = {"ms": round(t * 1000, 2), "MBps": round(n / t / 1e6, 1), "rounds": la.stats().rounds}
+            st = la.stats()
+            out[name] = {"ms": round(t * 1000, 2), "MBps": round(n / t / 1e6, 1), "rounds": st.rounds,
+                         "big_rounds": st.big_rounds, "big_members": st.big_elems,
+                         "pivot_rounds": st.big_pivot_rounds, "pivot_members": st.big_pivot_elems}
             log(f"degenerate {name}: {out[name]}")
     finally:
         lib.sagpu_free(dT)

@@ -69,6 +69,7 @@ struct RadixScratch {
     int variant;      // RADIX_ONESWEEP or RADIX_REDUCE_THEN_SCAN
     int items;        // keys per thread per tile: 8, 16 or 32 (tile = threads * items)
     int threads;      // onesweep workgroup size with 32 items: 256 or 512
+    u64 small_n;      // sorts of fewer keys take the smallest tiles (items and threads ignored)
     u32 *counts;      // [counts_cap] per-tile digit counts (reduce-then-scan)
     u64 counts_cap;
     ScanScratch scan;

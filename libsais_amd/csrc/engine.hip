@@ -367,6 +367,9 @@ int g_windowed_scatter = 1;
 // Big groups of rounds after the first take up to BP frequent keys per group out of
 // the sort when they hold at least this many members in all (0: never)
 int g_big_pivot_min = 1 << 20;
+// radix sorts of fewer keys use the smallest tiles (2048 keys), whatever radix_items
+// and radix_threads say
+int g_radix_small_n = 1 << 20;
 
 // A context is one stream with its own pooled buffers and scratch; calls on one
 // context are serialised, different contexts run concurrently on the device.
@@ -375,6 +378,7 @@ void apply_options(Ctx &c)
     c.rs.variant = g_radix_variant;
     c.rs.items = g_radix_items;
     c.rs.threads = g_radix_threads;
+    c.rs.small_n = (u64)g_radix_small_n;
     // the allocation-free build's look-back scratch is sized for 8-bit digits
     c.rs.bits = c.pool.arena ? 8 : g_radix_bits;
 }
@@ -2225,6 +2229,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "round_graphs")) {
         if (value != 0 && value != 1) return SAGPU_EINVAL;
         g_round_graphs = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "radix_small_n")) {
+        if (value < 0 || value > 0x7FFFFFFF) return SAGPU_EINVAL;
+        g_radix_small_n = (int)value;
         return 0;
     }
     if (!strcmp(name, "big_pivot_min")) {

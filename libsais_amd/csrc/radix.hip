@@ -470,7 +470,11 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
     const int npass = (end_bit - begin_bit + DB - 1) / DB;
     // 4- and 11-bit digits: onesweep only, 256-thread tiles
     const int variant = DB == 8 ? rs.variant : RADIX_ONESWEEP;
-    const int items = rs.items == 32 ? 32 : rs.items == 8 ? 8 : 16;
+    // short inputs (one string of a batch, a served job) take 2048-key tiles: more
+    // workgroups at once instead of a few long ones (a 16384-key tile takes about
+    // 25 us, so a 20 K-key sort would run its passes on two workgroups)
+    const bool small = n < rs.small_n;
+    const int items = small ? 8 : rs.items == 32 ? 32 : rs.items == 8 ? 8 : 16;
     const u64 tile_keys = (u64)RS_THREADS * items;
     const u64 tiles = (n + tile_keys - 1) / tile_keys;
     if (tiles * BINS > rs.lb_words || npass > RADIX_MAX_PASSES || (u64)npass * BINS > RADIX_HIST_WORDS ||
@@ -492,7 +496,10 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
     }
     SAGPU_TRY(launch_fill_u64(rs.hist, (u64)npass * BINS, 0, s));
     {
-        int blocks = (int)((n + 256 * 64 - 1) / (256 * 64));
+        // 64 keys per thread; 8 for short inputs, which would otherwise leave all
+        // but a few workgroups idle
+        const u64 per = small ? 256 * 8 : 256 * 64;
+        int blocks = (int)((n + per - 1) / per);
         if (blocks > 4096) blocks = 4096;
         if (gen)
             hipLaunchKernelGGL((k_radix_hist<K, true, DB>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit,
@@ -503,9 +510,13 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
         hipLaunchKernelGGL(k_radix_offsets, dim3(1), dim3(256), 0, s, rs.hist, npass, BINS);
         SAGPU_CHECK(hipGetLastError());
     }
-    // Histograms decide which passes can be skipped (all keys share the digit).
-    SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, (u64)npass * BINS * sizeof(u64), hipMemcpyDeviceToHost, s));
-    SAGPU_CHECK(hipStreamSynchronize(s));
+    // Histograms decide which passes can be skipped (all keys share the digit).  A
+    // short input runs every pass instead: a pass over it costs less than the wait
+    // for the histograms on the host.
+    if (!small) {
+        SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, (u64)npass * BINS * sizeof(u64), hipMemcpyDeviceToHost, s));
+        SAGPU_CHECK(hipStreamSynchronize(s));
+    }
 
     SAGPU_TRY(launch_fill_u32(rs.tile_ctr, RADIX_MAX_PASSES, 0, s));
     K *ki = k0, *ko = k1;
@@ -513,7 +524,7 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
     int alt = 0, ran = 0;
     for (int p = 0; p < npass; p++) {
         bool trivial = false;
-        for (int d = 0; d < BINS; d++)
+        for (int d = 0; d < BINS && !small; d++)
             if (rs.hist_host[p * BINS + d] == n) trivial = true;
         if (trivial) continue;
         rs.epoch = (rs.epoch + 1) & ((1u << 22) - 1);
@@ -547,7 +558,8 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
         const u32 *cnts = lb ? nullptr : rs.counts;
         // onesweep passes may use 512-thread workgroups (16384-key tiles with 32
         // items: twice as long digit runs per tile, same waves per CU); 8-bit digits
-        const int nt = (DB == 8 && sizeof(K) == 4 && lb && rs.threads == 512 && items == 32) ? 512 : RS_THREADS;
+        const int nt = (!small && DB == 8 && sizeof(K) == 4 && lb && rs.threads == 512 && items == 32) ? 512
+                                                                                                          : RS_THREADS;
         const u64 ptiles = (n + (u64)nt * items - 1) / ((u64)nt * items);
 #define RS_LAUNCH(HV, LB, IT)                                                                                     \
     hipLaunchKernelGGL((k_radix_pass<K, HV, LB, IT, false, RS_THREADS, DB>), dim3((unsigned)tiles), dim3(RS_THREADS), \

@@ -63,20 +63,23 @@ constexpr int PK_THREADS = 256, PK_ITEMS = 16, PK_TILE = PK_THREADS * PK_ITEMS, 
 __global__ __launch_bounds__(PK_THREADS) void k_pack_keys32(const uint8_t *__restrict__ T, u64 n,
                                                             const uint8_t *__restrict__ lut, int bits, int k,
                                                             int gsa, int cb, u32 *__restrict__ keys,
-                                                            u32 *__restrict__ vals)
+                                                            u32 *__restrict__ vals, int items)
 {
+    // items <= PK_ITEMS keys per thread (short texts take fewer, so more workgroups
+    // share the k-symbol loops)
     __shared__ uint8_t s_lut[256];
     __shared__ uint8_t s_c[PK_TILE + PK_HALO + 1];   // s_c[i] = code of T[base - 1 + i]
     s_lut[threadIdx.x] = lut[threadIdx.x];
     __syncthreads();
-    const u64 base = (u64)blockIdx.x * PK_TILE;
-    for (int i = threadIdx.x; i < PK_TILE + PK_HALO + 1; i += PK_THREADS) {
+    const int tile = PK_THREADS * items;
+    const u64 base = (u64)blockIdx.x * tile;
+    for (int i = threadIdx.x; i < tile + PK_HALO + 1; i += PK_THREADS) {
         u64 p = base + i;   // position base - 1 + i, shifted by one
         s_c[i] = (p >= 1 && p - 1 < n) ? s_lut[T[p - 1]] : 0;
     }
     __syncthreads();
 #pragma unroll 4
-    for (int j = 0; j < PK_ITEMS; j++) {
+    for (int j = 0; j < items; j++) {
         int li = j * PK_THREADS + threadIdx.x;
         u64 p = base + li;
         if (p >= n) break;
@@ -354,8 +357,9 @@ int launch_pack_keys32(const uint8_t *T, u64 n, const uint8_t *lut, int bits, in
         sagpu_set_error("pack32: bad symbol count", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
-    hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, PK_TILE)), dim3(PK_THREADS), 0, s, T, n, lut, bits, k, gsa,
-                       cb, keys, vals);
+    const int items = n < (1u << 20) ? 1 : PK_ITEMS;
+    hipLaunchKernelGGL(k_pack_keys32, dim3(ceil_div_u(n, (u64)PK_THREADS * items)), dim3(PK_THREADS), 0, s, T, n, lut,
+                       bits, k, gsa, cb, keys, vals, items);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
