@@ -108,6 +108,8 @@ struct PlcpHint {
 #define LCP_UNKNOWN 0x80000000u
 #define LCP_BOUND_CAP 0x40000000u
 enum { RT_SMALL = 256 };   // groups up to this size are sorted inside one workgroup
+// texts shorter than this are grouped in 512-position tiles instead of 4096
+constexpr u64 INIT_SMALL_N = 1ull << 20;
 
 struct WsScratch {
     void *tile_agg;   // [tiles_cap] 16-byte tile aggregates

@@ -955,7 +955,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     void *key2 = vA;
 
     WsScratch ws{};
-    ws.tiles_cap = n / 4096 + 2;
+    ws.tiles_cap = n / (n < INIT_SMALL_N ? 512 : 4096) + 2;   // tiles of the initial grouping
     {
         GET(tagg, u64, ws.tiles_cap * 2);
         ws.tile_agg = tagg;
@@ -972,6 +972,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     }
 
     int alt = 0;
+    bool a_known = false;   // the first list's length is on the host already
     RawKey1 rk1{};   // first round from the code stream (when there is one)
     ExactLcp ex1{};  // and its exact lcps across the splits
     if (!sid) {
@@ -1015,7 +1016,9 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             const u64 cnt = n < (u64)ksym ? n : (u64)ksym;
             u32 *hs = (u32 *)(c.pinned + 6144);   // past the histogram and LUT staging
             SAGPU_CHECK(hipMemcpyAsync(hs, dISA + (n - cnt2), cnt2 * sizeof(u32), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));   // the list length
             SAGPU_CHECK(hipStreamSynchronize(s));
+            a_known = true;
             rk1.pw = gen.pw;
             rk1.bits = bits;
             rk1.kbits = ksym * bits;
@@ -1046,9 +1049,21 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                  (u32 *)(alt ? k1 : k0), (u32 *)(alt ? k1 : k0) + n));
     }
     u64 a = 0;
-    SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));
-    SAGPU_CHECK(hipStreamSynchronize(s));
-    a = c.pinned[0];
+    // Short texts without a code-stream round skip the wait for the first list's
+    // length: the first round is launched for n entries and reads the length from
+    // the device (its kernels stop at the list's end); the host learns it with the
+    // round's counters.
+    const bool lazy_a = !a_known && n < INIT_SMALL_N && !rk1.pw;
+    u64 *h_a1 = c.pinned + 7204;   // past the round parameters and the pivot regions
+    if (lazy_a) {
+        a = n;
+    } else {
+        if (!a_known) {
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, ws.total, sizeof(u64), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipStreamSynchronize(s));
+        }
+        a = c.pinned[0];
+    }
 
     const ActCode ac{cb, cb ? seed->codes : nullptr};   // list entries carry BWT codes when seeded
     u64 h = (u64)ksym;
@@ -1082,9 +1097,11 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         h_rp[0] = a;
         h_rp[1] = h;
         const bool raw = st.rounds == 1 && rk1.pw;
+        const bool lazy_first = lazy_a && st.rounds == 1;
         // one round's work on the stream (launch geometry for a_grid entries)
         auto enqueue = [&](u64 a_grid, bool timed) -> int {
             SAGPU_CHECK(hipMemcpyAsync(d_rp, h_rp, 2 * sizeof(u64), hipMemcpyHostToDevice, s));
+            if (lazy_first) SAGPU_CHECK(hipMemcpyAsync(d_rp, ws.total, sizeof(u64), hipMemcpyDeviceToDevice, s));
             if (timed) ktime_begin(KI_ROUND_GATHER);
             SAGPU_TRY(round_gather(act_i[cur], act_g[cur], a_grid, d_rp, dISA, n, key2, wide, s, raw ? &rk1 : nullptr,
                                    ac.cb));
@@ -1098,6 +1115,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
                                  raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr));
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
+            if (lazy_first) SAGPU_CHECK(hipMemcpyAsync(h_a1, d_rp, sizeof(u64), hipMemcpyDeviceToHost, s));
             return SAGPU_OK;
         };
         {
@@ -1157,6 +1175,14 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             }
         }
         SAGPU_CHECK(hipStreamSynchronize(s));
+        if (lazy_first) {
+            a = *h_a1;
+            st.active[0] = (int64_t)a;
+            if (a == 0) {   // nothing was grouped: no round ran
+                st.rounds = 0;
+                break;
+            }
+        }
         u64 out_n = c.pinned[0];
         const u32 nb = ((u32 *)(c.pinned + 4))[0];
         if (nb > big_cap || ((u32 *)(c.pinned + 4))[1] != 0 || out_n > a) {

@@ -346,15 +346,19 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
                                                         VT *__restrict__ V, u32 *__restrict__ queue,
                                                         u32 *__restrict__ queue_l, u32 *__restrict__ queue_n,
                                                         u64 queue_cap, const u32 *__restrict__ pw, int bits,
-                                                        VT *__restrict__ tmax, int hint_off)
+                                                        VT *__restrict__ tmax, int hint_off, int split)
 {
+    // split > 1 (short texts): a tile's steps go to `split` workgroups, which raise
+    // the tile's maximum (zeroed first) with atomics instead of storing it
     constexpr u64 STEP = 256ull * PI;
     static_assert(MS_TILE % STEP == 0, "tile must be whole steps");
     __shared__ VT s_m[4];
     const u64 tiles = (n + MS_TILE - 1) / MS_TILE;
-    for (u64 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const u64 part_len = MS_TILE / (u64)split;
+    for (u64 wi = blockIdx.x; wi < tiles * (u64)split; wi += gridDim.x) {
+        const u64 tile = wi / (u64)split, p0 = tile * MS_TILE + (wi % (u64)split) * part_len;
         VT mx = 0;
-        for (u64 base = tile * MS_TILE; base < (tile + 1) * MS_TILE && base < n; base += STEP) {
+        for (u64 base = p0; base < p0 + part_len && base < n; base += STEP) {
             u32 r[PI], hint[PI];
             u64 w[PI];
             uint8_t a[PI];
@@ -410,7 +414,11 @@ __global__ __launch_bounds__(256) void k_plcp_irr_pairs(const uint8_t *__restric
         for (int d = 32; d >= 1; d >>= 1) mx = vmax(mx, (VT)__shfl_xor(mx, d, SAGPU_WAVE));
         if (lane_id() == 0) s_m[threadIdx.x / SAGPU_WAVE] = mx;
         __syncthreads();
-        if (threadIdx.x == 0) tmax[tile] = vmax(vmax(s_m[0], s_m[1]), vmax(s_m[2], s_m[3]));
+        if (threadIdx.x == 0) {
+            const VT m = vmax(vmax(s_m[0], s_m[1]), vmax(s_m[2], s_m[3]));
+            if (split == 1) tmax[tile] = m;
+            else atomicMax(&tmax[tile], m);
+        }
         __syncthreads();
     }
 }
@@ -781,10 +789,14 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
     const bool fused_max = pairs && W == 1;   // the pair-word pass writes the tile maxima itself
     if (fused_max) {
         if constexpr (W == 1) {
+            // short texts: one 1024-position step per workgroup
+            const int split = n < (1ull << 20) ? (int)(MS_TILE / (256 * PI)) : 1;
+            if (split > 1) SAGPU_TRY(launch_fill_u32((u32 *)tmax, tiles * sizeof(VT) / 4, 0, s));
             if (ls.hook) ls.hook(ls.hook_user, 0);
-            hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3((unsigned)(tiles < 65535 ? tiles : 65535)), dim3(256), 0, s,
-                               T, n, gsa, ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap,
-                               gsa ? nullptr : pw, bits, tmax, hint_off);
+            const u64 wg = tiles * (u64)split;
+            hipLaunchKernelGGL((k_plcp_irr_pairs<VT>), dim3((unsigned)(wg < 65535 ? wg : 65535)), dim3(256), 0, s, T, n,
+                               gsa, ISA, pairs, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap,
+                               gsa ? nullptr : pw, bits, tmax, hint_off, split);
             if (ls.hook) ls.hook(ls.hook_user, 1);
         }
     } else {

@@ -259,24 +259,26 @@ struct InitF {
     }
 };
 
-// The keys of one wave's 1024-element span (16 rows of 64) in registers, and the
-// keys just before and after the span.  Neighbours come from lane shuffles, so
-// every key is loaded once and all loads of a span are in flight together.
-template <typename K>
+// The keys of one wave's span (R rows of 64: 16, or 2 for short texts) in
+// registers, and the keys just before and after the span.  Neighbours come from
+// lane shuffles, so every key is loaded once and all loads of a span are in flight
+// together.
+template <typename K, int R>
 struct SpanKeys {
-    K k[WS_ROWS];
+    static constexpr int SPAN = R * SAGPU_WAVE;
+    K k[R];
     K before, after;
 
     __device__ void load(const K *keys, u64 n, u64 wbase)
     {
         const u32 lane = lane_id();
 #pragma unroll
-        for (int r = 0; r < WS_ROWS; r++) {
+        for (int r = 0; r < R; r++) {
             const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
             k[r] = e < n ? keys[e] : K(0);
         }
         before = (wbase > 0 && wbase - 1 < n) ? keys[wbase - 1] : K(0);
-        after = wbase + WS_SPAN < n ? keys[wbase + WS_SPAN] : K(0);
+        after = wbase + SPAN < n ? keys[wbase + SPAN] : K(0);
     }
     // Left and right neighbour keys of row r (r is a compile-time constant in the
     // unrolled callers).
@@ -287,26 +289,27 @@ struct SpanKeys {
         const K last = r > 0 ? __shfl(k[r - 1], SAGPU_WAVE - 1, SAGPU_WAVE) : before;
         kp = lane ? up : last;
         const K down = __shfl_down(k[r], 1, SAGPU_WAVE);
-        const K first = r + 1 < WS_ROWS ? __shfl(k[r + 1], 0, SAGPU_WAVE) : after;
+        const K first = r + 1 < R ? __shfl(k[r + 1 < R ? r + 1 : r], 0, SAGPU_WAVE) : after;
         kn = lane + 1 < SAGPU_WAVE ? down : first;
     }
 };
 
-// Initial grouping, pass 1: per tile, the number of grouped elements and the
-// position of the last group start.
-template <typename K>
+// Initial grouping, pass 1: per tile (4 spans of R rows), the number of grouped
+// elements and the position of the last group start.
+template <typename K, int R>
 __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__restrict__ tagg)
 {
+    constexpr int SPAN = R * SAGPU_WAVE;
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
-    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    const u64 wbase = (u64)blockIdx.x * (WS_WAVES * SPAN) + (u64)w * SPAN;
     __shared__ u32 s_cnt[IB_MAX];
     if (threadIdx.x < IB_MAX) s_cnt[threadIdx.x] = 0;
-    SpanKeys<K> sk;
+    SpanKeys<K, R> sk;
     sk.load(f.keys, f.n, wbase);
-    u32 vv[WS_ROWS];
+    u32 vv[R];
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         vv[r] = e < f.n ? f.vals[e] : 0u;
     }
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__r
     u64 s = 0;
     u32 m1 = 0;
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         K kp, kn;
         sk.neighbours(r, kp, kn);
@@ -347,33 +350,34 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_reduce(InitF<K> f, Agg *__r
 // Initial grouping, pass 2 (after k_ws_scan_tiles): singletons into SA, grouped
 // elements into the active list, LCP marks, BWT seeds, and the (suffix, group head)
 // pairs into their ISA buckets.
-template <typename K>
+template <typename K, int R>
 __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg *__restrict__ texcl)
 {
+    constexpr int SPAN = R * SAGPU_WAVE;
     __shared__ Agg s_w[WS_WAVES];
     __shared__ u32 s_cnt[IB_MAX];
     __shared__ u64 s_base[IB_MAX];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
-    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    const u64 wbase = (u64)blockIdx.x * (WS_WAVES * SPAN) + (u64)w * SPAN;
     const u64 n = f.n;
     const u32 nb = (u32)(((n - 1) >> f.wlog) + 1);
     if (threadIdx.x < IB_MAX) s_cnt[threadIdx.x] = 0;
-    SpanKeys<K> sk;
+    SpanKeys<K, R> sk;
     sk.load(f.keys, n, wbase);
-    u32 vv[WS_ROWS];
+    u32 vv[R];
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         vv[r] = e < n ? f.vals[e] : 0u;
     }
     __syncthreads();
     // st[r]: bit 31 grouped, bit 30 group start, low bits the rank inside the
     // workgroup's share of the element's ISA bucket
-    u32 st[WS_ROWS];
+    u32 st[R];
     u64 s = 0;
     u32 m1 = 0;
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         K kp, kn;
         sk.neighbours(r, kp, kn);
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_init_emit(InitF<K> f, const Agg 
     __syncthreads();
     const u32 cmask = (1u << f.cb) - 1u;
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         const bool act = st[r] >> 31;
         const u32 is = wave_incl_sum((u32)act);
@@ -917,7 +921,10 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
                hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, 0u, cb,
                seed ? seed->codes : nullptr};
     for (int i = 0; i < ksym; i++) f.lowbits |= 1u << (bits * i);
-    const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
+    // short texts: 512-position tiles (2 rows per wave), so more workgroups share them
+    const bool small = n < INIT_SMALL_N;
+    const u64 tile = small ? (u64)WS_WAVES * 2 * SAGPU_WAVE : (u64)WS_TILE;
+    const u64 tiles = (n + tile - 1) / tile;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
@@ -928,11 +935,16 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
         return SAGPU_EDEVICE;
     }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
-    hipLaunchKernelGGL(k_init_reduce<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    if (small) hipLaunchKernelGGL((k_init_reduce<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    else hipLaunchKernelGGL((k_init_reduce<K, WS_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
     SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, nb * tiles, s));
     ktime_begin(KI_INIT_EMIT);
-    hipLaunchKernelGGL(k_init_emit<K>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
+    if (small)
+        hipLaunchKernelGGL((k_init_emit<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
+    else
+        hipLaunchKernelGGL((k_init_emit<K, WS_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f,
+                           (const Agg *)tagg);
     // key and value read; SA, the (suffix, rank) pair, LCP mark and BWT code written
     // per position (the active-list entries are not counted)
     ktime_end(KI_INIT_EMIT, n * (sizeof(K) + 4 + 4 + 8 + (LCP ? 4 : 0) + (f.codes ? 1 : 0)));
