@@ -254,6 +254,8 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *   "windowed_scatter" 1 = texts over 2^22 write the initial ranks through a radix pass into
  *                    windows of the ISA and a scatter in which each XCD writes only its own windows
  *                    (default); 0 = one direct scatter
+ *   "radix_small_n"  radix sorts of fewer keys use 2048-key tiles and run every pass without
+ *                    waiting for the histograms on the host (default 2^20; 0 = never)
  *   "big_pivot_min"  rounds after the first whose big groups hold at least this many members in
  *                    all take up to 4 keys frequent in a 64-member sample of each group out of the
  *                    sort: those members become subgroups by a stable partition (default 2^20;
