@@ -139,6 +139,18 @@ struct BwtSeed {
 struct ActCode {
     int cb;
     uint8_t *codes;
+    // Group-word flag (0x80000000 for texts of at most 2^31 suffixes, else 0).  A
+    // group's members share one rank in ISA, any slot of the group's range: its
+    // first slot g, or, where the group word (act_g, g | flag) carries the flag, the
+    // value stored in SA[g + 1] (a group of two or more members has no suffix
+    // placed in its slots yet).  When a group splits, the subgroup whose slots hold
+    // that rank keeps it and the flag (storing it at its own first slot + 1), so
+    // its members' ISA entries are not rewritten; the others get new ranks.  A
+    // pivot subgroup (the few-key partition of big groups) takes the middle of its
+    // slots, flagged: on texts of long repeats it holds nearly all of the next
+    // round's group, whose surviving subgroup then keeps that rank round after
+    // round.  Other new ranks are first slots, unflagged.
+    u32 gf = 0;
 };
 // PV, PG: scratch of n entries each ((suffix, group head) pairs bucketed for the
 // ISA scatter).
@@ -194,12 +206,16 @@ int round_gather(const u32 *act_i, const u32 *act_g, u64 a_grid, const u64 *rp, 
                  int wide, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
 // The tile kernel's survivors go to 8 per-XCD segments of seg_i / seg_g (capacity
 // 8 * round_seg_cap(a_grid) each) and are concatenated into out_i / out_g, whose
-// length goes to *out_counter; seg_ctr: 128 u64.
+// length goes to *out_counter; seg_ctr: 128 u64.  Big-group heads go the same way
+// to 8 segments of bseg_h / bseg_g (round_bigseg_cap(n) entries each, n >= every
+// list length) and are concatenated into big_head / big_g / big_size, their count
+// to *big_counter.
 u64 round_seg_cap(u64 a);
+u64 round_bigseg_cap(u64 n);
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a_grid, const u64 *rp, u32 *SA,
                u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
-               u32 *seg_i, u32 *seg_g, u64 *seg_ctr);
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap);
 // ISA[PV[e]] = PG[e] for e < m, in the pairs' order.
 int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_t s);
 // The same for pairs grouped into windows of the target by a radix pass on bits
@@ -215,6 +231,9 @@ int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, 
 // Few-key partition of the big groups (pivots.hip): up to BP key2 values frequent
 // in a sample of each group become subgroups without sorting; the rest is sorted.
 enum { BP = 4 };
+// keepkey of a flagged big group whose rank lies in no subgroup of its sorted members
+// (key2 values are below 2^34)
+constexpr u64 BIG_NOKEY = ~0ull;
 struct PivRegions {
     u64 start[BP + 1];   // pivot class b: [start[b], start[b + 1]); start[BP] = pivot members
 };
@@ -225,7 +244,13 @@ struct BigPivots {
 };
 int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
              u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
-             const ExactLcp *ex = nullptr, const BigPivots *pv = nullptr);
+             const u32 *grank, const u64 *keepkey, const ExactLcp *ex = nullptr, const BigPivots *pv = nullptr);
+// For the flagged big groups of a round without pivot classes: grank[q] = the rank
+// the members hold (SA[g + 1]), keepkey[q] = the key2 of the subgroup whose slots
+// hold it (group q's members are sorted[boff[q], boff[q + 1])).  Pivot rounds get
+// both from big_piv_ranks.
+int big_keep(const void *sorted, int comp32, const u64 *boff, int kb, u32 nb, const u32 *gslot, u32 gf,
+             const u32 *SA, u32 *grank, u64 *keepkey, hipStream_t s);
 u64 big_part_tiles(u64 total);
 int big_pivots(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 *piv, u32 *npiv,
                hipStream_t s);
@@ -234,11 +259,14 @@ int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff
 int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb,
                      u64 total, int kb, int comp32, const u64 *piv, const u32 *npiv, const u32 *cscan, u64 npivot,
                      void *comp, u32 *vals, u32 *pq, u32 *ppv, hipStream_t s);
+// rk, rv: nb * BP entries (the pivot subgroups' group words and ranks); grank,
+// keepkey: as big_keep, for the flagged groups of a pivot round.
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
-                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 nb, u32 *eq, u32 *rk, u64 *out_counter,
-                  u64 *pbase, u32 *err, hipStream_t s);
-int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *gslot,
-                 const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s);
+                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
+                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 *pbase, u32 *err, hipStream_t s);
+int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *rv,
+                 const u32 *gslot, u32 gf, const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP,
+                 int cb, hipStream_t s);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);

@@ -285,7 +285,7 @@ constexpr int F_KERNEL0 = 64;   // + KernelId: one kernel's launches
 
 // Host copies of one round's big-group list (sa_build), kept with the context so
 // that their capacity is reused.
-constexpr int ROUND_SIG = 26;   // words of a round's launch signature (Ctx::round_sig)
+constexpr int ROUND_SIG = 30;   // words of a round's launch signature (Ctx::round_sig)
 
 struct BigHost {
     std::vector<u32> head, slot, size, order, shead, sslot;
@@ -367,6 +367,10 @@ int g_windowed_scatter = 1;
 // Big groups of rounds after the first take up to BP frequent keys per group out of
 // the sort when they hold at least this many members in all (0: never)
 int g_big_pivot_min = 1 << 20;
+// 1: texts of at most 2^31 suffixes flag group words whose members rank at the
+// group's last slot, so the subgroup holding that slot keeps its ISA entries when
+// the group splits; 0: every group ranks at its first slot
+int g_group_flags = 1;
 // radix sorts of fewer keys use the smallest tiles (2048 keys), whatever radix_items
 // and radix_threads say
 int g_radix_small_n = 1 << 20;
@@ -949,6 +953,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     GET(d_boff, u64, big_cap + 1);
     GET(d_bhead, u32, big_cap);
     GET(d_bslot, u32, big_cap);
+    // the tile kernel's big heads, in one segment per XCD before k_big_sizes joins them
+    const u64 bseg_cap = round_bigseg_cap(n);
+    GET(bseg_h, u32, 8 * bseg_cap);
+    GET(bseg_g, u32, 8 * bseg_cap);
     u64 *out_counter = d_cnt;
     u32 *big_counter = (u32 *)(d_cnt + 4), *err = (u32 *)(d_cnt + 4) + 1;
     u32 *act_i[2] = {aA, kA}, *act_g[2] = {aA + n, kA + n};
@@ -1065,7 +1073,9 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         a = c.pinned[0];
     }
 
-    const ActCode ac{cb, cb ? seed->codes : nullptr};   // list entries carry BWT codes when seeded
+    // list entries carry BWT codes when seeded; group words carry the last-slot flag
+    // when every slot fits 31 bits
+    const ActCode ac{cb, cb ? seed->codes : nullptr, g_group_flags && n <= (1ull << 31) ? 0x80000000u : 0u};
     u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
@@ -1113,7 +1123,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             // round is captured)
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a_grid, d_rp, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
-                                 raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr));
+                                 raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr, bseg_h, bseg_g, bseg_cap));
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             if (lazy_first) SAGPU_CHECK(hipMemcpyAsync(h_a1, d_rp, sizeof(u64), hipMemcpyDeviceToHost, s));
             return SAGPU_OK;
@@ -1127,7 +1137,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                             (u64)key2, (u64)dISA, (u64)dSA, (u64)dLCP, (u64)d_cnt, (u64)big_head,
                                             (u64)big_g, (u64)big_size, (u64)ac.codes, (u64)d_rp, (u64)c.pinned,
                                             (u64)seg_i, (u64)seg_g, (u64)seg_ctr, (u64)ac.cb, a, n, (u64)wide,
-                                            (u64)g_tile_range, big_cap, (u64)(dLCP != nullptr), (u64)exec};
+                                            (u64)g_tile_range, big_cap, (u64)(dLCP != nullptr), (u64)bseg_h,
+                                            (u64)bseg_g, bseg_cap, (u64)ac.gf, (u64)exec};
                 if (!captured[cur] && exec && !memcmp(sig, c.round_sig[cur], sizeof sig)) captured[cur] = true;
                 if (!captured[cur]) {
                     // capture this parity's round once per build; the executable graph of
@@ -1238,14 +1249,24 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             BigPivots bp{};
             PivRegions reg{};
             u64 nrest = total;
-            u32 *pq = nullptr, *ppv = nullptr, *d_rk = nullptr;
+            u32 *pq = nullptr, *ppv = nullptr, *d_rk = nullptr, *d_rv = nullptr;
             u64 *d_pbase = nullptr;
+            // flagged groups: the rank their members hold and the key of the sorted
+            // members' subgroup whose slots hold it
+            u32 *d_grank = nullptr;
+            u64 *d_keepkey = nullptr;
+            if (ac.gf) {
+                d_grank = rsc.get<u32>(nb);
+                d_keepkey = rsc.get<u64>(nb);
+                if (!d_grank || !d_keepkey) return SAGPU_ENOMEM;
+            }
             if (pivots) {
                 u64 *d_piv = rsc.get<u64>((u64)nb * BP);
                 u32 *d_np = rsc.get<u32>(nb), *d_eq = rsc.get<u32>((u64)nb * BP);
                 d_rk = rsc.get<u32>((u64)nb * BP);
+                d_rv = rsc.get<u32>((u64)nb * BP);
                 d_pbase = rsc.get<u64>(1);
-                if (!d_piv || !d_np || !d_eq || !d_rk || !d_pbase) return SAGPU_ENOMEM;
+                if (!d_piv || !d_np || !d_eq || !d_rk || !d_rv || !d_pbase) return SAGPU_ENOMEM;
                 const u64 ptiles = big_part_tiles(total);
                 {
                     Span sp(c, F_ROUNDS);
@@ -1288,15 +1309,19 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             {
                 Span sp(c, F_ROUNDS);
                 if (pivots)
-                    SAGPU_TRY(big_piv_ranks(alt ? b1 : b0, comp32, kb, nrest, pq, reg, bp.piv, bp.npiv, d_bslot, nb,
-                                            (u32 *)bp.eq, d_rk, out_counter, d_pbase, err, s));
+                    SAGPU_TRY(big_piv_ranks(alt ? b1 : b0, comp32, kb, nrest, pq, reg, bp.piv, bp.npiv, d_bslot, ac.gf,
+                                            dSA, d_grank, d_keepkey, nb, (u32 *)bp.eq, d_rk, d_rv, out_counter,
+                                            d_pbase, err, s));
+                else
+                    SAGPU_TRY(big_keep(alt ? b1 : b0, comp32, d_boff, kb, nb, d_bslot, ac.gf, dSA, d_grank, d_keepkey,
+                                       s));
                 if (nrest > 0)
                     SAGPU_TRY(big_emit(ws, alt ? b1 : b0, comp32, kb, alt ? bv1 : bv0, d_bslot, nrest, h, dSA, dISA,
-                                       act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac,
+                                       act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac, d_grank, d_keepkey,
                                        st.rounds == 1 && rk1.pw ? &ex1 : nullptr, pivots ? &bp : nullptr));
                 if (pivots)
-                    SAGPU_TRY(big_piv_emit(pq, ppv, reg, d_rk, d_bslot, d_pbase, h, dISA, act_i[nxt], act_g[nxt],
-                                           dLCP, ac.cb, s));
+                    SAGPU_TRY(big_piv_emit(pq, ppv, reg, d_rk, d_rv, d_bslot, ac.gf, d_pbase, h, dISA, act_i[nxt],
+                                           act_g[nxt], dLCP, ac.cb, s));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
@@ -2265,6 +2290,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "big_pivot_min")) {
         if (value < 0 || value > 0x7FFFFFFF) return SAGPU_EINVAL;
         g_big_pivot_min = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "group_flags")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_group_flags = (int)value;
         return 0;
     }
     if (!strcmp(name, "windowed_scatter")) {

@@ -227,31 +227,72 @@ __device__ u64 lower_bound_dev(const T *__restrict__ a, u64 n, u64 v)
 }
 
 // One thread per big group: members equal to each pivot (from the pivot classes,
-// grouped by dense group index) and the new subgroup's rank.  The first thread
-// reserves the pivot members' entries of the next active list.
+// grouped by dense group index), the new subgroup's word and rank.  The first
+// thread reserves the pivot members' entries of the next active list.
+//   The group's members hold one rank v: its first slot g, or, where the group word
+//   carries the flag gf, the value stored in SA[g + 1].  The subgroup whose slots
+//   hold v keeps it (no ISA writes).  A pivot subgroup that does not takes the
+//   middle of its slots as its rank when flags are on (flagged, stored at its first
+//   slot + 1): on texts of long repeats the pivot subgroup of one round holds
+//   nearly all of the next round's group, and so keeps the middle of it.
+//   rk[q, b] = the subgroup's first slot | flag, its group word in the next list;
+//   rv[q, b] = the rank its members take, PIV_KEEP where it keeps the group's.
+//   grank[q], keepkey[q] (flagged groups) = v and the key2 of the sorted rest's
+//   subgroup whose slots hold v (BIG_NOKEY if v is a pivot subgroup's).
+constexpr u32 PIV_KEEP = 0xFFFFFFFFu;   // a pivot subgroup's first slot is at most n - 2
 template <typename CT>
 __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, u64 nrest, int kb,
                                                    const u32 *__restrict__ pq, PivRegions reg,
                                                    const u64 *__restrict__ piv, const u32 *__restrict__ npiv,
-                                                   const u32 *__restrict__ gslot, u32 nb, u32 *__restrict__ eq,
-                                                   u32 *__restrict__ rk, u64 *__restrict__ out_counter,
-                                                   u64 *__restrict__ pbase, u32 *__restrict__ err)
+                                                   const u32 *__restrict__ gslot, u32 gf, u32 *__restrict__ SA,
+                                                   u32 *__restrict__ grank, u64 *__restrict__ keepkey, u32 nb,
+                                                   u32 *__restrict__ eq, u32 *__restrict__ rk, u32 *__restrict__ rv,
+                                                   u64 *__restrict__ out_counter, u64 *__restrict__ pbase,
+                                                   u32 *__restrict__ err)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *pbase = atomicAdd(out_counter, reg.start[BP]);
     const u32 q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nb) return;
     const u32 np = npiv[q];
+    const u32 gw = gslot[q], g = gw & ~gf;
+    const bool flagged = (gw & gf) != 0;
+    const u32 v = flagged ? SA[g + 1] : g;   // read before any store below
+    const u64 vr = (u64)(v - g);             // the rank's position among the group's slots
     const u64 g0 = (u64)q << kb;
     const u64 rest0 = lower_bound_dev(rest, nrest, g0);
-    u64 below = 0;   // pivot members of smaller keys
+    u64 below = 0;        // pivot members of smaller keys
+    u64 piv_before = 0;   // pivot members whose slots all lie before position vr
+    bool in_pivot = false;
     for (u32 b = 0; b < np; b++) {
-        const u64 lt = lower_bound_dev(rest, nrest, g0 | piv[(u64)q * BP + b]) - rest0;
+        const u64 pk = piv[(u64)q * BP + b];
+        const u64 lt = lower_bound_dev(rest, nrest, g0 | pk) - rest0;
         const u64 r0 = reg.start[b], len = reg.start[b + 1] - r0;
         const u64 c = lower_bound_dev(pq + r0, len, (u64)q + 1) - lower_bound_dev(pq + r0, len, (u64)q);
         if (c < 2) atomicOr(err, 4u);   // a pivot is sampled twice, so it has two members
+        const u64 pos = lt + below;     // the subgroup's first position among the group's slots
+        const u32 first = g + (u32)pos;
+        const bool kept = vr >= pos && vr < pos + c;
+        in_pivot = in_pivot || kept;
+        if (pos + c <= vr) piv_before += c;
         eq[(u64)q * BP + b] = (u32)c;
-        rk[(u64)q * BP + b] = gslot[q] + (u32)(lt + below);
+        if (kept) {
+            rk[(u64)q * BP + b] = first | (gw & gf);
+            rv[(u64)q * BP + b] = PIV_KEEP;
+            if (flagged && first != g && c >= 2) SA[first + 1] = v;
+        } else {
+            const u32 mid = first + (u32)((c - 1) / 2);
+            rk[(u64)q * BP + b] = first | (c >= 2 ? gf : 0u);
+            rv[(u64)q * BP + b] = gf && c >= 2 ? mid : first;
+            if (gf && c >= 2) SA[first + 1] = mid;
+        }
         below += c;
+    }
+    if (flagged) {
+        grank[q] = v;
+        // v among the sorted rest: its position minus the pivot members before it
+        const u64 rest1 = lower_bound_dev(rest, nrest, g0 + (1ull << kb));
+        const u64 t = rest0 + (vr - piv_before);
+        keepkey[q] = in_pivot || vr < piv_before || t >= rest1 ? BIG_NOKEY : (u64)rest[t] & ((1ull << kb) - 1);
     }
 }
 
@@ -260,9 +301,10 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
 // change, and the split's LCP mark at the subgroup's first slot.
 __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, const u32 *__restrict__ ppv,
                                                   PivRegions reg, const u32 *__restrict__ rk,
-                                                  const u32 *__restrict__ gslot, const u64 *__restrict__ pbase, u32 h,
-                                                  u32 *__restrict__ ISA, u32 *__restrict__ out_i,
-                                                  u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb)
+                                                  const u32 *__restrict__ rv, const u32 *__restrict__ gslot, u32 gf,
+                                                  const u64 *__restrict__ pbase, u32 h, u32 *__restrict__ ISA,
+                                                  u32 *__restrict__ out_i, u32 *__restrict__ out_g,
+                                                  u32 *__restrict__ LCP, int cb)
 {
     const u64 npv = reg.start[BP];
     const u64 base = *pbase;
@@ -272,11 +314,12 @@ __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, co
 #pragma unroll
         for (int t = 1; t < BP; t++) b += m >= reg.start[t];
         const u32 q = pq[m];
-        const u32 r = rk[(u64)q * BP + b], g = gslot[q];
+        const u32 w = rk[(u64)q * BP + b], r = w & ~gf, g = gslot[q] & ~gf;
+        const u32 nv = rv[(u64)q * BP + b];
         const u32 pv = ppv[m];
-        if (r != g) ISA[pv >> cb] = r;
+        if (nv != PIV_KEEP) ISA[pv >> cb] = nv;
         out_i[base + m] = pv;
-        out_g[base + m] = r;
+        out_g[base + m] = w;
         if (LCP && r != g && (m == reg.start[b] || pq[m - 1] != q))
             LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
     }
@@ -347,28 +390,33 @@ int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bh
 }
 
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
-                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 nb, u32 *eq, u32 *rk, u64 *out_counter,
-                  u64 *pbase, u32 *err, hipStream_t s)
+                  const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
+                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 *pbase, u32 *err, hipStream_t s)
 {
+    if (gf && (!grank || !keepkey)) {
+        sagpu_set_error("flagged groups need their ranks and kept keys", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
     const unsigned grid = (nb + 255) / 256 > 0 ? (nb + 255) / 256 : 1;
     if (comp32)
         hipLaunchKernelGGL(k_piv_ranks<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, nb, eq, rk, out_counter, pbase, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, pbase, err);
     else
         hipLaunchKernelGGL(k_piv_ranks<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, nb, eq, rk, out_counter, pbase, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, pbase, err);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *gslot,
-                 const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s)
+int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *rv,
+                 const u32 *gslot, u32 gf, const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP,
+                 int cb, hipStream_t s)
 {
     const u64 npv = reg.start[BP];
     if (npv == 0) return SAGPU_OK;
     const u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, ppv, reg, rk, gslot,
-                       pbase, hh, ISA, out_i, out_g, LCP, cb);
+    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, ppv, reg, rk, rv, gslot,
+                       gf, pbase, hh, ISA, out_i, out_g, LCP, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

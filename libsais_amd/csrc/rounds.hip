@@ -480,7 +480,7 @@ template <typename CT>
 struct BigF {
     const CT *comp;      // (dense big-group index << kb) | key2, sorted
     const u32 *vals;     // list entry: suffix index, or (suffix << cb) | BWT code
-    const u32 *gslot;    // first SA slot of each big group
+    const u32 *gslot;    // group word of each big group: first SA slot | flag
     u64 n;
     u32 h;
     int kb;              // key2 width
@@ -488,6 +488,10 @@ struct BigF {
     ActCode ac;
     ExactLcp ex;
     BigPivots bp;        // the pivot classes taken out of the sort (bp.eq null: none)
+    // flagged groups: the rank their members hold, and the key2 of the sorted
+    // members' subgroup whose slots hold it (BIG_NOKEY: none of them, or a pivot's)
+    const u32 *grank;
+    const u64 *keepkey;
 
     __device__ void scan_inputs(u64 e, bool &act, u32 &v1, u32 &v2) const
     {
@@ -513,19 +517,25 @@ struct BigF {
         const u32 v = pv >> ac.cb;
         const u64 c = comp[e];
         const u32 q = (u32)(c >> kb);
-        const u32 g = gslot[q];
+        const u32 gw = gslot[q];
+        const u32 g = gw & ~ac.gf;
+        const u64 key = c & ((1ull << kb) - 1);
         u32 r = g + (hstart - gstart);
         if (bp.eq) {
             // members of the group's pivot classes with smaller keys rank below
-            const u64 key = c & ((1ull << kb) - 1);
             const u32 np = bp.npiv[q];
             for (u32 b = 0; b < np; b++)
                 if (bp.piv[(u64)q * BP + b] < key) r += bp.eq[(u64)q * BP + b];
         }
-        if (r != g) ISA[v] = r;  // the first subgroup keeps the group's rank
+        // the subgroup whose slots hold the group's rank keeps it (and the flag, its
+        // rank then stored at its first slot + 1); new ranks here are first slots
+        const bool flagged = (gw & ac.gf) != 0;
+        const bool kept = flagged ? key == keepkey[q] : r == g;
+        if (!kept) ISA[v] = r;
         if (act) {
             out_i[o] = pv;
-            out_g[o] = r;
+            out_g[o] = kept ? r | (gw & ac.gf) : r;
+            if (kept && flagged && r != g && hstart == (u32)e) SA[r + 1] = grank[q];
         } else {
             SA[r] = v;
             if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
@@ -625,16 +635,33 @@ constexpr int RT_THREADS = 256;
 // big list, and the workgroup holding a big group's last entry writes that entry's
 // list index to SA[g] (the group's SA slots are free until its suffixes are
 // placed, and no workgroup places suffixes of a big group here), from which
-// k_big_sizes takes the size after this kernel.
+// k_big_sizes takes the size after this kernel.  Big heads are reserved on the
+// counter of the workgroup's XCD (seg_ctr[16 x + 8]) in that XCD's segment of
+// (bseg_h, bseg_g): on texts of long repeats a round can list 10^5 big groups, and
+// as many returning atomics on one address serialise for milliseconds.
+template <int RANGE>
+__device__ __forceinline__ void rt_big_head(u64 head, u32 g, u64 *__restrict__ seg_ctr, u32 *__restrict__ bseg_h,
+                                            u32 *__restrict__ bseg_g, u64 bseg_cap, u32 *__restrict__ err)
+{
+    const u32 x = blockIdx.x & 7u;
+    const u64 slot = atomicAdd(&seg_ctr[x * 16 + 8], 1ull);
+    if (slot < bseg_cap) {
+        bseg_h[(u64)x * bseg_cap + slot] = (u32)head;
+        bseg_g[(u64)x * bseg_cap + slot] = g;
+    } else {
+        atomicOr(err, 1u);
+    }
+}
+
 template <typename KT, int RANGE>
 __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict__ act_i, const u32 *__restrict__ act_g,
                                                            const KT *__restrict__ key2, const u64 *__restrict__ rp,
                                                            u32 *__restrict__ SA, u32 *__restrict__ ISA,
                                                            u32 *__restrict__ seg_i, u32 *__restrict__ seg_g,
                                                            u64 *__restrict__ seg_ctr, u64 seg_cap,
-                                                           u32 *__restrict__ err, u32 *__restrict__ big_head,
-                                                           u32 *__restrict__ big_g, u32 *__restrict__ big_counter,
-                                                           u64 big_cap, u32 *__restrict__ LCP, ActCode ac, ExactLcp ex)
+                                                           u32 *__restrict__ err, u32 *__restrict__ bseg_h,
+                                                           u32 *__restrict__ bseg_g, u64 bseg_cap,
+                                                           u32 *__restrict__ LCP, ActCode ac, ExactLcp ex)
 {
     constexpr int RT_CAP = RANGE + RT_SMALL, RT_PER = RT_CAP / RT_THREADS;
     constexpr int WIN = RANGE + RT_SMALL + 1;                  // act_g window [lo, lo + WIN)
@@ -675,6 +702,15 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     }
     for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE16;
     __syncthreads();
+    // A whole window inside one group (groups are contiguous runs of one act_g
+    // value): no group ends in [lo, hi) and no small group starts there; only a
+    // head at lo, of a group of at least WIN > RT_SMALL members, is recorded.  On
+    // texts of long repeats most windows are of this kind.
+    if (wlen == (u32)WIN && s_w[0] == s_w[WIN - 1]) {
+        if (tid == 0 && (lo == 0 || s_prev != s_w[0]))
+            rt_big_head<RANGE>(lo, s_w[0], seg_ctr, bseg_h, bseg_g, bseg_cap, err);
+        return;
+    }
     {
         u32 loc[W_PER];
         u32 m = 0;
@@ -715,7 +751,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 const u64 e = lo + x;
                 big = e >= (u64)RT_SMALL && act_g[e - RT_SMALL] == s_w[x];
             }
-            if (big) SA[s_w[x]] = (u32)(lo + x);
+            if (big) SA[s_w[x] & ~ac.gf] = (u32)(lo + x);
         }
     }
     __syncthreads();
@@ -730,11 +766,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 f = x < f ? x : f;
                 e = ge + 1 > e ? ge + 1 : e;
             } else {
-                const u32 slot = atomicAdd(big_counter, 1u);
-                if (slot < big_cap) {
-                    big_head[slot] = (u32)(lo + x);
-                    big_g[slot] = s_w[x];
-                }
+                rt_big_head<RANGE>(lo + x, s_w[x], seg_ctr, bseg_h, bseg_g, bseg_cap, err);
             }
         }
         for (int d = 32; d >= 1; d >>= 1) {
@@ -769,7 +801,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         const u32 ge = s_gend[hx];
         if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
         const u32 gsz = ge - hx + 1;
-        const u32 g = s_w[hx];
+        const u32 gw = s_w[hx];         // first slot | flag
+        const u32 g = gw & ~ac.gf;
         const KT key = s_k[x];
         u32 lt = 0, eq = 0, eqb = 0;
         KT pk = 0;   // the largest key below this one: the previous subgroup's
@@ -782,9 +815,16 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
         const u32 pos = hx + lt + eqb;
         const u32 r = g + lt;
+        // the subgroup whose slots hold the group's rank keeps rank and flag (a
+        // flagged group's rank is stored in SA[g + 1]; its other slots, and those of
+        // the next subgroup's first slot + 1, are unplaced until this round's
+        // writes); the others rank at their first slot, unflagged
+        const u32 gv = (gw & ac.gf) ? SA[g + 1] : g;
+        const bool kept = gv - g >= lt && gv - g < lt + eq;
         s_oi[pos] = s_i[x];
-        s_og[pos] = r;
-        s_os[pos] = (eq > 1 ? 2 : 1) | (lt == 0 ? 4 : 0);  // 4: rank unchanged, ISA already holds it
+        s_og[pos] = r | (kept ? (gw & ac.gf) : 0u);
+        s_os[pos] = (uint8_t)((eq > 1 ? 2 : 1) | (kept ? 4 : 0));  // 4: rank unchanged, ISA already holds it
+        if (kept && (gw & ac.gf) && lt > 0 && eq > 1 && eqb == 0) SA[r + 1] = gv;
         if (LCP && lt > 0 && eqb == 0) {
             u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
             if (ex.on && !ex.skipped(g)) mark = h + ex.shared((u64)key, (u64)pk);
@@ -818,10 +858,11 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         const u32 st = s_os[x];
         if (st == 0) continue;
         const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
-        if (!(st & 4)) ISA[v] = s_og[x];
+        const u32 r = s_og[x] & ~ac.gf;
+        if (!(st & 4)) ISA[v] = r;
         if ((st & 3) == 1) {
-            SA[s_og[x]] = v;
-            if (ac.codes) ac.codes[s_og[x]] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
+            SA[r] = v;
+            if (ac.codes) ac.codes[r] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
         }
     }
     __syncthreads();
@@ -863,16 +904,36 @@ __global__ __launch_bounds__(256) void k_seg_gather(const u32 *__restrict__ seg_
     }
 }
 
-// Sizes of the big groups the tile kernel listed: the entry holding a big group's
-// last list index wrote it to SA[g].  Stored minus one (a group may hold 2^32
-// members).
-__global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ big_head, const u32 *__restrict__ big_g,
-                                                   const u32 *__restrict__ big_counter, u64 big_cap,
-                                                   const u32 *__restrict__ SA, u32 *__restrict__ big_size)
+// The big groups the tile kernel listed, its 8 XCD segments (segment x: [x *
+// bseg_cap, x * bseg_cap + seg_ctr[16 x + 8])) concatenated in segment order, with
+// their sizes: the entry holding a big group's last list index wrote it to SA[g].
+// Sizes are stored minus one (a group may hold 2^32 members).  *big_counter = the
+// count (entries past big_cap are not written; the host rejects such a count).
+__global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ bseg_h, const u32 *__restrict__ bseg_g,
+                                                   const u64 *__restrict__ seg_ctr, u64 bseg_cap,
+                                                   u32 *__restrict__ big_head, u32 *__restrict__ big_g,
+                                                   u32 *__restrict__ big_counter, u64 big_cap,
+                                                   const u32 *__restrict__ SA, u32 *__restrict__ big_size, u32 gf)
 {
-    const u64 nb = *big_counter < big_cap ? *big_counter : big_cap;
-    for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (u64)gridDim.x * blockDim.x)
-        big_size[q] = SA[big_g[q]] - big_head[q];
+    u64 pre[9];
+    pre[0] = 0;
+#pragma unroll
+    for (int x = 0; x < 8; x++) {
+        const u64 c = seg_ctr[x * 16 + 8];
+        pre[x + 1] = pre[x] + (c < bseg_cap ? c : bseg_cap);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *big_counter = (u32)(pre[8] < 0xFFFFFFFFull ? pre[8] : 0xFFFFFFFFull);
+    const u64 nb = pre[8] < big_cap ? pre[8] : big_cap;
+    for (u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (u64)gridDim.x * blockDim.x) {
+        int x = 0;
+#pragma unroll
+        for (int t = 1; t < 8; t++) x += q >= pre[t];
+        const u64 src = (u64)x * bseg_cap + (q - pre[x]);
+        const u32 hd = bseg_h[src], g = bseg_g[src];
+        big_head[q] = hd;
+        big_g[q] = g;
+        big_size[q] = SA[g & ~gf] - hd;
+    }
 }
 
 // Copies the members of the big groups (sorted by list position) into the sort
@@ -895,6 +956,25 @@ __global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act
         comp[e] = (CT)(((u64)lo << kb) | (u64)key2[k]);
         vals[e] = act_i[k];
     }
+}
+
+// Flagged big groups without pivot classes (one thread per group): the rank the
+// members hold (stored in SA[g + 1]) and the key2 of the subgroup whose slots hold
+// it, the key of the group's member at sorted position rank - g (group q's members
+// are sorted[boff[q], boff[q + 1]), keyed (dense group index << kb) | key2).
+template <typename CT>
+__global__ __launch_bounds__(256) void k_big_keep(const CT *__restrict__ sorted, const u64 *__restrict__ boff, int kb,
+                                                  u32 nb, const u32 *__restrict__ gslot, u32 gf,
+                                                  const u32 *__restrict__ SA, u32 *__restrict__ grank,
+                                                  u64 *__restrict__ keepkey)
+{
+    const u32 q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb || !(gslot[q] & gf)) return;
+    const u32 g = gslot[q] & ~gf;
+    const u32 v = SA[g + 1];
+    const u64 size = boff[q + 1] - boff[q];
+    grank[q] = v;
+    keepkey[q] = (u64)(v - g) < size ? (u64)sorted[boff[q] + (v - g)] & ((1ull << kb) - 1) : BIG_NOKEY;
 }
 
 unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
@@ -1000,17 +1080,23 @@ int round_gather_t(const u32 *act_i, const u32 *act_g, u64 a, const u64 *rp, con
 // the tiles x, x + 8, ..., each emitting at most RANGE + RT_SMALL survivors.
 u64 round_seg_cap(u64 a) { return (ceil_div_u(a, (u64)g_tile_range) / 8 + 1) * (u64)(g_tile_range + RT_SMALL); }
 
+// Big-head segment capacity of each XCD for lists of at most n entries: heads of
+// groups of more than RT_SMALL = 256 members lie at least 257 entries apart, so a
+// workgroup of RANGE entries holds at most RANGE / 256 of them, and an XCD's
+// ceil(ceil(n / RANGE) / 8) workgroups at most n / 2048 + RANGE / 128.
+u64 round_bigseg_cap(u64 n) { return n / 2048 + 16; }
+
 template <typename KT>
 int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, const u64 *rp, u32 *SA, u32 *ISA,
                  u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                  u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp &ex,
-                 u32 *seg_i, u32 *seg_g, u64 *seg_ctr)
+                 u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap)
 {
     const u64 seg_cap = round_seg_cap(a);
     SAGPU_TRY(launch_fill_u64(seg_ctr, 8 * 16, 0, s));
 #define RT_LAUNCH(R)                                                                                              \
     hipLaunchKernelGGL((k_round_tile<KT, R>), dim3(ceil_div_u(a, R)), dim3(RT_THREADS), 0, s, act_i, act_g, key2, rp, \
-                       SA, ISA, seg_i, seg_g, seg_ctr, seg_cap, err, big_head, big_g, big_counter, big_cap, LCP, ac, ex)
+                       SA, ISA, seg_i, seg_g, seg_ctr, seg_cap, err, bseg_h, bseg_g, bseg_cap, LCP, ac, ex)
     // the kernel alone is timed: list entry and key read (8 + sizeof(KT) B), about
     // 16 B written per entry (ranks, placed suffixes, marks and survivors)
     ktime_begin(KI_ROUND_TILE);
@@ -1019,8 +1105,8 @@ int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, cons
     else RT_LAUNCH(512);
 #undef RT_LAUNCH
     ktime_end(KI_ROUND_TILE, a * (24 + sizeof(KT)));
-    hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, big_head, big_g,
-                       big_counter, big_cap, SA, big_size);
+    hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, bseg_h, bseg_g,
+                       (const u64 *)seg_ctr, bseg_cap, big_head, big_g, big_counter, big_cap, SA, big_size, ac.gf);
     hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, seg_i, seg_g,
                        (const u64 *)seg_ctr, seg_cap, out_i, out_g, out_counter);
     SAGPU_CHECK(hipGetLastError());
@@ -1048,15 +1134,15 @@ int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u64 *rp, const
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, const u64 *rp, u32 *SA,
                u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
-               u32 *seg_i, u32 *seg_g, u64 *seg_ctr)
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap)
 {
     const ExactLcp e = ex ? *ex : ExactLcp{};
     return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
                                     big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
-                                    seg_ctr)
+                                    seg_ctr, bseg_h, bseg_g, bseg_cap)
                 : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
                                     big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
-                                    seg_ctr);
+                                    seg_ctr, bseg_h, bseg_g, bseg_cap);
 }
 
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
@@ -1071,15 +1157,35 @@ int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, 
 
 int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *vals, const u32 *gslot, u64 total, u64 h,
              u32 *SA, u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *LCP, hipStream_t s, const ActCode &ac,
-             const ExactLcp *ex, const BigPivots *pv)
+             const u32 *grank, const u64 *keepkey, const ExactLcp *ex, const BigPivots *pv)
 {
     u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
     const ExactLcp e = ex ? *ex : ExactLcp{};
     const BigPivots p = pv ? *pv : BigPivots{};
+    if (ac.gf && (!grank || !keepkey)) {
+        sagpu_set_error("flagged groups need their ranks and kept keys", __FILE__, __LINE__);
+        return SAGPU_EDEVICE;
+    }
     if (comp32) {
-        BigF<u32> f{(const u32 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p};
+        BigF<u32> f{(const u32 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p, grank,
+                    keepkey};
         return ws_run(ws, f, total, out_counter, s);
     }
-    BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p};
+    BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p, grank, keepkey};
     return ws_run(ws, f, total, out_counter, s);
+}
+
+int big_keep(const void *sorted, int comp32, const u64 *boff, int kb, u32 nb, const u32 *gslot, u32 gf,
+             const u32 *SA, u32 *grank, u64 *keepkey, hipStream_t s)
+{
+    if (nb == 0 || !gf) return SAGPU_OK;
+    const unsigned grid = (nb + 255) / 256;
+    if (comp32)
+        hipLaunchKernelGGL(k_big_keep<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)sorted, boff, kb, nb, gslot, gf,
+                           SA, grank, keepkey);
+    else
+        hipLaunchKernelGGL(k_big_keep<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)sorted, boff, kb, nb, gslot, gf,
+                           SA, grank, keepkey);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
 }

@@ -459,11 +459,13 @@ def _pivot_cases():
 
 @pytest.mark.parametrize("name,T,degenerate", _pivot_cases(), ids=[c[0] for c in _pivot_cases()])
 @pytest.mark.parametrize("wide", [0, 1])
-def test_big_pivots(name, T, degenerate, wide):
+@pytest.mark.parametrize("flags", [1, 0])
+def test_big_pivots(name, T, degenerate, wide, flags):
     """Big groups with few distinct keys: up to 4 keys per group frequent in a sample
     skip the sort (the pivot classes) and the rest is sorted and ranked around them.
     Forced on at every size (big_pivot_min = 1), in the 32-bit and the wide (u64
-    key2) engine, the fused build equals the oracle; on repeat-rich texts the path
+    key2) engine, with group words flagged for last-slot ranks (the default) and
+    without, the fused build equals the oracle; on repeat-rich texts the path
     runs."""
     lib = la.load()
     rc, SA_ref = ol.sa(T)
@@ -473,9 +475,10 @@ def test_big_pivots(name, T, degenerate, wide):
     U_ref, idx_ref = ol.bwt(T, SA_ref)
     try:
         assert lib.sagpu_set_option(b"big_pivot_min", 1) == 0
+        assert lib.sagpu_set_option(b"group_flags", flags) == 0
         if wide:
             assert lib.sagpu_set_option(b"wide_from", 0) == 0
-        print(f"[pivots] {name} n={len(T)} wide={wide}", flush=True)
+        print(f"[pivots] {name} n={len(T)} wide={wide} flags={flags}", flush=True)
         SA, U, r, P, L, f = fused(T)
         st = la.stats()
         print(f"[pivots] {name}: rounds {st.rounds}, big rounds {st.big_rounds}, pivot rounds "
@@ -487,7 +490,28 @@ def test_big_pivots(name, T, degenerate, wide):
             assert st.big_pivot_rounds > 0 and st.big_pivot_elems > 0, name
     finally:
         assert lib.sagpu_set_option(b"big_pivot_min", 1 << 20) == 0
+        assert lib.sagpu_set_option(b"group_flags", 1) == 0
         assert lib.sagpu_set_option(b"wide_from", 0x7FFFFFFF) == 0
+
+
+@pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana", "acgt17"])
+def test_group_flags_off(name):
+    """Every group ranked at its first slot (group_flags = 0): the fused build
+    equals the oracle as with the default last-slot ranks of flagged groups."""
+    lib = la.load()
+    T = dict(CASES)[name]
+    rc, SA_ref = ol.sa(T)
+    assert rc == 0
+    P_ref = ol.plcp(T, SA_ref)
+    try:
+        assert lib.sagpu_set_option(b"group_flags", 0) == 0
+        SA, U, r, P, L, f = fused(T)
+        assert np.array_equal(SA, SA_ref), name
+        U_ref, idx_ref = ol.bwt(T, SA_ref)
+        assert r == idx_ref and np.array_equal(U, U_ref), name
+        assert np.array_equal(P, P_ref) and np.array_equal(L, ol.lcp(P_ref, SA_ref)), name
+    finally:
+        assert lib.sagpu_set_option(b"group_flags", 1) == 0
 
 
 def test_big_pivots_gsa():
