@@ -1249,8 +1249,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             BigPivots bp{};
             PivRegions reg{};
             u64 nrest = total;
-            u32 *pq = nullptr, *ppv = nullptr, *d_rk = nullptr, *d_rv = nullptr;
-            u64 *d_pbase = nullptr;
+            u32 *pq = nullptr, *d_rk = nullptr, *d_rv = nullptr;
             // flagged groups: the rank their members hold and the key of the sorted
             // members' subgroup whose slots hold it
             u32 *d_grank = nullptr;
@@ -1265,8 +1264,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 u32 *d_np = rsc.get<u32>(nb), *d_eq = rsc.get<u32>((u64)nb * BP);
                 d_rk = rsc.get<u32>((u64)nb * BP);
                 d_rv = rsc.get<u32>((u64)nb * BP);
-                d_pbase = rsc.get<u64>(1);
-                if (!d_piv || !d_np || !d_eq || !d_rk || !d_rv || !d_pbase) return SAGPU_ENOMEM;
+                if (!d_piv || !d_np || !d_eq || !d_rk || !d_rv) return SAGPU_ENOMEM;
                 const u64 ptiles = big_part_tiles(total);
                 {
                     Span sp(c, F_ROUNDS);
@@ -1286,8 +1284,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                     }
                 nrest = total - reg.start[BP];
                 pq = rsc.get<u32>(reg.start[BP] + 1);
-                ppv = rsc.get<u32>(reg.start[BP] + 1);
-                if (!pq || !ppv) return SAGPU_ENOMEM;
+                if (!pq) return SAGPU_ENOMEM;
                 bp = BigPivots{d_piv, d_np, d_eq};
                 st.big_pivot_rounds++;
                 st.big_pivot_elems += (int64_t)reg.start[BP];
@@ -1297,9 +1294,12 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             if (!b0 || !b1 || !bv0 || !bv1) return SAGPU_ENOMEM;
             {
                 Span sp(c, F_ROUNDS);
+                // the pivot members' list entries go straight to their range of the next
+                // list, behind the tile kernel's out_n survivors
                 if (pivots)
                     SAGPU_TRY(big_part_scatter(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32,
-                                               bp.piv, bp.npiv, ws.bcnt, reg.start[BP], b0, bv0, pq, ppv, s));
+                                               bp.piv, bp.npiv, ws.bcnt, reg.start[BP], b0, bv0, pq,
+                                               act_i[nxt] + out_n, s));
                 else
                     SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32, b0, bv0, s));
             }
@@ -1311,7 +1311,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 if (pivots)
                     SAGPU_TRY(big_piv_ranks(alt ? b1 : b0, comp32, kb, nrest, pq, reg, bp.piv, bp.npiv, d_bslot, ac.gf,
                                             dSA, d_grank, d_keepkey, nb, (u32 *)bp.eq, d_rk, d_rv, out_counter,
-                                            d_pbase, err, s));
+                                            out_n, err, s));
                 else
                     SAGPU_TRY(big_keep(alt ? b1 : b0, comp32, d_boff, kb, nb, d_bslot, ac.gf, dSA, d_grank, d_keepkey,
                                        s));
@@ -1320,7 +1320,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                        act_i[nxt], act_g[nxt], out_counter, dLCP, s, ac, d_grank, d_keepkey,
                                        st.rounds == 1 && rk1.pw ? &ex1 : nullptr, pivots ? &bp : nullptr));
                 if (pivots)
-                    SAGPU_TRY(big_piv_emit(pq, ppv, reg, d_rk, d_rv, d_bslot, ac.gf, d_pbase, h, dISA, act_i[nxt],
+                    SAGPU_TRY(big_piv_emit(pq, reg, d_rk, d_rv, d_bslot, ac.gf, out_n, h, dISA, act_i[nxt],
                                            act_g[nxt], dLCP, ac.cb, s));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));

@@ -145,8 +145,9 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(const KT *__restrict_
 }
 
 // Partition pass 2 (after the exclusive scan of ccnt): members in tile order into
-// their class.  Pivot classes go to (pq, ppv) = (dense group index, list entry) at
-// the scanned position; the rest to the sort buffers at position - npivot, keyed
+// their class.  Pivot classes go to pq (dense group index) and ppv (the list entry)
+// at the scanned position, ppv being the next active list's act_i from the range
+// the pivot members take there; the rest to the sort buffers at position - npivot, keyed
 // (dense group index << kb) | key2 as the plain extraction keys them.
 template <typename KT, typename CT>
 __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
@@ -247,10 +248,12 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
                                                    const u32 *__restrict__ gslot, u32 gf, u32 *__restrict__ SA,
                                                    u32 *__restrict__ grank, u64 *__restrict__ keepkey, u32 nb,
                                                    u32 *__restrict__ eq, u32 *__restrict__ rk, u32 *__restrict__ rv,
-                                                   u64 *__restrict__ out_counter, u64 *__restrict__ pbase,
+                                                   u64 *__restrict__ out_counter, u64 base,
                                                    u32 *__restrict__ err)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *pbase = atomicAdd(out_counter, reg.start[BP]);
+    // the pivot members' range of the next list starts at base (the tile kernel's
+    // survivors before it; the partition wrote their act_i there already)
+    if (blockIdx.x == 0 && threadIdx.x == 0 && atomicAdd(out_counter, reg.start[BP]) != base) atomicOr(err, 8u);
     const u32 q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nb) return;
     const u32 np = npiv[q];
@@ -296,18 +299,17 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
     }
 }
 
-// The pivot classes' members: one subgroup per (group, pivot), entered in the next
-// active list (the reserved range, in class order), ranks written where they
-// change, and the split's LCP mark at the subgroup's first slot.
-__global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, const u32 *__restrict__ ppv,
-                                                  PivRegions reg, const u32 *__restrict__ rk,
-                                                  const u32 *__restrict__ rv, const u32 *__restrict__ gslot, u32 gf,
-                                                  const u64 *__restrict__ pbase, u32 h, u32 *__restrict__ ISA,
-                                                  u32 *__restrict__ out_i, u32 *__restrict__ out_g,
-                                                  u32 *__restrict__ LCP, int cb)
+// The pivot classes' members: one subgroup per (group, pivot) in the next active
+// list (the reserved range at base, in class order, whose act_i the partition
+// wrote): the group words, ranks written where they change, and the split's LCP
+// mark at the subgroup's first slot.
+__global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, PivRegions reg,
+                                                  const u32 *__restrict__ rk, const u32 *__restrict__ rv,
+                                                  const u32 *__restrict__ gslot, u32 gf, u64 base, u32 h,
+                                                  u32 *__restrict__ ISA, const u32 *__restrict__ out_i,
+                                                  u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb)
 {
     const u64 npv = reg.start[BP];
-    const u64 base = *pbase;
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 m = (u64)blockIdx.x * blockDim.x + threadIdx.x; m < npv; m += stride) {
         u32 b = 0;
@@ -316,9 +318,7 @@ __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, co
         const u32 q = pq[m];
         const u32 w = rk[(u64)q * BP + b], r = w & ~gf, g = gslot[q] & ~gf;
         const u32 nv = rv[(u64)q * BP + b];
-        const u32 pv = ppv[m];
-        if (nv != PIV_KEEP) ISA[pv >> cb] = nv;
-        out_i[base + m] = pv;
+        if (nv != PIV_KEEP) ISA[out_i[base + m] >> cb] = nv;
         out_g[base + m] = w;
         if (LCP && r != g && (m == reg.start[b] || pq[m - 1] != q))
             LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
@@ -391,7 +391,7 @@ int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bh
 
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
                   const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
-                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 *pbase, u32 *err, hipStream_t s)
+                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, hipStream_t s)
 {
     if (gf && (!grank || !keepkey)) {
         sagpu_set_error("flagged groups need their ranks and kept keys", __FILE__, __LINE__);
@@ -400,23 +400,22 @@ int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq
     const unsigned grid = (nb + 255) / 256 > 0 ? (nb + 255) / 256 : 1;
     if (comp32)
         hipLaunchKernelGGL(k_piv_ranks<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, pbase, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err);
     else
         hipLaunchKernelGGL(k_piv_ranks<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, pbase, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
-int big_piv_emit(const u32 *pq, const u32 *ppv, const PivRegions &reg, const u32 *rk, const u32 *rv,
-                 const u32 *gslot, u32 gf, const u64 *pbase, u64 h, u32 *ISA, u32 *out_i, u32 *out_g, u32 *LCP,
-                 int cb, hipStream_t s)
+int big_piv_emit(const u32 *pq, const PivRegions &reg, const u32 *rk, const u32 *rv, const u32 *gslot, u32 gf,
+                 u64 base, u64 h, u32 *ISA, const u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s)
 {
     const u64 npv = reg.start[BP];
     if (npv == 0) return SAGPU_OK;
     const u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, ppv, reg, rk, rv, gslot,
-                       gf, pbase, hh, ISA, out_i, out_g, LCP, cb);
+    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, reg, rk, rv, gslot, gf,
+                       base, hh, ISA, out_i, out_g, LCP, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
