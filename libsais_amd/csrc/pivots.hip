@@ -23,6 +23,7 @@
 namespace {
 
 constexpr int PT_THREADS = 256, PT_ROWS = 16, PT_SPAN = 1024, PT_TILE = 4096;
+constexpr int PE_ROWS = 16;   // rows of 64 pivot members per wave in the emission
 constexpr u32 NOQ = 0xFFFFFFFFu;
 
 __device__ __forceinline__ u32 group_of(const u64 *__restrict__ boff, u32 nb, u64 e)
@@ -309,18 +310,39 @@ __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, Pi
                                                   u32 *__restrict__ ISA, const u32 *__restrict__ out_i,
                                                   u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb)
 {
+    // each wave takes PE_ROWS rows of 64 consecutive members, a lane the same column
+    // of every row: a lane's members mostly share one subgroup, whose word and rank
+    // it keeps in registers, and the member before each (for the run starts) is
+    // the lane to the left's, or lane 63's of the row above
     const u64 npv = reg.start[BP];
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 m = (u64)blockIdx.x * blockDim.x + threadIdx.x; m < npv; m += stride) {
+    const u32 lane = lane_id();
+    const u64 wave = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / SAGPU_WAVE;
+    const u64 c0 = wave * (u64)(PE_ROWS * SAGPU_WAVE);
+    if (c0 >= npv) return;   // whole waves leave together
+    u32 cq = NOQ, cbk = 0, w = 0, r = 0, g = 0, nv = 0;
+    u32 above = c0 > 0 ? pq[c0 - 1] : NOQ;   // the member before the first row's lane 0
+    for (int row = 0; row < PE_ROWS; row++) {
+        const u64 m = c0 + (u64)row * SAGPU_WAVE + lane;
+        const bool in = m < npv;
+        const u32 q = in ? pq[m] : NOQ;
+        const u32 left = __shfl_up(q, 1, SAGPU_WAVE);
+        const u32 qm1 = lane == 0 ? above : left;
+        above = __shfl(q, SAGPU_WAVE - 1, SAGPU_WAVE);
+        if (!in) continue;
         u32 b = 0;
 #pragma unroll
         for (int t = 1; t < BP; t++) b += m >= reg.start[t];
-        const u32 q = pq[m];
-        const u32 w = rk[(u64)q * BP + b], r = w & ~gf, g = gslot[q] & ~gf;
-        const u32 nv = rv[(u64)q * BP + b];
+        if (q != cq || b != cbk) {
+            cq = q;
+            cbk = b;
+            w = rk[(u64)q * BP + b];
+            r = w & ~gf;
+            g = gslot[q] & ~gf;
+            nv = rv[(u64)q * BP + b];
+        }
         if (nv != PIV_KEEP) ISA[out_i[base + m] >> cb] = nv;
         out_g[base + m] = w;
-        if (LCP && r != g && (m == reg.start[b] || pq[m - 1] != q))
+        if (LCP && r != g && (m == reg.start[b] || qm1 != q))
             LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
     }
 }
@@ -414,8 +436,9 @@ int big_piv_emit(const u32 *pq, const PivRegions &reg, const u32 *rk, const u32 
     const u64 npv = reg.start[BP];
     if (npv == 0) return SAGPU_OK;
     const u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
-    hipLaunchKernelGGL(k_piv_emit, dim3(grid_cap(npv, 256 * 4, 65535)), dim3(256), 0, s, pq, reg, rk, rv, gslot, gf,
-                       base, hh, ISA, out_i, out_g, LCP, cb);
+    const u64 per_block = 256 / SAGPU_WAVE * (u64)(PE_ROWS * SAGPU_WAVE);
+    hipLaunchKernelGGL(k_piv_emit, dim3((unsigned)((npv + per_block - 1) / per_block)), dim3(256), 0, s, pq, reg, rk,
+                       rv, gslot, gf, base, hh, ISA, out_i, out_g, LCP, cb);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }

@@ -625,6 +625,9 @@ __global__ __launch_bounds__(256) void k_round_gather(const u32 *__restrict__ ac
 }
 
 constexpr int RT_THREADS = 256;
+// A workgroup whose largest small group has more members ranks by sorting its range
+// in LDS instead of counting over each entry's group
+constexpr int RT_SORT_MIN = 96;
 
 // RANGE = list entries whose group heads one workgroup takes (256, 512 or 1024; the
 // "tile_range" option): smaller ranges use less LDS and fit more workgroups per CU.
@@ -673,12 +676,15 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     __shared__ u16 s_hx[WIN];       // local index of the position's group head (NONE16: head before lo)
     __shared__ u16 s_gend[RANGE];   // last local index of the group headed at local x (NONE16: past the window)
     // suffixes and keys of the tile's range, by local position (every small group
-    // of this workgroup lies in [lo, lo + RT_CAP)); outputs by local position
+    // of this workgroup lies in [lo, lo + RT_CAP)); outputs by local position (s_oi,
+    // s_og), which share their LDS with the sort keys of the ranking by sorting
     __shared__ u32 s_i[RT_CAP];
     __shared__ KT s_k[RT_CAP];
-    __shared__ u32 s_oi[RT_CAP], s_og[RT_CAP];
+    __shared__ u64 s_o[RT_CAP];
+    u32 *const s_oi = reinterpret_cast<u32 *>(s_o);
+    u32 *const s_og = reinterpret_cast<u32 *>(s_o) + RT_CAP;
     __shared__ uint8_t s_os[RT_CAP];
-    __shared__ u32 s_first, s_end, s_wsum[4], s_prev;
+    __shared__ u32 s_first, s_end, s_gmax, s_wsum[4], s_prev;
     __shared__ u64 s_base;
 
     const u64 a = rp[0];
@@ -698,6 +704,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     if (tid == 0) {
         s_first = NONE;
         s_end = 0;
+        s_gmax = 0;
         s_prev = lo > 0 ? act_g[lo - 1] : NONE;
     }
     for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE16;
@@ -758,25 +765,29 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     // 3. heads in [lo, hi): small groups define the tile's element range (a min and
     //    a max per wave, then one LDS atomic each), big ones go to the big list
     {
-        u32 f = NONE, e = 0;
+        u32 f = NONE, e = 0, gm = 0;
         for (u32 x = tid; x < rlen; x += RT_THREADS) {
             if (s_hx[x] != x) continue;
             const u32 ge = s_gend[x];
             if (ge != NONE16 && ge - x + 1 <= (u32)RT_SMALL) {
                 f = x < f ? x : f;
                 e = ge + 1 > e ? ge + 1 : e;
+                gm = ge - x + 1 > gm ? ge - x + 1 : gm;
             } else {
                 rt_big_head<RANGE>(lo + x, s_w[x], seg_ctr, bseg_h, bseg_g, bseg_cap, err);
             }
         }
         for (int d = 32; d >= 1; d >>= 1) {
             const u32 of = __shfl_xor(f, d, SAGPU_WAVE), oe = __shfl_xor(e, d, SAGPU_WAVE);
+            const u32 om = __shfl_xor(gm, d, SAGPU_WAVE);
             f = of < f ? of : f;
             e = oe > e ? oe : e;
+            gm = om > gm ? om : gm;
         }
         if (lane_id() == 0 && f != NONE) {
             atomicMin(&s_first, f);
             atomicMax(&s_end, e);
+            atomicMax(&s_gmax, gm);
         }
     }
     __syncthreads();
@@ -794,25 +805,12 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     }
     __syncthreads();
 
-    // 4. rank inside the group by key2 (count smaller, equal, equal-before); local
-    //    positions throughout
-    for (u32 x = first + tid; x < end; x += RT_THREADS) {
-        const u32 hx = s_hx[x];
-        const u32 ge = s_gend[hx];
-        if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
-        const u32 gsz = ge - hx + 1;
-        const u32 gw = s_w[hx];         // first slot | flag
+    // 4. rank inside the group by key2: lt smaller, eq equal, eqb equal before (in
+    //    list order), pk the largest smaller key; local positions throughout.  The
+    //    entry goes to position hx + lt + eqb, its subgroup's first slot is g + lt.
+    auto place = [&](u32 x, u32 hx, u32 lt, u32 eq, u32 eqb, KT key, KT pk) {
+        const u32 gw = s_w[hx];   // first slot | flag
         const u32 g = gw & ~ac.gf;
-        const KT key = s_k[x];
-        u32 lt = 0, eq = 0, eqb = 0;
-        KT pk = 0;   // the largest key below this one: the previous subgroup's
-        for (u32 y = hx; y < hx + gsz; y++) {
-            const KT ky = s_k[y];
-            lt += ky < key;
-            eq += ky == key;
-            eqb += (ky == key) & (y < x);
-            pk = (ky < key && ky > pk) ? ky : pk;
-        }
         const u32 pos = hx + lt + eqb;
         const u32 r = g + lt;
         // the subgroup whose slots hold the group's rank keeps rank and flag (a
@@ -830,6 +828,106 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             if (ex.on && !ex.skipped(g)) mark = h + ex.shared((u64)key, (u64)pk);
             LCP[r] = mark;
         }
+    };
+    if (s_gmax <= (u32)RT_SORT_MIN) {
+        // small groups: each entry counts over its group (O(size) per entry)
+        for (u32 x = first + tid; x < end; x += RT_THREADS) {
+            const u32 hx = s_hx[x];
+            const u32 ge = s_gend[hx];
+            if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
+            const u32 gsz = ge - hx + 1;
+            const KT key = s_k[x];
+            u32 lt = 0, eq = 0, eqb = 0;
+            KT pk = 0;
+            for (u32 y = hx; y < hx + gsz; y++) {
+                const KT ky = s_k[y];
+                lt += ky < key;
+                eq += ky == key;
+                eqb += (ky == key) & (y < x);
+                pk = (ky < key && ky > pk) ? ky : pk;
+            }
+            place(x, hx, lt, eq, eqb, key, pk);
+        }
+    } else {
+        // groups of more than RT_SORT_MIN members: the range [first, end) sorted by
+        // (group head, key2, position) in LDS (a bitonic network whose comparators
+        // all put the smaller at the lower index, so the positions past the range
+        // act as +inf and are never touched), then every entry finds its subgroup's
+        // bounds by binary search inside its group's sorted stretch
+        constexpr int XB = 11;   // position and head fields (RT_CAP <= 2048)
+        constexpr u64 KMASK = (1ull << (64 - 2 * XB)) - 1;
+        static_assert(RT_CAP <= (1 << XB), "sort fields too narrow");
+        const u32 cnt = end - first;
+        for (u32 p = tid; p < cnt; p += RT_THREADS)
+            s_o[p] = ((u64)s_hx[first + p] << (64 - XB)) | ((u64)s_k[first + p] << XB) | p;
+        __syncthreads();
+        u32 lg = 0;
+        while ((1u << lg) < cnt) lg++;
+        const u32 npair = (1u << lg) >> 1;
+        for (u32 kl = 1; kl <= lg; kl++) {
+            // mirror step: i against the mirrored partner in its block of 2^kl
+            for (u32 pi = tid; pi < npair; pi += RT_THREADS) {
+                const u32 blk = (pi >> (kl - 1)) << kl, j = pi & ((1u << (kl - 1)) - 1);
+                const u32 i = blk + j, l = blk + (1u << kl) - 1 - j;
+                if (l < cnt) {
+                    const u64 u = s_o[i], w = s_o[l];
+                    if (u > w) {
+                        s_o[i] = w;
+                        s_o[l] = u;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int dl = (int)kl - 2; dl >= 0; dl--) {
+                for (u32 pi = tid; pi < npair; pi += RT_THREADS) {
+                    const u32 i = ((pi >> dl) << (dl + 1)) | (pi & ((1u << dl) - 1)), l = i + (1u << dl);
+                    if (l < cnt) {
+                        const u64 u = s_o[i], w = s_o[l];
+                        if (u > w) {
+                            s_o[i] = w;
+                            s_o[l] = u;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        u32 rx[RT_PER], rh[RT_PER], rlt[RT_PER], req[RT_PER], reqb[RT_PER];
+        KT rkey[RT_PER], rpk[RT_PER];
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            const u32 p = tid + (u32)j * RT_THREADS;
+            rx[j] = NONE;
+            if (p >= cnt) continue;
+            const u64 c = s_o[p];
+            const u32 hx = (u32)(c >> (64 - XB));
+            const u32 ge = s_gend[hx];
+            if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
+            const u64 ck = c >> XB;   // (head, key)
+            u32 a0 = hx - first, a1 = p;   // first entry of the subgroup: in [group start, p]
+            while (a0 < a1) {
+                const u32 m = (a0 + a1) >> 1;
+                if ((s_o[m] >> XB) < ck) a0 = m + 1;
+                else a1 = m;
+            }
+            u32 b0 = p + 1, b1 = ge - first + 1;   // past its last: in (p, group end]
+            while (b0 < b1) {
+                const u32 m = (b0 + b1) >> 1;
+                if ((s_o[m] >> XB) <= ck) b0 = m + 1;
+                else b1 = m;
+            }
+            rx[j] = first + (u32)(c & ((1u << XB) - 1));
+            rh[j] = hx;
+            rlt[j] = a0 - (hx - first);
+            req[j] = b0 - a0;
+            reqb[j] = p - a0;
+            rkey[j] = (KT)(ck & KMASK);
+            rpk[j] = rlt[j] > 0 ? (KT)((s_o[a0 - 1] >> XB) & KMASK) : (KT)0;
+        }
+        __syncthreads();   // the sort keys are read; their LDS takes the outputs now
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++)
+            if (rx[j] != NONE) place(rx[j], rh[j], rlt[j], req[j], reqb[j], rkey[j], rpk[j]);
     }
     __syncthreads();
 
