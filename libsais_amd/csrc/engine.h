@@ -224,6 +224,9 @@ int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_
 // own windows.  bounds: 256 * nc + 1 u64; ctr: 256 u32.
 int launch_scatter_windows(const u32 *PV, const u32 *PG, u64 m, int s1, u32 nc, int s2, u32 *ISA, u64 *bounds,
                            u32 *ctr, hipStream_t s);
+// boff[q] = the members of the big groups before q (sizes stored minus one),
+// boff[nb] = all of them; err bit 2 when a listed group has at most RT_SMALL members.
+int launch_big_offsets(const u32 *big_size, u32 nb, u64 *boff, u32 *err, hipStream_t s);
 // Big groups: comp = (dense group index << kb) | key2, u32 when comp32 (index and
 // key2 fit 32 bits together), else u64.
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,

@@ -283,26 +283,13 @@ CopyPool &copies()
 enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_PLCP_PAIRS, F_COUNT };
 constexpr int F_KERNEL0 = 64;   // + KernelId: one kernel's launches
 
-// Host copies of one round's big-group list (sa_build), kept with the context so
-// that their capacity is reused.
 constexpr int ROUND_SIG = 30;   // words of a round's launch signature (Ctx::round_sig)
-
-struct BigHost {
-    std::vector<u32> head, slot, size, order, shead, sslot;
-    std::vector<u64> off;
-    void reserve(size_t nb)
-    {
-        head.reserve(nb); slot.reserve(nb); size.reserve(nb); order.reserve(nb);
-        shead.reserve(nb); sslot.reserve(nb); off.reserve(nb + 1);
-    }
-};
 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool;
     RadixScratch rs{};
-    BigHost bigh;
     bool stats_fresh = false;   // stats were reset by the current call (copied out on release)
     u64 *pinned = nullptr;      // small pinned host staging (counts, histograms)
     uint8_t *stage = nullptr;   // large pinned staging for batched jobs
@@ -951,8 +938,6 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     GET(big_size, u32, big_cap);
     GET(d_cnt, u64, 8);
     GET(d_boff, u64, big_cap + 1);
-    GET(d_bhead, u32, big_cap);
-    GET(d_bslot, u32, big_cap);
     // the tile kernel's big heads, in one segment per XCD before k_big_sizes joins them
     const u64 bseg_cap = round_bigseg_cap(n);
     GET(bseg_h, u32, 8 * bseg_cap);
@@ -1079,11 +1064,6 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     u64 h = (u64)ksym;
     int cur = 0;
     st.rounds = 0;
-    // the host copies of each round's big-group list keep their capacity in the
-    // context
-    std::vector<u32> &bh = c.bigh.head, &bg = c.bigh.slot, &bs = c.bigh.size, &order = c.bigh.order,
-                     &hhead = c.bigh.shead, &hslot = c.bigh.sslot;
-    std::vector<u64> &hoff = c.bigh.off;
     // The round's list length and offset reach the kernels through device memory
     // (d_rp), copied from pinned memory as the round's first operation, so rounds
     // from the second on replay one captured graph per buffer parity: parameter copy,
@@ -1201,39 +1181,19 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             return SAGPU_EDEVICE;
         }
         if (nb > 0) {
-            // Big groups, in list order so the extraction is deterministic.
-            bh.resize(nb);
-            bg.resize(nb);
-            bs.resize(nb);
-            SAGPU_CHECK(hipMemcpyAsync(bh.data(), big_head, nb * 4, hipMemcpyDeviceToHost, s));
-            SAGPU_CHECK(hipMemcpyAsync(bg.data(), big_g, nb * 4, hipMemcpyDeviceToHost, s));
-            SAGPU_CHECK(hipMemcpyAsync(bs.data(), big_size, nb * 4, hipMemcpyDeviceToHost, s));
+            // Big groups in the order k_big_sizes listed them (the outputs do not
+            // depend on the order of the groups): their offsets in the concatenation
+            // of their members by a scan on the device; the host reads the total
+            SAGPU_TRY(launch_big_offsets(big_size, nb, d_boff, err, s));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_boff + nb, sizeof(u64), hipMemcpyDeviceToHost, s));
+            SAGPU_CHECK(hipMemcpyAsync(c.pinned + 1, d_cnt + 4, sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));
-            order.resize(nb);
-            for (u32 q = 0; q < nb; q++) order[q] = q;
-            std::sort(order.begin(), order.end(), [&](u32 x, u32 y) { return bh[x] < bh[y]; });
-            hoff.resize(nb + 1);
-            hhead.resize(nb);
-            hslot.resize(nb);
-            u64 total = 0;
-            for (u32 q = 0; q < nb; q++) {
-                hoff[q] = total;
-                hhead[q] = bh[order[q]];
-                hslot[q] = bg[order[q]];
-                total += (u64)bs[order[q]] + 1;  // sizes are stored minus one
-            }
-            hoff[nb] = total;
-            for (u32 q = 0; q < nb; q++) {
-                u64 hq = hhead[q], sq = hoff[q + 1] - hoff[q];
-                if (sq <= RT_SMALL || hq + sq > a || (q + 1 < nb && hq + sq > hhead[q + 1])) total = ~0ull;
-            }
-            if (total > a) {
+            const u64 total = c.pinned[0];
+            if (total > a || ((u32 *)(c.pinned + 1))[1] != 0) {
                 sagpu_set_error("big-group list inconsistent", __FILE__, __LINE__);
                 return SAGPU_EDEVICE;
             }
-            SAGPU_CHECK(hipMemcpyAsync(d_boff, hoff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
-            SAGPU_CHECK(hipMemcpyAsync(d_bhead, hhead.data(), nb * 4, hipMemcpyHostToDevice, s));
-            SAGPU_CHECK(hipMemcpyAsync(d_bslot, hslot.data(), nb * 4, hipMemcpyHostToDevice, s));
+            const u32 *d_bhead = big_head, *d_bslot = big_g;   // first list entries, group words
             Scope rsc(c);  // this round's big-group sort buffers
             // key2 < h + n in rank form; the first round's raw keys have rk1.kbits bits
             int kb = bits_for(h + n - 1);
@@ -2523,7 +2483,6 @@ int noalloc_setup(int dev)
         ctx_destroy(c);
         return rc;
     }
-    c->bigh.reserve(SAGPU_NOALLOC_MAX_N / (RT_SMALL + 1) + 16);
     d.ptr = (uint8_t *)ptr;
     d.c = c;
     return SAGPU_OK;

@@ -347,14 +347,6 @@ __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, Pi
     }
 }
 
-unsigned grid_cap(u64 work, unsigned per_block, unsigned cap)
-{
-    u64 b = (work + per_block - 1) / per_block;
-    if (b < 1) b = 1;
-    if (b > cap) b = cap;
-    return (unsigned)b;
-}
-
 } // namespace
 
 u64 big_part_tiles(u64 total) { return (total + PT_TILE - 1) / PT_TILE; }

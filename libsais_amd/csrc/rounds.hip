@@ -1034,8 +1034,45 @@ __global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ bseg_
     }
 }
 
-// Copies the members of the big groups (sorted by list position) into the sort
-// buffers: comp = (dense group index << 32) | key2, val = suffix index.
+// Offsets of the big groups in the concatenation of their members, in the order
+// k_big_sizes listed them: boff[q] = the members of the groups before q, boff[nb] =
+// all of them (one workgroup: each thread sums a contiguous chunk, then a scan of
+// the 1024 chunk sums).  A listed group of at most RT_SMALL members is an
+// inconsistency (err bit 2).
+__global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ big_size, u32 nb,
+                                                      u64 *__restrict__ boff, u32 *__restrict__ err)
+{
+    __shared__ u64 s_sum[1024];
+    const u32 t = threadIdx.x;
+    const u32 chunk = (nb + 1023) / 1024;
+    const u64 b0 = (u64)t * chunk < nb ? (u64)t * chunk : nb;
+    const u64 b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    u64 sum = 0;
+    bool bad = false;
+    for (u64 q = b0; q < b1; q++) {
+        const u64 sz = (u64)big_size[q] + 1;   // sizes are stored minus one
+        bad = bad || sz <= (u64)RT_SMALL;
+        sum += sz;
+    }
+    s_sum[t] = sum;
+    __syncthreads();
+    for (u32 d = 1; d < 1024; d <<= 1) {
+        const u64 v = t >= d ? s_sum[t - d] : 0ull;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
+    }
+    u64 off = s_sum[t] - sum;
+    for (u64 q = b0; q < b1; q++) {
+        boff[q] = off;
+        off += (u64)big_size[q] + 1;
+    }
+    if (t == 1023) boff[nb] = s_sum[1023];
+    if (bad) atomicOr(err, 2u);
+}
+
+// Copies the members of the big groups into the sort buffers: comp = (dense group
+// index << kb) | key2, val = the list entry.
 template <typename KT, typename CT>
 __global__ __launch_bounds__(256) void k_big_extract(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
                                                      const u32 *__restrict__ bhead, const u64 *__restrict__ boff,
@@ -1271,6 +1308,14 @@ int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *val
     }
     BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p, grank, keepkey};
     return ws_run(ws, f, total, out_counter, s);
+}
+
+int launch_big_offsets(const u32 *big_size, u32 nb, u64 *boff, u32 *err, hipStream_t s)
+{
+    if (nb == 0) return SAGPU_OK;
+    hipLaunchKernelGGL(k_big_offsets, dim3(1), dim3(1024), 0, s, big_size, nb, boff, err);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
 }
 
 int big_keep(const void *sorted, int comp32, const u64 *boff, int kb, u32 nb, const u32 *gslot, u32 gf,
