@@ -209,13 +209,15 @@ int round_gather(const u32 *act_i, const u32 *act_g, u64 a_grid, const u64 *rp, 
 // length goes to *out_counter; seg_ctr: 128 u64.  Big-group heads go the same way
 // to 8 segments of bseg_h / bseg_g (round_bigseg_cap(n) entries each, n >= every
 // list length) and are concatenated into big_head / big_g / big_size, their count
-// to *big_counter.
+// to *big_counter; boff (big_cap + 1 entries) gets their offsets in the
+// concatenation of their members, *big_total the member count.
 u64 round_seg_cap(u64 a);
 u64 round_bigseg_cap(u64 n);
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a_grid, const u64 *rp, u32 *SA,
                u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
-               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap);
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap, u64 *boff,
+               u64 *big_total);
 // ISA[PV[e]] = PG[e] for e < m, in the pairs' order.
 int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_t s);
 // The same for pairs grouped into windows of the target by a radix pass on bits
@@ -224,9 +226,6 @@ int launch_isa_scatter(const u32 *PV, const u32 *PG, u64 m, u32 *ISA, hipStream_
 // own windows.  bounds: 256 * nc + 1 u64; ctr: 256 u32.
 int launch_scatter_windows(const u32 *PV, const u32 *PG, u64 m, int s1, u32 nc, int s2, u32 *ISA, u64 *bounds,
                            u32 *ctr, hipStream_t s);
-// boff[q] = the members of the big groups before q (sizes stored minus one),
-// boff[nb] = all of them; err bit 2 when a listed group has at most RT_SMALL members.
-int launch_big_offsets(const u32 *big_size, u32 nb, u64 *boff, u32 *err, hipStream_t s);
 // Big groups: comp = (dense group index << kb) | key2, u32 when comp32 (index and
 // key2 fit 32 bits together), else u64.
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,

@@ -283,7 +283,7 @@ CopyPool &copies()
 enum Family { F_ONESWEEP = 0, F_ONESWEEP32, F_INITIAL, F_ROUNDS, F_BWT, F_PLCP, F_LCP, F_TOTAL, F_PLCP_PAIRS, F_COUNT };
 constexpr int F_KERNEL0 = 64;   // + KernelId: one kernel's launches
 
-constexpr int ROUND_SIG = 30;   // words of a round's launch signature (Ctx::round_sig)
+constexpr int ROUND_SIG = 31;   // words of a round's launch signature (Ctx::round_sig)
 
 struct Ctx {
     int device = 0;
@@ -1103,7 +1103,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             // round is captured)
             SAGPU_TRY(round_tile(act_i[cur], act_g[cur], key2, wide, a_grid, d_rp, dSA, dISA, act_i[nxt], act_g[nxt],
                                  out_counter, err, big_head, big_g, big_size, big_counter, big_cap, dLCP, s, ac,
-                                 raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr, bseg_h, bseg_g, bseg_cap));
+                                 raw ? &ex1 : nullptr, seg_i, seg_g, seg_ctr, bseg_h, bseg_g, bseg_cap, d_boff,
+                                 d_cnt + 5));
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             if (lazy_first) SAGPU_CHECK(hipMemcpyAsync(h_a1, d_rp, sizeof(u64), hipMemcpyDeviceToHost, s));
             return SAGPU_OK;
@@ -1118,7 +1119,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                             (u64)big_g, (u64)big_size, (u64)ac.codes, (u64)d_rp, (u64)c.pinned,
                                             (u64)seg_i, (u64)seg_g, (u64)seg_ctr, (u64)ac.cb, a, n, (u64)wide,
                                             (u64)g_tile_range, big_cap, (u64)(dLCP != nullptr), (u64)bseg_h,
-                                            (u64)bseg_g, bseg_cap, (u64)ac.gf, (u64)exec};
+                                            (u64)bseg_g, bseg_cap, (u64)ac.gf, (u64)d_boff, (u64)exec};
                 if (!captured[cur] && exec && !memcmp(sig, c.round_sig[cur], sizeof sig)) captured[cur] = true;
                 if (!captured[cur]) {
                     // capture this parity's round once per build; the executable graph of
@@ -1182,14 +1183,11 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         }
         if (nb > 0) {
             // Big groups in the order k_big_sizes listed them (the outputs do not
-            // depend on the order of the groups): their offsets in the concatenation
-            // of their members by a scan on the device; the host reads the total
-            SAGPU_TRY(launch_big_offsets(big_size, nb, d_boff, err, s));
-            SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_boff + nb, sizeof(u64), hipMemcpyDeviceToHost, s));
-            SAGPU_CHECK(hipMemcpyAsync(c.pinned + 1, d_cnt + 4, sizeof(u64), hipMemcpyDeviceToHost, s));
-            SAGPU_CHECK(hipStreamSynchronize(s));
-            const u64 total = c.pinned[0];
-            if (total > a || ((u32 *)(c.pinned + 1))[1] != 0) {
+            // depend on the order of the groups), with their offsets in the
+            // concatenation of their members from the round's scan (d_boff) and the
+            // member count among the round's counters
+            const u64 total = c.pinned[5];
+            if (total > a) {
                 sagpu_set_error("big-group list inconsistent", __FILE__, __LINE__);
                 return SAGPU_EDEVICE;
             }

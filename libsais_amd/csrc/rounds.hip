@@ -1035,14 +1035,17 @@ __global__ __launch_bounds__(256) void k_big_sizes(const u32 *__restrict__ bseg_
 }
 
 // Offsets of the big groups in the concatenation of their members, in the order
-// k_big_sizes listed them: boff[q] = the members of the groups before q, boff[nb] =
-// all of them (one workgroup: each thread sums a contiguous chunk, then a scan of
-// the 1024 chunk sums).  A listed group of at most RT_SMALL members is an
-// inconsistency (err bit 2).
-__global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ big_size, u32 nb,
-                                                      u64 *__restrict__ boff, u32 *__restrict__ err)
+// k_big_sizes listed them (nb = *big_counter of them, at most big_cap): boff[q] =
+// the members of the groups before q, boff[nb] = *total = all of them (one
+// workgroup: each thread sums a contiguous chunk, then a scan of the 1024 chunk
+// sums).  A listed group of at most RT_SMALL members is an inconsistency (err bit 2).
+__global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ big_size,
+                                                      const u32 *__restrict__ big_counter, u64 big_cap,
+                                                      u64 *__restrict__ boff, u64 *__restrict__ total,
+                                                      u32 *__restrict__ err)
 {
     __shared__ u64 s_sum[1024];
+    const u32 nb = (u32)((u64)*big_counter < big_cap ? (u64)*big_counter : big_cap);
     const u32 t = threadIdx.x;
     const u32 chunk = (nb + 1023) / 1024;
     const u64 b0 = (u64)t * chunk < nb ? (u64)t * chunk : nb;
@@ -1067,7 +1070,10 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ bi
         boff[q] = off;
         off += (u64)big_size[q] + 1;
     }
-    if (t == 1023) boff[nb] = s_sum[1023];
+    if (t == 1023) {
+        boff[nb] = s_sum[1023];
+        *total = s_sum[1023];
+    }
     if (bad) atomicOr(err, 2u);
 }
 
@@ -1225,7 +1231,8 @@ template <typename KT>
 int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, const u64 *rp, u32 *SA, u32 *ISA,
                  u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                  u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp &ex,
-                 u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap)
+                 u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap, u64 *boff,
+                 u64 *big_total)
 {
     const u64 seg_cap = round_seg_cap(a);
     SAGPU_TRY(launch_fill_u64(seg_ctr, 8 * 16, 0, s));
@@ -1242,6 +1249,7 @@ int round_tile_t(const u32 *act_i, const u32 *act_g, const KT *key2, u64 a, cons
     ktime_end(KI_ROUND_TILE, a * (24 + sizeof(KT)));
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(big_cap, 256, 1024)), dim3(256), 0, s, bseg_h, bseg_g,
                        (const u64 *)seg_ctr, bseg_cap, big_head, big_g, big_counter, big_cap, SA, big_size, ac.gf);
+    hipLaunchKernelGGL(k_big_offsets, dim3(1), dim3(1024), 0, s, big_size, big_counter, big_cap, boff, big_total, err);
     hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(a, 256 * 4, 65535)), dim3(256), 0, s, seg_i, seg_g,
                        (const u64 *)seg_ctr, seg_cap, out_i, out_g, out_counter);
     SAGPU_CHECK(hipGetLastError());
@@ -1269,15 +1277,16 @@ int round_gather(const u32 *act_i, const u32 *act_g, u64 a, const u64 *rp, const
 int round_tile(const u32 *act_i, const u32 *act_g, const void *key2, int wide, u64 a, const u64 *rp, u32 *SA,
                u32 *ISA, u32 *out_i, u32 *out_g, u64 *out_counter, u32 *err, u32 *big_head, u32 *big_g, u32 *big_size,
                u32 *big_counter, u64 big_cap, u32 *LCP, hipStream_t s, const ActCode &ac, const ExactLcp *ex,
-               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap)
+               u32 *seg_i, u32 *seg_g, u64 *seg_ctr, u32 *bseg_h, u32 *bseg_g, u64 bseg_cap, u64 *boff,
+               u64 *big_total)
 {
     const ExactLcp e = ex ? *ex : ExactLcp{};
     return wide ? round_tile_t<u64>(act_i, act_g, (const u64 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
                                     big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
-                                    seg_ctr, bseg_h, bseg_g, bseg_cap)
+                                    seg_ctr, bseg_h, bseg_g, bseg_cap, boff, big_total)
                 : round_tile_t<u32>(act_i, act_g, (const u32 *)key2, a, rp, SA, ISA, out_i, out_g, out_counter, err,
                                     big_head, big_g, big_size, big_counter, big_cap, LCP, s, ac, e, seg_i, seg_g,
-                                    seg_ctr, bseg_h, bseg_g, bseg_cap);
+                                    seg_ctr, bseg_h, bseg_g, bseg_cap, boff, big_total);
 }
 
 int big_extract(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total,
@@ -1308,14 +1317,6 @@ int big_emit(WsScratch &ws, const void *comp, int comp32, int kb, const u32 *val
     }
     BigF<u64> f{(const u64 *)comp, vals, gslot, total, hh, kb, SA, ISA, out_i, out_g, LCP, ac, e, p, grank, keepkey};
     return ws_run(ws, f, total, out_counter, s);
-}
-
-int launch_big_offsets(const u32 *big_size, u32 nb, u64 *boff, u32 *err, hipStream_t s)
-{
-    if (nb == 0) return SAGPU_OK;
-    hipLaunchKernelGGL(k_big_offsets, dim3(1), dim3(1024), 0, s, big_size, nb, boff, err);
-    SAGPU_CHECK(hipGetLastError());
-    return SAGPU_OK;
 }
 
 int big_keep(const void *sorted, int comp32, const u64 *boff, int kb, u32 nb, const u32 *gslot, u32 gf,
