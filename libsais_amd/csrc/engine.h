@@ -300,8 +300,11 @@ int launch_localize(const u32 *SA, const u32 *sid, const int64_t *offsets, u64 n
 struct LcpScratch {
     u32 *queue;       // [cap] positions whose lcp did not finish in the per-thread pass
     u32 *queue_l;     // [cap] lcp matched so far
-    u32 *queue_n;     // [1]
+    u32 *queue_n;     // [2]: queued positions, then those of the very-long queue
     u64 queue_cap;
+    u32 *vq_pos;      // [256] very-long queue: positions still matching after the
+    u64 *vq_off;      // [256] first pass's steps, their offset reached,
+    u64 *vq_res;      // [256] and first mismatch
     void *tile_max;   // [tiles] per-tile max for the prefix-max scan (u32 or u64)
     u64 tiles_cap;
     // optional: called with 0 / 1 around the launch of the pair-word text-order pass

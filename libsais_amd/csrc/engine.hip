@@ -1325,6 +1325,12 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     GET(ql, u32, ls.queue_cap);
     GET(qn, u32, 64);
     GET(tm, u64, ls.tiles_cap);
+    GET(vqp, u32, 256);
+    GET(vqo, u64, 256);
+    GET(vqr, u64, 256);
+    ls.vq_pos = vqp;
+    ls.vq_off = vqo;
+    ls.vq_res = vqr;
     ls.queue = q;
     ls.queue_l = ql;
     ls.queue_n = qn;

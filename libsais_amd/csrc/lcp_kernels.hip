@@ -469,66 +469,105 @@ __global__ __launch_bounds__(256) void k_plcp_pairs(const u32 *__restrict__ SA, 
 
 // One workgroup per queued position: 256 lanes x LW words of 8 bytes per step, first
 // mismatch found with a workgroup min-reduction (byte offsets; W bytes per symbol).
+// A position still matching after LONG_STEPS steps goes to the very-long queue
+// (vq: its position and the offset reached), whose matches many workgroups compare
+// together (k_plcp_vlong): a text with a few positions of lcp near n would
+// otherwise keep one workgroup stepping through megabytes while the rest idle.
 constexpr int LW = 8;
+constexpr u64 LONG_STEP = 256ull * 8 * LW;   // bytes per workgroup step
+constexpr int LONG_STEPS = 32;
+constexpr u32 VQ_CAP = 256;                  // very-long queue entries
+struct VLongQueue {
+    u32 *n;       // [1] entries
+    u32 *pos;     // [VQ_CAP] text position
+    u64 *off;     // [VQ_CAP] byte offset matched so far
+    u64 *res;     // [VQ_CAP] first mismatch found (atomic min), ~0 while none
+};
+
+// The smallest byte offset in [l, l + LONG_STEP) where the suffixes at i and k
+// differ (or, in GSA mode, hold a separator), capped at m; ~0 if none.  Called by
+// all threads of the workgroup.
+template <int W>
+__device__ u64 long_step(const uint8_t *__restrict__ T, int gsa, u64 i, u64 k, u64 l, u64 m, u64 *s_min)
+{
+    u64 first = ~0ull;
+#pragma unroll
+    for (int u = 0; u < LW; u++) {
+        const u64 off = l + ((u64)u * 256 + threadIdx.x) * 8;
+        u64 f = m;
+        if (off < m) {
+            const u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
+            u64 stop = x ^ y;
+            if (gsa) stop |= zero_lanes<W>(x);
+            f = ~0ull;
+            if (stop) {
+                f = off + (u64)(__builtin_ctzll(stop) >> 3);
+                if (f > m) f = m;
+            }
+        }
+        first = f < first ? f : first;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        const u64 o = __shfl_xor(first, d, SAGPU_WAVE);
+        first = first < o ? first : o;
+    }
+    if (lane_id() == 0) s_min[threadIdx.x / SAGPU_WAVE] = first;
+    __syncthreads();
+    u64 r = s_min[0];
+    for (int w = 1; w < 4; w++) r = r < s_min[w] ? r : s_min[w];
+    __syncthreads();   // s_min is reused by the next step
+    return r;
+}
+
+// The suffix that precedes suffix i in SA order (n for the first), as Φ gives it.
+__device__ __forceinline__ u64 phi_of(const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
+                                      const u32 *__restrict__ SA, u64 n, u64 i)
+{
+    u64 k;
+    if (PHI) {
+        k = PHI[i];
+        if (k == i) k = n;
+    } else {
+        const u32 r = isa_rank(ISA[i], n);
+        k = r == 0 ? n : SA[r - 1];
+    }
+    return k > n ? n : k;
+}
+
 template <typename VT, int W>
 __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T, u64 n, int gsa,
                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
                                                    const u32 *__restrict__ SA, VT *__restrict__ V,
                                                    const u32 *__restrict__ queue, const u32 *__restrict__ queue_l,
                                                    const u32 *__restrict__ queue_n, u64 queue_cap,
-                                                   VT *__restrict__ tmax)
+                                                   VT *__restrict__ tmax, VLongQueue vq)
 {
     __shared__ u64 s_min[4];
-    __shared__ u64 s_res;
+    __shared__ u32 s_slot;
     const u32 cnt = (u32)((u64)*queue_n < queue_cap ? (u64)*queue_n : queue_cap);
     for (u32 q = blockIdx.x; q < cnt; q += gridDim.x) {
         const u64 i = queue[q];
-        u64 k;
-        if (PHI) {
-            k = PHI[i];
-            if (k == i) k = n;
-        } else {
-            u32 r = isa_rank(ISA[i], n);
-            k = r == 0 ? n : SA[r - 1];
-        }
-        if (k > n) k = n;
+        const u64 k = phi_of(PHI, ISA, SA, n, i);
         const u64 m = W * (n - (i > k ? i : k));
         u64 l = W * (u64)queue_l[q];
-        for (;;) {
-            // LW words of 8 bytes per lane and step (16 KiB per workgroup), all
-            // loads in flight together; the step's answer is the smallest offset
-            u64 first = ~0ull;
-#pragma unroll
-            for (int u = 0; u < LW; u++) {
-                const u64 off = l + ((u64)u * 256 + threadIdx.x) * 8;
-                u64 f = m;
-                if (off < m) {
-                    const u64 x = load8u(T + W * i + off), y = load8u(T + W * k + off);
-                    u64 stop = x ^ y;
-                    if (gsa) stop |= zero_lanes<W>(x);
-                    f = ~0ull;
-                    if (stop) {
-                        f = off + (u64)(__builtin_ctzll(stop) >> 3);
-                        if (f > m) f = m;
+        for (int step = 0;; step++) {
+            if (step == LONG_STEPS && vq.n) {
+                // hand the rest of the match to the very-long pass (when it has room)
+                if (threadIdx.x == 0) {
+                    const u32 slot = atomicAdd(vq.n, 1u);
+                    if (slot < VQ_CAP) {
+                        vq.pos[slot] = (u32)i;
+                        vq.off[slot] = l;
+                        vq.res[slot] = ~0ull;
                     }
+                    s_slot = slot;
                 }
-                first = f < first ? f : first;
+                __syncthreads();
+                const u32 slot = s_slot;
+                __syncthreads();
+                if (slot < VQ_CAP) break;
             }
-            // workgroup min
-            for (int d = 32; d >= 1; d >>= 1) {
-                u64 o = __shfl_xor(first, d, SAGPU_WAVE);
-                first = first < o ? first : o;
-            }
-            if (lane_id() == 0) s_min[threadIdx.x / SAGPU_WAVE] = first;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                u64 r = s_min[0];
-                for (int w = 1; w < 4; w++) r = r < s_min[w] ? r : s_min[w];
-                s_res = r;
-            }
-            __syncthreads();
-            u64 r = s_res;
-            __syncthreads();
+            const u64 r = long_step<W>(T, gsa, i, k, l, m, s_min);
             if (r != ~0ull) {
                 if (threadIdx.x == 0) {
                     V[i] = (VT)(r / W + i);
@@ -537,9 +576,50 @@ __global__ __launch_bounds__(256) void k_plcp_long(const uint8_t *__restrict__ T
                 }
                 break;
             }
-            l += 256ull * 8 * LW;
+            l += LONG_STEP;
         }
     }
+}
+
+// The very-long matches: the workgroups split evenly over the queued positions;
+// workgroup g of a position compares its steps g, g + G, g + 2G, ... in order until
+// a mismatch at or before its step is known (atomic min of the first mismatch).
+template <int W>
+__global__ __launch_bounds__(256) void k_plcp_vlong(const uint8_t *__restrict__ T, u64 n, int gsa,
+                                                    const u32 *__restrict__ PHI, const u32 *__restrict__ ISA,
+                                                    const u32 *__restrict__ SA, VLongQueue vq)
+{
+    __shared__ u64 s_min[4];
+    const u32 cnt = *vq.n < VQ_CAP ? *vq.n : VQ_CAP;
+    if (cnt == 0) return;
+    const u32 G = gridDim.x / cnt > 0 ? gridDim.x / cnt : 1u;
+    const u32 g = blockIdx.x % G;
+    for (u32 v = blockIdx.x / G; v < cnt; v += (gridDim.x / G > 0 ? gridDim.x / G : 1u)) {
+        const u64 i = vq.pos[v];
+        const u64 k = phi_of(PHI, ISA, SA, n, i);
+        const u64 m = W * (n - (i > k ? i : k));
+        for (u64 l = vq.off[v] + (u64)g * LONG_STEP;; l += (u64)G * LONG_STEP) {
+            if (*(volatile u64 *)&vq.res[v] <= l) break;   // a mismatch before this step is known
+            const u64 r = long_step<W>(T, gsa, i, k, l, m, s_min);
+            if (r != ~0ull) {
+                if (threadIdx.x == 0) atomicMin((unsigned long long *)&vq.res[v], (unsigned long long)r);
+                break;
+            }
+        }
+    }
+}
+
+// V (and the tile maxima) of the very-long matches.
+template <typename VT, int W>
+__global__ __launch_bounds__(256) void k_plcp_vlong_fin(VT *__restrict__ V, VT *__restrict__ tmax, VLongQueue vq)
+{
+    const u32 cnt = *vq.n < VQ_CAP ? *vq.n : VQ_CAP;
+    const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= cnt) return;
+    const u64 i = vq.pos[v];
+    const VT x = (VT)(vq.res[v] / W + i);
+    V[i] = x;
+    if (tmax) atomicMax(&tmax[i / MS_TILE], x);
 }
 
 // Prefix max over V, then PLCP[i] = max - i.  VT = u32 works in place (V is the
@@ -785,7 +865,7 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
         return SAGPU_EDEVICE;
     }
     VT *tmax = (VT *)ls.tile_max;
-    SAGPU_TRY(launch_fill_u32(ls.queue_n, 1, 0, s));
+    SAGPU_TRY(launch_fill_u32(ls.queue_n, 2, 0, s));
     const bool fused_max = pairs && W == 1;   // the pair-word pass writes the tile maxima itself
     if (fused_max) {
         if constexpr (W == 1) {
@@ -803,8 +883,14 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
         hipLaunchKernelGGL((k_plcp_irr<VT, W>), dim3(grid_for(n, 256 * 8, 65535)), dim3(256), 0, s, T, n, gsa, PHI, ISA,
                            SA, V, ls.queue, ls.queue_l, ls.queue_n, ls.queue_cap, marks);
     }
+    const VLongQueue vq{ls.vq_pos ? ls.queue_n + 1 : nullptr, ls.vq_pos, ls.vq_off, ls.vq_res};
     hipLaunchKernelGGL((k_plcp_long<VT, W>), dim3(4096), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, V, ls.queue, ls.queue_l,
-                       ls.queue_n, ls.queue_cap, fused_max ? tmax : (VT *)nullptr);
+                       ls.queue_n, ls.queue_cap, fused_max ? tmax : (VT *)nullptr, vq);
+    if (vq.n) {
+        hipLaunchKernelGGL((k_plcp_vlong<W>), dim3(2048), dim3(256), 0, s, T, n, gsa, PHI, ISA, SA, vq);
+        hipLaunchKernelGGL((k_plcp_vlong_fin<VT, W>), dim3(VQ_CAP / 256), dim3(256), 0, s, V,
+                           fused_max ? tmax : (VT *)nullptr, vq);
+    }
     if (!fused_max) hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
     hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
     hipLaunchKernelGGL(k_max_apply<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax, PLCP);

@@ -1047,6 +1047,13 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ bi
     __shared__ u64 s_sum[1024];
     const u32 nb = (u32)((u64)*big_counter < big_cap ? (u64)*big_counter : big_cap);
     const u32 t = threadIdx.x;
+    if (nb == 0) {   // most rounds of most texts: nothing to scan
+        if (t == 0) {
+            boff[0] = 0;
+            *total = 0;
+        }
+        return;
+    }
     const u32 chunk = (nb + 1023) / 1024;
     const u64 b0 = (u64)t * chunk < nb ? (u64)t * chunk : nb;
     const u64 b1 = b0 + chunk < nb ? b0 + chunk : nb;

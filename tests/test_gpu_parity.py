@@ -66,6 +66,9 @@ def _cases():
         cases.append((f"runs{run}", T))
     rep = workloads.iid(3000, seed=99, alphabet=b"ACGT")
     cases.append(("copies300", np.tile(rep, 300)))
+    # two copies of a 1.5 MiB block: lcps of up to 1.5 M symbols, past what one
+    # workgroup of the long-match pass takes before the very-long pass splits them
+    cases.append(("copies2_1536k", np.tile(workloads.iid(3 << 19, seed=98, alphabet=b"ACGT"), 2)))
     cases.append(("banana", np.frombuffer(b"banana", np.uint8).copy()))
     cases.append(("mississippi", np.frombuffer(b"mississippi", np.uint8).copy()))
     return cases
