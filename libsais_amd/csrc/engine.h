@@ -261,6 +261,8 @@ int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff
 int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb,
                      u64 total, int kb, int comp32, const u64 *piv, const u32 *npiv, const u32 *cscan, u64 npivot,
                      void *comp, u32 *vals, u32 *pq, u32 *ppv, hipStream_t s);
+// starts[b] (b = 0..BP) = the first member of class b after the scan of the counts
+int big_part_starts(const u32 *cscan, u64 total, u32 *starts, hipStream_t s);
 // rk, rv: nb * BP entries (the pivot subgroups' group words and ranks); grank,
 // keepkey: as big_keep, for the flagged groups of a pivot round.  The pivot members
 // take the next active list's entries [base, base + npivot) (base = its length

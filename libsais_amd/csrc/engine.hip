@@ -1231,8 +1231,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                     SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, (BP + 1) * ptiles, s));
                 }
                 u32 *preg = (u32 *)(c.pinned + 7200);   // past the round parameters
-                for (int b = 0; b <= BP; b++)
-                    SAGPU_CHECK(hipMemcpyAsync(preg + b, ws.bcnt + (u64)b * ptiles, 4, hipMemcpyDeviceToHost, s));
+                u32 *d_starts = rsc.get<u32>(BP + 1);
+                if (!d_starts) return SAGPU_ENOMEM;
+                SAGPU_TRY(big_part_starts(ws.bcnt, total, d_starts, s));
+                SAGPU_CHECK(hipMemcpyAsync(preg, d_starts, (BP + 1) * sizeof(u32), hipMemcpyDeviceToHost, s));
                 SAGPU_CHECK(hipStreamSynchronize(s));
                 for (int b = 0; b <= BP; b++) reg.start[b] = preg[b];
                 for (int b = 0; b < BP; b++)

@@ -215,22 +215,32 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restri
     }
 }
 
-// First index in sorted a[0, n) with a[i] >= v.
+// First index in sorted a[0, n) with a[i] >= v, searched by the whole wave: 64
+// probes per step, so about log64(n) dependent reads instead of log2(n).
 template <typename T>
 __device__ u64 lower_bound_dev(const T *__restrict__ a, u64 n, u64 v)
 {
+    const u32 lane = lane_id();
     u64 lo = 0, hi = n;
-    while (lo < hi) {
-        const u64 mid = (lo + hi) >> 1;
-        if ((u64)a[mid] < v) lo = mid + 1;
-        else hi = mid;
+    while (hi - lo > SAGPU_WAVE) {
+        const u64 step = (hi - lo + SAGPU_WAVE - 1) / SAGPU_WAVE;
+        const u64 p = lo + (u64)lane * step;
+        const u64 m = __ballot(p >= hi || (u64)a[p] >= v);
+        const u32 f = m ? (u32)__builtin_ctzll(m) : (u32)SAGPU_WAVE;   // first probe at or above v
+        if (f == 0) return lo;
+        const u64 nhi = f < (u32)SAGPU_WAVE ? lo + (u64)f * step : hi;
+        lo = lo + (u64)(f - 1) * step + 1;   // the probe before holds less than v
+        hi = nhi < hi ? nhi : hi;
     }
-    return lo;
+    const u64 p = lo + lane;
+    const u64 m = __ballot(p >= hi || (u64)a[p] >= v);
+    return m ? lo + (u64)__builtin_ctzll(m) : hi;
 }
 
-// One thread per big group: members equal to each pivot (from the pivot classes,
-// grouped by dense group index), the new subgroup's word and rank.  The first
-// thread reserves the pivot members' entries of the next active list.
+// One wave per big group (every lane computes the same values, lane 0 writes):
+// members equal to each pivot (from the pivot classes, grouped by dense group
+// index), the new subgroup's word and rank.  The first wave reserves the pivot
+// members' entries of the next active list.
 //   The group's members hold one rank v: its first slot g, or, where the group word
 //   carries the flag gf, the value stored in SA[g + 1].  The subgroup whose slots
 //   hold v keeps it (no ISA writes).  A pivot subgroup that does not takes the
@@ -255,8 +265,9 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
     // the pivot members' range of the next list starts at base (the tile kernel's
     // survivors before it; the partition wrote their act_i there already)
     if (blockIdx.x == 0 && threadIdx.x == 0 && atomicAdd(out_counter, reg.start[BP]) != base) atomicOr(err, 8u);
-    const u32 q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nb) return;
+    const u32 q = (blockIdx.x * blockDim.x + threadIdx.x) / SAGPU_WAVE;
+    if (q >= nb) return;   // whole waves leave together
+    const bool w0 = lane_id() == 0;
     const u32 np = npiv[q];
     const u32 gw = gslot[q], g = gw & ~gf;
     const bool flagged = (gw & gf) != 0;
@@ -272,31 +283,35 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
         const u64 lt = lower_bound_dev(rest, nrest, g0 | pk) - rest0;
         const u64 r0 = reg.start[b], len = reg.start[b + 1] - r0;
         const u64 c = lower_bound_dev(pq + r0, len, (u64)q + 1) - lower_bound_dev(pq + r0, len, (u64)q);
-        if (c < 2) atomicOr(err, 4u);   // a pivot is sampled twice, so it has two members
+        if (c < 2 && w0) atomicOr(err, 4u);   // a pivot is sampled twice, so it has two members
         const u64 pos = lt + below;     // the subgroup's first position among the group's slots
         const u32 first = g + (u32)pos;
         const bool kept = vr >= pos && vr < pos + c;
         in_pivot = in_pivot || kept;
         if (pos + c <= vr) piv_before += c;
-        eq[(u64)q * BP + b] = (u32)c;
-        if (kept) {
-            rk[(u64)q * BP + b] = first | (gw & gf);
-            rv[(u64)q * BP + b] = PIV_KEEP;
-            if (flagged && first != g && c >= 2) SA[first + 1] = v;
-        } else {
-            const u32 mid = first + (u32)((c - 1) / 2);
-            rk[(u64)q * BP + b] = first | (c >= 2 ? gf : 0u);
-            rv[(u64)q * BP + b] = gf && c >= 2 ? mid : first;
-            if (gf && c >= 2) SA[first + 1] = mid;
+        if (w0) {
+            eq[(u64)q * BP + b] = (u32)c;
+            if (kept) {
+                rk[(u64)q * BP + b] = first | (gw & gf);
+                rv[(u64)q * BP + b] = PIV_KEEP;
+                if (flagged && first != g && c >= 2) SA[first + 1] = v;
+            } else {
+                const u32 mid = first + (u32)((c - 1) / 2);
+                rk[(u64)q * BP + b] = first | (c >= 2 ? gf : 0u);
+                rv[(u64)q * BP + b] = gf && c >= 2 ? mid : first;
+                if (gf && c >= 2) SA[first + 1] = mid;
+            }
         }
         below += c;
     }
     if (flagged) {
-        grank[q] = v;
         // v among the sorted rest: its position minus the pivot members before it
         const u64 rest1 = lower_bound_dev(rest, nrest, g0 + (1ull << kb));
         const u64 t = rest0 + (vr - piv_before);
-        keepkey[q] = in_pivot || vr < piv_before || t >= rest1 ? BIG_NOKEY : (u64)rest[t] & ((1ull << kb) - 1);
+        if (w0) {
+            grank[q] = v;
+            keepkey[q] = in_pivot || vr < piv_before || t >= rest1 ? BIG_NOKEY : (u64)rest[t] & ((1ull << kb) - 1);
+        }
     }
 }
 
@@ -403,6 +418,21 @@ int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bh
     return SAGPU_OK;
 }
 
+// The class regions' starts after the exclusive scan of the class-major tile
+// counts: starts[b] = ccnt[b * tiles] for b = 0..BP (one copy to the host instead of
+// BP + 1).
+__global__ void k_part_starts(const u32 *__restrict__ ccnt, u64 tiles, u32 *__restrict__ starts)
+{
+    if (threadIdx.x <= BP) starts[threadIdx.x] = ccnt[(u64)threadIdx.x * tiles];
+}
+
+int big_part_starts(const u32 *cscan, u64 total, u32 *starts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(64), 0, s, cscan, big_part_tiles(total), starts);
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
+}
+
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
                   const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
                   u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, hipStream_t s)
@@ -411,7 +441,8 @@ int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq
         sagpu_set_error("flagged groups need their ranks and kept keys", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
-    const unsigned grid = (nb + 255) / 256 > 0 ? (nb + 255) / 256 : 1;
+    const unsigned waves = 256 / SAGPU_WAVE;
+    const unsigned grid = (nb + waves - 1) / waves > 0 ? (nb + waves - 1) / waves : 1;
     if (comp32)
         hipLaunchKernelGGL(k_piv_ranks<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)rest, nrest, kb, pq, reg, piv,
                            npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err);
