@@ -96,9 +96,11 @@ int radix_sort_u32(RadixScratch &rs, u32 *k0, u32 *v0, u32 *k1, u32 *v1, u64 n, 
 // longer texts nothing is packed and the word is the rank.
 __host__ __device__ __forceinline__ u32 isa_rank(u32 x, u64 n) { return n <= (1ull << 28) ? (x & 0x0FFFFFFFu) : x; }
 __host__ __device__ __forceinline__ u32 isa_hint(u32 x, u64 n) { return n <= (1ull << 28) ? (x >> 28) : 0u; }
-// A hint h (1..15) stands for the lcp off + h - 1: off = max(0, ksym - 15) puts the
-// 15 values on the largest marks an initial key allows (marks of DNA with 16-symbol
-// keys: 1..15).  on (in) asks for the hints; on (out) tells whether they were packed.
+// A hint h (1..15) stands for the lcp off + h - 1: off = max(0, ksym - 10) puts the
+// 15 values on the largest marks an initial key allows and the exact marks of the
+// first doubling round (ksym plus the few symbols two raw keys share; DNA with
+// 16-symbol keys: 6..20).  on (in) asks for the hints; on (out) tells whether they
+// were packed.
 struct PlcpHint {
     int on = 0;
     int off = 0;
@@ -184,6 +186,13 @@ struct ExactLcp {
     int on = 0;
     int bits = 0, kbits = 0, nskip = 0;
     u32 skip[64] = {};
+    // PLCP hints (PlcpHint) for the suffixes the first round places with an exact
+    // mark: the hint bits of the rank written to ISA
+    int hint = 0, hint_off = 0;
+    __device__ u32 hint_bits(u32 mark) const
+    {
+        return hint && mark >= (u32)hint_off && mark - (u32)hint_off < 15u ? (mark - (u32)hint_off + 1u) << 28 : 0u;
+    }
     __device__ bool skipped(u32 g) const
     {
         for (int t = 0; t < nskip; t++)

@@ -995,7 +995,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         Span sp(c, F_INITIAL);
         if (hint) {   // byte texts without separators, up to 2^28 suffixes
             hint->on = hint->on && !gsa && !dSym && n <= (1ull << 28);
-            hint->off = ksym > 15 ? ksym - 15 : 0;
+            // the 15 hint values cover marks [off, off + 14]: the largest marks of the
+            // initial grouping (below ksym) and the first round's exact marks (ksym
+            // and a few above)
+            hint->off = ksym > 10 ? ksym - 10 : 0;
         }
         // the unsorted key buffer is free now: it holds the per-slot group heads
         SAGPU_TRY(group_init(ws, alt ? k1 : k0, alt ? v1 : v0, n, bits, ksym, gsa, dSA, dISA, act_i[0], act_g[0], dLCP,
@@ -1024,6 +1027,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 ex1.kbits = ksym * bits;
                 ex1.nskip = (int)cnt2;
                 for (u64 t = 0; t < cnt2; t++) ex1.skip[t] = hs[t];
+                // suffixes placed by the first round with an exact mark carry it as a
+                // PLCP hint too (their ISA words are written anyway)
+                ex1.hint = hint && hint->on ? 1 : 0;
+                ex1.hint_off = hint ? hint->off : 0;
             }
         }
     } else {

@@ -531,15 +531,7 @@ struct BigF {
         // rank then stored at its first slot + 1); new ranks here are first slots
         const bool flagged = (gw & ac.gf) != 0;
         const bool kept = flagged ? key == keepkey[q] : r == g;
-        if (!kept) ISA[v] = r;
-        if (act) {
-            out_i[o] = pv;
-            out_g[o] = kept ? r | (gw & ac.gf) : r;
-            if (kept && flagged && r != g && hstart == (u32)e) SA[r + 1] = grank[q];
-        } else {
-            SA[r] = v;
-            if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
-        }
+        u32 hint = 0;   // PLCP hint of a suffix placed with an exact mark (ISA bits 28-31)
         if (LCP && hstart == (u32)e && r != g) {
             // a subgroup after the group's first: e - 1 is the last entry of the
             // previous subgroup (exact marks come only without pivot classes)
@@ -547,8 +539,18 @@ struct BigF {
             if (ex.on && !ex.skipped(g) && hstart != gstart) {
                 const u64 km = (1ull << kb) - 1;
                 mark = h + ex.shared(c & km, (u64)comp[e - 1] & km);
+                if (!act) hint = ex.hint_bits(mark);
             }
             LCP[r] = mark;
+        }
+        if (!kept) ISA[v] = r | hint;
+        if (act) {
+            out_i[o] = pv;
+            out_g[o] = kept ? r | (gw & ac.gf) : r;
+            if (kept && flagged && r != g && hstart == (u32)e) SA[r + 1] = grank[q];
+        } else {
+            SA[r] = v;
+            if (ac.codes) ac.codes[r] = (uint8_t)(pv & ((1u << ac.cb) - 1u));
         }
     }
 };
@@ -819,15 +821,21 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         // writes); the others rank at their first slot, unflagged
         const u32 gv = (gw & ac.gf) ? SA[g + 1] : g;
         const bool kept = gv - g >= lt && gv - g < lt + eq;
-        s_oi[pos] = s_i[x];
-        s_og[pos] = r | (kept ? (gw & ac.gf) : 0u);
-        s_os[pos] = (uint8_t)((eq > 1 ? 2 : 1) | (kept ? 4 : 0));  // 4: rank unchanged, ISA already holds it
+        u32 hint = 0;   // PLCP hint of a suffix placed with an exact mark (ISA bits 28-31)
         if (kept && (gw & ac.gf) && lt > 0 && eq > 1 && eqb == 0) SA[r + 1] = gv;
         if (LCP && lt > 0 && eqb == 0) {
             u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
-            if (ex.on && !ex.skipped(g)) mark = h + ex.shared((u64)key, (u64)pk);
+            if (ex.on && !ex.skipped(g)) {
+                mark = h + ex.shared((u64)key, (u64)pk);
+                if (eq == 1) hint = ex.hint_bits(mark) >> 28;
+            }
             LCP[r] = mark;
         }
+        s_oi[pos] = s_i[x];
+        s_og[pos] = r | (kept ? (gw & ac.gf) : 0u);
+        // status: 1 placed, 2 survives; 4 rank unchanged (ISA already holds it);
+        // the hint from bit 3
+        s_os[pos] = (uint8_t)((eq > 1 ? 2 : 1) | (kept ? 4 : 0) | (hint << 3));
     };
     if (s_gmax <= (u32)RT_SORT_MIN) {
         // small groups: each entry counts over its group (O(size) per entry)
@@ -957,7 +965,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         if (st == 0) continue;
         const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
         const u32 r = s_og[x] & ~ac.gf;
-        if (!(st & 4)) ISA[v] = r;
+        if (!(st & 4)) ISA[v] = r | ((st >> 3) << 28);
         if ((st & 3) == 1) {
             SA[r] = v;
             if (ac.codes) ac.codes[r] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
