@@ -270,8 +270,12 @@ int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff
 int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb,
                      u64 total, int kb, int comp32, const u64 *piv, const u32 *npiv, const u32 *cscan, u64 npivot,
                      void *comp, u32 *vals, u32 *pq, u32 *ppv, hipStream_t s);
+// Rounds of at most 32 big groups count the partition per class and group, ccnt[(c *
+// nb + q) * tiles + t], and write no per-member group index (pq); big_part_groups(nb)
+// = nb then, else 0 (counts per class, ccnt[c * tiles + t]).
+u32 big_part_groups(u32 nb);
 // starts[b] (b = 0..BP) = the first member of class b after the scan of the counts
-int big_part_starts(const u32 *cscan, u64 total, u32 *starts, hipStream_t s);
+int big_part_starts(const u32 *cscan, u64 total, u32 nb, u32 *starts, hipStream_t s);
 // rk, rv: nb * BP entries (the pivot subgroups' group words and ranks); grank,
 // keepkey: as big_keep, for the flagged groups of a pivot round.  The pivot members
 // take the next active list's entries [base, base + npivot) (base = its length
@@ -279,9 +283,11 @@ int big_part_starts(const u32 *cscan, u64 total, u32 *starts, hipStream_t s);
 // base); the partition wrote their act_i there (ppv = out_i + base).
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
                   const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
-                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, hipStream_t s);
+                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, const u32 *cscan,
+                  u64 total, hipStream_t s);
 int big_piv_emit(const u32 *pq, const PivRegions &reg, const u32 *rk, const u32 *rv, const u32 *gslot, u32 gf,
-                 u64 base, u64 h, u32 *ISA, const u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s);
+                 u64 base, u64 h, u32 *ISA, const u32 *out_i, u32 *out_g, u32 *LCP, int cb, const u32 *cscan, u32 nb,
+                 u64 total, hipStream_t s);
 
 // ------------------------------------------------------------ SA kernels
 int launch_histogram(const uint8_t *T, u64 n, int gsa, u64 *hist256, u32 *gsa_bad, hipStream_t s);

@@ -1214,7 +1214,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
             BigPivots bp{};
             PivRegions reg{};
             u64 nrest = total;
-            u32 *pq = nullptr, *d_rk = nullptr, *d_rv = nullptr;
+            u32 *pq = nullptr, *d_rk = nullptr, *d_rv = nullptr, *d_ccnt = ws.bcnt;
             // flagged groups: the rank their members hold and the key of the sorted
             // members' subgroup whose slots hold it
             u32 *d_grank = nullptr;
@@ -1231,16 +1231,22 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 d_rv = rsc.get<u32>((u64)nb * BP);
                 if (!d_piv || !d_np || !d_eq || !d_rk || !d_rv) return SAGPU_ENOMEM;
                 const u64 ptiles = big_part_tiles(total);
+                // per class (and per group, for few groups) and tile
+                const u64 ncnt = (BP + 1) * (u64)(big_part_groups(nb) ? big_part_groups(nb) : 1u) * ptiles;
+                if (ncnt > ws.bcnt_cap) {
+                    d_ccnt = rsc.get<u32>(ncnt);
+                    if (!d_ccnt) return SAGPU_ENOMEM;
+                }
                 {
                     Span sp(c, F_ROUNDS);
                     SAGPU_TRY(big_pivots(key2, wide, d_bhead, d_boff, nb, d_piv, d_np, s));
-                    SAGPU_TRY(big_part_count(key2, wide, d_bhead, d_boff, nb, total, d_piv, d_np, ws.bcnt, s));
-                    SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, (BP + 1) * ptiles, s));
+                    SAGPU_TRY(big_part_count(key2, wide, d_bhead, d_boff, nb, total, d_piv, d_np, d_ccnt, s));
+                    SAGPU_TRY(scan_exclusive_u32(ws.scan, d_ccnt, ncnt, s));
                 }
                 u32 *preg = (u32 *)(c.pinned + 7200);   // past the round parameters
                 u32 *d_starts = rsc.get<u32>(BP + 1);
                 if (!d_starts) return SAGPU_ENOMEM;
-                SAGPU_TRY(big_part_starts(ws.bcnt, total, d_starts, s));
+                SAGPU_TRY(big_part_starts(d_ccnt, total, nb, d_starts, s));
                 SAGPU_CHECK(hipMemcpyAsync(preg, d_starts, (BP + 1) * sizeof(u32), hipMemcpyDeviceToHost, s));
                 SAGPU_CHECK(hipStreamSynchronize(s));
                 for (int b = 0; b <= BP; b++) reg.start[b] = preg[b];
@@ -1250,8 +1256,10 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                         return SAGPU_EDEVICE;
                     }
                 nrest = total - reg.start[BP];
-                pq = rsc.get<u32>(reg.start[BP] + 1);
-                if (!pq) return SAGPU_ENOMEM;
+                if (!big_part_groups(nb)) {   // few groups: their counts replace the index
+                    pq = rsc.get<u32>(reg.start[BP] + 1);
+                    if (!pq) return SAGPU_ENOMEM;
+                }
                 bp = BigPivots{d_piv, d_np, d_eq};
                 st.big_pivot_rounds++;
                 st.big_pivot_elems += (int64_t)reg.start[BP];
@@ -1265,7 +1273,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 // list, behind the tile kernel's out_n survivors
                 if (pivots)
                     SAGPU_TRY(big_part_scatter(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32,
-                                               bp.piv, bp.npiv, ws.bcnt, reg.start[BP], b0, bv0, pq,
+                                               bp.piv, bp.npiv, d_ccnt, reg.start[BP], b0, bv0, pq,
                                                act_i[nxt] + out_n, s));
                 else
                     SAGPU_TRY(big_extract(act_i[cur], key2, wide, d_bhead, d_boff, nb, total, kb, comp32, b0, bv0, s));
@@ -1278,7 +1286,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                 if (pivots)
                     SAGPU_TRY(big_piv_ranks(alt ? b1 : b0, comp32, kb, nrest, pq, reg, bp.piv, bp.npiv, d_bslot, ac.gf,
                                             dSA, d_grank, d_keepkey, nb, (u32 *)bp.eq, d_rk, d_rv, out_counter,
-                                            out_n, err, s));
+                                            out_n, err, d_ccnt, total, s));
                 else
                     SAGPU_TRY(big_keep(alt ? b1 : b0, comp32, d_boff, kb, nb, d_bslot, ac.gf, dSA, d_grank, d_keepkey,
                                        s));
@@ -1288,7 +1296,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
                                        st.rounds == 1 && rk1.pw ? &ex1 : nullptr, pivots ? &bp : nullptr));
                 if (pivots)
                     SAGPU_TRY(big_piv_emit(pq, reg, d_rk, d_rv, d_bslot, ac.gf, out_n, h, dISA, act_i[nxt],
-                                           act_g[nxt], dLCP, ac.cb, s));
+                                           act_g[nxt], dLCP, ac.cb, d_ccnt, nb, total, s));
             }
             SAGPU_CHECK(hipMemcpyAsync(c.pinned, d_cnt, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
             SAGPU_CHECK(hipStreamSynchronize(s));

@@ -24,6 +24,9 @@ namespace {
 
 constexpr int PT_THREADS = 256, PT_ROWS = 16, PT_SPAN = 1024, PT_TILE = 4096;
 constexpr int PE_ROWS = 16;   // rows of 64 pivot members per wave in the emission
+// Rounds with at most this many big groups count the partition per class and group
+// (no per-member group index)
+constexpr u32 PQ_GROUPS = 32;
 constexpr u32 NOQ = 0xFFFFFFFFu;
 
 __device__ __forceinline__ u32 group_of(const u64 *__restrict__ boff, u32 nb, u64 e)
@@ -113,14 +116,18 @@ __global__ __launch_bounds__(256) void k_big_pivots(const KT *__restrict__ key2,
 }
 
 // Partition pass 1: members of each class per tile, class-major: ccnt[c * tiles + t].
+// With gq > 0 (gq = nb groups, at most PQ_GROUPS) the counts are per class and group,
+// ccnt[(c * gq + q) * tiles + t]: after the scan, the pivot subgroups' sizes and
+// starts come from the counts, and no per-member group index (pq) is written.
 template <typename KT>
 __global__ __launch_bounds__(PT_THREADS) void k_part_count(const KT *__restrict__ key2, const u32 *__restrict__ bhead,
                                                            const u64 *__restrict__ boff, u32 nb, u64 total,
                                                            const u64 *__restrict__ piv, const u32 *__restrict__ npiv,
-                                                           u32 *__restrict__ ccnt, u64 tiles)
+                                                           u32 *__restrict__ ccnt, u64 tiles, u32 gq)
 {
-    __shared__ u32 s_c[BP + 1];
-    if (threadIdx.x <= BP) s_c[threadIdx.x] = 0;
+    __shared__ u32 s_c[(BP + 1) * PQ_GROUPS];
+    const u32 nc = (BP + 1) * (gq ? gq : 1u);
+    for (u32 x = threadIdx.x; x < nc; x += PT_THREADS) s_c[x] = 0;
     __syncthreads();
     const u32 lane = lane_id();
     const u64 wbase = (u64)blockIdx.x * PT_TILE + (u64)(threadIdx.x / SAGPU_WAVE) * PT_SPAN;
@@ -130,26 +137,45 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(const KT *__restrict_
     for (int b = 0; b <= BP; b++) cnt[b] = 0;
     for (int r = 0; r < PT_ROWS; r++) {
         const u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
-        u32 c = BP + 1;
+        u32 c = BP + 1, q = 0;
         if (e < total) {
             cur.to(boff, bhead, piv, npiv, nb, e);
             c = cur.cls((u64)key2[cur.head + (e - cur.beg)]);
+            q = cur.q;
         }
+        if (gq) {
+            // a row of 64 consecutive members spans at most two groups (big groups
+            // hold more than 64 members)
+            const u32 q0 = __shfl(q, 0, SAGPU_WAVE);
+            const u64 lo = __ballot(e < total && q == q0), hi = __ballot(e < total && q != q0);
 #pragma unroll
-        for (int b = 0; b <= BP; b++) cnt[b] += (u32)__popcll(__ballot(c == (u32)b));
+            for (int b = 0; b <= BP; b++) {
+                const u64 m = __ballot(c == (u32)b);
+                if (lane == 0) {
+                    const u32 c0 = (u32)__popcll(m & lo), c1 = (u32)__popcll(m & hi);
+                    if (c0) atomicAdd(&s_c[(u32)b * gq + q0], c0);
+                    if (c1) atomicAdd(&s_c[(u32)b * gq + q0 + 1], c1);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b <= BP; b++) cnt[b] += (u32)__popcll(__ballot(c == (u32)b));
+        }
     }
-    if (lane == 0)
+    if (!gq && lane == 0)
         for (int b = 0; b <= BP; b++)
             if (cnt[b]) atomicAdd(&s_c[b], cnt[b]);
     __syncthreads();
-    if (threadIdx.x <= BP) ccnt[(u64)threadIdx.x * tiles + blockIdx.x] = s_c[threadIdx.x];
+    for (u32 x = threadIdx.x; x < nc; x += PT_THREADS) ccnt[(u64)x * tiles + blockIdx.x] = s_c[x];
 }
 
 // Partition pass 2 (after the exclusive scan of ccnt): members in tile order into
-// their class.  Pivot classes go to pq (dense group index) and ppv (the list entry)
-// at the scanned position, ppv being the next active list's act_i from the range
-// the pivot members take there; the rest to the sort buffers at position - npivot, keyed
-// (dense group index << kb) | key2 as the plain extraction keys them.
+// their class.  Pivot classes go to pq (dense group index; not with gq > 0) and ppv
+// (the list entry) at the scanned position, ppv being the next active list's act_i
+// from the range the pivot members take there; the rest to the sort buffers at
+// position - npivot, keyed (dense group index << kb) | key2 as the plain extraction
+// keys them.  Groups are contiguous in member order, so with per-group counts the
+// tile's class-c members start at the scanned count of (c, the tile's first group).
 template <typename KT, typename CT>
 __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restrict__ act_i, const KT *__restrict__ key2,
                                                              const u32 *__restrict__ bhead,
@@ -158,7 +184,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restri
                                                              const u32 *__restrict__ npiv,
                                                              const u32 *__restrict__ cscan, u64 tiles, u64 npivot,
                                                              CT *__restrict__ comp, u32 *__restrict__ vals,
-                                                             u32 *__restrict__ pq, u32 *__restrict__ ppv)
+                                                             u32 *__restrict__ pq, u32 *__restrict__ ppv, u32 gq)
 {
     __shared__ u32 s_wc[PT_THREADS / SAGPU_WAVE][BP + 1];
     const u32 lane = lane_id(), w = threadIdx.x / SAGPU_WAVE;
@@ -185,9 +211,10 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restri
         for (int b = 0; b <= BP; b++) s_wc[w][b] = cnt[b];
     __syncthreads();
     u64 pos[BP + 1];   // next output position of each class for this wave
+    const u32 qt = gq ? group_of(boff, nb, (u64)blockIdx.x * PT_TILE) : 0u;   // the tile's first group
 #pragma unroll
     for (int b = 0; b <= BP; b++) {
-        u64 o = cscan[(u64)b * tiles + blockIdx.x];
+        u64 o = cscan[((u64)b * (gq ? gq : 1u) + qt) * tiles + blockIdx.x];
         for (u32 x = 0; x < w; x++) o += s_wc[x][b];
         pos[b] = o;
     }
@@ -205,7 +232,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(const u32 *__restri
         cur.to(boff, bhead, piv, npiv, nb, e);
         const u64 k = cur.head + (e - cur.beg);
         if (c < BP) {
-            pq[mine] = cur.q;
+            if (!gq) pq[mine] = cur.q;
             ppv[mine] = act_i[k];
         } else {
             const u64 o = mine - npivot;
@@ -260,7 +287,8 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
                                                    u32 *__restrict__ grank, u64 *__restrict__ keepkey, u32 nb,
                                                    u32 *__restrict__ eq, u32 *__restrict__ rk, u32 *__restrict__ rv,
                                                    u64 *__restrict__ out_counter, u64 base,
-                                                   u32 *__restrict__ err)
+                                                   u32 *__restrict__ err, const u32 *__restrict__ cscan, u64 tiles,
+                                                   u32 gq)
 {
     // the pivot members' range of the next list starts at base (the tile kernel's
     // survivors before it; the partition wrote their act_i there already)
@@ -281,8 +309,13 @@ __global__ __launch_bounds__(256) void k_piv_ranks(const CT *__restrict__ rest, 
     for (u32 b = 0; b < np; b++) {
         const u64 pk = piv[(u64)q * BP + b];
         const u64 lt = lower_bound_dev(rest, nrest, g0 | pk) - rest0;
-        const u64 r0 = reg.start[b], len = reg.start[b + 1] - r0;
-        const u64 c = lower_bound_dev(pq + r0, len, (u64)q + 1) - lower_bound_dev(pq + r0, len, (u64)q);
+        u64 c;
+        if (gq) {   // per-group counts: the scanned counts of (b, q) and (b, q + 1)
+            c = (u64)cscan[((u64)b * gq + q + 1) * tiles] - (u64)cscan[((u64)b * gq + q) * tiles];
+        } else {
+            const u64 r0 = reg.start[b], len = reg.start[b + 1] - r0;
+            c = lower_bound_dev(pq + r0, len, (u64)q + 1) - lower_bound_dev(pq + r0, len, (u64)q);
+        }
         if (c < 2 && w0) atomicOr(err, 4u);   // a pivot is sampled twice, so it has two members
         const u64 pos = lt + below;     // the subgroup's first position among the group's slots
         const u32 first = g + (u32)pos;
@@ -323,8 +356,51 @@ __global__ __launch_bounds__(256) void k_piv_emit(const u32 *__restrict__ pq, Pi
                                                   const u32 *__restrict__ rk, const u32 *__restrict__ rv,
                                                   const u32 *__restrict__ gslot, u32 gf, u64 base, u32 h,
                                                   u32 *__restrict__ ISA, const u32 *__restrict__ out_i,
-                                                  u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb)
+                                                  u32 *__restrict__ out_g, u32 *__restrict__ LCP, int cb,
+                                                  const u32 *__restrict__ cscan, u64 tiles, u32 gq)
 {
+    if (gq) {
+        // per-group counts: run ri = b * gq + q of the pivot region is [R(ri), R(ri + 1))
+        // with R(ri) = cscan[ri * tiles]; a lane's members only move forward
+        const u64 npv = reg.start[BP];
+        const u32 lane = lane_id();
+        const u64 wave = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / SAGPU_WAVE;
+        const u64 c0 = wave * (u64)(PE_ROWS * SAGPU_WAVE);
+        if (c0 >= npv) return;   // whole waves leave together
+        const u32 nrun = BP * gq;
+        u32 ri = 0, w = 0, r = 0, g = 0, nv = 0;
+        u64 rlo = 0, rhi = 0;   // the current run
+        bool have = false;
+        for (int row = 0; row < PE_ROWS; row++) {
+            const u64 m = c0 + (u64)row * SAGPU_WAVE + lane;
+            if (m >= npv) break;
+            if (!have || m >= rhi) {
+                if (!have) {   // the last run starting at or before m
+                    u32 lo = 0, hi = nrun - 1;
+                    while (lo < hi) {
+                        const u32 mid = (lo + hi + 1) >> 1;
+                        if ((u64)cscan[(u64)mid * tiles] <= m) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    ri = lo;
+                } else {
+                    while ((u64)cscan[(u64)(ri + 1) * tiles] <= m) ri++;
+                }
+                have = true;
+                rlo = cscan[(u64)ri * tiles];
+                rhi = cscan[(u64)(ri + 1) * tiles];
+                const u32 b = ri / gq, q = ri % gq;
+                w = rk[(u64)q * BP + b];
+                r = w & ~gf;
+                g = gslot[q] & ~gf;
+                nv = rv[(u64)q * BP + b];
+            }
+            if (nv != PIV_KEEP) ISA[out_i[base + m] >> cb] = nv;
+            out_g[base + m] = w;
+            if (LCP && r != g && m == rlo) LCP[r] = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
+        }
+        return;
+    }
     // each wave takes PE_ROWS rows of 64 consecutive members, a lane the same column
     // of every row: a lane's members mostly share one subgroup, whose word and rank
     // it keeps in registers, and the member before each (for the run starts) is
@@ -381,17 +457,20 @@ int big_pivots(const void *key2, int wide, const u32 *bhead, const u64 *boff, u3
     return SAGPU_OK;
 }
 
+u32 big_part_groups(u32 nb) { return nb <= PQ_GROUPS ? nb : 0u; }
+
 int big_part_count(const void *key2, int wide, const u32 *bhead, const u64 *boff, u32 nb, u64 total, const u64 *piv,
                    const u32 *npiv, u32 *ccnt, hipStream_t s)
 {
     const u64 tiles = big_part_tiles(total);
     if (total == 0 || nb == 0) return SAGPU_OK;
+    const u32 gq = big_part_groups(nb);
     if (wide)
         hipLaunchKernelGGL(k_part_count<u64>, dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, (const u64 *)key2, bhead,
-                           boff, nb, total, piv, npiv, ccnt, tiles);
+                           boff, nb, total, piv, npiv, ccnt, tiles, gq);
     else
         hipLaunchKernelGGL(k_part_count<u32>, dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, (const u32 *)key2, bhead,
-                           boff, nb, total, piv, npiv, ccnt, tiles);
+                           boff, nb, total, piv, npiv, ccnt, tiles, gq);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
@@ -405,7 +484,7 @@ int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bh
 #define PT_SCATTER(KT, CT)                                                                                          \
     hipLaunchKernelGGL((k_part_scatter<KT, CT>), dim3((unsigned)tiles), dim3(PT_THREADS), 0, s, act_i,            \
                        (const KT *)key2, bhead, boff, nb, total, kb, piv, npiv, cscan, tiles, npivot, (CT *)comp, \
-                       vals, pq, ppv)
+                       vals, pq, ppv, big_part_groups(nb))
     if (wide) {
         if (comp32) PT_SCATTER(u64, u32);
         else PT_SCATTER(u64, u64);
@@ -421,22 +500,26 @@ int big_part_scatter(const u32 *act_i, const void *key2, int wide, const u32 *bh
 // The class regions' starts after the exclusive scan of the class-major tile
 // counts: starts[b] = ccnt[b * tiles] for b = 0..BP (one copy to the host instead of
 // BP + 1).
-__global__ void k_part_starts(const u32 *__restrict__ ccnt, u64 tiles, u32 *__restrict__ starts)
+__global__ void k_part_starts(const u32 *__restrict__ ccnt, u64 tiles, u32 gq, u32 *__restrict__ starts)
 {
-    if (threadIdx.x <= BP) starts[threadIdx.x] = ccnt[(u64)threadIdx.x * tiles];
+    if (threadIdx.x <= BP) starts[threadIdx.x] = ccnt[(u64)threadIdx.x * (gq ? gq : 1u) * tiles];
 }
 
-int big_part_starts(const u32 *cscan, u64 total, u32 *starts, hipStream_t s)
+int big_part_starts(const u32 *cscan, u64 total, u32 nb, u32 *starts, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(64), 0, s, cscan, big_part_tiles(total), starts);
+    hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(64), 0, s, cscan, big_part_tiles(total), big_part_groups(nb),
+                       starts);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
 int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq, const PivRegions &reg,
                   const u64 *piv, const u32 *npiv, const u32 *gslot, u32 gf, u32 *SA, u32 *grank, u64 *keepkey,
-                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, hipStream_t s)
+                  u32 nb, u32 *eq, u32 *rk, u32 *rv, u64 *out_counter, u64 base, u32 *err, const u32 *cscan,
+                  u64 total, hipStream_t s)
 {
+    const u64 tiles = big_part_tiles(total);
+    const u32 gq = big_part_groups(nb);
     if (gf && (!grank || !keepkey)) {
         sagpu_set_error("flagged groups need their ranks and kept keys", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
@@ -445,23 +528,25 @@ int big_piv_ranks(const void *rest, int comp32, int kb, u64 nrest, const u32 *pq
     const unsigned grid = (nb + waves - 1) / waves > 0 ? (nb + waves - 1) / waves : 1;
     if (comp32)
         hipLaunchKernelGGL(k_piv_ranks<u32>, dim3(grid), dim3(256), 0, s, (const u32 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err, cscan, tiles, gq);
     else
         hipLaunchKernelGGL(k_piv_ranks<u64>, dim3(grid), dim3(256), 0, s, (const u64 *)rest, nrest, kb, pq, reg, piv,
-                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err);
+                           npiv, gslot, gf, SA, grank, keepkey, nb, eq, rk, rv, out_counter, base, err, cscan, tiles, gq);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
 
 int big_piv_emit(const u32 *pq, const PivRegions &reg, const u32 *rk, const u32 *rv, const u32 *gslot, u32 gf,
-                 u64 base, u64 h, u32 *ISA, const u32 *out_i, u32 *out_g, u32 *LCP, int cb, hipStream_t s)
+                 u64 base, u64 h, u32 *ISA, const u32 *out_i, u32 *out_g, u32 *LCP, int cb, const u32 *cscan, u32 nb,
+                 u64 total, hipStream_t s)
 {
     const u64 npv = reg.start[BP];
     if (npv == 0) return SAGPU_OK;
     const u32 hh = h < LCP_BOUND_CAP ? (u32)h : LCP_BOUND_CAP;
     const u64 per_block = 256 / SAGPU_WAVE * (u64)(PE_ROWS * SAGPU_WAVE);
     hipLaunchKernelGGL(k_piv_emit, dim3((unsigned)((npv + per_block - 1) / per_block)), dim3(256), 0, s, pq, reg, rk,
-                       rv, gslot, gf, base, hh, ISA, out_i, out_g, LCP, cb);
+                       rv, gslot, gf, base, hh, ISA, out_i, out_g, LCP, cb, cscan, big_part_tiles(total),
+                       big_part_groups(nb));
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
 }
