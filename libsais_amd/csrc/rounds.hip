@@ -674,7 +674,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     static_assert(RT_CAP % RT_THREADS == 0, "tile capacity must split evenly");
     constexpr u32 NONE = 0xFFFFFFFFu;
     constexpr u16 NONE16 = 0xFFFFu;   // local indexes are below 2048: 16 bits each
-    __shared__ u32 s_w[WIN];        // act_g over the window
+    // act_g over the window, and over the RT_SMALL entries before it: whether the
+    // group straddling lo has more than RT_SMALL members is then an LDS compare
+    __shared__ u32 s_wx[RT_SMALL + WIN];
+    u32 *const s_w = s_wx + RT_SMALL;   // s_w[x] = act_g[lo + x], x >= -RT_SMALL
     __shared__ u16 s_hx[WIN];       // local index of the position's group head (NONE16: head before lo)
     __shared__ u16 s_gend[RANGE];   // last local index of the group headed at local x (NONE16: past the window)
     // suffixes and keys of the tile's range, by local position (every small group
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     u32 *const s_oi = reinterpret_cast<u32 *>(s_o);
     u32 *const s_og = reinterpret_cast<u32 *>(s_o) + RT_CAP;
     __shared__ uint8_t s_os[RT_CAP];
-    __shared__ u32 s_first, s_end, s_gmax, s_wsum[4], s_prev;
+    __shared__ u32 s_first, s_end, s_gmax, s_wsum[4];
     __shared__ u64 s_base;
 
     const u64 a = rp[0];
@@ -701,13 +704,13 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
 
     // 1. the window, the entry before it, and the group head of every position (a
     //    max-scan over head flags: local index + 1, 0 = head before lo)
-    for (u32 x = tid; x < wlen; x += RT_THREADS) s_w[x] = act_g[lo + x];
+    for (u32 x = tid; x < wlen + (u32)RT_SMALL; x += RT_THREADS)
+        s_wx[x] = lo + x >= (u64)RT_SMALL ? act_g[lo + x - RT_SMALL] : NONE;
     for (u32 x = tid; x < (u32)RT_CAP; x += RT_THREADS) s_os[x] = 0;
     if (tid == 0) {
         s_first = NONE;
         s_end = 0;
         s_gmax = 0;
-        s_prev = lo > 0 ? act_g[lo - 1] : NONE;
     }
     for (u32 x = tid; x < (u32)RANGE; x += RT_THREADS) s_gend[x] = NONE16;
     __syncthreads();
@@ -716,7 +719,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     // head at lo, of a group of at least WIN > RT_SMALL members, is recorded.  On
     // texts of long repeats most windows are of this kind.
     if (wlen == (u32)WIN && s_w[0] == s_w[WIN - 1]) {
-        if (tid == 0 && (lo == 0 || s_prev != s_w[0]))
+        if (tid == 0 && (lo == 0 || s_w[-1] != s_w[0]))
             rt_big_head<RANGE>(lo, s_w[0], seg_ctr, bseg_h, bseg_g, bseg_cap, err);
         return;
     }
@@ -726,7 +729,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
 #pragma unroll
         for (int j = 0; j < W_PER; j++) {
             const u32 x = tid * W_PER + j;
-            const bool head = x < wlen && (x == 0 ? (lo == 0 || s_prev != s_w[0]) : s_w[x - 1] != s_w[x]);
+            const bool head = x < wlen && (x == 0 ? (lo == 0 || s_w[-1] != s_w[0]) : s_w[x - 1] != s_w[x]);
             if (head) m = x + 1;
             loc[j] = m;
         }
@@ -758,7 +761,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             else {
                 // the group holding position lo has its head before lo
                 const u64 e = lo + x;
-                big = e >= (u64)RT_SMALL && act_g[e - RT_SMALL] == s_w[x];
+                big = e >= (u64)RT_SMALL && s_w[(int)x - RT_SMALL] == s_w[x];
             }
             if (big) SA[s_w[x] & ~ac.gf] = (u32)(lo + x);
         }
