@@ -678,17 +678,22 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
     // group straddling lo has more than RT_SMALL members is then an LDS compare
     __shared__ u32 s_wx[RT_SMALL + WIN];
     u32 *const s_w = s_wx + RT_SMALL;   // s_w[x] = act_g[lo + x], x >= -RT_SMALL
-    __shared__ u16 s_hx[WIN];       // local index of the position's group head (NONE16: head before lo)
     __shared__ u16 s_gend[RANGE];   // last local index of the group headed at local x (NONE16: past the window)
-    // suffixes and keys of the tile's range, by local position (every small group
-    // of this workgroup lies in [lo, lo + RT_CAP)); outputs by local position (s_oi,
-    // s_og), which share their LDS with the sort keys of the ranking by sorting
-    __shared__ u32 s_i[RT_CAP];
-    __shared__ KT s_k[RT_CAP];
-    __shared__ u64 s_o[RT_CAP];
-    u32 *const s_oi = reinterpret_cast<u32 *>(s_o);
-    u32 *const s_og = reinterpret_cast<u32 *>(s_o) + RT_CAP;
-    __shared__ uint8_t s_os[RT_CAP];
+    // One block for: the keys of the range by local position (every small group of
+    // this workgroup lies in [lo, lo + RT_CAP)), the local index of each position's
+    // group head (NONE16: head before lo) and the survivor flags by output position;
+    // the sort keys of the ranking by sorting take the whole block, and the
+    // survivors' output ranks take the keys' place once the ranking is done.  The
+    // list entries stay in registers of the threads that rank them, so a workgroup
+    // needs about 15 B of LDS per list entry and 8 workgroups of 1024 entries fit a CU.
+    constexpr u32 RAW_A = (u32)sizeof(KT) * RT_CAP + 2u * ((WIN + 1) & ~1) + RT_CAP;
+    constexpr u32 RAW = RAW_A > 8u * RT_CAP ? RAW_A : 8u * RT_CAP;
+    __shared__ __align__(8) uint8_t s_raw[RAW];
+    KT *const s_k = reinterpret_cast<KT *>(s_raw);
+    u16 *const s_hx = reinterpret_cast<u16 *>(s_raw + sizeof(KT) * RT_CAP);
+    uint8_t *const s_os = s_raw + sizeof(KT) * RT_CAP + 2u * ((WIN + 1) & ~1);
+    u64 *const s_o = reinterpret_cast<u64 *>(s_raw);   // sort keys
+    u16 *const s_rk = reinterpret_cast<u16 *>(s_raw);  // survivors' output ranks
     __shared__ u32 s_first, s_end, s_gmax, s_wsum[4];
     __shared__ u64 s_base;
 
@@ -803,25 +808,35 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
-    // the range's suffixes and keys
-    for (u32 x = first + tid; x < end; x += RT_THREADS) {
-        s_i[x] = act_i[lo + x];
-        s_k[x] = key2[lo + x];
+    // the range's keys into LDS; each thread's own entries (positions first + tid +
+    // j * RT_THREADS) into its registers
+    u32 e_ai[RT_PER];
+#pragma unroll
+    for (int j = 0; j < RT_PER; j++) {
+        const u32 x = first + tid + (u32)j * RT_THREADS;
+        e_ai[j] = 0;
+        if (x < end) {
+            s_k[x] = key2[lo + x];
+            e_ai[j] = act_i[lo + x];
+        }
     }
     __syncthreads();
 
     // 4. rank inside the group by key2: lt smaller, eq equal, eqb equal before (in
     //    list order), pk the largest smaller key; local positions throughout.  The
     //    entry goes to position hx + lt + eqb, its subgroup's first slot is g + lt.
-    auto place = [&](u32 x, u32 hx, u32 lt, u32 eq, u32 eqb, KT key, KT pk) {
+    //    Per entry of the thread: output position (NONE: none), group word for the
+    //    next list, the rank written to ISA, flags (1: survives, 2: ISA written).
+    u32 e_pos[RT_PER], e_word[RT_PER], e_isa[RT_PER], e_fl[RT_PER];
+    auto place = [&](int j, u32 hx, u32 lt, u32 eq, u32 eqb, KT key, KT pk) {
         const u32 gw = s_w[hx];   // first slot | flag
         const u32 g = gw & ~ac.gf;
-        const u32 pos = hx + lt + eqb;
         const u32 r = g + lt;
         // the subgroup whose slots hold the group's rank keeps rank and flag (a
         // flagged group's rank is stored in SA[g + 1]; its other slots, and those of
         // the next subgroup's first slot + 1, are unplaced until this round's
-        // writes); the others rank at their first slot, unflagged
+        // writes, which come after a barrier); the others rank at their first slot,
+        // unflagged
         const u32 gv = (gw & ac.gf) ? SA[g + 1] : g;
         const bool kept = gv - g >= lt && gv - g < lt + eq;
         u32 hint = 0;   // PLCP hint of a suffix placed with an exact mark (ISA bits 28-31)
@@ -830,19 +845,23 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             u32 mark = LCP_UNKNOWN | (h < LCP_BOUND_CAP ? h : LCP_BOUND_CAP);
             if (ex.on && !ex.skipped(g)) {
                 mark = h + ex.shared((u64)key, (u64)pk);
-                if (eq == 1) hint = ex.hint_bits(mark) >> 28;
+                if (eq == 1) hint = ex.hint_bits(mark);
             }
             LCP[r] = mark;
         }
-        s_oi[pos] = s_i[x];
-        s_og[pos] = r | (kept ? (gw & ac.gf) : 0u);
-        // status: 1 placed, 2 survives; 4 rank unchanged (ISA already holds it);
-        // the hint from bit 3
-        s_os[pos] = (uint8_t)((eq > 1 ? 2 : 1) | (kept ? 4 : 0) | (hint << 3));
+        e_pos[j] = hx + lt + eqb;
+        e_word[j] = r | (kept ? (gw & ac.gf) : 0u);
+        e_isa[j] = r | hint;
+        e_fl[j] = (eq > 1 ? 1u : 0u) | (kept ? 0u : 2u);
     };
+#pragma unroll
+    for (int j = 0; j < RT_PER; j++) e_pos[j] = NONE;
     if (s_gmax <= (u32)RT_SORT_MIN) {
         // small groups: each entry counts over its group (O(size) per entry)
-        for (u32 x = first + tid; x < end; x += RT_THREADS) {
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            const u32 x = first + tid + (u32)j * RT_THREADS;
+            if (x >= end) continue;
             const u32 hx = s_hx[x];
             const u32 ge = s_gend[hx];
             if (ge == NONE16 || ge - hx + 1 > (u32)RT_SMALL) continue;   // big group: not ours
@@ -857,20 +876,32 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 eqb += (ky == key) & (y < x);
                 pk = (ky < key && ky > pk) ? ky : pk;
             }
-            place(x, hx, lt, eq, eqb, key, pk);
+            place(j, hx, lt, eq, eqb, key, pk);
         }
     } else {
         // groups of more than RT_SORT_MIN members: the range [first, end) sorted by
         // (group head, key2, position) in LDS (a bitonic network whose comparators
         // all put the smaller at the lower index, so the positions past the range
         // act as +inf and are never touched), then every entry finds its subgroup's
-        // bounds by binary search inside its group's sorted stretch
+        // bounds by binary search inside its group's sorted stretch.  The entry at
+        // sorted position p is not this thread's own: its list word is read again
+        // (it was read by this workgroup just before).
         constexpr int XB = 11;   // position and head fields (RT_CAP <= 2048)
         constexpr u64 KMASK = (1ull << (64 - 2 * XB)) - 1;
         static_assert(RT_CAP <= (1 << XB), "sort fields too narrow");
         const u32 cnt = end - first;
-        for (u32 p = tid; p < cnt; p += RT_THREADS)
-            s_o[p] = ((u64)s_hx[first + p] << (64 - XB)) | ((u64)s_k[first + p] << XB) | p;
+        u64 comp[RT_PER];
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            const u32 p = tid + (u32)j * RT_THREADS;
+            comp[j] = p < cnt ? ((u64)s_hx[first + p] << (64 - XB)) | ((u64)s_k[first + p] << XB) | p : 0ull;
+        }
+        __syncthreads();   // the keys and heads are read; the sort keys take their LDS
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            const u32 p = tid + (u32)j * RT_THREADS;
+            if (p < cnt) s_o[p] = comp[j];
+        }
         __syncthreads();
         u32 lg = 0;
         while ((1u << lg) < cnt) lg++;
@@ -878,8 +909,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         for (u32 kl = 1; kl <= lg; kl++) {
             // mirror step: i against the mirrored partner in its block of 2^kl
             for (u32 pi = tid; pi < npair; pi += RT_THREADS) {
-                const u32 blk = (pi >> (kl - 1)) << kl, j = pi & ((1u << (kl - 1)) - 1);
-                const u32 i = blk + j, l = blk + (1u << kl) - 1 - j;
+                const u32 blk = (pi >> (kl - 1)) << kl, jj = pi & ((1u << (kl - 1)) - 1);
+                const u32 i = blk + jj, l = blk + (1u << kl) - 1 - jj;
                 if (l < cnt) {
                     const u64 u = s_o[i], w = s_o[l];
                     if (u > w) {
@@ -903,12 +934,12 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 __syncthreads();
             }
         }
-        u32 rx[RT_PER], rh[RT_PER], rlt[RT_PER], req[RT_PER], reqb[RT_PER];
+        u32 rh[RT_PER], rlt[RT_PER], req[RT_PER], reqb[RT_PER];
         KT rkey[RT_PER], rpk[RT_PER];
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
             const u32 p = tid + (u32)j * RT_THREADS;
-            rx[j] = NONE;
+            rh[j] = NONE;
             if (p >= cnt) continue;
             const u64 c = s_o[p];
             const u32 hx = (u32)(c >> (64 - XB));
@@ -927,7 +958,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
                 if ((s_o[m] >> XB) <= ck) b0 = m + 1;
                 else b1 = m;
             }
-            rx[j] = first + (u32)(c & ((1u << XB) - 1));
+            e_ai[j] = act_i[lo + first + (u32)(c & ((1u << XB) - 1))];
             rh[j] = hx;
             rlt[j] = a0 - (hx - first);
             req[j] = b0 - a0;
@@ -935,56 +966,69 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
             rkey[j] = (KT)(ck & KMASK);
             rpk[j] = rlt[j] > 0 ? (KT)((s_o[a0 - 1] >> XB) & KMASK) : (KT)0;
         }
-        __syncthreads();   // the sort keys are read; their LDS takes the outputs now
+        __syncthreads();   // the sort keys are read; the survivor flags take their LDS again
+        for (u32 x = tid; x < (u32)RT_CAP; x += RT_THREADS) s_os[x] = 0;
 #pragma unroll
         for (int j = 0; j < RT_PER; j++)
-            if (rx[j] != NONE) place(rx[j], rh[j], rlt[j], req[j], reqb[j], rkey[j], rpk[j]);
+            if (rh[j] != NONE) place(j, rh[j], rlt[j], req[j], reqb[j], rkey[j], rpk[j]);
+    }
+    __syncthreads();   // every read of a flagged group's stored rank is done
+
+    // 5. ranks and placed suffixes; survivor flags by output position
+#pragma unroll
+    for (int j = 0; j < RT_PER; j++) {
+        if (e_pos[j] == NONE) continue;
+        const u32 v = e_ai[j] >> ac.cb;   // list entries may carry the BWT code
+        if (e_fl[j] & 2u) ISA[v] = e_isa[j];
+        if (!(e_fl[j] & 1u)) {
+            const u32 r = e_word[j] & ~ac.gf;
+            SA[r] = v;
+            if (ac.codes) ac.codes[r] = (uint8_t)(e_ai[j] & ((1u << ac.cb) - 1u));
+        } else {
+            s_os[e_pos[j]] = 1;
+        }
     }
     __syncthreads();
 
-    // 5. survivors: counted first, and their range reserved in this XCD's segment
-    //    (one atomic on the XCD's counter, in flight while this workgroup writes
-    //    ranks and placed suffixes).  A counter shared by all workgroups serialises
-    //    them: 300 K returning atomics on one address take 3.6 ms, on 8 addresses
-    //    0.57 ms; workgroups go to the XCDs round-robin (blockIdx % 8).
-    u32 cnt[RT_PER];
+    // 6. survivors: output ranks by position, their range reserved in this XCD's
+    //    segment (one atomic on the XCD's counter).  A counter shared by all
+    //    workgroups serialises them: 300 K returning atomics on one address take
+    //    3.6 ms, on 8 addresses 0.57 ms; workgroups go to the XCDs round-robin
+    //    (blockIdx % 8).
+    u32 cnt_s[RT_PER];
     u32 tot = 0;
 #pragma unroll
     for (int j = 0; j < RT_PER; j++) {
         const u32 x = first + tid * RT_PER + j;
-        cnt[j] = (x < end && (s_os[x] & 3) == 2) ? 1u : 0u;
-        tot += cnt[j];
+        cnt_s[j] = (x < end && s_os[x]) ? 1u : 0u;
+        tot += cnt_s[j];
     }
     const u32 inc = wave_incl_sum(tot);
     if (lane_id() == SAGPU_WAVE - 1) s_wsum[tid / SAGPU_WAVE] = inc;
     __syncthreads();
     if (tid == 0) {
         const u32 all = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-        const u32 x = blockIdx.x & 7u;
-        s_base = (u64)x * seg_cap + (all ? atomicAdd(&seg_ctr[x * 16], (u64)all) : 0ull);
+        const u32 xcd = blockIdx.x & 7u;
+        s_base = (u64)xcd * seg_cap + (all ? atomicAdd(&seg_ctr[xcd * 16], (u64)all) : 0ull);
     }
-    for (u32 x = first + tid; x < end; x += RT_THREADS) {
-        const u32 st = s_os[x];
-        if (st == 0) continue;
-        const u32 v = s_oi[x] >> ac.cb;   // list entries may carry the BWT code
-        const u32 r = s_og[x] & ~ac.gf;
-        if (!(st & 4)) ISA[v] = r | ((st >> 3) << 28);
-        if ((st & 3) == 1) {
-            SA[r] = v;
-            if (ac.codes) ac.codes[r] = (uint8_t)(s_oi[x] & ((1u << ac.cb) - 1u));
+    {
+        u32 o = inc - tot;
+        for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
+#pragma unroll
+        for (int j = 0; j < RT_PER; j++) {
+            const u32 x = first + tid * RT_PER + j;
+            if (x < end) s_rk[x] = (u16)o;
+            o += cnt_s[j];
         }
     }
     __syncthreads();
-    u64 o = s_base + inc - tot;
-    for (u32 q = 0; q < tid / SAGPU_WAVE; q++) o += s_wsum[q];
+    const u64 base = s_base;
 #pragma unroll
     for (int j = 0; j < RT_PER; j++) {
-        const u32 x = first + tid * RT_PER + j;
-        if (cnt[j]) {
-            seg_i[o] = s_oi[x];
-            seg_g[o] = s_og[x];
-            o++;
-        }
+        if (e_pos[j] == NONE || !(e_fl[j] & 1u)) continue;
+        const u64 o = base + s_rk[e_pos[j]];
+        seg_i[o] = e_ai[j];
+        seg_g[o] = e_word[j];
     }
 }
 
