@@ -115,7 +115,7 @@ constexpr u64 INIT_SMALL_N = 1ull << 20;
 
 struct WsScratch {
     void *tile_agg;   // [tiles_cap] 16-byte tile aggregates
-    u64 tiles_cap;    // tiles of 4096 elements
+    u64 tiles_cap;    // tiles of at least 2048 elements (512 below INIT_SMALL_N)
     u64 *total;       // [1]
     u64 *base;        // [1] reserved output base
     u32 *bcnt;        // [bcnt_cap] ISA bucket counts per tile, then their scan (initial grouping)
@@ -210,7 +210,10 @@ struct ExactLcp {
 // The round's list length a and offset h are read from rp[0], rp[1] in device memory
 // (the launch geometry is sized on the host for a list of a_grid entries, which
 // must be at least rp[0]), so one captured launch serves every later round.
-extern int g_tile_range;   // list entries per workgroup of the tile kernel (the "tile_range" option)
+extern int g_tile_range;
+// rows of 64 positions per wave span in the initial grouping of texts of at least
+// INIT_SMALL_N suffixes (8 or 16: tiles of 2048 or 4096 positions; the "init_rows" option)
+extern int g_init_rows;   // list entries per workgroup of the tile kernel (the "tile_range" option)
 int round_gather(const u32 *act_i, const u32 *act_g, u64 a_grid, const u64 *rp, const u32 *ISA, u64 n, void *key2,
                  int wide, hipStream_t s, const RawKey1 *rk = nullptr, int cb = 0);
 // The tile kernel's survivors go to 8 per-XCD segments of seg_i / seg_g (capacity

@@ -948,7 +948,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     void *key2 = vA;
 
     WsScratch ws{};
-    ws.tiles_cap = n / (n < INIT_SMALL_N ? 512 : 4096) + 2;   // tiles of the initial grouping
+    ws.tiles_cap = n / (n < INIT_SMALL_N ? 512 : 2048) + 2;   // tiles of the initial grouping
     {
         GET(tagg, u64, ws.tiles_cap * 2);
         ws.tile_agg = tagg;
@@ -957,7 +957,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     }
     SAGPU_TRY(ensure_radix_scratch(c, sc, n));
     {
-        // bucket counts of the initial grouping: at most 64 buckets per 4096-entry tile
+        // bucket counts of the initial grouping: at most 64 buckets per tile
         ws.bcnt_cap = 64 * ws.tiles_cap;
         GET(bcnt, u32, ws.bcnt_cap);
         ws.bcnt = bcnt;
@@ -2291,6 +2291,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "bwt_variant")) {
         if (value < 0 || value > 2) return SAGPU_EINVAL;
         g_bwt_variant = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "init_rows")) {
+        if (value != 8 && value != 16) return SAGPU_EINVAL;
+        g_init_rows = (int)value;
         return 0;
     }
     if (!strcmp(name, "tile_range")) {

@@ -23,9 +23,9 @@
 namespace {
 
 // ---------------------------------------------------- warp-striped segmented scan
-// A tile of 4096 elements is split into four 1024-element wave spans, each walked
-// as 16 rows of 64 consecutive elements, so every load and store of a row is
-// coalesced.  The scan carries one sum (output index of active elements) and two
+// A tile is split into four wave spans, each walked as R rows of 64 consecutive
+// elements (R = 8 or 16: tiles of 2048 or 4096 elements; 2 for short texts), so
+// every load and store of a row is coalesced.  The scan carries one sum (output index of active elements) and two
 // max-scans (positions of the latest group starts).  Functors are called by every
 // lane of a row (lane l holds element row_base + l), also past the end, so they can
 // take neighbour values from the adjacent lanes.
@@ -37,17 +37,18 @@ struct Agg {
 
 __device__ __forceinline__ u32 umax(u32 a, u32 b) { return a > b ? a : b; }
 
-constexpr int WS_THREADS = 256, WS_ROWS = 16, WS_TILE = 4096, WS_SPAN = 1024, WS_WAVES = 4;
+constexpr int WS_THREADS = 256, WS_ROWS = 16, WS_WAVES = 4;
 
-template <class F>
+template <class F, int R>
 __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__ tagg)
 {
+    constexpr int SPAN = R * SAGPU_WAVE;
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
-    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
+    const u64 wbase = (u64)blockIdx.x * (WS_WAVES * SPAN) + (u64)w * SPAN;
     u64 s = 0;
     u32 m1 = 0, m2 = 0;
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         bool act;
         u32 v1, v2;
@@ -122,18 +123,19 @@ __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u6
     }
 }
 
-template <class F>
+template <class F, int R>
 __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restrict__ texcl, const u64 *__restrict__ base)
 {
+    constexpr int SPAN = R * SAGPU_WAVE;
     __shared__ Agg s_w[WS_WAVES];
     const u32 w = threadIdx.x / SAGPU_WAVE, lane = lane_id();
-    const u64 wbase = (u64)blockIdx.x * WS_TILE + (u64)w * WS_SPAN;
-    bool act[WS_ROWS];
-    u32 v1[WS_ROWS], v2[WS_ROWS];
+    const u64 wbase = (u64)blockIdx.x * (WS_WAVES * SPAN) + (u64)w * SPAN;
+    bool act[R];
+    u32 v1[R], v2[R];
     u64 s = 0;
     u32 m1 = 0, m2 = 0;
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         f.scan_inputs(e, act[r], v1[r], v2[r]);
         s += act[r];
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_emit(F f, const Agg *__restri
     }
     const u64 ob = base ? *base : 0ull;
 #pragma unroll
-    for (int r = 0; r < WS_ROWS; r++) {
+    for (int r = 0; r < R; r++) {
         u64 e = wbase + (u64)r * SAGPU_WAVE + lane;
         u32 is = wave_incl_sum((u32)act[r]);
         u32 i1 = umax(c.m1, wave_incl_max(v1[r]));
@@ -197,7 +199,11 @@ struct InitF {
     // form, and the rounds write the code of every suffix they place.
     int cb;
     uint8_t *codes;
+    // x / bits for 0 <= x <= 64 as (x * bmul) >> 16, bmul = 2^16 / bits + 1 (exact: the
+    // error x * (bmul - 2^16 / bits) stays below 2^16 / bits for bits <= 32)
+    u32 bmul;
 
+    __device__ u32 div_bits(u32 x) const { return (x * bmul) >> 16; }
     __device__ bool sep(u32 k) const { return gsa && (k & sepmask) == 0; }
     // Group flags of element e from its key and its neighbours' keys.
     __device__ void flags(u64 e, K k, K kp, K kn, bool &act, u32 &v1) const
@@ -213,7 +219,7 @@ struct InitF {
     {
         u32 l = (u32)ksym;
         u32 x = k ^ kp;
-        if (x) l = (u32)(__clz(x) - (32 - ksym * bits)) / (u32)bits;
+        if (x) l = div_bits((u32)(__clz(x) - (32 - ksym * bits)));
         if (gsa) {
             // the first separator (code 0) from the top: fold each symbol's bits of ~k
             // into its lowest bit, keep those bits, take the highest
@@ -222,7 +228,7 @@ struct InitF {
             for (int b = 1; b < bits; b++) m &= z >> b;
             m &= lowbits;
             if (m) {
-                const u32 t = (u32)(ksym - 1) - (u32)(31 - __clz(m)) / (u32)bits;
+                const u32 t = (u32)(ksym - 1) - div_bits((u32)(31 - __clz(m)));
                 l = t < l ? t : l;
             }
         }
@@ -233,7 +239,7 @@ struct InitF {
     {
         const u32 kb = (u32)(ksym * bits);
         const u32 kk = kb >= 32 ? k : (k & ((1u << kb) - 1u));
-        return kk ? (u32)__builtin_ctz(kk) / (u32)bits : (u32)ksym;
+        return kk ? div_bits((u32)__builtin_ctz(kk)) : (u32)ksym;
     }
     // LCP mark of slot e >= 1: the lcp when the keys decide it, else LCP_UNKNOWN |
     // lower bound.  one / onep: slot e / e-1 is alone in its group, so it keeps its
@@ -555,18 +561,23 @@ struct BigF {
     }
 };
 
+// Rows per wave span of the segmented scans over big-group members: 8 (2048-element
+// tiles) keeps the emit pass's registers at 4 x 8 values per lane, 16 rows held
+// 116 VGPRs (4 waves per SIMD).
+constexpr int WS_RUN_ROWS = 8;
 template <class F>
 int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
 {
-    const u64 tiles = (n + WS_TILE - 1) / WS_TILE;
+    constexpr u64 tile = (u64)WS_WAVES * WS_RUN_ROWS * SAGPU_WAVE;
+    const u64 tiles = (n + tile - 1) / tile;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
         return SAGPU_EDEVICE;
     }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
-    hipLaunchKernelGGL(k_ws_reduce<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    hipLaunchKernelGGL((k_ws_reduce<F, WS_RUN_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, out_counter, ws.base);
-    hipLaunchKernelGGL(k_ws_emit<F>, dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg,
+    hipLaunchKernelGGL((k_ws_emit<F, WS_RUN_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg,
                        out_counter ? ws.base : (const u64 *)nullptr);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
@@ -880,59 +891,75 @@ __global__ __launch_bounds__(RT_THREADS) void k_round_tile(const u32 *__restrict
         }
     } else {
         // groups of more than RT_SORT_MIN members: the range [first, end) sorted by
-        // (group head, key2, position) in LDS (a bitonic network whose comparators
-        // all put the smaller at the lower index, so the positions past the range
-        // act as +inf and are never touched), then every entry finds its subgroup's
-        // bounds by binary search inside its group's sorted stretch.  The entry at
+        // (group head, key2, position): 64-entry chunks sorted in registers, then
+        // merged pairwise in LDS; every entry then finds its subgroup's bounds by
+        // binary search inside its group's sorted stretch.  The entry at
         // sorted position p is not this thread's own: its list word is read again
         // (it was read by this workgroup just before).
         constexpr int XB = 11;   // position and head fields (RT_CAP <= 2048)
         constexpr u64 KMASK = (1ull << (64 - 2 * XB)) - 1;
         static_assert(RT_CAP <= (1 << XB), "sort fields too narrow");
         const u32 cnt = end - first;
+        const u32 lane = lane_id();
         u64 comp[RT_PER];
+        u32 pos[RT_PER];
 #pragma unroll
         for (int j = 0; j < RT_PER; j++) {
             const u32 p = tid + (u32)j * RT_THREADS;
-            comp[j] = p < cnt ? ((u64)s_hx[first + p] << (64 - XB)) | ((u64)s_k[first + p] << XB) | p : 0ull;
+            // past the range: +inf, which the chunk sort moves to the chunk's end
+            comp[j] = p < cnt ? ((u64)s_hx[first + p] << (64 - XB)) | ((u64)s_k[first + p] << XB) | p : ~0ull;
+            pos[j] = p;
+        }
+        // each wave sorts its 64-entry chunks (positions j * RT_THREADS + 64 w + lane) in
+        // registers, a bitonic network over the lanes
+#pragma unroll
+        for (u32 k = 2; k <= (u32)SAGPU_WAVE; k <<= 1) {
+#pragma unroll
+            for (u32 d = k >> 1; d > 0; d >>= 1) {
+                const bool keep_min = ((lane & d) == 0) == ((lane & k) == 0);
+#pragma unroll
+                for (int j = 0; j < RT_PER; j++) {
+                    const u64 o = __shfl_xor(comp[j], (int)d, SAGPU_WAVE);
+                    comp[j] = keep_min ? (o < comp[j] ? o : comp[j]) : (o > comp[j] ? o : comp[j]);
+                }
+            }
         }
         __syncthreads();   // the keys and heads are read; the sort keys take their LDS
 #pragma unroll
-        for (int j = 0; j < RT_PER; j++) {
-            const u32 p = tid + (u32)j * RT_THREADS;
-            if (p < cnt) s_o[p] = comp[j];
-        }
+        for (int j = 0; j < RT_PER; j++)
+            if (pos[j] < cnt) s_o[pos[j]] = comp[j];
         __syncthreads();
-        u32 lg = 0;
-        while ((1u << lg) < cnt) lg++;
-        const u32 npair = (1u << lg) >> 1;
-        for (u32 kl = 1; kl <= lg; kl++) {
-            // mirror step: i against the mirrored partner in its block of 2^kl
-            for (u32 pi = tid; pi < npair; pi += RT_THREADS) {
-                const u32 blk = (pi >> (kl - 1)) << kl, jj = pi & ((1u << (kl - 1)) - 1);
-                const u32 i = blk + jj, l = blk + (1u << kl) - 1 - jj;
-                if (l < cnt) {
-                    const u64 u = s_o[i], w = s_o[l];
-                    if (u > w) {
-                        s_o[i] = w;
-                        s_o[l] = u;
-                    }
+        // sorted runs of L entries merged pairwise: an entry moves to its offset in its
+        // run plus the number of entries of the partner run below it (the composite
+        // keys are distinct), found by binary search; then all entries are stored
+        for (u32 L = SAGPU_WAVE; L < cnt; L <<= 1) {
+            u32 np[RT_PER];
+#pragma unroll
+            for (int j = 0; j < RT_PER; j++) {
+                np[j] = pos[j];
+                if (pos[j] >= cnt) continue;
+                const u32 r0 = pos[j] & ~(2u * L - 1u);
+                const bool second = (pos[j] & L) != 0;
+                const u32 q0 = second ? r0 : r0 + L;
+                if (q0 >= cnt) continue;   // no partner run: in place
+                const u32 q1 = q0 + L < cnt ? q0 + L : cnt;
+                const u64 v = comp[j];
+                u32 a0 = q0, a1 = q1;
+                while (a0 < a1) {
+                    const u32 m = (a0 + a1) >> 1;
+                    if (s_o[m] < v) a0 = m + 1;
+                    else a1 = m;
                 }
+                np[j] = pos[j] - (second ? L : 0u) + (a0 - q0);
+            }
+            __syncthreads();   // the searches of this level are done
+#pragma unroll
+            for (int j = 0; j < RT_PER; j++) {
+                if (pos[j] >= cnt) continue;
+                s_o[np[j]] = comp[j];
+                pos[j] = np[j];
             }
             __syncthreads();
-            for (int dl = (int)kl - 2; dl >= 0; dl--) {
-                for (u32 pi = tid; pi < npair; pi += RT_THREADS) {
-                    const u32 i = ((pi >> dl) << (dl + 1)) | (pi & ((1u << dl) - 1)), l = i + (1u << dl);
-                    if (l < cnt) {
-                        const u64 u = s_o[i], w = s_o[l];
-                        if (u > w) {
-                            s_o[i] = w;
-                            s_o[l] = u;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
         }
         u32 rh[RT_PER], rlt[RT_PER], req[RT_PER], reqb[RT_PER];
         KT rkey[RT_PER], rpk[RT_PER];
@@ -1192,6 +1219,7 @@ unsigned grid_for(u64 work, unsigned per_block, unsigned cap)
 
 int g_isa_window_log2 = 25;
 int g_tile_range = 512;
+int g_init_rows = 8;
 
 template <typename K>
 int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits, int ksym, int gsa, u32 *SA, u32 *ISA,
@@ -1202,11 +1230,12 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     const int wlog = isa_bucket_log2(n);
     InitF<K> f{keys, vals, n, (1u << bits) - 1u, bits, ksym, gsa, SA, act_i, act_g, LCP, PV, PG, ws.bcnt, wlog,
                hint && hint->on && n <= (1ull << 28) ? 1 : 0, hint ? hint->off : 0, 0u, cb,
-               seed ? seed->codes : nullptr};
+               seed ? seed->codes : nullptr, 65536u / (u32)bits + 1u};
     for (int i = 0; i < ksym; i++) f.lowbits |= 1u << (bits * i);
     // short texts: 512-position tiles (2 rows per wave), so more workgroups share them
     const bool small = n < INIT_SMALL_N;
-    const u64 tile = small ? (u64)WS_WAVES * 2 * SAGPU_WAVE : (u64)WS_TILE;
+    const int rows = small ? 2 : g_init_rows;
+    const u64 tile = (u64)WS_WAVES * rows * SAGPU_WAVE;
     const u64 tiles = (n + tile - 1) / tile;
     if (tiles > ws.tiles_cap) {
         sagpu_set_error("segment scratch too small", __FILE__, __LINE__);
@@ -1218,13 +1247,16 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
         return SAGPU_EDEVICE;
     }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
-    if (small) hipLaunchKernelGGL((k_init_reduce<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    if (rows == 2) hipLaunchKernelGGL((k_init_reduce<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
+    else if (rows == 8) hipLaunchKernelGGL((k_init_reduce<K, 8>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     else hipLaunchKernelGGL((k_init_reduce<K, WS_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
     SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, nb * tiles, s));
     ktime_begin(KI_INIT_EMIT);
-    if (small)
+    if (rows == 2)
         hipLaunchKernelGGL((k_init_emit<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
+    else if (rows == 8)
+        hipLaunchKernelGGL((k_init_emit<K, 8>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, (const Agg *)tagg);
     else
         hipLaunchKernelGGL((k_init_emit<K, WS_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f,
                            (const Agg *)tagg);
