@@ -411,8 +411,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on exactly one GPU; multi-GPU runs are not part of "
-                                   "this build (see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "this engine runs on exactly one GPU: sharding a text over several GPUs "
+                                   "is designed but not built, so only --gpus 1 is measured",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
