@@ -25,6 +25,9 @@ struct KeyGen {
     int bits, kbits, cb;
 };
 u64 pack_stream_words(u64 n, int bits);
+// 1: the upfront histograms of generated keys whose digits hold whole symbols come
+// from one histogram of the stream's 8-bit windows (the "hist_windows" option)
+extern int g_hist_windows;
 int launch_pack_stream(const uint8_t *T, u64 n, const uint8_t *lut, int bits, u32 *pw, hipStream_t s);
 
 // Onesweep LSD radix sort, 8-bit digits, stable.  Keys are u64 or u32, values u32.

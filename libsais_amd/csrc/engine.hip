@@ -2293,6 +2293,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
         g_bwt_variant = (int)value;
         return 0;
     }
+    if (!strcmp(name, "hist_windows")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_hist_windows = (int)value;
+        return 0;
+    }
     if (!strcmp(name, "init_rows")) {
         if (value != 8 && value != 16) return SAGPU_EINVAL;
         g_init_rows = (int)value;

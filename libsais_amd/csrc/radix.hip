@@ -16,6 +16,8 @@
 
 #include <string.h>
 
+int g_hist_windows = 1;
+
 namespace {
 
 constexpr int RS_WAVES = RS_THREADS / SAGPU_WAVE;
@@ -85,6 +87,55 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
         u32 v = 0;
         for (int c = 0; c < C; c++) v += s_hc[p * C + c][d];
         if (v) atomicAdd(&hist[i], (u64)v);
+    }
+}
+
+// Keys generated from the code stream with whole symbols per 8-bit digit (bits in
+// {1, 2, 4, 8}): digit p (from the low end) of key(i) is D(i + o_p), the s = 8 / bits
+// symbols from position i + o_p, o_p = ksym - s (p + 1).  So every pass's histogram
+// is the histogram H of D(j) over j in [0, n), less D(j) for j < min(o_p, n), plus
+// min(o_p, n) zeros for the positions past the end (their symbols are code 0): one count per
+// position instead of one per position and pass.  k_hist_windows counts H into
+// hist[0, 256); k_hist_passes derives the passes' histograms from it.
+__global__ __launch_bounds__(256) void k_hist_windows(KeyGen gen, u64 n, u64 *__restrict__ hist)
+{
+    constexpr int C = 4;   // copies picked by lane % 4, as k_radix_hist
+    __shared__ u32 s_hc[C][256];
+    for (int i = threadIdx.x; i < C * 256; i += blockDim.x) (&s_hc[0][0])[i] = 0;
+    __syncthreads();
+    const u32 copy = threadIdx.x & (C - 1);
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    // thread per stream word: the 32 / bits positions starting in word w take their
+    // windows from words w and w + 1
+    const u32 per = 32u / (u32)gen.bits;
+    const u64 words = (n + per - 1) / per;
+    for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        const u64 x = ((u64)gen.pw[w] << 32) | (u64)gen.pw[w + 1];
+        const u64 j0 = w * per;
+        const u32 m = n - j0 < (u64)per ? (u32)(n - j0) : per;
+        for (u32 q = 0; q < m; q++) atomicAdd(&s_hc[copy][(u32)((x << (q * (u32)gen.bits)) >> 56)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < 256; d += blockDim.x) {
+        const u32 v = s_hc[0][d] + s_hc[1][d] + s_hc[2][d] + s_hc[3][d];
+        if (v) atomicAdd(&hist[d], (u64)v);
+    }
+}
+
+// One workgroup of 256 threads: thread d fills bin d of every pass from H.
+__global__ __launch_bounds__(256) void k_hist_passes(KeyGen gen, u64 n, int npass, u64 *__restrict__ hist)
+{
+    const u32 d = threadIdx.x;
+    const u64 h = hist[d];
+    const int s = 8 / gen.bits, ksym = gen.kbits / gen.bits;
+    __syncthreads();   // every thread has read H before pass 0's bins are rewritten
+    for (int p = 0; p < npass; p++) {
+        const int o = ksym - s * (p + 1);
+        u64 c = h;
+        for (int j = 0; j < o && (u64)j < n; j++)
+            if (stream_bits(gen.pw, (u64)j * (u64)gen.bits, 8) == d) c--;
+        if (d == 0) c += (u64)o < n ? (u64)o : n;   // positions past the end among [o, n + o)
+        hist[p * 256 + d] = c;
     }
 }
 
@@ -501,7 +552,13 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
         const u64 per = small ? 256 * 8 : 256 * 64;
         int blocks = (int)((n + per - 1) / per);
         if (blocks > 4096) blocks = 4096;
-        if (gen)
+        // generated keys cut into whole-symbol digits: one window histogram for all passes
+        const bool windows = gen && DB == 8 && begin_bit == 0 && end_bit == g0.kbits && g0.kbits % 8 == 0 &&
+                             8 % g0.bits == 0 && g_hist_windows;
+        if (windows) {
+            hipLaunchKernelGGL(k_hist_windows, dim3(blocks), dim3(256), 0, s, g0, n, rs.hist);
+            hipLaunchKernelGGL(k_hist_passes, dim3(1), dim3(256), 0, s, g0, n, npass, rs.hist);
+        } else if (gen)
             hipLaunchKernelGGL((k_radix_hist<K, true, DB>), dim3(blocks), dim3(256), 0, s, k0, g0, n, begin_bit,
                                end_bit, npass, rs.hist);
         else

@@ -75,21 +75,32 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
     }
 }
 
-// Exclusive scan of tile aggregates (one workgroup).  Writes the grand total of
-// active elements and, when out_counter is given, reserves that many slots in the
-// output list: *base = atomicAdd(out_counter, total).
+// Exclusive scan of tile aggregates (one workgroup of 1024 threads, SC_K consecutive
+// tiles per thread and step: 8192 tiles per step, so a 2^28-position text's 131072
+// tiles take 16 steps).  Writes the grand total of active elements and, when
+// out_counter is given, reserves that many slots in the output list:
+// *base = atomicAdd(out_counter, total).
+constexpr int SC_K = 8;
 __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u64 *total, u64 *out_counter, u64 *base)
 {
     __shared__ Agg s_w[16];
     __shared__ Agg s_carry;
     if (threadIdx.x == 0) s_carry = Agg{0, 0, 0};
     __syncthreads();
-    for (u64 b0 = 0; b0 < tiles; b0 += 1024) {
-        u64 i = b0 + threadIdx.x;
-        Agg v = i < tiles ? tagg[i] : Agg{0, 0, 0};
-        u64 is = wave_incl_sum64(v.s);
-        u32 i1 = wave_incl_max(v.m1), i2 = wave_incl_max(v.m2);
-        const u32 w = threadIdx.x / SAGPU_WAVE;
+    const u32 w = threadIdx.x / SAGPU_WAVE;
+    for (u64 b0 = 0; b0 < tiles; b0 += 1024 * SC_K) {
+        const u64 i0 = b0 + (u64)threadIdx.x * SC_K;
+        Agg v[SC_K];
+        Agg t{0, 0, 0};   // this thread's SC_K tiles combined
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            v[q] = i0 + q < tiles ? tagg[i0 + q] : Agg{0, 0, 0};
+            t.s += v[q].s;
+            t.m1 = umax(t.m1, v[q].m1);
+            t.m2 = umax(t.m2, v[q].m2);
+        }
+        const u64 is = wave_incl_sum64(t.s);
+        const u32 i1 = wave_incl_max(t.m1), i2 = wave_incl_max(t.m2);
         if (lane_id() == SAGPU_WAVE - 1) s_w[w] = Agg{is, i1, i2};
         __syncthreads();
         Agg c = s_carry;
@@ -98,23 +109,21 @@ __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u6
             c.m1 = umax(c.m1, s_w[q].m1);
             c.m2 = umax(c.m2, s_w[q].m2);
         }
-        u32 p1 = __shfl_up(i1, 1, SAGPU_WAVE), p2 = __shfl_up(i2, 1, SAGPU_WAVE);
+        const u32 p1 = __shfl_up(i1, 1, SAGPU_WAVE), p2 = __shfl_up(i2, 1, SAGPU_WAVE);
         if (lane_id() > 0) {
             c.m1 = umax(c.m1, p1);
             c.m2 = umax(c.m2, p2);
         }
-        c.s += is - v.s;
-        if (i < tiles) tagg[i] = c;
-        __syncthreads();
-        if (threadIdx.x == 1023) {
-            Agg t = s_carry;
-            for (int q = 0; q < 16; q++) {
-                t.s += s_w[q].s;
-                t.m1 = umax(t.m1, s_w[q].m1);
-                t.m2 = umax(t.m2, s_w[q].m2);
-            }
-            s_carry = t;
+        c.s += is - t.s;
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            if (i0 + q < tiles) tagg[i0 + q] = c;
+            c.s += v[q].s;
+            c.m1 = umax(c.m1, v[q].m1);
+            c.m2 = umax(c.m2, v[q].m2);
         }
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = c;   // the last thread's running value covers the whole step
         __syncthreads();
     }
     if (threadIdx.x == 0) {
