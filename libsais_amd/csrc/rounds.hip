@@ -1135,7 +1135,7 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ bi
                                                       u64 *__restrict__ boff, u64 *__restrict__ total,
                                                       u32 *__restrict__ err)
 {
-    __shared__ u64 s_sum[1024];
+    __shared__ u64 s_sum[1024 / SAGPU_WAVE];
     const u32 nb = (u32)((u64)*big_counter < big_cap ? (u64)*big_counter : big_cap);
     const u32 t = threadIdx.x;
     if (nb == 0) {   // most rounds of most texts: nothing to scan
@@ -1145,32 +1145,40 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const u32 *__restrict__ bi
         }
         return;
     }
-    const u32 chunk = (nb + 1023) / 1024;
-    const u64 b0 = (u64)t * chunk < nb ? (u64)t * chunk : nb;
-    const u64 b1 = b0 + chunk < nb ? b0 + chunk : nb;
-    u64 sum = 0;
-    bool bad = false;
-    for (u64 q = b0; q < b1; q++) {
-        const u64 sz = (u64)big_size[q] + 1;   // sizes are stored minus one
-        bad = bad || sz <= (u64)RT_SMALL;
-        sum += sz;
-    }
-    s_sum[t] = sum;
+    // SC_K consecutive sizes per thread and step (coalesced reads; a Fibonacci round
+    // lists 10^5 big groups)
+    __shared__ u64 s_carry;
+    if (t == 0) s_carry = 0;
     __syncthreads();
-    for (u32 d = 1; d < 1024; d <<= 1) {
-        const u64 v = t >= d ? s_sum[t - d] : 0ull;
+    const u32 w = t / SAGPU_WAVE;
+    bool bad = false;
+    for (u64 b0 = 0; b0 < nb; b0 += 1024 * SC_K) {
+        const u64 i0 = b0 + (u64)t * SC_K;
+        u64 v[SC_K];
+        u64 ts = 0;
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            v[q] = i0 + q < nb ? (u64)big_size[i0 + q] + 1 : 0ull;   // sizes are stored minus one
+            bad = bad || (i0 + q < nb && v[q] <= (u64)RT_SMALL);
+            ts += v[q];
+        }
+        const u64 is = wave_incl_sum64(ts);
+        if (lane_id() == SAGPU_WAVE - 1) s_sum[w] = is;
         __syncthreads();
-        s_sum[t] += v;
+        u64 c = s_carry + is - ts;
+        for (u32 q = 0; q < w; q++) c += s_sum[q];
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            if (i0 + q < nb) boff[i0 + q] = c;
+            c += v[q];
+        }
+        __syncthreads();
+        if (t == 1023) s_carry = c;
         __syncthreads();
     }
-    u64 off = s_sum[t] - sum;
-    for (u64 q = b0; q < b1; q++) {
-        boff[q] = off;
-        off += (u64)big_size[q] + 1;
-    }
-    if (t == 1023) {
-        boff[nb] = s_sum[1023];
-        *total = s_sum[1023];
+    if (t == 0) {
+        boff[nb] = s_carry;
+        *total = s_carry;
     }
     if (bad) atomicOr(err, 2u);
 }
