@@ -260,10 +260,16 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    all take up to 4 keys frequent in a 64-member sample of each group out of the
  *                    sort: those members become subgroups by a stable partition (default 2^20;
  *                    0 = never)
- *   "group_flags"    1 = texts of at most 2^31 suffixes let a group's members rank at its last
- *                    slot (a flag in the group's list word), so that the subgroup holding that
- *                    slot keeps its ranks when the group splits (default); 0 = every group ranks
- *                    at its first slot
+ *   "group_flags"    1 = texts of at most 2^31 suffixes let a group's members hold any slot of
+ *                    the group's range as their rank (a flag in the group's list word, the rank
+ *                    stored at the group's first slot + 1), so that the subgroup whose slots hold
+ *                    it keeps its ranks when the group splits (default); 0 = every group ranks at
+ *                    its first slot
+ *   "init_rows"      rows of 64 positions per wave in the initial grouping of texts of at least
+ *                    2^20 suffixes: 8 (2048-position tiles, default) or 16
+ *   "hist_windows"   1 = the initial sort's histograms, when its keys come from the code stream
+ *                    with whole symbols per 8-bit digit, are derived from one histogram of the
+ *                    stream's 8-bit windows (default); 0 = counted per pass
  *   "api_slots"      contexts that calls made without a libsais context share, 1-16
  *                    (default 8): that many such calls run at once, each on its own stream
  * Returns 0, or -1 for an unknown name or value.

@@ -354,9 +354,10 @@ int g_windowed_scatter = 1;
 // Big groups of rounds after the first take up to BP frequent keys per group out of
 // the sort when they hold at least this many members in all (0: never)
 int g_big_pivot_min = 1 << 20;
-// 1: texts of at most 2^31 suffixes flag group words whose members rank at the
-// group's last slot, so the subgroup holding that slot keeps its ISA entries when
-// the group splits; 0: every group ranks at its first slot
+// 1: texts of at most 2^31 suffixes flag group words whose members rank at another
+// slot of the group's range (stored at the first slot + 1), so the subgroup whose
+// slots hold that rank keeps its ISA entries when the group splits; 0: every group
+// ranks at its first slot
 int g_group_flags = 1;
 // radix sorts of fewer keys use the smallest tiles (2048 keys), whatever radix_items
 // and radix_threads say
@@ -1065,7 +1066,7 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
         a = c.pinned[0];
     }
 
-    // list entries carry BWT codes when seeded; group words carry the last-slot flag
+    // list entries carry BWT codes when seeded; group words carry the stored-rank flag
     // when every slot fits 31 bits
     const ActCode ac{cb, cb ? seed->codes : nullptr, g_group_flags && n <= (1ull << 31) ? 0x80000000u : 0u};
     u64 h = (u64)ksym;

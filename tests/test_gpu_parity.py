@@ -467,7 +467,7 @@ def test_big_pivots(name, T, degenerate, wide, flags):
     """Big groups with few distinct keys: up to 4 keys per group frequent in a sample
     skip the sort (the pivot classes) and the rest is sorted and ranked around them.
     Forced on at every size (big_pivot_min = 1), in the 32-bit and the wide (u64
-    key2) engine, with group words flagged for last-slot ranks (the default) and
+    key2) engine, with group words flagged for stored ranks (the default) and
     without, the fused build equals the oracle; on repeat-rich texts the path
     runs."""
     lib = la.load()
@@ -500,7 +500,7 @@ def test_big_pivots(name, T, degenerate, wide, flags):
 @pytest.mark.parametrize("name", ["acgt200003", "bytes_k256", "all_a_70000", "dna_like_1m", "banana", "acgt17"])
 def test_group_flags_off(name):
     """Every group ranked at its first slot (group_flags = 0): the fused build
-    equals the oracle as with the default last-slot ranks of flagged groups."""
+    equals the oracle as with the default stored ranks of flagged groups."""
     lib = la.load()
     T = dict(CASES)[name]
     rc, SA_ref = ol.sa(T)
@@ -515,6 +515,30 @@ def test_group_flags_off(name):
         assert np.array_equal(P, P_ref) and np.array_equal(L, ol.lcp(P_ref, SA_ref)), name
     finally:
         assert lib.sagpu_set_option(b"group_flags", 1) == 0
+
+
+@pytest.mark.parametrize("name", ["acgt5", "acgt17", "acgt4097", "bytes_k2", "bytes_k5", "bytes_k16_n16385",
+                                  "bytes_k256", "dna_like_1m", "all_a_70000"])
+@pytest.mark.parametrize("option,value", [(b"hist_windows", 0), (b"init_rows", 16)])
+def test_initial_sort_forms(name, option, value):
+    """The radix histograms counted per pass instead of from the stream's 8-bit
+    windows (hist_windows = 0), and the initial grouping in 4096-position tiles
+    (init_rows = 16): the fused build equals the oracle."""
+    lib = la.load()
+    T = dict(CASES)[name]
+    rc, SA_ref = ol.sa(T)
+    assert rc == 0
+    P_ref = ol.plcp(T, SA_ref)
+    default = 1 if option == b"hist_windows" else 8
+    try:
+        assert lib.sagpu_set_option(option, value) == 0
+        SA, U, r, P, L, f = fused(T)
+        assert np.array_equal(SA, SA_ref), name
+        U_ref, idx_ref = ol.bwt(T, SA_ref)
+        assert r == idx_ref and np.array_equal(U, U_ref), name
+        assert np.array_equal(P, P_ref) and np.array_equal(L, ol.lcp(P_ref, SA_ref)), name
+    finally:
+        assert lib.sagpu_set_option(option, default) == 0
 
 
 def test_big_pivots_gsa():
