@@ -590,6 +590,7 @@ __global__ __launch_bounds__(256) void k_plcp_vlong(const uint8_t *__restrict__ 
                                                     const u32 *__restrict__ SA, VLongQueue vq)
 {
     __shared__ u64 s_min[4];
+    __shared__ u64 s_res;
     const u32 cnt = *vq.n < VQ_CAP ? *vq.n : VQ_CAP;
     if (cnt == 0) return;
     const u32 G = gridDim.x / cnt > 0 ? gridDim.x / cnt : 1u;
@@ -599,7 +600,18 @@ __global__ __launch_bounds__(256) void k_plcp_vlong(const uint8_t *__restrict__ 
         const u64 k = phi_of(PHI, ISA, SA, n, i);
         const u64 m = W * (n - (i > k ? i : k));
         for (u64 l = vq.off[v] + (u64)g * LONG_STEP;; l += (u64)G * LONG_STEP) {
-            if (*(volatile u64 *)&vq.res[v] <= l) break;   // a mismatch before this step is known
+            // Whether a mismatch before this step is known, read once for the whole
+            // workgroup: waves reading it separately can see different values (other
+            // workgroups lower it meanwhile), and a wave that left the loop would
+            // leave its s_min slot stale for the others' step (uninitialised on the
+            // first, so any value, 0 included, could win the min).  The second
+            // barrier keeps thread 0 from rewriting s_res (next step or next
+            // position) before every thread has read it.
+            if (threadIdx.x == 0) s_res = *(volatile u64 *)&vq.res[v];
+            __syncthreads();
+            const u64 known = s_res;
+            __syncthreads();
+            if (known <= l) break;
             const u64 r = long_step<W>(T, gsa, i, k, l, m, s_min);
             if (r != ~0ull) {
                 if (threadIdx.x == 0) atomicMin((unsigned long long *)&vq.res[v], (unsigned long long)r);
