@@ -267,6 +267,9 @@ SAGPU_API int32_t sagpu_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals, int64
  *                    its first slot
  *   "init_rows"      rows of 64 positions per wave in the initial grouping of texts of at least
  *                    2^20 suffixes: 8 (2048-position tiles, default) or 16
+ *   "perm_hist"      1 = the radix pass that groups the initial ranks into ISA windows takes its
+ *                    digit histogram from the closed form of a permutation of [0, n) (default);
+ *                    0 = counted
  *   "hist_windows"   1 = the initial sort's histograms, when its keys come from the code stream
  *                    with whole symbols per 8-bit digit, are derived from one histogram of the
  *                    stream's 8-bit windows (default); 0 = counted per pass

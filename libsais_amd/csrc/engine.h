@@ -81,6 +81,11 @@ struct RadixScratch {
     // gen_pass is 1 while that pass is bracketed by the hook.
     const KeyGen *gen;
     int gen_pass;
+    // u32 sorts only: when set, the keys are a permutation of [0, n) (each value once,
+    // as the suffixes of the ISA pairs), so every digit histogram has a closed form:
+    // it is computed on the device without reading the keys, and the host decides
+    // the constant digits without reading it back.
+    int perm_keys;
 };
 
 enum { RADIX_ONESWEEP = 0, RADIX_REDUCE_THEN_SCAN = 1 };

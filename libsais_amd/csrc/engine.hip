@@ -359,6 +359,9 @@ int g_big_pivot_min = 1 << 20;
 // slots hold that rank keeps its ISA entries when the group splits; 0: every group
 // ranks at its first slot
 int g_group_flags = 1;
+// 1: the radix pass that groups the ISA pairs into windows takes its digit histogram
+// from the closed form of a permutation's (the "perm_hist" option)
+int g_perm_hist = 1;
 // radix sorts of fewer keys use the smallest tiles (2048 keys), whatever radix_items
 // and radix_threads say
 int g_radix_small_n = 1 << 20;
@@ -821,7 +824,11 @@ int isa_from_pairs(Ctx &c, Scope &sc, const u32 *PV, const u32 *PG, u64 n, u32 *
     const u64 nc = ((n - 1) >> cw) + 1;
     if (g_windowed_scatter && n > (1ull << 22) && nc <= 8 && cw >= 16) {
         int alt = 0;
-        SAGPU_TRY(sort_u32(c, (u32 *)PV, (u32 *)PG, K1, V1, n, cw - 8, cw, &alt));
+        // PV holds every suffix once: the partition pass's histogram has a closed form
+        c.rs.perm_keys = g_perm_hist;
+        const int src = sort_u32(c, (u32 *)PV, (u32 *)PG, K1, V1, n, cw - 8, cw, &alt);
+        c.rs.perm_keys = 0;
+        SAGPU_TRY(src);
         GET(bounds, u64, 256 * nc + 1);
         GET(ctr, u32, 256);
         ktime_begin(KI_ISA_SCATTER);
@@ -2272,6 +2279,11 @@ extern "C" int32_t sagpu_set_option(const char *name, int64_t value)
     if (!strcmp(name, "big_pivot_min")) {
         if (value < 0 || value > 0x7FFFFFFF) return SAGPU_EINVAL;
         g_big_pivot_min = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "perm_hist")) {
+        if (value != 0 && value != 1) return SAGPU_EINVAL;
+        g_perm_hist = (int)value;
         return 0;
     }
     if (!strcmp(name, "group_flags")) {

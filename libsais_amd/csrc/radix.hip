@@ -139,6 +139,32 @@ __global__ __launch_bounds__(256) void k_hist_passes(KeyGen gen, u64 n, int npas
     }
 }
 
+// Number of v in [0, n) whose digit (v >> shift) & dmask is d (dmask = 2^b - 1):
+// every full period of 2^(shift + b) values holds each digit 2^shift times, and the
+// last, partial period holds d's stretch [d << shift, (d + 1) << shift) up to n.
+__host__ __device__ inline u64 perm_digit_count(u64 n, int shift, u32 dmask, u32 d)
+{
+    const int b = __builtin_popcount(dmask);
+    const u64 period = 1ull << (shift + b);
+    const u64 full = n >> (shift + b), rest = n & (period - 1);
+    const u64 lo = (u64)d << shift, hi = (u64)(d + 1) << shift;
+    const u64 part = rest > lo ? (rest < hi ? rest : hi) - lo : 0;
+    return full * (1ull << shift) + part;
+}
+
+// Digit histograms of keys that are a permutation of [0, n), from the closed form.
+__global__ __launch_bounds__(256) void k_perm_hist(u64 n, int begin_bit, int end_bit, int npass, int db,
+                                                   u64 *__restrict__ hist)
+{
+    const u32 bins = 1u << db;
+    for (u32 i = threadIdx.x; i < (u32)npass * bins; i += blockDim.x) {
+        const int p = (int)(i / bins), shift = begin_bit + db * p;
+        const u32 dmask = end_bit - shift >= db ? bins - 1u : ((1u << (end_bit - shift)) - 1u);
+        const u32 d = i % bins;
+        hist[i] = d <= dmask ? perm_digit_count(n, shift, dmask, d) : 0ull;
+    }
+}
+
 // Exclusive scan of each pass's histogram (bins per pass) into bucket offsets at
 // hist + RADIX_HIST_WORDS.  Thread t takes bins [t * per, (t + 1) * per).
 __global__ __launch_bounds__(256) void k_radix_offsets(u64 *hist, int npass, int bins)
@@ -545,7 +571,9 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
         }
         gen = nullptr;
     }
-    SAGPU_TRY(launch_fill_u64(rs.hist, (u64)npass * BINS, 0, s));
+    // keys known to be a permutation of [0, n): closed-form histograms (u32 keys)
+    const bool perm = rs.perm_keys && sizeof(K) == 4 && !gen && n < (1ull << 32);
+    if (!perm) SAGPU_TRY(launch_fill_u64(rs.hist, (u64)npass * BINS, 0, s));
     {
         // 64 keys per thread; 8 for short inputs, which would otherwise leave all
         // but a few workgroups idle
@@ -555,7 +583,9 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
         // generated keys cut into whole-symbol digits: one window histogram for all passes
         const bool windows = gen && DB == 8 && begin_bit == 0 && end_bit == g0.kbits && g0.kbits % 8 == 0 &&
                              8 % g0.bits == 0 && g_hist_windows;
-        if (windows) {
+        if (perm) {
+            hipLaunchKernelGGL(k_perm_hist, dim3(1), dim3(256), 0, s, n, begin_bit, end_bit, npass, DB, rs.hist);
+        } else if (windows) {
             hipLaunchKernelGGL(k_hist_windows, dim3(blocks), dim3(256), 0, s, g0, n, rs.hist);
             hipLaunchKernelGGL(k_hist_passes, dim3(1), dim3(256), 0, s, g0, n, npass, rs.hist);
         } else if (gen)
@@ -570,7 +600,14 @@ int radix_sort_db(RadixScratch &rs, K *k0, u32 *v0, K *k1, u32 *v1, u64 n, int b
     // Histograms decide which passes can be skipped (all keys share the digit).  A
     // short input runs every pass instead: a pass over it costs less than the wait
     // for the histograms on the host.
-    if (!small) {
+    if (perm) {
+        for (int p = 0; p < npass; p++) {
+            const int shift = begin_bit + DB * p;
+            const u32 dmask = end_bit - shift >= DB ? (u32)(BINS - 1) : ((1u << (end_bit - shift)) - 1u);
+            for (u32 d = 0; d < (u32)BINS; d++)
+                rs.hist_host[p * BINS + d] = d <= dmask ? perm_digit_count(n, shift, dmask, d) : 0ull;
+        }
+    } else if (!small) {
         SAGPU_CHECK(hipMemcpyAsync(rs.hist_host, rs.hist, (u64)npass * BINS * sizeof(u64), hipMemcpyDeviceToHost, s));
         SAGPU_CHECK(hipStreamSynchronize(s));
     }

@@ -519,17 +519,19 @@ def test_group_flags_off(name):
 
 @pytest.mark.parametrize("name", ["acgt5", "acgt17", "acgt4097", "bytes_k2", "bytes_k5", "bytes_k16_n16385",
                                   "bytes_k256", "dna_like_1m", "all_a_70000"])
-@pytest.mark.parametrize("option,value", [(b"hist_windows", 0), (b"init_rows", 16)])
+@pytest.mark.parametrize("option,value", [(b"hist_windows", 0), (b"init_rows", 16), (b"perm_hist", 0)])
 def test_initial_sort_forms(name, option, value):
     """The radix histograms counted per pass instead of from the stream's 8-bit
-    windows (hist_windows = 0), and the initial grouping in 4096-position tiles
-    (init_rows = 16): the fused build equals the oracle."""
+    windows (hist_windows = 0), the initial grouping in 4096-position tiles
+    (init_rows = 16), and the ISA pairs' partition pass counting its digits instead
+    of taking a permutation's closed form (perm_hist = 0): the fused build equals
+    the oracle."""
     lib = la.load()
     T = dict(CASES)[name]
     rc, SA_ref = ol.sa(T)
     assert rc == 0
     P_ref = ol.plcp(T, SA_ref)
-    default = 1 if option == b"hist_windows" else 8
+    default = {b"hist_windows": 1, b"init_rows": 8, b"perm_hist": 1}[option]
     try:
         assert lib.sagpu_set_option(option, value) == 0
         SA, U, r, P, L, f = fused(T)
