@@ -55,6 +55,8 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
     constexpr int BINS = 1 << DB;
     constexpr int C = DB > 8 ? 1 : 4;
     constexpr int NP = (8 * (int)sizeof(K) + DB - 1) / DB;
+    // copy c keeps digit d at column (d + c) % BINS, so the copies of one digit sit in
+    // different LDS banks
     __shared__ u32 s_hc[NP * C][BINS];
     for (int i = threadIdx.x; i < NP * C * BINS; i += blockDim.x) (&s_hc[0][0])[i] = 0;
     __syncthreads();
@@ -78,14 +80,14 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
                     continue;
                 }
             }
-            atomicAdd(&s_hc[p * C + copy][d], 1u);
+            atomicAdd(&s_hc[p * C + copy][(d + copy) & (u32)(BINS - 1)], 1u);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < npass * BINS; i += blockDim.x) {
         const int p = i / BINS, d = i % BINS;
         u32 v = 0;
-        for (int c = 0; c < C; c++) v += s_hc[p * C + c][d];
+        for (int c = 0; c < C; c++) v += s_hc[p * C + c][(d + c) & (BINS - 1)];
         if (v) atomicAdd(&hist[i], (u64)v);
     }
 }
@@ -99,9 +101,9 @@ __global__ __launch_bounds__(256) void k_radix_hist(const K *__restrict__ keys, 
 // hist[0, 256); k_hist_passes derives the passes' histograms from it.
 __global__ __launch_bounds__(256) void k_hist_windows(KeyGen gen, u64 n, u64 *__restrict__ hist)
 {
-    constexpr int C = 4;   // copies picked by lane % 4, as k_radix_hist
-    __shared__ u32 s_hc[C][256];
-    for (int i = threadIdx.x; i < C * 256; i += blockDim.x) (&s_hc[0][0])[i] = 0;
+    constexpr int C = 4;   // copies picked by lane % 4, as k_radix_hist; rows of 257 words
+    __shared__ u32 s_hc[C][257];   // put the copies of one window value in different banks
+    for (int i = threadIdx.x; i < C * 257; i += blockDim.x) (&s_hc[0][0])[i] = 0;
     __syncthreads();
     const u32 copy = threadIdx.x & (C - 1);
     const u64 stride = (u64)gridDim.x * blockDim.x;

@@ -11,9 +11,10 @@ __global__ __launch_bounds__(256) void k_histogram(const uint8_t *__restrict__ T
                                                    u64 *__restrict__ hist, u32 *__restrict__ bad)
 {
     // 16 sub-histograms: lanes of a wave spread over 4 copies (lane % 4), so a text of
-    // few distinct bytes (DNA) has a quarter of the same-word atomics per step
-    __shared__ u32 s_h[16][256];
-    for (int i = threadIdx.x; i < 16 * 256; i += 256) (&s_h[0][0])[i] = 0;
+    // few distinct bytes (DNA) has a quarter of the same-word atomics per step.  Rows
+    // of 257 words put the copies of one byte value in different LDS banks.
+    __shared__ u32 s_h[16][257];
+    for (int i = threadIdx.x; i < 16 * 257; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
     const u32 w = (threadIdx.x / SAGPU_WAVE) * 4 + (threadIdx.x & 3);
     u32 nbad = 0;
