@@ -368,17 +368,21 @@ def test_windowed_scatter_option(name):
     L_ref = ol.lcp(P_ref, SA_ref)
     lib = la.load()
     try:
-        for on in (1, 0):
-            print(f"[windowed] {name} n={len(T)} windowed_scatter={on}", flush=True)
+        # windowed on with the partition pass's closed-form histogram, on with a
+        # counted one, and off
+        for on, perm in ((1, 1), (1, 0), (0, 1)):
+            print(f"[windowed] {name} n={len(T)} windowed_scatter={on} perm_hist={perm}", flush=True)
             assert lib.sagpu_set_option(b"windowed_scatter", on) == 0
+            assert lib.sagpu_set_option(b"perm_hist", perm) == 0
             SA, U, r, P, L, f = fused(T, gsa=gsa)
-            assert np.array_equal(SA, SA_ref), (name, on)
-            assert np.array_equal(P, P_ref) and np.array_equal(L, L_ref), (name, on)
+            assert np.array_equal(SA, SA_ref), (name, on, perm)
+            assert np.array_equal(P, P_ref) and np.array_equal(L, L_ref), (name, on, perm)
             if not gsa:
                 U_ref, idx_ref = ol.bwt(T, SA_ref)
-                assert r == idx_ref and np.array_equal(U, U_ref), (name, on)
+                assert r == idx_ref and np.array_equal(U, U_ref), (name, on, perm)
     finally:
         assert lib.sagpu_set_option(b"windowed_scatter", 1) == 0
+        assert lib.sagpu_set_option(b"perm_hist", 1) == 0
 
 
 @pytest.mark.parametrize("bits", [4, 11])
@@ -519,19 +523,18 @@ def test_group_flags_off(name):
 
 @pytest.mark.parametrize("name", ["acgt5", "acgt17", "acgt4097", "bytes_k2", "bytes_k5", "bytes_k16_n16385",
                                   "bytes_k256", "dna_like_1m", "all_a_70000"])
-@pytest.mark.parametrize("option,value", [(b"hist_windows", 0), (b"init_rows", 16), (b"perm_hist", 0)])
+@pytest.mark.parametrize("option,value", [(b"hist_windows", 0), (b"init_rows", 16)])
 def test_initial_sort_forms(name, option, value):
     """The radix histograms counted per pass instead of from the stream's 8-bit
-    windows (hist_windows = 0), the initial grouping in 4096-position tiles
-    (init_rows = 16), and the ISA pairs' partition pass counting its digits instead
-    of taking a permutation's closed form (perm_hist = 0): the fused build equals
-    the oracle."""
+    windows (hist_windows = 0), and the initial grouping in 4096-position tiles
+    (init_rows = 16): the fused build equals the oracle.  (The closed-form partition
+    histogram, perm_hist, runs above 2^22 suffixes: test_windowed_scatter_option.)"""
     lib = la.load()
     T = dict(CASES)[name]
     rc, SA_ref = ol.sa(T)
     assert rc == 0
     P_ref = ol.plcp(T, SA_ref)
-    default = {b"hist_windows": 1, b"init_rows": 8, b"perm_hist": 1}[option]
+    default = {b"hist_windows": 1, b"init_rows": 8}[option]
     try:
         assert lib.sagpu_set_option(option, value) == 0
         SA, U, r, P, L, f = fused(T)
