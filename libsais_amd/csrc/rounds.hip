@@ -75,26 +75,33 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
     }
 }
 
-// Exclusive scan of tile aggregates (one workgroup of 1024 threads, SC_K consecutive
-// tiles per thread and step: 8192 tiles per step, so a 2^28-position text's 131072
-// tiles take 16 steps).  Writes the grand total of active elements and, when
-// out_counter is given, reserves that many slots in the output list:
-// *base = atomicAdd(out_counter, total).
+// Exclusive scan of tile aggregates (one workgroup of 1024 threads; SC_K consecutive
+// tiles per thread and step, 8192 tiles per step, so a 2^28-position text's 131072
+// tiles take 16 steps).  Each step's tiles pass through LDS: loaded and stored with
+// consecutive lanes on consecutive tiles, and read by their owner thread from rows of
+// SC_K + 1 (padding against bank conflicts).  Writes the grand total of active
+// elements and, when out_counter is given, reserves that many slots in the output
+// list: *base = atomicAdd(out_counter, total).
 constexpr int SC_K = 8;
 __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u64 *total, u64 *out_counter, u64 *base)
 {
+    __shared__ Agg s_t[1024 * (SC_K + 1)];
     __shared__ Agg s_w[16];
     __shared__ Agg s_carry;
-    if (threadIdx.x == 0) s_carry = Agg{0, 0, 0};
-    __syncthreads();
-    const u32 w = threadIdx.x / SAGPU_WAVE;
+    const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE;
+    if (tid == 0) s_carry = Agg{0, 0, 0};
     for (u64 b0 = 0; b0 < tiles; b0 += 1024 * SC_K) {
-        const u64 i0 = b0 + (u64)threadIdx.x * SC_K;
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            const u32 j = (u32)q * 1024 + tid;   // tile b0 + j, owned by thread j / SC_K
+            s_t[(j / SC_K) * (SC_K + 1) + j % SC_K] = b0 + j < tiles ? tagg[b0 + j] : Agg{0, 0, 0};
+        }
+        __syncthreads();
         Agg v[SC_K];
         Agg t{0, 0, 0};   // this thread's SC_K tiles combined
 #pragma unroll
         for (int q = 0; q < SC_K; q++) {
-            v[q] = i0 + q < tiles ? tagg[i0 + q] : Agg{0, 0, 0};
+            v[q] = s_t[tid * (SC_K + 1) + q];
             t.s += v[q].s;
             t.m1 = umax(t.m1, v[q].m1);
             t.m2 = umax(t.m2, v[q].m2);
@@ -116,17 +123,22 @@ __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u6
         }
         c.s += is - t.s;
 #pragma unroll
-        for (int q = 0; q < SC_K; q++) {
-            if (i0 + q < tiles) tagg[i0 + q] = c;
+        for (int q = 0; q < SC_K; q++) {   // exclusive prefixes into the thread's own row
+            s_t[tid * (SC_K + 1) + q] = c;
             c.s += v[q].s;
             c.m1 = umax(c.m1, v[q].m1);
             c.m2 = umax(c.m2, v[q].m2);
         }
-        __syncthreads();
-        if (threadIdx.x == 1023) s_carry = c;   // the last thread's running value covers the whole step
+        __syncthreads();   // every row written; s_carry and s_w read
+#pragma unroll
+        for (int q = 0; q < SC_K; q++) {
+            const u32 j = (u32)q * 1024 + tid;
+            if (b0 + j < tiles) tagg[b0 + j] = s_t[(j / SC_K) * (SC_K + 1) + j % SC_K];
+        }
+        if (tid == 1023) s_carry = c;   // the last thread's running value covers the whole step
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         *total = s_carry.s;
         if (out_counter) *base = atomicAdd(out_counter, s_carry.s);
     }
