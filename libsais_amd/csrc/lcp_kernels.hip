@@ -672,27 +672,36 @@ __global__ __launch_bounds__(MS_THREADS) void k_max_reduce(const VT *__restrict_
 template <typename VT>
 __global__ __launch_bounds__(1024) void k_max_scan_tiles(VT *tmax, u64 tiles)
 {
+    // MX_K consecutive tiles per thread and step (8192 tiles per step: 8 steps for the
+    // 65536 tiles of a 2^28-position text instead of 64)
+    constexpr int MX_K = 8;
     __shared__ VT s_w[16];
     __shared__ VT s_carry;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
-    for (u64 base = 0; base < tiles; base += 1024) {
-        u64 i = base + threadIdx.x;
-        VT v = i < tiles ? tmax[i] : (VT)0;
-        VT inc = wave_incl_max_t(v);
+    for (u64 base = 0; base < tiles; base += 1024 * MX_K) {
+        const u64 i0 = base + (u64)threadIdx.x * MX_K;
+        VT v[MX_K];
+        VT t = 0;
+#pragma unroll
+        for (int q = 0; q < MX_K; q++) {
+            v[q] = i0 + q < tiles ? tmax[i0 + q] : (VT)0;
+            t = vmax(t, v[q]);
+        }
+        const VT inc = wave_incl_max_t(t);
         if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
         __syncthreads();
         VT c = s_carry;
         for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
-        VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
+        const VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
         if (lane_id() > 0) c = vmax(c, prev);
-        if (i < tiles) tmax[i] = c;
-        __syncthreads();
-        if (threadIdx.x == 1023) {
-            VT t = s_carry;
-            for (int q = 0; q < 16; q++) t = vmax(t, s_w[q]);
-            s_carry = t;
+#pragma unroll
+        for (int q = 0; q < MX_K; q++) {   // exclusive max of the tiles before i0 + q
+            if (i0 + q < tiles) tmax[i0 + q] = c;
+            c = vmax(c, v[q]);
         }
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = c;   // the last thread's running max covers the step
         __syncthreads();
     }
 }
