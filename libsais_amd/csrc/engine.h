@@ -122,7 +122,7 @@ enum { RT_SMALL = 256 };   // groups up to this size are sorted inside one workg
 constexpr u64 INIT_SMALL_N = 1ull << 20;
 
 struct WsScratch {
-    void *tile_agg;   // [tiles_cap] 16-byte tile aggregates
+    void *tile_agg;   // [tiles_cap] 16-byte tile aggregates, then [tiles_cap / 8192 + 2] chunk aggregates
     u64 tiles_cap;    // tiles of at least 2048 elements (512 below INIT_SMALL_N)
     u64 *total;       // [1]
     u64 *base;        // [1] reserved output base

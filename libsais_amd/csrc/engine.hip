@@ -958,7 +958,8 @@ int sa_build(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, u32 *dSA, u32
     WsScratch ws{};
     ws.tiles_cap = n / (n < INIT_SMALL_N ? 512 : 2048) + 2;   // tiles of the initial grouping
     {
-        GET(tagg, u64, ws.tiles_cap * 2);
+        // the tile aggregates, then the aggregates of their 8192-tile chunks
+        GET(tagg, u64, (ws.tiles_cap + ws.tiles_cap / 8192 + 2) * 2);
         ws.tile_agg = tagg;
         ws.total = d_cnt + 2;
         ws.base = d_cnt + 3;

@@ -75,73 +75,145 @@ __global__ __launch_bounds__(WS_THREADS) void k_ws_reduce(F f, Agg *__restrict__
     }
 }
 
-// Exclusive scan of tile aggregates (one workgroup of 1024 threads; SC_K consecutive
-// tiles per thread and step, 8192 tiles per step, so a 2^28-position text's 131072
-// tiles take 16 steps).  Each step's tiles pass through LDS: loaded and stored with
-// consecutive lanes on consecutive tiles, and read by their owner thread from rows of
-// SC_K + 1 (padding against bank conflicts).  Writes the grand total of active
-// elements and, when out_counter is given, reserves that many slots in the output
-// list: *base = atomicAdd(out_counter, total).
-constexpr int SC_K = 8;
+// Exclusive scan of tile aggregates, in steps of 8192 tiles (SC_K consecutive tiles
+// per thread of a 1024-thread workgroup).  A step's tiles pass through LDS: loaded and
+// stored with consecutive lanes on consecutive tiles, and read by their owner thread
+// from rows of SC_K + 1 (padding against bank conflicts).  Up to one step, one
+// workgroup scans them (k_ws_scan_tiles); beyond, every step is a chunk of its own:
+// k_ws_chunk_reduce combines each chunk, k_ws_scan_tiles scans the chunk aggregates,
+// and k_ws_chunk_apply scans each chunk from its prefix, all chunks at once (the
+// 131072 tiles of a 2^28-position grouping took 0.145 ms in one workgroup).
+constexpr int SC_K = 8, SC_STEP = 1024 * SC_K;
+
+// One step: tiles [b0, b0 + SC_STEP) of tagg replaced by their exclusive prefixes
+// after `carry`; returns the inclusive aggregate through the step's last tile.
+__device__ Agg scan_step(Agg *tagg, u64 tiles, u64 b0, Agg carry, Agg *s_t, Agg *s_w, Agg *s_end)
+{
+    const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE;
+#pragma unroll
+    for (int q = 0; q < SC_K; q++) {
+        const u32 j = (u32)q * 1024 + tid;   // tile b0 + j, owned by thread j / SC_K
+        s_t[(j / SC_K) * (SC_K + 1) + j % SC_K] = b0 + j < tiles ? tagg[b0 + j] : Agg{0, 0, 0};
+    }
+    __syncthreads();
+    Agg v[SC_K];
+    Agg t{0, 0, 0};   // this thread's SC_K tiles combined
+#pragma unroll
+    for (int q = 0; q < SC_K; q++) {
+        v[q] = s_t[tid * (SC_K + 1) + q];
+        t.s += v[q].s;
+        t.m1 = umax(t.m1, v[q].m1);
+        t.m2 = umax(t.m2, v[q].m2);
+    }
+    const u64 is = wave_incl_sum64(t.s);
+    const u32 i1 = wave_incl_max(t.m1), i2 = wave_incl_max(t.m2);
+    if (lane_id() == SAGPU_WAVE - 1) s_w[w] = Agg{is, i1, i2};
+    __syncthreads();
+    Agg c = carry;
+    for (u32 q = 0; q < w; q++) {
+        c.s += s_w[q].s;
+        c.m1 = umax(c.m1, s_w[q].m1);
+        c.m2 = umax(c.m2, s_w[q].m2);
+    }
+    const u32 p1 = __shfl_up(i1, 1, SAGPU_WAVE), p2 = __shfl_up(i2, 1, SAGPU_WAVE);
+    if (lane_id() > 0) {
+        c.m1 = umax(c.m1, p1);
+        c.m2 = umax(c.m2, p2);
+    }
+    c.s += is - t.s;
+#pragma unroll
+    for (int q = 0; q < SC_K; q++) {   // exclusive prefixes into the thread's own row
+        s_t[tid * (SC_K + 1) + q] = c;
+        c.s += v[q].s;
+        c.m1 = umax(c.m1, v[q].m1);
+        c.m2 = umax(c.m2, v[q].m2);
+    }
+    __syncthreads();   // every row written; s_w read
+#pragma unroll
+    for (int q = 0; q < SC_K; q++) {
+        const u32 j = (u32)q * 1024 + tid;
+        if (b0 + j < tiles) tagg[b0 + j] = s_t[(j / SC_K) * (SC_K + 1) + j % SC_K];
+    }
+    if (tid == 1023) *s_end = c;   // the last thread's running value covers the whole step
+    __syncthreads();
+    const Agg e = *s_end;
+    __syncthreads();   // s_t, s_w and s_end are free for the next step
+    return e;
+}
+
 __global__ __launch_bounds__(1024) void k_ws_scan_tiles(Agg *tagg, u64 tiles, u64 *total, u64 *out_counter, u64 *base)
 {
     __shared__ Agg s_t[1024 * (SC_K + 1)];
     __shared__ Agg s_w[16];
-    __shared__ Agg s_carry;
-    const u32 tid = threadIdx.x, w = tid / SAGPU_WAVE;
-    if (tid == 0) s_carry = Agg{0, 0, 0};
-    for (u64 b0 = 0; b0 < tiles; b0 += 1024 * SC_K) {
+    __shared__ Agg s_end;
+    Agg carry{0, 0, 0};
+    for (u64 b0 = 0; b0 < tiles; b0 += SC_STEP) carry = scan_step(tagg, tiles, b0, carry, s_t, s_w, &s_end);
+    if (threadIdx.x == 0) {
+        *total = carry.s;
+        if (out_counter) *base = atomicAdd(out_counter, carry.s);
+    }
+}
+
+// The aggregate of chunk blockIdx.x (tiles [b SC_STEP, (b + 1) SC_STEP)).
+__global__ __launch_bounds__(1024) void k_ws_chunk_reduce(const Agg *__restrict__ tagg, u64 tiles,
+                                                          Agg *__restrict__ cagg)
+{
+    __shared__ Agg s_w[16];
+    const u64 b0 = (u64)blockIdx.x * SC_STEP;
+    Agg t{0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < SC_K; q++) {
-            const u32 j = (u32)q * 1024 + tid;   // tile b0 + j, owned by thread j / SC_K
-            s_t[(j / SC_K) * (SC_K + 1) + j % SC_K] = b0 + j < tiles ? tagg[b0 + j] : Agg{0, 0, 0};
+    for (int q = 0; q < SC_K; q++) {
+        const u64 i = b0 + (u64)q * 1024 + threadIdx.x;
+        if (i < tiles) {
+            const Agg v = tagg[i];
+            t.s += v.s;
+            t.m1 = umax(t.m1, v.m1);
+            t.m2 = umax(t.m2, v.m2);
         }
-        __syncthreads();
-        Agg v[SC_K];
-        Agg t{0, 0, 0};   // this thread's SC_K tiles combined
-#pragma unroll
-        for (int q = 0; q < SC_K; q++) {
-            v[q] = s_t[tid * (SC_K + 1) + q];
-            t.s += v[q].s;
-            t.m1 = umax(t.m1, v[q].m1);
-            t.m2 = umax(t.m2, v[q].m2);
-        }
-        const u64 is = wave_incl_sum64(t.s);
-        const u32 i1 = wave_incl_max(t.m1), i2 = wave_incl_max(t.m2);
-        if (lane_id() == SAGPU_WAVE - 1) s_w[w] = Agg{is, i1, i2};
-        __syncthreads();
-        Agg c = s_carry;
-        for (u32 q = 0; q < w; q++) {
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        t.s += __shfl_xor(t.s, d, SAGPU_WAVE);
+        t.m1 = umax(t.m1, __shfl_xor(t.m1, d, SAGPU_WAVE));
+        t.m2 = umax(t.m2, __shfl_xor(t.m2, d, SAGPU_WAVE));
+    }
+    if (lane_id() == 0) s_w[threadIdx.x / SAGPU_WAVE] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Agg c{0, 0, 0};
+        for (int q = 0; q < 16; q++) {
             c.s += s_w[q].s;
             c.m1 = umax(c.m1, s_w[q].m1);
             c.m2 = umax(c.m2, s_w[q].m2);
         }
-        const u32 p1 = __shfl_up(i1, 1, SAGPU_WAVE), p2 = __shfl_up(i2, 1, SAGPU_WAVE);
-        if (lane_id() > 0) {
-            c.m1 = umax(c.m1, p1);
-            c.m2 = umax(c.m2, p2);
-        }
-        c.s += is - t.s;
-#pragma unroll
-        for (int q = 0; q < SC_K; q++) {   // exclusive prefixes into the thread's own row
-            s_t[tid * (SC_K + 1) + q] = c;
-            c.s += v[q].s;
-            c.m1 = umax(c.m1, v[q].m1);
-            c.m2 = umax(c.m2, v[q].m2);
-        }
-        __syncthreads();   // every row written; s_carry and s_w read
-#pragma unroll
-        for (int q = 0; q < SC_K; q++) {
-            const u32 j = (u32)q * 1024 + tid;
-            if (b0 + j < tiles) tagg[b0 + j] = s_t[(j / SC_K) * (SC_K + 1) + j % SC_K];
-        }
-        if (tid == 1023) s_carry = c;   // the last thread's running value covers the whole step
-        __syncthreads();
+        cagg[blockIdx.x] = c;
     }
-    if (tid == 0) {
-        *total = s_carry.s;
-        if (out_counter) *base = atomicAdd(out_counter, s_carry.s);
+}
+
+// Chunk blockIdx.x scanned from its exclusive prefix cagg[b].
+__global__ __launch_bounds__(1024) void k_ws_chunk_apply(Agg *tagg, u64 tiles, const Agg *__restrict__ cagg)
+{
+    __shared__ Agg s_t[1024 * (SC_K + 1)];
+    __shared__ Agg s_w[16];
+    __shared__ Agg s_end;
+    scan_step(tagg, tiles, (u64)blockIdx.x * SC_STEP, cagg[blockIdx.x], s_t, s_w, &s_end);
+}
+
+// Exclusive scan of `tiles` aggregates in ws.tile_agg (chunk aggregates after the
+// tiles_cap entries); total, and the reserved base when out_counter is given.
+int ws_scan(WsScratch &ws, u64 tiles, u64 *out_counter, hipStream_t s)
+{
+    Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
+    if (tiles <= (u64)SC_STEP) {
+        hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, out_counter, ws.base);
+    } else {
+        const u64 nch = (tiles + SC_STEP - 1) / SC_STEP;
+        Agg *cagg = tagg + ws.tiles_cap;
+        hipLaunchKernelGGL(k_ws_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, s, (const Agg *)tagg, tiles, cagg);
+        hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, cagg, nch, ws.total, out_counter, ws.base);
+        hipLaunchKernelGGL(k_ws_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, s, tagg, tiles, (const Agg *)cagg);
     }
+    SAGPU_CHECK(hipGetLastError());
+    return SAGPU_OK;
 }
 
 template <class F, int R>
@@ -597,7 +669,7 @@ int ws_run(WsScratch &ws, const F &f, u64 n, u64 *out_counter, hipStream_t s)
     }
     Agg *tagg = reinterpret_cast<Agg *>(ws.tile_agg);
     hipLaunchKernelGGL((k_ws_reduce<F, WS_RUN_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
-    hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, out_counter, ws.base);
+    SAGPU_TRY(ws_scan(ws, tiles, out_counter, s));
     hipLaunchKernelGGL((k_ws_emit<F, WS_RUN_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg,
                        out_counter ? ws.base : (const u64 *)nullptr);
     SAGPU_CHECK(hipGetLastError());
@@ -1279,7 +1351,7 @@ int group_init_t(WsScratch &ws, const K *keys, const u32 *vals, u64 n, int bits,
     if (rows == 2) hipLaunchKernelGGL((k_init_reduce<K, 2>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     else if (rows == 8) hipLaunchKernelGGL((k_init_reduce<K, 8>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
     else hipLaunchKernelGGL((k_init_reduce<K, WS_ROWS>), dim3((unsigned)tiles), dim3(WS_THREADS), 0, s, f, tagg);
-    hipLaunchKernelGGL(k_ws_scan_tiles, dim3(1), dim3(1024), 0, s, tagg, tiles, ws.total, (u64 *)nullptr, ws.base);
+    SAGPU_TRY(ws_scan(ws, tiles, (u64 *)nullptr, s));
     SAGPU_TRY(scan_exclusive_u32(ws.scan, ws.bcnt, nb * tiles, s));
     ktime_begin(KI_INIT_EMIT);
     if (rows == 2)
