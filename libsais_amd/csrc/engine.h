@@ -333,7 +333,7 @@ struct LcpScratch {
     u32 *vq_pos;      // [256] very-long queue: positions still matching after the
     u64 *vq_off;      // [256] first pass's steps, their offset reached,
     u64 *vq_res;      // [256] and first mismatch
-    void *tile_max;   // [tiles] per-tile max for the prefix-max scan (u32 or u64)
+    void *tile_max;   // [tiles_cap] per-tile max for the prefix-max scan (u32 or u64), then chunk maxima
     u64 tiles_cap;
     // optional: called with 0 / 1 around the launch of the pair-word text-order pass
     void (*hook)(void *user, int phase);

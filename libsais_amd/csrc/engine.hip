@@ -1350,7 +1350,7 @@ int plcp_run(Ctx &c, Scope &sc, const uint8_t *dT, u64 n, int gsa, const u32 *PH
     GET(q, u32, ls.queue_cap);
     GET(ql, u32, ls.queue_cap);
     GET(qn, u32, 64);
-    GET(tm, u64, ls.tiles_cap);
+    GET(tm, u64, ls.tiles_cap + ls.tiles_cap / 8192 + 2);   // tile maxima, then chunk maxima
     GET(vqp, u32, 256);
     GET(vqo, u64, 256);
     GET(vqr, u64, 256);

@@ -669,41 +669,83 @@ __global__ __launch_bounds__(MS_THREADS) void k_max_reduce(const VT *__restrict_
     }
 }
 
+// Exclusive max-scan of the tile maxima in steps of 8192 tiles (MX_K consecutive
+// tiles per thread of a 1024-thread workgroup).  Up to one step, one workgroup scans
+// them (k_max_scan_tiles); beyond, each step is a chunk: k_max_chunk_reduce takes each
+// chunk's max, k_max_scan_tiles scans those, and k_max_chunk_apply scans every chunk
+// from its prefix, all at once (65536 tiles of a 2^28-position text: 61 us in one
+// workgroup).
+constexpr int MX_K = 8, MX_STEP = 1024 * MX_K;
+
+// One step: tiles [base, base + MX_STEP) replaced by the max of everything before
+// each (from `carry`); returns the max through the step's last tile.
+template <typename VT>
+__device__ VT max_step(VT *tmax, u64 tiles, u64 base, VT carry, VT *s_w, VT *s_end)
+{
+    const u64 i0 = base + (u64)threadIdx.x * MX_K;
+    VT v[MX_K];
+    VT t = 0;
+#pragma unroll
+    for (int q = 0; q < MX_K; q++) {
+        v[q] = i0 + q < tiles ? tmax[i0 + q] : (VT)0;
+        t = vmax(t, v[q]);
+    }
+    const VT inc = wave_incl_max_t(t);
+    if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
+    __syncthreads();
+    VT c = carry;
+    for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
+    const VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
+    if (lane_id() > 0) c = vmax(c, prev);
+#pragma unroll
+    for (int q = 0; q < MX_K; q++) {   // exclusive max of the tiles before i0 + q
+        if (i0 + q < tiles) tmax[i0 + q] = c;
+        c = vmax(c, v[q]);
+    }
+    if (threadIdx.x == 1023) *s_end = c;   // the last thread's running max covers the step
+    __syncthreads();
+    const VT e = *s_end;
+    __syncthreads();   // s_w and s_end are free for the next step
+    return e;
+}
+
 template <typename VT>
 __global__ __launch_bounds__(1024) void k_max_scan_tiles(VT *tmax, u64 tiles)
 {
-    // MX_K consecutive tiles per thread and step (8192 tiles per step: 8 steps for the
-    // 65536 tiles of a 2^28-position text instead of 64)
-    constexpr int MX_K = 8;
     __shared__ VT s_w[16];
-    __shared__ VT s_carry;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (u64 base = 0; base < tiles; base += 1024 * MX_K) {
-        const u64 i0 = base + (u64)threadIdx.x * MX_K;
-        VT v[MX_K];
-        VT t = 0;
+    __shared__ VT s_end;
+    VT carry = 0;
+    for (u64 base = 0; base < tiles; base += MX_STEP) carry = max_step(tmax, tiles, base, carry, s_w, &s_end);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(1024) void k_max_chunk_reduce(const VT *__restrict__ tmax, u64 tiles,
+                                                           VT *__restrict__ cmax)
+{
+    __shared__ VT s_w[16];
+    const u64 b0 = (u64)blockIdx.x * MX_STEP;
+    VT t = 0;
 #pragma unroll
-        for (int q = 0; q < MX_K; q++) {
-            v[q] = i0 + q < tiles ? tmax[i0 + q] : (VT)0;
-            t = vmax(t, v[q]);
-        }
-        const VT inc = wave_incl_max_t(t);
-        if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = inc;
-        __syncthreads();
-        VT c = s_carry;
-        for (u32 q = 0; q < threadIdx.x / SAGPU_WAVE; q++) c = vmax(c, s_w[q]);
-        const VT prev = __shfl_up(inc, 1, SAGPU_WAVE);
-        if (lane_id() > 0) c = vmax(c, prev);
-#pragma unroll
-        for (int q = 0; q < MX_K; q++) {   // exclusive max of the tiles before i0 + q
-            if (i0 + q < tiles) tmax[i0 + q] = c;
-            c = vmax(c, v[q]);
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) s_carry = c;   // the last thread's running max covers the step
-        __syncthreads();
+    for (int q = 0; q < MX_K; q++) {
+        const u64 i = b0 + (u64)q * 1024 + threadIdx.x;
+        if (i < tiles) t = vmax(t, tmax[i]);
     }
+    t = wave_incl_max_t(t);
+    if (lane_id() == SAGPU_WAVE - 1) s_w[threadIdx.x / SAGPU_WAVE] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        VT c = 0;
+        for (int q = 0; q < 16; q++) c = vmax(c, s_w[q]);
+        cmax[blockIdx.x] = c;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(1024) void k_max_chunk_apply(VT *tmax, u64 tiles, const VT *__restrict__ cmax)
+{
+    __shared__ VT s_w[16];
+    __shared__ VT s_end;
+    max_step(tmax, tiles, (u64)blockIdx.x * MX_STEP, cmax[blockIdx.x], s_w, &s_end);
 }
 
 // Elements 4*lane .. 4*lane+3 of a 256-element row: one 16-byte (u32) or two
@@ -913,7 +955,15 @@ int launch_plcp_t(LcpScratch &ls, const uint8_t *T, u64 n, int gsa, const u32 *P
                            fused_max ? tmax : (VT *)nullptr, vq);
     }
     if (!fused_max) hipLaunchKernelGGL(k_max_reduce<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax);
-    hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
+    if (tiles <= (u64)MX_STEP) {
+        hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, tmax, tiles);
+    } else {
+        const u64 nch = (tiles + MX_STEP - 1) / MX_STEP;
+        VT *cmax = (VT *)((u64 *)ls.tile_max + ls.tiles_cap);   // chunk maxima after the tile maxima
+        hipLaunchKernelGGL(k_max_chunk_reduce<VT>, dim3((unsigned)nch), dim3(1024), 0, s, (const VT *)tmax, tiles, cmax);
+        hipLaunchKernelGGL(k_max_scan_tiles<VT>, dim3(1), dim3(1024), 0, s, cmax, nch);
+        hipLaunchKernelGGL(k_max_chunk_apply<VT>, dim3((unsigned)nch), dim3(1024), 0, s, tmax, tiles, (const VT *)cmax);
+    }
     hipLaunchKernelGGL(k_max_apply<VT>, dim3((unsigned)tiles), dim3(MS_THREADS), 0, s, V, n, tmax, PLCP);
     SAGPU_CHECK(hipGetLastError());
     return SAGPU_OK;
